@@ -1,0 +1,174 @@
+/*
+ * sdpd.h -- C-ABI of the MI355X SDPD step (drop-in for MCF's per-step
+ * density / force / integration loops).
+ *
+ * The reference has no FFI layer: its boundary is the set of Fortran
+ * module procedures that marching_integrate_VV calls on TYPE(Particles)
+ * and TYPE(Technique).  Each entry point below names the procedure it
+ * replaces.  A Fortran host binds them with iso_c_binding
+ * (mcf_amd/fortran/sdpd_c.f90).
+ *
+ * Conventions
+ *  - every call returns 0 on success and a negative code on failure,
+ *    mirroring the reference's INTEGER stat_info (0 / -1); the message is
+ *    available from sdpd_last_error();
+ *  - host arrays use the reference's column-major layout: x(num_dim, N),
+ *    id(2, N) with pid in row 1 and sid in row 2, ct(num_dim*num_dim, N);
+ *  - the context owns every device buffer; host arrays are copied only by
+ *    sdpd_upload / sdpd_download / sdpd_stats;
+ *  - one host thread and one HIP stream per context, calls are not
+ *    reentrant on the same context;
+ *  - particles are kept in cell order on the device; downloads return them
+ *    in upload order.
+ */
+#ifndef SDPD_H
+#define SDPD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDPD_OK 0
+#define SDPD_ERR_ARG -1      /* bad argument or parameter                 */
+#define SDPD_ERR_STATE -2    /* call out of order (e.g. no particles yet) */
+#define SDPD_ERR_DEVICE -3   /* HIP runtime error                          */
+#define SDPD_ERR_PHYSICS -4  /* numerical check failed (e.g. n <= 0)       */
+
+#define SDPD_PRECISION_FP64 0
+#define SDPD_PRECISION_MIXED 1
+
+/* Physics and control parameters; names follow the reference's config keys
+ * (mcf/mcf_config/ctrl.mcf:44-215, physics_config.mcf). */
+typedef struct sdpd_params {
+  int32_t num_dim;            /* 2 or 3                                     */
+  int32_t symmetry;           /* image convention of the reference's mode   */
+  int32_t kernel_type;        /* 1 quintic spline, 2 Lucy                    */
+  int32_t rhs_density_type;   /* 1 mass density, 2 number density            */
+  int32_t rhs_force_type;     /* 3 Hu-Adams, 4 Hu-Adams angular              */
+  int32_t state_equation_type;/* 1 Morris, 2 Batchelor (Tait)                */
+  int32_t newtonian;          /* 0: Oldroyd-B conformation tensor            */
+  int32_t brownian;           /* thermal noise                               */
+  int32_t body_force_type;    /* 0 none, 1 constant                          */
+  int32_t precision;          /* SDPD_PRECISION_FP64 or _MIXED               */
+  double min_phys[3];
+  double max_phys[3];
+  double cut_off;
+  double c, rho, rho_ref, gamma;
+  double eta, eta_coef, kt, dt;
+  double tau, n_p, kt_p;
+  double body_force[3];
+  uint64_t seed;              /* key of the counter-based noise stream       */
+} sdpd_params;
+
+typedef struct sdpd_ctx sdpd_ctx;
+
+/* Replaces mcf.F90 start-up of Particles/Technique for this path
+ * (particles_new, technique_init_ppm: mcf.F90:616-649). */
+int sdpd_create(const sdpd_params *params, sdpd_ctx **ctx);
+
+/* Replaces particles_init_global_exter / io_read_particles hand-over
+ * (io_read_particles.F90:162-240).  x, v: (num_dim, n); m: n;
+ * id: (2, n) = (pid, sid).  Only fluid particles (sid == 0). */
+int sdpd_upload(sdpd_ctx *ctx, const double *x, const double *v,
+                const double *m, const int32_t *id, int32_t n);
+
+/* Oldroyd-B initial conformation ct(num_dim*num_dim, n); the default
+ * after sdpd_upload is the identity (particles_init_partial_inter.F90:145-149). */
+int sdpd_upload_ct(sdpd_ctx *ctx, const double *ct);
+
+/* Replaces technique_build_list (technique_build_list.F90:1-127 ->
+ * ppm_neighlist_clist + ppm_util_rank3d + MkNeighIdx): sorts particles
+ * into cells with a stable counting sort. */
+int sdpd_build_cells(sdpd_ctx *ctx);
+
+/* Replaces particles_compute_density (particles_compute_density.F90:1-903).
+ * Also evaluates the equation of state (particles_compute_pressure,
+ * particles_compute_pressure.F90:1-106) in the same pass. */
+int sdpd_density(sdpd_ctx *ctx);
+
+/* Replaces particles_compute_pressure when called on its own (the pressure
+ * is already current after sdpd_density; this recomputes it from rho). */
+int sdpd_eos(sdpd_ctx *ctx);
+
+/* Replaces particles_compute_interaction (particles_compute_interaction.F90)
+ * plus particles_apply_body_force (particles_apply_body_force.F90:96-405)
+ * and, when non-Newtonian, particles_compute_act
+ * (particles_compute_act.F90:112-156).  Noise uses the context step. */
+int sdpd_forces(sdpd_ctx *ctx);
+
+/* Replaces particles_integrate_velocity (particles_integrate_velocity.F90:84-94):
+ * v += lambda * f * dt for fluid particles. */
+int sdpd_kick(sdpd_ctx *ctx, double lambda);
+
+/* Replaces particles_integrate_position (order 1) followed by
+ * particles_adjust_particles (particles_integrate_position.F90:82-123,
+ * particles_adjust_particles.F90:154-161,272-279): x += v*dt, periodic wrap. */
+int sdpd_drift_wrap(sdpd_ctx *ctx);
+
+/* Replaces particles_integrate_ct (particles_integrate_ct.F90:66-68):
+ * ct += lambda * dt * act. */
+int sdpd_conformation(sdpd_ctx *ctx, double lambda);
+
+/* Replaces statistic_compute_statistic (statistic_compute_statistic.F90:230-283):
+ * kinetic energy and momentum of fluid particles, FP64 compensated sums. */
+int sdpd_stats(sdpd_ctx *ctx, double *mom /*[3]*/, double *ekin);
+
+/* Copies state back in upload order.  Any pointer may be NULL.
+ * x, v, f: (num_dim, n); rho, p: n; ct, vgt: (num_dim*num_dim, n). */
+int sdpd_download(sdpd_ctx *ctx, double *x, double *v, double *rho,
+                  double *p, double *f);
+int sdpd_download_tensors(sdpd_ctx *ctx, double *ct, double *vgt);
+
+/* Replaces marching_marching's loop body with integrate_type = 2
+ * (marching_integrate.F90:53-61 -> marching_integrate_VV.F90:303-1619):
+ * advances nsteps velocity-Verlet steps.  The first call after upload
+ * performs the initial evaluation of marching_marching.F90:528-1123
+ * unless sdpd_forces has run since. */
+int sdpd_step(sdpd_ctx *ctx, int32_t nsteps);
+
+/* Noise step counter (the reference's marching step). */
+int sdpd_set_step(sdpd_ctx *ctx, int64_t step);
+int sdpd_get_step(sdpd_ctx *ctx, int64_t *step);
+
+/* Cell list in upload indices, for checking against the reference's
+ * ppm_util_rank3d output: lhbx(ncell+1) offsets, lpdx(n) upload indices,
+ * nm[3] cells per dimension. */
+int sdpd_get_cells(sdpd_ctx *ctx, int32_t *nm, int32_t *lhbx, int32_t *lpdx);
+
+/* Number of in-cutoff ordered pairs seen by the last force pass and the
+ * number of candidate distance tests resolved in FP64 (mixed mode). */
+int sdpd_get_counters(sdpd_ctx *ctx, int64_t *pairs, int64_t *fp64_rechecks);
+
+/* Neighbour-set signature of the current cell list, in upload order: the
+ * number of in-cutoff neighbours of each particle and the wrapping sum of
+ * a 32-bit hash of (pid_i, pid_j) over them.  mixed_decision = 1 uses the
+ * mixed-precision in/out decision (FP32 test, FP64 inside the band). */
+int sdpd_neighbor_signature(sdpd_ctx *ctx, int32_t mixed_decision,
+                            int32_t *count, uint32_t *hash);
+
+/* Counter-based noise primitive, exposed for checking the stream. */
+int sdpd_pair_normals(const uint32_t *pid_lo, const uint32_t *pid_hi,
+                      int32_t count, uint64_t seed, int64_t step, float *g);
+
+/* Per-phase kernel timing with HIP events on the context's stream (the
+ * counterpart of the reference's VV phase timers with debug_flag = 3,
+ * marching_integrate_VV.F90:264-277,1679-1709).  Phases:
+ * 0 step boundary / kicks, 1 cell sort, 2 density + EOS, 3 pair forces,
+ * 4 statistics.  sdpd_get_phase_times waits for the stream, returns the
+ * summed milliseconds and launch counts per phase and clears them. */
+#define SDPD_NUM_PHASES 5
+int sdpd_set_timing(sdpd_ctx *ctx, int32_t enable);
+int sdpd_get_phase_times(sdpd_ctx *ctx, double *ms, int64_t *launches);
+
+/* Waits for all work queued on the context's stream. */
+int sdpd_sync(sdpd_ctx *ctx);
+
+const char *sdpd_last_error(const sdpd_ctx *ctx);
+int sdpd_destroy(sdpd_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,728 @@
+// sdpd_api.hip -- the C-ABI declared in include/sdpd.h: context, device
+// buffers, host <-> device copies and the velocity-Verlet step driver.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/sdpd.h"
+#include "sdpd_kernels.h"
+
+using sdpd::DevParams;
+using sdpd::DeviceState;
+
+struct sdpd_ctx {
+  sdpd_params prm{};
+  DevParams P{};
+  DeviceState S{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int n = 0;
+  int stats_nb = 1;
+  int64_t step = 0;
+  bool cells_valid = false;  // particles sorted for the current positions
+  bool rho_valid = false;    // rho / p current for the current order
+  bool f_valid = false;      // f (and act) current for the current order
+  // phase timing: pairs of events recorded around each phase's launches
+  bool timing = false;
+  struct Mark {
+    int phase;
+    hipEvent_t a, b;
+  };
+  std::vector<Mark> marks;
+  std::vector<hipEvent_t> event_pool;
+  std::string err;
+};
+
+namespace {
+
+constexpr double kPi = 3.141592653589793238;
+constexpr double kMachineZero = 4.440892098500626e-16;
+
+int fail(sdpd_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, call)                                                         \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail((ctx), SDPD_ERR_DEVICE,                                          \
+                  std::string(#call) + ": " + hipGetErrorString(e_));              \
+  } while (0)
+
+int check_launch(sdpd_ctx* c, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return fail(c, SDPD_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+  return SDPD_OK;
+}
+
+// Derived constants, with the reference's start-up rules.
+int derive(const sdpd_params& p, DevParams& P, std::string& err) {
+  std::memset(&P, 0, sizeof(P));
+  if (p.num_dim != 2 && p.num_dim != 3) return err = "num_dim must be 2 or 3", SDPD_ERR_ARG;
+  if (!(p.cut_off > 0)) return err = "cut_off must be positive", SDPD_ERR_ARG;
+  if (p.kernel_type != 1 && p.kernel_type != 2)
+    return err = "kernel_type must be 1 (quintic spline) or 2 (Lucy)", SDPD_ERR_ARG;
+  if (p.rhs_density_type != 1 && p.rhs_density_type != 2)
+    return err = "rhs_density_type must be 1 or 2", SDPD_ERR_ARG;
+  if (p.rhs_force_type != 3 && p.rhs_force_type != 4)
+    return err = "rhs_force_type must be 3 (Hu-Adams) or 4 (angular)", SDPD_ERR_ARG;
+  if (p.state_equation_type != 1 && p.state_equation_type != 2)
+    return err = "stateEquation_type must be 1 or 2", SDPD_ERR_ARG;
+  if (p.state_equation_type == 1 && p.c == 0.0)
+    return err = "stateEquation_init_Morris: sound speed should not be zero", SDPD_ERR_ARG;
+  if (p.state_equation_type == 2 && (p.rho_ref == 0.0 || p.gamma == 0.0))
+    return err = "stateEquation_init_Batchelor: neither rho_ref nor gamma can be zero",
+           SDPD_ERR_ARG;
+  if (!p.newtonian && p.rhs_force_type != 3)
+    return err = "non-Newtonian flow needs rhs_force_type = 3", SDPD_ERR_ARG;
+  if (!p.newtonian && std::fabs(p.tau) < kMachineZero)
+    return err = "particles_compute_act: tau should not be zero", SDPD_ERR_ARG;
+  if (p.brownian && p.rhs_density_type != 2)
+    return err = "Brownian motion requires rhs_density_type = 2", SDPD_ERR_ARG;
+  if (p.eta < 0) return err = "dissipative force should be non-positive (eta >= 0)", SDPD_ERR_ARG;
+  if (!(p.dt > 0)) return err = "dt must be positive", SDPD_ERR_ARG;
+  if (p.precision != SDPD_PRECISION_FP64 && p.precision != SDPD_PRECISION_MIXED)
+    return err = "precision must be 0 (FP64) or 1 (mixed)", SDPD_ERR_ARG;
+
+  P.dim = p.num_dim;
+  P.symmetry = p.symmetry ? 1 : 0;
+  P.kernel_type = p.kernel_type;
+  P.density_type = p.rhs_density_type;
+  P.force_type = p.rhs_force_type;
+  P.eos_type = p.state_equation_type;
+  P.newtonian = p.newtonian ? 1 : 0;
+  const double kt = p.brownian ? p.kt : 0.0;  // physics_adjust_parameters.F90:415-417
+  P.noise = (p.brownian && kt > kMachineZero) ? 1 : 0;
+  P.body_force = 0;
+  if (p.body_force_type == 1)
+    for (int d = 0; d < p.num_dim; ++d)
+      if (std::fabs(p.body_force[d]) > kMachineZero) P.body_force = 1;
+  const double cell = p.cut_off * 1.0001;  // mcf.F90:617
+  int64_t ncell = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (d < p.num_dim) {
+      double L = p.max_phys[d] - p.min_phys[d];
+      if (!(L > 0)) return err = "max_phys must exceed min_phys", SDPD_ERR_ARG;
+      int nm = (int)(L / cell);
+      if (nm < 1) nm = 1;
+      if (nm < 3)
+        return err = "the box must span at least 3 cells of 1.0001*cut_off per dimension",
+               SDPD_ERR_ARG;
+      P.nm[d] = nm;
+      P.min_phys[d] = p.min_phys[d];
+      P.length[d] = L;
+      P.rd[d] = (double)nm / L;
+      P.cs[d] = L / (double)nm;
+    } else {
+      P.nm[d] = 1;
+      P.min_phys[d] = 0.0;
+      P.length[d] = 1.0;
+      P.rd[d] = 1.0;
+      P.cs[d] = 1.0;
+    }
+    ncell *= P.nm[d];
+  }
+  if (ncell > (int64_t)1 << 30) return err = "too many cells", SDPD_ERR_ARG;
+  P.ncell = (int32_t)ncell;
+  P.cut_off = p.cut_off;
+  P.cut2 = p.cut_off * p.cut_off;
+  // kernel_new.F90:47-93
+  if (p.kernel_type == 1) {
+    P.h = p.cut_off / 3.0;
+    P.coef = p.num_dim == 2 ? 7.0 / (478.0 * kPi * P.h * P.h)
+                            : 3.0 / (359.0 * kPi * P.h * P.h * P.h);
+  } else {
+    P.h = p.cut_off;
+    P.coef = p.num_dim == 2 ? 5.0 / (kPi * P.h * P.h)
+                            : 105.0 / (16.0 * kPi * P.h * P.h * P.h);
+  }
+  P.coef_grad = -P.coef / P.h;
+  P.w0 = p.kernel_type == 1 ? P.coef * (243.0 + -192.0 + 15.0) : P.coef;
+  P.c2 = p.c * p.c;
+  P.rho_ref = p.rho_ref;
+  P.gamma = p.gamma;
+  P.p0 = p.state_equation_type == 2
+             ? p.c * p.c * p.rho * std::pow(p.rho_ref / p.rho, p.gamma) / p.gamma
+             : 0.0;  // stateEquation_new.F90:102
+  P.eta = p.eta;
+  P.eta_coef = p.eta_coef;
+  P.kt = kt;
+  P.dt = p.dt;
+  P.sqrt_dt = std::sqrt(p.dt);
+  P.tau = p.tau;
+  P.G = p.n_p * p.kt_p;  // particles_compute_pressure_tensor.F90:94
+  for (int d = 0; d < 3; ++d) P.bf[d] = p.body_force[d];
+  P.seed = p.seed;
+  P.step = 0;
+  const float c2f = (float)P.cut2;
+  P.cut2_lo_f32 = c2f * (1.0f - 2e-5f);
+  P.cut2_hi_f32 = c2f * (1.0f + 2e-5f);
+  return SDPD_OK;
+}
+
+template <typename T>
+int dalloc(sdpd_ctx* c, T** p, size_t count) {
+  if (count == 0) count = 1;
+  HIP_TRY(c, hipMalloc((void**)p, count * sizeof(T)));
+  return SDPD_OK;
+}
+
+void free_state(DeviceState& S) {
+  void* ptrs[] = {S.x,        S.v,        S.m,        S.pid,   S.perm,     S.x_alt,   S.v_alt,
+                  S.m_alt,    S.pid_alt,  S.perm_alt, S.f,     S.rho,      S.p,       S.ct,
+                  S.ct_alt,   S.act,      S.vgt,      S.key,   S.slot,     S.count,   S.start,
+                  S.unsorted, S.key_sorted, S.xr,     S.vr,    S.br,       S.partials, S.stats_out,
+                  S.nbr_count, S.recheck, S.nbr_hash};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  std::memset(&S, 0, sizeof(S));
+}
+
+int alloc_state(sdpd_ctx* c, int n) {
+  DeviceState& S = c->S;
+  const DevParams& P = c->P;
+  const size_t dim = P.dim, d2 = dim * dim, N = (size_t)n;
+  int rc;
+#define ALLOC(ptr, cnt) \
+  if ((rc = dalloc(c, &S.ptr, (cnt))) != SDPD_OK) return rc;
+  ALLOC(x, dim * N);
+  ALLOC(v, dim * N);
+  ALLOC(m, N);
+  ALLOC(pid, N);
+  ALLOC(perm, N);
+  ALLOC(x_alt, dim * N);
+  ALLOC(v_alt, dim * N);
+  ALLOC(m_alt, N);
+  ALLOC(pid_alt, N);
+  ALLOC(perm_alt, N);
+  ALLOC(f, dim * N);
+  ALLOC(rho, N);
+  ALLOC(p, N);
+  if (!P.newtonian) {
+    ALLOC(ct, d2 * N);
+    ALLOC(ct_alt, d2 * N);
+    ALLOC(act, d2 * N);
+    ALLOC(vgt, d2 * N);
+  }
+  ALLOC(key, N);
+  ALLOC(slot, N);
+  ALLOC(count, (size_t)P.ncell);
+  ALLOC(start, (size_t)P.ncell + 1);
+  ALLOC(unsorted, N);
+  ALLOC(key_sorted, N);
+  if (c->prm.precision == SDPD_PRECISION_MIXED) {
+    ALLOC(xr, N);
+    ALLOC(vr, N);
+    ALLOC(br, N);
+  }
+  c->stats_nb = sdpd::stats_blocks(n);
+  ALLOC(partials, (size_t)c->stats_nb * 4);
+  ALLOC(stats_out, 8);
+  ALLOC(nbr_count, N);
+  ALLOC(recheck, N);
+  ALLOC(nbr_hash, N);
+#undef ALLOC
+  return SDPD_OK;
+}
+
+void swap_sorted(DeviceState& S) {
+  std::swap(S.x, S.x_alt);
+  std::swap(S.v, S.v_alt);
+  std::swap(S.m, S.m_alt);
+  std::swap(S.pid, S.pid_alt);
+  std::swap(S.perm, S.perm_alt);
+  std::swap(S.ct, S.ct_alt);
+}
+
+hipEvent_t take_event(sdpd_ctx* c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// RAII bracket of one phase on the context stream (no-op unless timing)
+struct PhaseMark {
+  sdpd_ctx* c;
+  int phase;
+  hipEvent_t a = nullptr;
+  PhaseMark(sdpd_ctx* ctx, int ph) : c(ctx), phase(ph) {
+    if (c->timing && (a = take_event(c))) (void)hipEventRecord(a, c->stream);
+  }
+  ~PhaseMark() {
+    if (!a) return;
+    hipEvent_t b = take_event(c);
+    if (!b) {
+      c->event_pool.push_back(a);
+      return;
+    }
+    (void)hipEventRecord(b, c->stream);
+    c->marks.push_back({phase, a, b});
+  }
+};
+
+void release_events(sdpd_ctx* c) {
+  for (auto& m : c->marks) {
+    c->event_pool.push_back(m.a);
+    c->event_pool.push_back(m.b);
+  }
+  c->marks.clear();
+}
+
+int need_particles(sdpd_ctx* c) {
+  if (!c) return SDPD_ERR_ARG;
+  if (c->n <= 0) return fail(c, SDPD_ERR_STATE, "no particles: call sdpd_upload first");
+  return SDPD_OK;
+}
+
+// K1 for the current positions (keys, histogram, scan, scatter, place)
+int run_sort(sdpd_ctx* c, bool keys_done) {
+  c->P.step = c->step;
+  {
+    PhaseMark pm(c, 1);
+    if (!keys_done) sdpd::launch_cell_keys(c->stream, c->P, c->S);
+    sdpd::launch_cell_sort_after_keys(c->stream, c->P, c->S);
+  }
+  int rc = check_launch(c, "cell sort");
+  if (rc) return rc;
+  swap_sorted(c->S);
+  c->cells_valid = true;
+  c->rho_valid = false;
+  c->f_valid = false;
+  return SDPD_OK;
+}
+
+int run_density(sdpd_ctx* c) {
+  c->P.step = c->step;
+  {
+    PhaseMark pm(c, 2);
+    sdpd::launch_density_f64(c->stream, c->P, c->S);
+  }
+  int rc = check_launch(c, "density");
+  if (rc) return rc;
+  c->rho_valid = true;
+  return SDPD_OK;
+}
+
+int run_forces(sdpd_ctx* c) {
+  c->P.step = c->step;
+  {
+    PhaseMark pm(c, 3);
+    sdpd::launch_forces_f64(c->stream, c->P, c->S);
+  }
+  int rc = check_launch(c, "forces");
+  if (rc) return rc;
+  c->f_valid = true;
+  return SDPD_OK;
+}
+
+// column-major host (dim, n) in upload order  <->  device SoA in cell order
+int download_vec(sdpd_ctx* c, const double* dev, int comps, double* host) {
+  const int n = c->n;
+  std::vector<double> tmp((size_t)comps * n);
+  std::vector<int32_t> perm(n);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpy(tmp.data(), dev, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(perm.data(), c->S.perm, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; ++i) {
+    int32_t u = perm[i];
+    if (u < 0 || u >= n) return fail(c, SDPD_ERR_STATE, "corrupt permutation");
+    for (int q = 0; q < comps; ++q) host[(size_t)q + (size_t)comps * u] = tmp[(size_t)q * n + i];
+  }
+  return SDPD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sdpd_create(const sdpd_params* params, sdpd_ctx** out) {
+  if (!params || !out) return SDPD_ERR_ARG;
+  *out = nullptr;
+  sdpd_ctx* c = new sdpd_ctx();
+  c->prm = *params;
+  std::string err;
+  int rc = derive(c->prm, c->P, err);
+  if (rc != SDPD_OK) {
+    std::fprintf(stderr, "sdpd_create : %s\n", err.c_str());
+    delete c;
+    return rc;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    std::fprintf(stderr, "sdpd_create : no HIP device available\n");
+    delete c;
+    return SDPD_ERR_DEVICE;
+  }
+  if (hipGetDevice(&c->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    std::fprintf(stderr, "sdpd_create : stream creation failed\n");
+    delete c;
+    return SDPD_ERR_DEVICE;
+  }
+  *out = c;
+  return SDPD_OK;
+}
+
+int sdpd_destroy(sdpd_ctx* c) {
+  if (!c) return SDPD_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  release_events(c);
+  for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+  c->event_pool.clear();
+  free_state(c->S);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return SDPD_OK;
+}
+
+const char* sdpd_last_error(const sdpd_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int sdpd_upload(sdpd_ctx* c, const double* x, const double* v, const double* m,
+                const int32_t* id, int32_t n) {
+  if (!c) return SDPD_ERR_ARG;
+  if (n <= 0 || !x || !v || !m || !id) return fail(c, SDPD_ERR_ARG, "sdpd_upload: bad arguments");
+  const int dim = c->P.dim;
+  for (int32_t i = 0; i < n; ++i) {
+    if (id[2 * (size_t)i + 1] != 0)
+      return fail(c, SDPD_ERR_ARG, "sdpd_upload: only fluid particles (sid = 0) are supported");
+    if (!(m[i] > kMachineZero))
+      return fail(c, SDPD_ERR_PHYSICS, "sdpd_upload: particle mass must be positive");
+    for (int d = 0; d < dim; ++d) {
+      double xv = x[(size_t)d + (size_t)dim * i];
+      if (!(xv >= c->P.min_phys[d] && xv < c->P.min_phys[d] + c->P.length[d]))
+        return fail(c, SDPD_ERR_ARG, "sdpd_upload: particle outside [min_phys, max_phys)");
+    }
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  free_state(c->S);
+  c->n = 0;
+  c->P.n = n;
+  int rc = alloc_state(c, n);
+  if (rc) {
+    free_state(c->S);
+    return rc;
+  }
+  std::vector<double> xs((size_t)dim * n), vs((size_t)dim * n);
+  std::vector<int32_t> pid(n), perm(n);
+  for (int32_t i = 0; i < n; ++i) {
+    for (int d = 0; d < dim; ++d) {
+      xs[(size_t)d * n + i] = x[(size_t)d + (size_t)dim * i];
+      vs[(size_t)d * n + i] = v[(size_t)d + (size_t)dim * i];
+    }
+    pid[i] = id[2 * (size_t)i];
+    perm[i] = i;
+  }
+  HIP_TRY(c, hipMemcpy(c->S.x, xs.data(), xs.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->S.v, vs.data(), vs.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->S.m, m, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->S.pid, pid.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->S.perm, perm.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
+  sdpd::launch_fill_f64(c->stream, c->S.f, 0.0, (int64_t)dim * n);
+  sdpd::launch_fill_f64(c->stream, c->S.rho, 0.0, n);
+  sdpd::launch_fill_f64(c->stream, c->S.p, 0.0, n);
+  if (!c->P.newtonian) {
+    // ct = I (particles_init_partial_inter.F90:145-149), act = vgt = 0
+    const int d2 = dim * dim;
+    for (int q = 0; q < d2; ++q) {
+      double val = (q % (dim + 1) == 0) ? 1.0 : 0.0;
+      sdpd::launch_fill_f64(c->stream, c->S.ct + (size_t)q * n, val, n);
+    }
+    sdpd::launch_fill_f64(c->stream, c->S.act, 0.0, (int64_t)d2 * n);
+    sdpd::launch_fill_f64(c->stream, c->S.vgt, 0.0, (int64_t)d2 * n);
+  }
+  rc = check_launch(c, "sdpd_upload");
+  if (rc) return rc;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->n = n;
+  c->cells_valid = c->rho_valid = c->f_valid = false;
+  return SDPD_OK;
+}
+
+int sdpd_upload_ct(sdpd_ctx* c, const double* ct) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  if (c->P.newtonian) return fail(c, SDPD_ERR_STATE, "sdpd_upload_ct: Newtonian context");
+  if (!ct) return fail(c, SDPD_ERR_ARG, "sdpd_upload_ct: null array");
+  const int n = c->n, d2 = c->P.dim * c->P.dim;
+  std::vector<int32_t> perm(n);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpy(perm.data(), c->S.perm, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  std::vector<double> tmp((size_t)d2 * n);
+  for (int i = 0; i < n; ++i)
+    for (int q = 0; q < d2; ++q) tmp[(size_t)q * n + i] = ct[(size_t)q + (size_t)d2 * perm[i]];
+  HIP_TRY(c, hipMemcpy(c->S.ct, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+  return SDPD_OK;
+}
+
+int sdpd_build_cells(sdpd_ctx* c) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  return run_sort(c, false);
+}
+
+int sdpd_density(sdpd_ctx* c) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  if (!c->cells_valid) return fail(c, SDPD_ERR_STATE, "sdpd_density: build the cell list first");
+  return run_density(c);
+}
+
+int sdpd_eos(sdpd_ctx* c) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  if (!c->rho_valid) return fail(c, SDPD_ERR_STATE, "sdpd_eos: density is not current");
+  sdpd::launch_eos(c->stream, c->P, c->S);
+  return check_launch(c, "eos");
+}
+
+int sdpd_forces(sdpd_ctx* c) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  if (!c->cells_valid || !c->rho_valid)
+    return fail(c, SDPD_ERR_STATE, "sdpd_forces: density is not current");
+  return run_forces(c);
+}
+
+int sdpd_kick(sdpd_ctx* c, double lambda) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  if (!c->f_valid) return fail(c, SDPD_ERR_STATE, "sdpd_kick: forces are not current");
+  sdpd::launch_kick(c->stream, c->P, c->S, lambda);
+  return check_launch(c, "kick");
+}
+
+int sdpd_drift_wrap(sdpd_ctx* c) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  sdpd::launch_drift_wrap(c->stream, c->P, c->S);
+  c->cells_valid = false;
+  return check_launch(c, "drift_wrap");
+}
+
+int sdpd_conformation(sdpd_ctx* c, double lambda) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  if (c->P.newtonian) return fail(c, SDPD_ERR_STATE, "sdpd_conformation: Newtonian context");
+  if (!c->f_valid) return fail(c, SDPD_ERR_STATE, "sdpd_conformation: act is not current");
+  sdpd::launch_conformation(c->stream, c->P, c->S, lambda);
+  return check_launch(c, "conformation");
+}
+
+int sdpd_stats(sdpd_ctx* c, double* mom, double* ekin) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  if (!mom || !ekin) return fail(c, SDPD_ERR_ARG, "sdpd_stats: null output");
+  {
+    PhaseMark pm(c, 4);
+    sdpd::launch_stats(c->stream, c->P, c->S, c->stats_nb);
+  }
+  rc = check_launch(c, "stats");
+  if (rc) return rc;
+  double out[4];
+  HIP_TRY(c, hipMemcpyAsync(out, c->S.stats_out, sizeof(out), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  *ekin = out[0];
+  for (int d = 0; d < c->P.dim; ++d) mom[d] = out[1 + d];
+  for (int d = c->P.dim; d < 3; ++d) mom[d] = 0.0;
+  return SDPD_OK;
+}
+
+int sdpd_download(sdpd_ctx* c, double* x, double* v, double* rho, double* p, double* f) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  const int dim = c->P.dim;
+  if ((rho || p) && !c->rho_valid)
+    return fail(c, SDPD_ERR_STATE, "sdpd_download: density is not current");
+  if (f && !c->f_valid) return fail(c, SDPD_ERR_STATE, "sdpd_download: forces are not current");
+  if (x && (rc = download_vec(c, c->S.x, dim, x))) return rc;
+  if (v && (rc = download_vec(c, c->S.v, dim, v))) return rc;
+  if (rho && (rc = download_vec(c, c->S.rho, 1, rho))) return rc;
+  if (p && (rc = download_vec(c, c->S.p, 1, p))) return rc;
+  if (f && (rc = download_vec(c, c->S.f, dim, f))) return rc;
+  return SDPD_OK;
+}
+
+int sdpd_download_tensors(sdpd_ctx* c, double* ct, double* vgt) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  if (c->P.newtonian) return fail(c, SDPD_ERR_STATE, "sdpd_download_tensors: Newtonian context");
+  const int d2 = c->P.dim * c->P.dim;
+  if (ct && (rc = download_vec(c, c->S.ct, d2, ct))) return rc;
+  if (vgt) {
+    if (!c->f_valid) return fail(c, SDPD_ERR_STATE, "sdpd_download_tensors: vgt not current");
+    if ((rc = download_vec(c, c->S.vgt, d2, vgt))) return rc;
+  }
+  return SDPD_OK;
+}
+
+int sdpd_step(sdpd_ctx* c, int32_t nsteps) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  if (nsteps < 0) return fail(c, SDPD_ERR_ARG, "sdpd_step: nsteps < 0");
+  if (!c->f_valid) {
+    // initial evaluation (marching_marching.F90:528-1123)
+    if ((rc = run_sort(c, false))) return rc;
+    if ((rc = run_density(c))) return rc;
+    if ((rc = run_forces(c))) return rc;
+  }
+  for (int32_t s = 0; s < nsteps; ++s) {
+    c->step += 1;
+    c->P.step = c->step;
+    {
+      PhaseMark pm(c, 0);
+      sdpd::launch_boundary(c->stream, c->P, c->S, s > 0 ? 1 : 0, 1);
+    }
+    if ((rc = check_launch(c, "step boundary"))) return rc;
+    c->cells_valid = false;
+    if ((rc = run_sort(c, true))) return rc;
+    if ((rc = run_density(c))) return rc;
+    if ((rc = run_forces(c))) return rc;
+  }
+  if (nsteps > 0) {
+    // closing half kick (and conformation half update) of the last step
+    PhaseMark pm(c, 0);
+    if (!c->P.newtonian) sdpd::launch_conformation(c->stream, c->P, c->S, 0.5);
+    sdpd::launch_kick(c->stream, c->P, c->S, 0.5);
+    if ((rc = check_launch(c, "closing kick"))) return rc;
+  }
+  return SDPD_OK;
+}
+
+int sdpd_set_step(sdpd_ctx* c, int64_t step) {
+  if (!c) return SDPD_ERR_ARG;
+  c->step = step;
+  return SDPD_OK;
+}
+
+int sdpd_get_step(sdpd_ctx* c, int64_t* step) {
+  if (!c || !step) return SDPD_ERR_ARG;
+  *step = c->step;
+  return SDPD_OK;
+}
+
+int sdpd_get_cells(sdpd_ctx* c, int32_t* nm, int32_t* lhbx, int32_t* lpdx) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  if (!c->cells_valid) return fail(c, SDPD_ERR_STATE, "sdpd_get_cells: cells are not current");
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (nm)
+    for (int d = 0; d < 3; ++d) nm[d] = c->P.nm[d];
+  if (lhbx)
+    HIP_TRY(c, hipMemcpy(lhbx, c->S.start, ((size_t)c->P.ncell + 1) * sizeof(int32_t),
+                         hipMemcpyDeviceToHost));
+  if (lpdx)
+    HIP_TRY(c, hipMemcpy(lpdx, c->S.perm, (size_t)c->n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return SDPD_OK;
+}
+
+int sdpd_get_counters(sdpd_ctx* c, int64_t* pairs, int64_t* rechecks) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  std::vector<int32_t> a(c->n), b(c->n);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpy(a.data(), c->S.nbr_count, c->n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(b.data(), c->S.recheck, c->n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  int64_t sa = 0, sb = 0;
+  for (int i = 0; i < c->n; ++i) {
+    sa += a[i];
+    sb += b[i];
+  }
+  if (pairs) *pairs = sa;
+  if (rechecks) *rechecks = sb;
+  return SDPD_OK;
+}
+
+int sdpd_neighbor_signature(sdpd_ctx* c, int32_t mixed_decision, int32_t* count,
+                            uint32_t* hash) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  if (!c->cells_valid)
+    return fail(c, SDPD_ERR_STATE, "sdpd_neighbor_signature: cells are not current");
+  if (mixed_decision)
+    return fail(c, SDPD_ERR_ARG, "sdpd_neighbor_signature: mixed decision not available");
+  sdpd::launch_signature_f64(c->stream, c->P, c->S);
+  if ((rc = check_launch(c, "signature"))) return rc;
+  const int n = c->n;
+  std::vector<int32_t> cnt(n), perm(n);
+  std::vector<uint32_t> hs(n);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpy(cnt.data(), c->S.nbr_count, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(hs.data(), c->S.nbr_hash, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(perm.data(), c->S.perm, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; ++i) {
+    if (count) count[perm[i]] = cnt[i];
+    if (hash) hash[perm[i]] = hs[i];
+  }
+  return SDPD_OK;
+}
+
+int sdpd_pair_normals(const uint32_t* lo, const uint32_t* hi, int32_t count, uint64_t seed,
+                      int64_t step, float* g) {
+  if (count <= 0 || !lo || !hi || !g) return SDPD_ERR_ARG;
+  uint32_t *dlo = nullptr, *dhi = nullptr;
+  float* dg = nullptr;
+  hipError_t e = hipMalloc((void**)&dlo, count * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc((void**)&dhi, count * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc((void**)&dg, (size_t)count * 4 * sizeof(float));
+  if (e == hipSuccess) e = hipMemcpy(dlo, lo, count * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dhi, hi, count * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    sdpd::launch_pair_normals(nullptr, dlo, dhi, count, seed, step, dg);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess)
+    e = hipMemcpy(g, dg, (size_t)count * 4 * sizeof(float), hipMemcpyDeviceToHost);
+  if (dlo) (void)hipFree(dlo);
+  if (dhi) (void)hipFree(dhi);
+  if (dg) (void)hipFree(dg);
+  return e == hipSuccess ? SDPD_OK : SDPD_ERR_DEVICE;
+}
+
+int sdpd_set_timing(sdpd_ctx* c, int32_t enable) {
+  if (!c) return SDPD_ERR_ARG;
+  c->timing = enable != 0;
+  return SDPD_OK;
+}
+
+int sdpd_get_phase_times(sdpd_ctx* c, double* ms, int64_t* launches) {
+  if (!c) return SDPD_ERR_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  double acc[SDPD_NUM_PHASES] = {0};
+  int64_t cnt[SDPD_NUM_PHASES] = {0};
+  for (auto& m : c->marks) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, m.a, m.b) == hipSuccess && m.phase >= 0 &&
+        m.phase < SDPD_NUM_PHASES) {
+      acc[m.phase] += t;
+      cnt[m.phase] += 1;
+    }
+  }
+  release_events(c);
+  for (int q = 0; q < SDPD_NUM_PHASES; ++q) {
+    if (ms) ms[q] = acc[q];
+    if (launches) launches[q] = cnt[q];
+  }
+  return SDPD_OK;
+}
+
+int sdpd_sync(sdpd_ctx* c) {
+  if (!c) return SDPD_ERR_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return SDPD_OK;
+}
+
+}  // extern "C"

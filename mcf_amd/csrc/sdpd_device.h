@@ -1,0 +1,256 @@
+// sdpd_device.h -- device-side parameters and pair arithmetic shared by the
+// SDPD kernels (gfx950).  Everything here is FP64 unless the name says f32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sdpd {
+
+// Launch-time constants of one box.  Passed by value to every kernel.
+struct DevParams {
+  int32_t dim;
+  int32_t symmetry;
+  int32_t kernel_type;
+  int32_t density_type;
+  int32_t force_type;
+  int32_t eos_type;
+  int32_t newtonian;
+  int32_t noise;          // Brownian and kt > machine zero
+  int32_t body_force;     // constant body force present
+  int32_t n;              // particles
+  int32_t nm[3];          // cells per dimension (>= 3)
+  int32_t ncell;
+  double min_phys[3];
+  double length[3];
+  double rd[3];           // nm / length (PPM's rdx)
+  double cs[3];           // cell edge length / nm
+  double cut_off, cut2;
+  double h, coef, coef_grad, w0;
+  double c2, rho_ref, gamma, p0;
+  double eta, eta_coef, kt, dt, sqrt_dt;
+  double tau, G;
+  double bf[3];
+  uint64_t seed;
+  int64_t step;
+  // mixed precision: relative band around cut2 resolved in FP64
+  float cut2_lo_f32, cut2_hi_f32;
+};
+
+// ---------------------------------------------------------------------
+// smoothing kernel (kernel_kernel_quintic_spline.F90:160-215,
+// kernel_kernel_Lucy.F90), same operation order as the reference
+// ---------------------------------------------------------------------
+__device__ __forceinline__ void kernel_w_gradw(const DevParams& P, double r,
+                                               double& w, double& gw) {
+  double s = r / P.h;
+  if (P.kernel_type == 1) {
+    double s1 = 1.0 - s, s2 = 2.0 - s, s3 = 3.0 - s;
+    double s1_4 = (s1 * s1) * (s1 * s1);
+    double s2_4 = (s2 * s2) * (s2 * s2);
+    double s3_4 = (s3 * s3) * (s3 * s3);
+    double s1_5 = 15.0 * s1_4 * s1;
+    double s2_5 = -6.0 * s2_4 * s2;
+    double s3_5 = s3_4 * s3;
+    double g1 = 75.0 * s1_4, g2 = -30.0 * s2_4, g3 = 5.0 * s3_4;
+    if (s < 1.0) {
+      w = P.coef * (s3_5 + s2_5 + s1_5);
+      gw = P.coef_grad * (g3 + g2 + g1);
+    } else if (s < 2.0) {
+      w = P.coef * (s3_5 + s2_5);
+      gw = P.coef_grad * (g3 + g2);
+    } else if (s < 3.0) {
+      w = P.coef * s3_5;
+      gw = P.coef_grad * g3;
+    } else {
+      w = 0.0;
+      gw = 0.0;
+    }
+  } else {
+    if (s <= 1.0) {
+      double s1 = 1.0 - s;
+      w = P.coef * (1.0 + 3.0 * s) * (s1 * s1 * s1);
+      gw = P.coef_grad * 12.0 * s * (s1 * s1);
+    } else {
+      w = 0.0;
+      gw = 0.0;
+    }
+  }
+}
+
+__device__ __forceinline__ void kernel_w_gradw_f32(int kernel_type, float s, float coef,
+                                                   float coef_grad, float& w, float& gw) {
+  if (kernel_type == 1) {
+    float s1 = 1.0f - s, s2 = 2.0f - s, s3 = 3.0f - s;
+    float s1_4 = (s1 * s1) * (s1 * s1);
+    float s2_4 = (s2 * s2) * (s2 * s2);
+    float s3_4 = (s3 * s3) * (s3 * s3);
+    float wv = s3_4 * s3, gv = 5.0f * s3_4;
+    if (s < 2.0f) {
+      wv = fmaf(-6.0f * s2_4, s2, wv);
+      gv = fmaf(-30.0f, s2_4, gv);
+    }
+    if (s < 1.0f) {
+      wv = fmaf(15.0f * s1_4, s1, wv);
+      gv = fmaf(75.0f, s1_4, gv);
+    }
+    w = coef * wv;
+    gw = coef_grad * gv;
+  } else {
+    float s1 = 1.0f - s;
+    w = coef * (1.0f + 3.0f * s) * (s1 * s1 * s1);
+    gw = coef_grad * 12.0f * s * (s1 * s1);
+  }
+}
+
+// ---------------------------------------------------------------------
+// equation of state (stateEquation_compute_pressure.F90:158,176); rho is
+// the mass density (number density times mass for density type 2)
+// ---------------------------------------------------------------------
+__device__ __forceinline__ double eos_pressure(const DevParams& P, double rho) {
+  if (P.eos_type == 1) return P.c2 * (pow(rho, P.gamma) - P.rho_ref);
+  return P.p0 * (pow(rho / P.rho_ref, P.gamma) - 1.0);
+}
+
+// ---------------------------------------------------------------------
+// counter-based noise: Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
+// keyed by the seed, counter (min pid, max pid, step); normals by
+// Box-Muller with log / sin / cos from Taylor series in exactly rounded
+// float operations (no contraction), so that host and device agree
+// bit for bit.
+// ---------------------------------------------------------------------
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
+                                              uint32_t c3, uint32_t k0, uint32_t k1,
+                                              uint32_t out[4]) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c0 = n0;
+    c1 = (uint32_t)p1;
+    c2 = n2;
+    c3 = (uint32_t)p0;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+__device__ __forceinline__ float noise_log(float u) {
+#pragma clang fp contract(off)
+  uint32_t b = __float_as_uint(u);
+  int e = (int)(b >> 23) - 127;
+  float m = __uint_as_float((b & 0x7fffffu) | 0x3f800000u);
+  if (m > 1.41421356f) {
+    m = m * 0.5f;
+    e = e + 1;
+  }
+  float t = (m - 1.0f) / (m + 1.0f);
+  float s = t * t;
+  float q = fmaf(s, 0.111111111f, 0.142857143f);
+  q = fmaf(s, q, 0.2f);
+  q = fmaf(s, q, 0.333333333f);
+  q = fmaf(s, q, 1.0f);
+  float lnm = (2.0f * t) * q;
+  return fmaf((float)e, 0.693147182f, lnm);
+}
+
+__device__ __forceinline__ void noise_sincos(uint32_t k, float& cs, float& sn) {
+#pragma clang fp contract(off)
+  uint32_t qn = (k + (1u << 21)) >> 22;
+  int32_t rem = (int32_t)k - (int32_t)(qn << 22);
+  float th = (float)rem * 0x1.921fb6p-22f;
+  float s = th * th;
+  float ps = fmaf(s, 2.75573192e-6f, -1.98412698e-4f);
+  ps = fmaf(s, ps, 8.33333333e-3f);
+  ps = fmaf(s, ps, -1.66666667e-1f);
+  float sv = fmaf(th * s, ps, th);
+  float pc = fmaf(s, -2.75573192e-7f, 2.48015873e-5f);
+  pc = fmaf(s, pc, -1.38888889e-3f);
+  pc = fmaf(s, pc, 4.16666667e-2f);
+  pc = fmaf(s, pc, -0.5f);
+  float cv = fmaf(s, pc, 1.0f);
+  uint32_t q = qn & 3u;
+  float c_out = (q == 0) ? cv : (q == 1) ? -sv : (q == 2) ? -cv : sv;
+  float s_out = (q == 0) ? sv : (q == 1) ? cv : (q == 2) ? -sv : -cv;
+  cs = c_out;
+  sn = s_out;
+}
+
+// Four N(0,1) deviates for the unordered pair (lo, hi) at a step.
+__device__ __forceinline__ void pair_normals(uint64_t seed, uint32_t lo, uint32_t hi,
+                                             int64_t step, float g[4]) {
+#pragma clang fp contract(off)
+  uint32_t w[4];
+  philox4x32_10(lo, hi, (uint32_t)(uint64_t)step, (uint32_t)((uint64_t)step >> 32),
+                (uint32_t)seed, (uint32_t)(seed >> 32), w);
+#pragma unroll
+  for (int hh = 0; hh < 2; ++hh) {
+    float u = (float)((w[2 * hh] >> 8) + 1u) * 0x1p-24f;
+    float r = sqrtf(-2.0f * noise_log(u));
+    float cs, sn;
+    noise_sincos(w[2 * hh + 1] >> 8, cs, sn);
+    g[2 * hh] = r * cs;
+    g[2 * hh + 1] = r * sn;
+  }
+}
+
+// Squared distance with two rounded operations per term (no FMA), so the
+// in/out decision equals the reference's DOT_PRODUCT on the same values
+// (pp_interaction.inc:37-46).
+__device__ __forceinline__ double dist2_exact(double a, double b, double c) {
+#pragma clang fp contract(off)
+  double s = a * a;
+  s = s + b * b;
+  s = s + c * c;
+  return s;
+}
+
+__device__ __forceinline__ double dist2_exact2(double a, double b) {
+#pragma clang fp contract(off)
+  double s = a * a;
+  s = s + b * b;
+  return s;
+}
+
+// Per-dimension image of a neighbour cell: returns the wrapped cell index
+// and how the coordinate difference is formed.
+//   mode 0: xi - xj
+//   mode 1: xi - (xj + L)   (neighbour across the upper face)
+//   mode 2: (xi + L) - xj   (symmetric mode, home particle on the lower side)
+//   mode 3: xi - (xj - L)   (full mode, neighbour across the lower face)
+// The symmetric convention reproduces PPM's upper-only ghost layer, where
+// the particle near the lower face is the one shifted by +L
+// (ppm_bcdef_periodic_x.inc:37-60); the full convention reproduces ghosts
+// on both faces (ppm_bcdef_periodic_x.inc:61-90).
+__device__ __forceinline__ int wrap_cell(int c, int nm, int symmetry, int& mode) {
+  if (c < 0) {
+    mode = symmetry ? 2 : 3;
+    return c + nm;
+  }
+  if (c >= nm) {
+    mode = 1;
+    return c - nm;
+  }
+  mode = 0;
+  return c;
+}
+
+__device__ __forceinline__ double image_diff(double xi, double xj, double L, int mode) {
+#pragma clang fp contract(off)
+  switch (mode) {
+    case 1: return xi - (xj + L);
+    case 2: return (xi + L) - xj;
+    case 3: return xi - (xj - L);
+    default: return xi - xj;
+  }
+}
+
+}  // namespace sdpd

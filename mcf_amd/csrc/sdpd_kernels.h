@@ -1,0 +1,60 @@
+// sdpd_kernels.h -- device buffers of one box and the kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sdpd_device.h"
+
+namespace sdpd {
+
+// Device buffers.  Per-particle arrays are structure-of-arrays in cell
+// order: component d of particle i lives at a[d * n + i].
+struct DeviceState {
+  // particle state (cell order) and the ping-pong copies the sort writes
+  double *x, *v, *m;
+  int32_t *pid, *perm;   // perm: upload index of the particle in this slot
+  double *x_alt, *v_alt, *m_alt;
+  int32_t *pid_alt, *perm_alt;
+  double *f;             // acceleration (dim, n)
+  double *rho, *p;       // density (number density for type 2), pressure
+  // Oldroyd-B: conformation, its rate, velocity gradient (dim*dim, n)
+  double *ct, *ct_alt, *act, *vgt;
+  // cell list
+  int32_t *key, *slot, *count, *start, *unsorted, *key_sorted;
+  // mixed precision records (cell order)
+  float4 *xr;            // position relative to the cell corner, w = pid bits
+  float4 *vr;            // velocity, w = A = p / n^2
+  float *br;             // B = 1 / n^2
+  // reductions and counters
+  double *partials;      // stats partial sums (blocks x 8)
+  double *stats_out;     // 8 doubles
+  int32_t *nbr_count;    // per-particle in-cutoff count of the last pass
+  int32_t *recheck;      // per-particle FP64 re-decisions (mixed mode)
+  uint32_t *nbr_hash;    // per-particle sum of pair hashes (checking)
+};
+
+void launch_fill_i32(hipStream_t s, int32_t* a, int32_t v, int64_t n);
+void launch_fill_f64(hipStream_t s, double* a, double v, int64_t n);
+// key + histogram (counts zeroed first), then scan, scatter and place into
+// the *_alt buffers; the caller swaps the ping-pong pointers afterwards
+void launch_cell_keys(hipStream_t s, const DevParams& P, const DeviceState& S);
+void launch_cell_sort_after_keys(hipStream_t s, const DevParams& P, const DeviceState& S);
+
+void launch_density_f64(hipStream_t s, const DevParams& P, const DeviceState& S);
+void launch_eos(hipStream_t s, const DevParams& P, const DeviceState& S);
+void launch_forces_f64(hipStream_t s, const DevParams& P, const DeviceState& S);
+void launch_signature_f64(hipStream_t s, const DevParams& P, const DeviceState& S);
+
+void launch_kick(hipStream_t s, const DevParams& P, const DeviceState& S, double lambda);
+void launch_drift_wrap(hipStream_t s, const DevParams& P, const DeviceState& S);
+void launch_conformation(hipStream_t s, const DevParams& P, const DeviceState& S, double lambda);
+void launch_boundary(hipStream_t s, const DevParams& P, const DeviceState& S, int closing,
+                     int opening);
+void launch_stats(hipStream_t s, const DevParams& P, const DeviceState& S, int nblocks);
+void launch_pair_normals(hipStream_t s, const uint32_t* lo, const uint32_t* hi, int count,
+                         uint64_t seed, int64_t step, float* g);
+
+int stats_blocks(int n);
+
+}  // namespace sdpd

@@ -1,0 +1,191 @@
+// sdpd_update.hip -- K5 velocity-Verlet / Oldroyd-B streaming updates and
+// K6 compensated statistics (gfx950).
+//
+// Replaces particles_integrate_velocity (particles_integrate_velocity.F90:84-94),
+// particles_integrate_position (particles_integrate_position.F90:82-123),
+// particles_adjust_particles' periodic branch (particles_adjust_particles.F90:
+// 154-161, 272-279), particles_integrate_ct (particles_integrate_ct.F90:66-68)
+// and statistic_compute_statistic (statistic_compute_statistic.F90:230-283).
+// Expressions keep the reference's evaluation order: v + (lambda*f)*dt,
+// x + v*dt, ct + (act*dt)*lambda.
+#include "sdpd_kernels.h"
+
+namespace sdpd {
+
+__global__ void k_kick(DevParams P, DeviceState S, double lambda) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const int64_t n = P.n;
+  for (int d = 0; d < P.dim; ++d)
+    S.v[d * n + i] = S.v[d * n + i] + lambda * S.f[d * n + i] * P.dt;
+}
+
+__device__ __forceinline__ double wrap_coord(double x, double lo, double L) {
+  if (x < lo) x = x + L;
+  if (x >= lo + L) x = x - L;
+  return x;
+}
+
+__global__ void k_drift_wrap(DevParams P, DeviceState S) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const int64_t n = P.n;
+  for (int d = 0; d < P.dim; ++d) {
+    double xv = S.x[d * n + i] + S.v[d * n + i] * P.dt;
+    S.x[d * n + i] = wrap_coord(xv, P.min_phys[d], P.length[d]);
+  }
+}
+
+__global__ void k_conformation(DevParams P, DeviceState S, double lambda) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const int64_t n = P.n;
+  const int d2 = P.dim * P.dim;
+  for (int q = 0; q < d2; ++q)
+    S.ct[q * n + i] = S.ct[q * n + i] + S.act[q * n + i] * P.dt * lambda;
+}
+
+__device__ __forceinline__ int cell_coord_u(double xv, double x0, double rd, double xmax, int nm) {
+  int k = (int)floor((xv - x0) * rd);
+  if (xv >= xmax && k < nm) k = nm;
+  if (xv < xmax && k >= nm) k = nm - 1;
+  if (k < 0) k = 0;
+  if (k >= nm) k = nm - 1;
+  return k;
+}
+
+// Step boundary: the closing half kick of the previous step, then the
+// opening half kick, drift, Oldroyd-B half update and periodic wrap of the
+// next one, then the cell key and cell histogram of the next sort.
+// Cell counts must be zero on entry.
+__global__ __launch_bounds__(256) void k_boundary(DevParams P, DeviceState S, int closing,
+                                                  int opening) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const int64_t n = P.n;
+  const int dim = P.dim;
+  double xv[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < dim; ++d) {
+    double vv = S.v[d * n + i];
+    double ff = S.f[d * n + i];
+    if (closing) vv = vv + 0.5 * ff * P.dt;
+    if (opening) vv = vv + 0.5 * ff * P.dt;
+    S.v[d * n + i] = vv;
+    double x = S.x[d * n + i];
+    if (opening) x = wrap_coord(x + vv * P.dt, P.min_phys[d], P.length[d]);
+    S.x[d * n + i] = x;
+    xv[d] = x;
+  }
+  if (!P.newtonian) {
+    const int d2 = dim * dim;
+    for (int q = 0; q < d2; ++q) {
+      double c = S.ct[q * n + i];
+      double a = S.act[q * n + i];
+      if (closing) c = c + a * P.dt * 0.5;
+      if (opening) c = c + a * P.dt * 0.5;
+      S.ct[q * n + i] = c;
+    }
+  }
+  int cx = cell_coord_u(xv[0], P.min_phys[0], P.rd[0], P.min_phys[0] + P.length[0], P.nm[0]);
+  int cy = cell_coord_u(xv[1], P.min_phys[1], P.rd[1], P.min_phys[1] + P.length[1], P.nm[1]);
+  int cz = dim == 3
+               ? cell_coord_u(xv[2], P.min_phys[2], P.rd[2], P.min_phys[2] + P.length[2], P.nm[2])
+               : 0;
+  int32_t k = cx + P.nm[0] * (cy + P.nm[1] * cz);
+  S.key[i] = k;
+  S.slot[i] = atomicAdd(&S.count[k], 1);
+}
+
+// Neumaier-compensated sum helpers
+struct KSum {
+  double s, c;
+};
+__device__ __forceinline__ void kadd(KSum& k, double v) {
+  double t = k.s + v;
+  if (fabs(k.s) >= fabs(v))
+    k.c += (k.s - t) + v;
+  else
+    k.c += (v - t) + k.s;
+  k.s = t;
+}
+
+// Block partial sums of (E_k, P_x, P_y, P_z) over fluid particles; a
+// second pass reduces the partials in a fixed order.
+__global__ __launch_bounds__(256) void k_stats_partial(DevParams P, DeviceState S) {
+  __shared__ double sh[4][256];
+  const int64_t n = P.n;
+  const int dim = P.dim;
+  KSum acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    double mi = S.m[i];
+    double v2 = 0.0;
+    for (int d = 0; d < dim; ++d) {
+      double vv = S.v[d * n + i];
+      v2 = v2 + vv * vv;
+      kadd(acc[1 + d], mi * vv);
+    }
+    kadd(acc[0], 0.5 * mi * v2);
+  }
+  for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] = acc[q].s + acc[q].c;
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    KSum t = {0, 0};
+    for (int k = 0; k < 256; ++k) kadd(t, sh[threadIdx.x][k]);
+    S.partials[(int64_t)blockIdx.x * 4 + threadIdx.x] = t.s + t.c;
+  }
+}
+
+__global__ void k_stats_final(int nblocks, DeviceState S) {
+  if (threadIdx.x < 4) {
+    KSum t = {0, 0};
+    for (int b = 0; b < nblocks; ++b) kadd(t, S.partials[(int64_t)b * 4 + threadIdx.x]);
+    S.stats_out[threadIdx.x] = t.s + t.c;
+  }
+}
+
+__global__ void k_pair_normals(const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
+                               int count, uint64_t seed, int64_t step, float* __restrict__ g) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float r[4];
+  pair_normals(seed, lo[i], hi[i], step, r);
+  for (int q = 0; q < 4; ++q) g[4 * (int64_t)i + q] = r[q];
+}
+
+static inline int nblk(int64_t n) { return (int)((n + 255) / 256); }
+
+int stats_blocks(int n) {
+  int b = nblk(n);
+  return b > 1024 ? 1024 : (b < 1 ? 1 : b);
+}
+
+void launch_kick(hipStream_t s, const DevParams& P, const DeviceState& S, double lambda) {
+  hipLaunchKernelGGL(k_kick, dim3(nblk(P.n)), dim3(256), 0, s, P, S, lambda);
+}
+
+void launch_drift_wrap(hipStream_t s, const DevParams& P, const DeviceState& S) {
+  hipLaunchKernelGGL(k_drift_wrap, dim3(nblk(P.n)), dim3(256), 0, s, P, S);
+}
+
+void launch_conformation(hipStream_t s, const DevParams& P, const DeviceState& S, double lambda) {
+  hipLaunchKernelGGL(k_conformation, dim3(nblk(P.n)), dim3(256), 0, s, P, S, lambda);
+}
+
+void launch_boundary(hipStream_t s, const DevParams& P, const DeviceState& S, int closing,
+                     int opening) {
+  launch_fill_i32(s, S.count, 0, P.ncell);
+  hipLaunchKernelGGL(k_boundary, dim3(nblk(P.n)), dim3(256), 0, s, P, S, closing, opening);
+}
+
+void launch_stats(hipStream_t s, const DevParams& P, const DeviceState& S, int nblocks) {
+  hipLaunchKernelGGL(k_stats_partial, dim3(nblocks), dim3(256), 0, s, P, S);
+  hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, s, nblocks, S);
+}
+
+void launch_pair_normals(hipStream_t s, const uint32_t* lo, const uint32_t* hi, int count,
+                         uint64_t seed, int64_t step, float* g) {
+  hipLaunchKernelGGL(k_pair_normals, dim3(nblk(count)), dim3(256), 0, s, lo, hi, count, seed,
+                     step, g);
+}
+
+}  // namespace sdpd

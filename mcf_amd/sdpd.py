@@ -1,0 +1,357 @@
+"""Host-side mirror of the reference's per-step interface, over the C-ABI.
+
+The reference's boundary is the set of module procedures that
+marching_integrate_VV calls on TYPE(Particles) and TYPE(Technique)
+(mcf/src/Class_Particles.F90:43-311, Class_Technique.F90:40-97).  The
+class below keeps those names and their argument meaning:
+
+    technique_build_list            -> build_list()
+    particles_compute_density       -> compute_density()
+    particles_compute_pressure      -> compute_pressure()
+    particles_compute_interaction   -> compute_interaction()
+    particles_integrate_velocity    -> integrate_velocity(lambda_)
+    particles_integrate_position +
+    particles_adjust_particles      -> integrate_position_and_adjust()
+    particles_integrate_ct          -> integrate_ct(lambda_)
+    statistic_compute_statistic     -> compute_statistic()
+    marching_integrate_VV           -> integrate_VV(nsteps)
+
+Errors: the reference returns stat_info = -1 and prints; here a non-zero
+return code raises SDPDError carrying the library's message.  Everything
+runs in libsdpd.so on the GPU; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .params import SDPDParams
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsdpd.so")
+
+
+class SDPDError(RuntimeError):
+    pass
+
+
+class CParams(C.Structure):
+    """Mirror of sdpd_params in include/sdpd.h."""
+
+    _fields_ = [
+        ("num_dim", C.c_int32),
+        ("symmetry", C.c_int32),
+        ("kernel_type", C.c_int32),
+        ("rhs_density_type", C.c_int32),
+        ("rhs_force_type", C.c_int32),
+        ("state_equation_type", C.c_int32),
+        ("newtonian", C.c_int32),
+        ("brownian", C.c_int32),
+        ("body_force_type", C.c_int32),
+        ("precision", C.c_int32),
+        ("min_phys", C.c_double * 3),
+        ("max_phys", C.c_double * 3),
+        ("cut_off", C.c_double),
+        ("c", C.c_double),
+        ("rho", C.c_double),
+        ("rho_ref", C.c_double),
+        ("gamma", C.c_double),
+        ("eta", C.c_double),
+        ("eta_coef", C.c_double),
+        ("kt", C.c_double),
+        ("dt", C.c_double),
+        ("tau", C.c_double),
+        ("n_p", C.c_double),
+        ("kt_p", C.c_double),
+        ("body_force", C.c_double * 3),
+        ("seed", C.c_uint64),
+    ]
+
+
+# (name, restype, argtypes) of every symbol include/sdpd.h declares
+_P = C.c_void_p
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int32)
+_UP = C.POINTER(C.c_uint32)
+_LP = C.POINTER(C.c_int64)
+_FP = C.POINTER(C.c_float)
+SYMBOLS = [
+    ("sdpd_create", C.c_int, [C.POINTER(CParams), C.POINTER(C.c_void_p)]),
+    ("sdpd_upload", C.c_int, [_P, _DP, _DP, _DP, _IP, C.c_int32]),
+    ("sdpd_upload_ct", C.c_int, [_P, _DP]),
+    ("sdpd_build_cells", C.c_int, [_P]),
+    ("sdpd_density", C.c_int, [_P]),
+    ("sdpd_eos", C.c_int, [_P]),
+    ("sdpd_forces", C.c_int, [_P]),
+    ("sdpd_kick", C.c_int, [_P, C.c_double]),
+    ("sdpd_drift_wrap", C.c_int, [_P]),
+    ("sdpd_conformation", C.c_int, [_P, C.c_double]),
+    ("sdpd_stats", C.c_int, [_P, _DP, _DP]),
+    ("sdpd_download", C.c_int, [_P, _DP, _DP, _DP, _DP, _DP]),
+    ("sdpd_download_tensors", C.c_int, [_P, _DP, _DP]),
+    ("sdpd_step", C.c_int, [_P, C.c_int32]),
+    ("sdpd_set_step", C.c_int, [_P, C.c_int64]),
+    ("sdpd_get_step", C.c_int, [_P, _LP]),
+    ("sdpd_get_cells", C.c_int, [_P, _IP, _IP, _IP]),
+    ("sdpd_get_counters", C.c_int, [_P, _LP, _LP]),
+    ("sdpd_neighbor_signature", C.c_int, [_P, C.c_int32, _IP, _UP]),
+    ("sdpd_pair_normals", C.c_int, [_UP, _UP, C.c_int32, C.c_uint64, C.c_int64, _FP]),
+    ("sdpd_set_timing", C.c_int, [_P, C.c_int32]),
+    ("sdpd_get_phase_times", C.c_int, [_P, _DP, _LP]),
+    ("sdpd_sync", C.c_int, [_P]),
+    ("sdpd_last_error", C.c_char_p, [_P]),
+    ("sdpd_destroy", C.c_int, [_P]),
+]
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libsdpd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise SDPDError(
+            f"{path} is missing: build the HIP library first (python -m mcf_amd.build)")
+    lib = C.CDLL(path)
+    for name, res, args in SYMBOLS:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def to_c(prm: SDPDParams) -> CParams:
+    c = CParams()
+    c.num_dim = int(prm.num_dim)
+    c.symmetry = 1 if prm.symmetry else 0
+    c.kernel_type = int(prm.kernel_type)
+    c.rhs_density_type = int(prm.rhs_density_type)
+    c.rhs_force_type = int(prm.rhs_force_type)
+    c.state_equation_type = int(prm.stateEquation_type)
+    c.newtonian = 1 if prm.Newtonian else 0
+    c.brownian = 1 if prm.Brownian else 0
+    c.body_force_type = int(prm.body_force_type)
+    c.precision = int(prm.precision)
+    for d in range(3):
+        c.min_phys[d] = float(prm.min_phys[d])
+        c.max_phys[d] = float(prm.max_phys[d])
+        c.body_force[d] = float(prm.body_force[d])
+    c.cut_off = float(prm.cut_off)
+    c.c = float(prm.c)
+    c.rho = float(prm.rho)
+    c.rho_ref = float(prm.rho_ref)
+    c.gamma = float(prm.gamma)
+    c.eta = float(prm.eta)
+    c.eta_coef = float(prm.eta_coef)
+    c.kt = float(prm.kt)
+    c.dt = float(prm.dt)
+    c.tau = float(prm.tau)
+    c.n_p = float(prm.n_p)
+    c.kt_p = float(prm.kt_p)
+    c.seed = int(prm.random_seed) & 0xFFFFFFFFFFFFFFFF
+    return c
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_IP)
+
+
+class SDPD:
+    """One box of particles on the GPU (one sdpd_ctx)."""
+
+    def __init__(self, prm: SDPDParams):
+        self.lib = load_library()
+        self.prm = prm
+        self._cp = to_c(prm)
+        h = C.c_void_p()
+        rc = self.lib.sdpd_create(C.byref(self._cp), C.byref(h))
+        if rc != 0 or not h.value:
+            raise SDPDError(f"sdpd_create failed (stat_info={rc}); see stderr")
+        self.h = h
+        self.n = 0
+
+    # -- lifetime -------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.sdpd_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            msg = self.lib.sdpd_last_error(self.h)
+            raise SDPDError(f"{what}: stat_info={rc}: {msg.decode() if msg else ''}")
+
+    # -- data -----------------------------------------------------------
+    def upload(self, x, v, m, pid, sid=None) -> None:
+        x = np.ascontiguousarray(x, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        m = np.ascontiguousarray(m, np.float64)
+        n = x.shape[0]
+        ids = np.zeros((n, 2), np.int32)
+        ids[:, 0] = pid
+        if sid is not None:
+            ids[:, 1] = sid
+        self._check(self.lib.sdpd_upload(self.h, _dp(x), _dp(v), _dp(m), _ip(ids), n), "sdpd_upload")
+        self.n = n
+
+    def upload_ct(self, ct) -> None:
+        ct = np.ascontiguousarray(ct, np.float64)
+        self._check(self.lib.sdpd_upload_ct(self.h, _dp(ct)), "sdpd_upload_ct")
+
+    def download(self):
+        dim = self.prm.num_dim
+        x = np.zeros((self.n, dim))
+        v = np.zeros((self.n, dim))
+        self._check(self.lib.sdpd_download(self.h, _dp(x), _dp(v), None, None, None), "sdpd_download")
+        return x, v
+
+    def download_density(self):
+        rho = np.zeros(self.n)
+        p = np.zeros(self.n)
+        self._check(self.lib.sdpd_download(self.h, None, None, _dp(rho), _dp(p), None), "sdpd_download")
+        return rho, p
+
+    def download_force(self):
+        f = np.zeros((self.n, self.prm.num_dim))
+        self._check(self.lib.sdpd_download(self.h, None, None, None, None, _dp(f)), "sdpd_download")
+        return f
+
+    def download_tensors(self, vgt=True):
+        d2 = self.prm.num_dim ** 2
+        ct = np.zeros((self.n, d2))
+        g = np.zeros((self.n, d2)) if vgt else None
+        self._check(self.lib.sdpd_download_tensors(self.h, _dp(ct), _dp(g) if vgt else None),
+                    "sdpd_download_tensors")
+        return ct, g
+
+    # -- the reference's per-step procedures ------------------------------
+    def build_list(self) -> None:
+        self._check(self.lib.sdpd_build_cells(self.h), "technique_build_list")
+
+    def compute_density(self) -> None:
+        self._check(self.lib.sdpd_density(self.h), "particles_compute_density")
+
+    def compute_pressure(self) -> None:
+        self._check(self.lib.sdpd_eos(self.h), "particles_compute_pressure")
+
+    def compute_interaction(self) -> None:
+        self._check(self.lib.sdpd_forces(self.h), "particles_compute_interaction")
+
+    def integrate_velocity(self, lambda_: float) -> None:
+        self._check(self.lib.sdpd_kick(self.h, float(lambda_)), "particles_integrate_velocity")
+
+    def integrate_position_and_adjust(self) -> None:
+        self._check(self.lib.sdpd_drift_wrap(self.h), "particles_integrate_position")
+
+    def integrate_ct(self, lambda_: float) -> None:
+        self._check(self.lib.sdpd_conformation(self.h, float(lambda_)), "particles_integrate_ct")
+
+    def compute_statistic(self):
+        mom = np.zeros(3)
+        e = C.c_double()
+        self._check(self.lib.sdpd_stats(self.h, _dp(mom), C.byref(e)), "statistic_compute_statistic")
+        return e.value, mom[: self.prm.num_dim]
+
+    def integrate_VV(self, nsteps: int = 1) -> None:
+        self._check(self.lib.sdpd_step(self.h, int(nsteps)), "marching_integrate_VV")
+
+    def evaluate(self) -> None:
+        """Initial evaluation of marching_marching (cells, density, forces)."""
+        self.build_list()
+        self.compute_density()
+        self.compute_interaction()
+
+    # -- helpers ----------------------------------------------------------
+    @property
+    def step(self) -> int:
+        s = C.c_int64()
+        self._check(self.lib.sdpd_get_step(self.h, C.byref(s)), "sdpd_get_step")
+        return s.value
+
+    @step.setter
+    def step(self, value: int) -> None:
+        self._check(self.lib.sdpd_set_step(self.h, int(value)), "sdpd_set_step")
+
+    def cells(self):
+        nm = np.zeros(3, np.int32)
+        self._check(self.lib.sdpd_get_cells(self.h, _ip(nm), None, None), "sdpd_get_cells")
+        ncell = int(np.prod(nm))
+        lhbx = np.zeros(ncell + 1, np.int32)
+        lpdx = np.zeros(self.n, np.int32)
+        self._check(self.lib.sdpd_get_cells(self.h, _ip(nm), _ip(lhbx), _ip(lpdx)), "sdpd_get_cells")
+        return nm, lhbx, lpdx
+
+    def counters(self):
+        a = C.c_int64()
+        b = C.c_int64()
+        self._check(self.lib.sdpd_get_counters(self.h, C.byref(a), C.byref(b)), "sdpd_get_counters")
+        return a.value, b.value
+
+    def neighbor_signature(self, mixed_decision: bool = False):
+        cnt = np.zeros(self.n, np.int32)
+        hs = np.zeros(self.n, np.uint32)
+        self._check(self.lib.sdpd_neighbor_signature(self.h, 1 if mixed_decision else 0, _ip(cnt),
+                                                     hs.ctypes.data_as(_UP)),
+                    "sdpd_neighbor_signature")
+        return cnt, hs
+
+    PHASES = ("boundary", "sort", "density", "forces", "stats")
+
+    def set_timing(self, enable: bool) -> None:
+        self._check(self.lib.sdpd_set_timing(self.h, 1 if enable else 0), "sdpd_set_timing")
+
+    def phase_times(self):
+        ms = np.zeros(len(self.PHASES))
+        cnt = np.zeros(len(self.PHASES), np.int64)
+        self._check(self.lib.sdpd_get_phase_times(self.h, _dp(ms), cnt.ctypes.data_as(_LP)),
+                    "sdpd_get_phase_times")
+        return {k: (float(ms[q]), int(cnt[q])) for q, k in enumerate(self.PHASES)}
+
+    def sync(self) -> None:
+        self._check(self.lib.sdpd_sync(self.h), "sdpd_sync")
+
+
+def pair_normals(lo, hi, seed: int, step: int):
+    """Device evaluation of the counter-based noise stream (4 normals each)."""
+    lib = load_library()
+    lo = np.ascontiguousarray(lo, np.uint32)
+    hi = np.ascontiguousarray(hi, np.uint32)
+    g = np.zeros((lo.shape[0], 4), np.float32)
+    rc = lib.sdpd_pair_normals(lo.ctypes.data_as(_UP), hi.ctypes.data_as(_UP), lo.shape[0],
+                               int(seed) & 0xFFFFFFFFFFFFFFFF, int(step), g.ctypes.data_as(_FP))
+    if rc != 0:
+        raise SDPDError(f"sdpd_pair_normals failed (stat_info={rc})")
+    return g
+
+
+def pair_hash(a, b):
+    """The 32-bit pair hash of the neighbour-set signature (host side)."""
+    a = np.asarray(a, np.uint32)
+    b = np.asarray(b, np.uint32)
+    with np.errstate(over="ignore"):
+        h = (a * np.uint32(0x9E3779B1)) ^ ((b + np.uint32(0x7F4A7C15)) * np.uint32(0x85EBCA77))
+        h ^= h >> np.uint32(15)
+        h = h * np.uint32(0x2C1B3C6D)
+        h ^= h >> np.uint32(12)
+    return h.astype(np.uint32)

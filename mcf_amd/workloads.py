@@ -1,0 +1,155 @@
+"""Synthetic initial conditions for the SDPD benchmark and test boxes.
+
+The lattice rule follows the reference's square / simple-cubic lattice
+(mcf/src/particles_init_global_inter_square.F90:133-154,
+particles_init_global_inter_cubic.F90:133-163): site centres at
+(i - 1/2) dx.  Each component gets a seeded uniform jitter of +-0.05 dx.
+Masses follow particles_compute_mass.F90:426-450 for a lattice:
+m = rho * dx^dim.  Particle ids are 1..N with species id 0 (fluid), as
+assign_id gives them.
+
+Workloads
+---------
+shear_wave_2d  2D periodic box [0,1]^2, 64^2 particles, quintic kernel,
+               cut_off 3 dx, p = c^2 rho (Morris EOS, gamma 1, rho_ref 0),
+               c = 10, eta = 0.1, u_x = 0.1 sin(2 pi y), dt = 1e-4.
+tait_box_3d    3D periodic box of 128^3 particles, dx = 1 (m = 1),
+               cut_off 3, Tait EOS gamma 7, c = 10, eta = 0.1,
+               Maxwellian velocities at kT = 1e-3, dt = 5e-5, Brownian.
+oldroyd_box_3d Oldroyd-B variant of tait_box_3d (tau 0.1, eta_p 0.05).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .params import PRECISION_FP64, PRECISION_MIXED, SDPDParams
+
+
+def lattice(num_dim: int, n_per_dim, dx: float, jitter: float, seed: int,
+            origin=None) -> np.ndarray:
+    """Jittered square/cubic lattice, particle-major (N, dim)."""
+    if np.isscalar(n_per_dim):
+        n_per_dim = [int(n_per_dim)] * num_dim
+    origin = [0.0] * num_dim if origin is None else origin
+    axes = [origin[d] + (np.arange(n_per_dim[d]) + 0.5) * dx for d in range(num_dim)]
+    # x runs fastest, like the reference's lattice loops
+    grids = np.meshgrid(*axes, indexing="ij")
+    pts = np.stack([g.transpose(tuple(reversed(range(num_dim)))).ravel() for g in grids], axis=1)
+    if jitter > 0:
+        rng = np.random.default_rng(seed)
+        pts = pts + rng.uniform(-jitter, jitter, size=pts.shape) * dx
+    return np.ascontiguousarray(pts)
+
+
+def wrap(x: np.ndarray, prm: SDPDParams) -> np.ndarray:
+    x = x.copy()
+    for d in range(prm.num_dim):
+        L = prm.max_phys[d] - prm.min_phys[d]
+        x[:, d] = np.where(x[:, d] < prm.min_phys[d], x[:, d] + L, x[:, d])
+        x[:, d] = np.where(x[:, d] >= prm.max_phys[d], x[:, d] - L, x[:, d])
+    return x
+
+
+class Box:
+    """A workload: parameters plus initial particle arrays."""
+
+    def __init__(self, name, prm, x, v, m, pid, sid=None):
+        self.name = name
+        self.prm = prm
+        self.x = np.ascontiguousarray(x, np.float64)
+        self.v = np.ascontiguousarray(v, np.float64)
+        self.m = np.ascontiguousarray(m, np.float64)
+        self.pid = np.ascontiguousarray(pid, np.int32)
+        n = self.x.shape[0]
+        self.sid = np.zeros(n, np.int32) if sid is None else np.ascontiguousarray(sid, np.int32)
+
+    @property
+    def n(self) -> int:
+        return self.x.shape[0]
+
+
+def shear_wave_2d(n_side: int = 64, brownian: bool = False, kt: float = 1e-4,
+                  seed: int = 1, symmetry: bool = True) -> Box:
+    dx = 1.0 / n_side
+    prm = SDPDParams(num_dim=2, min_phys=[0, 0, 0], max_phys=[1, 1, 0], cut_off=3 * dx,
+                     symmetry=symmetry, kernel_type=1, rhs_density_type=2, rhs_force_type=3,
+                     stateEquation_type=1, c=10.0, rho=1.0, rho_ref=0.0, gamma=1.0, eta=0.1,
+                     Newtonian=True, Brownian=brownian, kt=kt if brownian else 0.0, dt=1e-4,
+                     random_seed=seed)
+    x = wrap(lattice(2, n_side, dx, 0.05, seed), prm)
+    v = np.zeros_like(x)
+    v[:, 0] = 0.1 * np.sin(2 * math.pi * x[:, 1])
+    m = np.full(x.shape[0], prm.rho * dx * dx)
+    pid = np.arange(1, x.shape[0] + 1, dtype=np.int32)
+    return Box("shear_wave_2d", prm, x, v, m, pid)
+
+
+def tait_box_3d(n_side: int = 128, precision: int = PRECISION_FP64, kt: float = 1e-3,
+                brownian: bool = True, seed: int = 1, eta: float = 0.1,
+                symmetry: bool = True) -> Box:
+    dx = 1.0
+    L = n_side * dx
+    prm = SDPDParams(num_dim=3, min_phys=[0, 0, 0], max_phys=[L, L, L], cut_off=3 * dx,
+                     symmetry=symmetry, kernel_type=1, rhs_density_type=2, rhs_force_type=3,
+                     stateEquation_type=2, c=10.0, rho=1.0, rho_ref=1.0, gamma=7.0, eta=eta,
+                     Newtonian=True, Brownian=brownian, kt=kt if brownian else 0.0, dt=5e-5,
+                     random_seed=seed, precision=precision)
+    x = wrap(lattice(3, n_side, dx, 0.05, seed), prm)
+    m = np.full(x.shape[0], prm.rho * dx ** 3)
+    rng = np.random.default_rng(seed + 7919)
+    sigma = math.sqrt(max(kt, 0.0) / m[0]) if kt > 0 else 0.0
+    v = rng.normal(0.0, 1.0, size=x.shape) * sigma
+    v -= v.mean(axis=0, keepdims=True)
+    pid = np.arange(1, x.shape[0] + 1, dtype=np.int32)
+    return Box("tait_box_3d", prm, x, v, m, pid)
+
+
+def oldroyd_box_3d(n_side: int = 24, precision: int = PRECISION_FP64, kt: float = 1e-3,
+                   brownian: bool = True, seed: int = 1) -> Box:
+    box = tait_box_3d(n_side, precision, kt, brownian, seed)
+    # eta_p = G tau with G = n_p kt_p (particles_compute_pressure_tensor.F90:94)
+    tau, eta_p = 0.1, 0.05
+    box.prm = box.prm.replace(Newtonian=False, tau=tau, n_p=1.0, kt_p=eta_p / tau)
+    box.name = "oldroyd_box_3d"
+    return box
+
+
+def mixed(box: Box) -> Box:
+    b = Box(box.name + "_mixed", box.prm.replace(precision=PRECISION_MIXED), box.x, box.v, box.m,
+            box.pid, box.sid)
+    return b
+
+
+def write_external(path: str, box: Box) -> None:
+    """Write the reference's read_external particle file
+    (mcf/src/io_read_particles.F90:162-204): N, then x v rho m pid sid."""
+    dim = box.prm.num_dim
+    with open(path, "w") as fh:
+        fh.write(f"{box.n}\n")
+        rho = np.full(box.n, box.prm.rho)
+        for i in range(box.n):
+            vals = list(box.x[i, :dim]) + list(box.v[i, :dim]) + [rho[i], box.m[i]]
+            fh.write(" ".join(f"{v:.17e}" for v in vals) + f" {int(box.pid[i])} {int(box.sid[i])}\n")
+
+
+def read_external(path: str, num_dim: int):
+    """Read a read_external particle file; returns x, v, rho, m, pid, sid.
+    Errors mirror io_read_particles.F90:121-240 (missing file, short file)."""
+    with open(path) as fh:
+        first = fh.readline()
+        if not first:
+            raise ValueError("reading particles has problem at line 1")
+        n = int(float(first.split()[0]))
+        data = np.loadtxt(fh, ndmin=2) if n > 0 else np.zeros((0, 2 * num_dim + 4))
+    if data.shape[0] != n:
+        raise ValueError("actual number of lines is not equal to number given !")
+    d = num_dim
+    x = data[:, 0:d]
+    v = data[:, d:2 * d]
+    rho = data[:, 2 * d]
+    m = data[:, 2 * d + 1]
+    pid = data[:, 2 * d + 2].astype(np.int32)
+    sid = data[:, 2 * d + 3].astype(np.int32)
+    return x, v, rho, m, pid, sid

@@ -1,0 +1,281 @@
+"""ctypes wrapper of the FP64 CPU oracle (liboracle_sdpd.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP product.  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import it.
+
+Arrays are numpy, particle-major in Python (shape (N, dim)); they are
+handed to C as the reference's column-major (dim, N) layout, which is the
+same memory.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_sdpd.so")
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("num_dim", C.c_int32),
+        ("symmetry", C.c_int32),
+        ("kernel_type", C.c_int32),
+        ("rhs_density_type", C.c_int32),
+        ("rhs_force_type", C.c_int32),
+        ("eos_type", C.c_int32),
+        ("newtonian", C.c_int32),
+        ("brownian", C.c_int32),
+        ("body_force_type", C.c_int32),
+        ("pad0", C.c_int32),
+        ("min_phys", C.c_double * 3),
+        ("max_phys", C.c_double * 3),
+        ("cut_off", C.c_double),
+        ("c", C.c_double),
+        ("rho", C.c_double),
+        ("rho_ref", C.c_double),
+        ("gamma", C.c_double),
+        ("eta", C.c_double),
+        ("eta_coef", C.c_double),
+        ("kt", C.c_double),
+        ("dt", C.c_double),
+        ("tau", C.c_double),
+        ("n_p", C.c_double),
+        ("kt_p", C.c_double),
+        ("body_force", C.c_double * 3),
+        ("seed", C.c_uint64),
+    ]
+
+
+def build() -> None:
+    """Compile the oracle with its Makefile (gcc, no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        P = C.POINTER(OrcParams)
+        L.orc_kernel.argtypes = [P, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_rank.argtypes = [C.c_int32, dp, C.c_int32, dp, dp, ip, ip, ip, ip, C.POINTER(C.c_int32)]
+        L.orc_cell_counts.argtypes = [P, ip]
+        L.orc_ghost_get.argtypes = [P, dp, C.c_int32, dp, ip, C.c_int32, C.POINTER(C.c_int32)]
+        L.orc_neighbor_pairs.argtypes = [P, dp, C.c_int32, ip, C.c_int64, C.POINTER(C.c_int64)]
+        L.orc_brute_pairs.argtypes = [P, dp, C.c_int32, ip, C.c_int64, C.POINTER(C.c_int64)]
+        L.orc_density.argtypes = [P, dp, dp, ip, C.c_int32, dp]
+        L.orc_pressure.argtypes = [P, dp, dp, C.c_int32, dp]
+        L.orc_interaction.argtypes = [P, C.c_int64, dp, dp, dp, dp, dp, ip, ip, C.c_void_p, C.c_int32, dp, C.c_void_p]
+        L.orc_compute_act.argtypes = [P, dp, dp, C.c_int32, dp]
+        L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.orc_pair_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, C.POINTER(C.c_float)]
+        L.orc_pair_normals.restype = None
+        L.orc_philox4x32_10.restype = None
+        state = [P, C.c_int64, C.c_int32, dp, dp, dp, ip, ip, dp, dp, dp, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_prologue.argtypes = state
+        L.orc_vv_step.argtypes = state
+        L.orc_statistics.argtypes = [P, dp, dp, ip, C.c_int32, C.POINTER(C.c_double), dp]
+        _lib = L
+    return _lib
+
+
+def to_orc(prm) -> OrcParams:
+    """Convert an SDPDParams-like object into the oracle's record."""
+    o = OrcParams()
+    o.num_dim = int(prm.num_dim)
+    o.symmetry = 1 if prm.symmetry else 0
+    o.kernel_type = int(prm.kernel_type)
+    o.rhs_density_type = int(prm.rhs_density_type)
+    o.rhs_force_type = int(prm.rhs_force_type)
+    o.eos_type = int(prm.stateEquation_type)
+    o.newtonian = 1 if prm.Newtonian else 0
+    o.brownian = 1 if prm.Brownian else 0
+    o.body_force_type = int(prm.body_force_type)
+    for d in range(3):
+        o.min_phys[d] = float(prm.min_phys[d])
+        o.max_phys[d] = float(prm.max_phys[d])
+        o.body_force[d] = float(prm.body_force[d])
+    o.cut_off = float(prm.cut_off)
+    o.c = float(prm.c)
+    o.rho = float(prm.rho)
+    o.rho_ref = float(prm.rho_ref)
+    o.gamma = float(prm.gamma)
+    o.eta = float(prm.eta)
+    o.eta_coef = float(prm.eta_coef)
+    o.kt = float(prm.kt) if prm.Brownian else 0.0
+    o.dt = float(prm.dt)
+    o.tau = float(prm.tau)
+    o.n_p = float(prm.n_p)
+    o.kt_p = float(prm.kt_p)
+    o.seed = int(prm.random_seed) & 0xFFFFFFFFFFFFFFFF
+    return o
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise RuntimeError(f"oracle {what} failed (stat_info={rc})")
+
+
+def _f64(a, dim=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def kernel(prm, r: float):
+    w = C.c_double()
+    g = C.c_double()
+    _check(lib().orc_kernel(C.byref(to_orc(prm)), float(r), C.byref(w), C.byref(g)), "kernel")
+    return w.value, g.value
+
+
+def cell_counts(prm):
+    nm = np.zeros(3, np.int32)
+    lib().orc_cell_counts(C.byref(to_orc(prm)), nm)
+    return nm
+
+
+def rank(dim, x, xmin, xmax, nm, ngl):
+    x = _f64(x)
+    n = x.shape[0]
+    nmt = [int(nm[d]) + int(ngl[d]) + int(ngl[dim + d]) for d in range(dim)]
+    nbox = int(np.prod(nmt))
+    lhbx = np.zeros(nbox + 1, np.int32)
+    lpdx = np.zeros(max(n, 1), np.int32)
+    cnt = C.c_int32()
+    _check(lib().orc_rank(dim, x, n, _f64(xmin), _f64(xmax), _i32(nm), _i32(ngl), lhbx, lpdx,
+                          C.byref(cnt)), "rank")
+    return lhbx, lpdx[: cnt.value]
+
+
+def ghosts(prm, x):
+    x = _f64(x)
+    n, dim = x.shape
+    cap = 8 * n + 16
+    xg = np.zeros((cap, dim))
+    owner = np.zeros(cap, np.int32)
+    ng = C.c_int32()
+    _check(lib().orc_ghost_get(C.byref(to_orc(prm)), x, n, xg, owner, cap, C.byref(ng)), "ghost_get")
+    return xg[: ng.value].copy(), owner[: ng.value].copy()
+
+
+def neighbor_pairs(prm, x, brute=False):
+    x = _f64(x)
+    n = x.shape[0]
+    cap = max(1024, n * 200)
+    pairs = np.zeros(2 * cap, np.int32)
+    np_ = C.c_int64()
+    fn = lib().orc_brute_pairs if brute else lib().orc_neighbor_pairs
+    _check(fn(C.byref(to_orc(prm)), x, n, pairs, cap, C.byref(np_)), "pairs")
+    return pairs[: 2 * np_.value].reshape(-1, 2).copy()
+
+
+def density(prm, x, m, sid=None):
+    x = _f64(x)
+    n = x.shape[0]
+    sid = np.zeros(n, np.int32) if sid is None else _i32(sid)
+    rho = np.zeros(n)
+    _check(lib().orc_density(C.byref(to_orc(prm)), x, _f64(m), sid, n, rho), "density")
+    return rho
+
+
+def pressure(prm, rho, m):
+    rho = _f64(rho)
+    p = np.zeros_like(rho)
+    _check(lib().orc_pressure(C.byref(to_orc(prm)), rho, _f64(m), rho.shape[0], p), "pressure")
+    return p
+
+
+def interaction(prm, step, x, v, rho, press, m, pid, sid=None, ct=None):
+    x = _f64(x)
+    n, dim = x.shape
+    sid = np.zeros(n, np.int32) if sid is None else _i32(sid)
+    f = np.zeros((n, dim))
+    vgt = np.zeros((n, dim * dim))
+    ctp = None
+    if ct is not None:
+        ct = _f64(ct)
+        ctp = ct.ctypes.data_as(C.c_void_p)
+    _check(lib().orc_interaction(C.byref(to_orc(prm)), int(step), x, _f64(v), _f64(rho), _f64(press),
+                                 _f64(m), _i32(pid), sid, ctp, n, f,
+                                 vgt.ctypes.data_as(C.c_void_p)), "interaction")
+    return f, vgt
+
+
+def compute_act(prm, ct, vgt):
+    ct = _f64(ct)
+    act = np.zeros_like(ct)
+    _check(lib().orc_compute_act(C.byref(to_orc(prm)), ct, _f64(vgt), ct.shape[0], act), "act")
+    return act
+
+
+def philox4x32_10(ctr, key):
+    c = (C.c_uint32 * 4)(*[int(v) & 0xFFFFFFFF for v in ctr])
+    k = (C.c_uint32 * 2)(*[int(v) & 0xFFFFFFFF for v in key])
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return [int(v) for v in o]
+
+
+def pair_normals(seed, pid_lo, pid_hi, step):
+    g = (C.c_float * 4)()
+    lib().orc_pair_normals(int(seed) & 0xFFFFFFFFFFFFFFFF, int(pid_lo), int(pid_hi), int(step), g)
+    return np.array([g[i] for i in range(4)], np.float32)
+
+
+class State:
+    """Particle state for multi-step oracle runs (particle-major arrays)."""
+
+    def __init__(self, prm, x, v, m, pid, sid=None, ct=None):
+        self.prm = prm
+        self.x = np.array(x, np.float64, copy=True)
+        n, dim = self.x.shape
+        self.v = np.array(v, np.float64, copy=True)
+        self.m = np.ascontiguousarray(m, np.float64)
+        self.pid = _i32(pid)
+        self.sid = np.zeros(n, np.int32) if sid is None else _i32(sid)
+        self.rho = np.zeros(n)
+        self.p = np.zeros(n)
+        self.f = np.zeros((n, dim))
+        d2 = dim * dim
+        if ct is None:
+            ct = np.tile(np.eye(dim).reshape(-1, order="F"), (n, 1))
+        self.ct = np.array(ct, np.float64, copy=True).reshape(n, d2)
+        self.act = np.zeros((n, d2))
+        self.vgt = np.zeros((n, d2))
+
+    def _args(self):
+        vp = C.c_void_p
+        return (self.rho, self.p, self.f, self.ct.ctypes.data_as(vp), self.act.ctypes.data_as(vp),
+                self.vgt.ctypes.data_as(vp))
+
+    def prologue(self, step=0):
+        n = self.x.shape[0]
+        _check(lib().orc_prologue(C.byref(to_orc(self.prm)), int(step), n, self.x, self.v, self.m,
+                                  self.pid, self.sid, *self._args()), "prologue")
+
+    def vv_step(self, step):
+        n = self.x.shape[0]
+        _check(lib().orc_vv_step(C.byref(to_orc(self.prm)), int(step), n, self.x, self.v, self.m,
+                                 self.pid, self.sid, *self._args()), "vv_step")
+
+    def statistics(self):
+        n = self.x.shape[0]
+        e = C.c_double()
+        mom = np.zeros(3)
+        _check(lib().orc_statistics(C.byref(to_orc(self.prm)), self.v, self.m, self.sid, n,
+                                    C.byref(e), mom), "statistics")
+        return e.value, mom[: self.x.shape[1]]

@@ -1,0 +1,123 @@
+/*
+ * sdpd_oracle.h -- CPU FP64 restatement of MCF's per-step SDPD path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * product in mcf_amd/: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  Nothing in mcf_amd/ links,
+ * imports or calls it.
+ *
+ * Arrays follow the reference's Fortran layout, 0-based in C:
+ *   x(dim, N) column-major  -> x[d + dim*i]
+ *   ct/act/vgt(dim*dim, N)  -> t[a + dim*b + dim*dim*i]   (a = row, b = column)
+ * Every function returns 0 on success and -1 on failure, like the
+ * reference's stat_info.
+ */
+#ifndef SDPD_ORACLE_H
+#define SDPD_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_params {
+  int32_t num_dim;          /* 2 or 3                                          */
+  int32_t symmetry;         /* 1: half stencil + upper ghost layer (reference default) */
+  int32_t kernel_type;      /* 1 quintic spline, 2 Lucy                        */
+  int32_t rhs_density_type; /* 1 mass density, 2 number density                */
+  int32_t rhs_force_type;   /* 3 Hu-Adams, 4 Hu-Adams angular                  */
+  int32_t eos_type;         /* 1 Morris, 2 Batchelor (Tait)                    */
+  int32_t newtonian;        /* 0: Oldroyd-B conformation tensor                */
+  int32_t brownian;         /* thermal noise on/off                            */
+  int32_t body_force_type;  /* 0 none, 1 constant per direction                */
+  int32_t pad0;
+  double min_phys[3];
+  double max_phys[3];
+  double cut_off;
+  double c, rho, rho_ref, gamma;
+  double eta, eta_coef, kt, dt;
+  double tau, n_p, kt_p;
+  double body_force[3];
+  uint64_t seed;
+} orc_params;
+
+/* smoothing kernel W(r), W'(r) */
+int orc_kernel(const orc_params *p, double r, double *w, double *gradw);
+
+/* PPM ppm_util_rank3d / rank2d: stable counting sort of points into boxes.
+ * nm[d] are the interior cell counts, ngl[0..dim-1] lower and ngl[dim..2dim-1]
+ * upper ghost layers.  lhbx has nbox+1 entries, lpdx receives the point
+ * indices (0-based) box by box; both are 0-based.  *count = sorted points.  */
+int orc_rank(int32_t dim, const double *x, int32_t n, const double *xmin,
+             const double *xmax, const int32_t *nm, const int32_t *ngl,
+             int32_t *lhbx, int32_t *lpdx, int32_t *count);
+
+/* interior cell counts Nm = INT(L / (1.0001 cut_off)) */
+int orc_cell_counts(const orc_params *p, int32_t *nm);
+
+/* periodic ghost images appended after the real particles (PPM ghost_get,
+ * periodic bcdef).  Returns the number of ghosts in *nghost; xg receives
+ * (dim, nghost) and owner the real index each ghost copies.  cap bounds
+ * both outputs.                                                           */
+int orc_ghost_get(const orc_params *p, const double *x, int32_t n,
+                  double *xg, int32_t *owner, int32_t cap, int32_t *nghost);
+
+/* In-cutoff pairs visited by the reference's cell-list traversal, mapped
+ * back to real owners: pairs[2k], pairs[2k+1].                           */
+int orc_neighbor_pairs(const orc_params *p, const double *x, int32_t n,
+                       int32_t *pairs, int64_t cap, int64_t *npairs);
+
+/* O(N^2) enumeration of in-cutoff pairs (i < j) with the same periodic
+ * image convention.                                                      */
+int orc_brute_pairs(const orc_params *p, const double *x, int32_t n,
+                    int32_t *pairs, int64_t cap, int64_t *npairs);
+
+/* density pass: rho(N) (number density for type 2, mass density for 1) */
+int orc_density(const orc_params *p, const double *x, const double *m,
+                const int32_t *sid, int32_t n, double *rho);
+
+/* equation of state p(rho) */
+int orc_pressure(const orc_params *p, const double *rho, const double *m,
+                 int32_t n, double *press);
+
+/* pair forces (acceleration) f(dim, N) and, when non-Newtonian, the
+ * velocity gradient tensor vgt(dim*dim, N).  ct may be NULL when Newtonian. */
+int orc_interaction(const orc_params *p, int64_t step, const double *x,
+                    const double *v, const double *rho, const double *press,
+                    const double *m, const int32_t *pid, const int32_t *sid,
+                    const double *ct, int32_t n, double *f, double *vgt);
+
+/* Oldroyd-B: act = c L^T + L c - (c - I)/tau ; ct += lambda*dt*act */
+int orc_compute_act(const orc_params *p, const double *ct, const double *vgt,
+                    int32_t n, double *act);
+
+/* Counter-based noise stream shared with the device path. */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
+                       uint32_t out[4]);
+void orc_pair_normals(uint64_t seed, uint32_t pid_lo, uint32_t pid_hi,
+                      int64_t step, float g[4]);
+
+/* Initial evaluation (marching_marching prologue): density, pressure,
+ * forces, Oldroyd-B act.  step = noise step of this evaluation.           */
+int orc_prologue(const orc_params *p, int64_t step, int32_t n, double *x,
+                 double *v, const double *m, const int32_t *pid,
+                 const int32_t *sid, double *rho, double *press, double *f,
+                 double *ct, double *act, double *vgt);
+
+/* One marching_integrate_VV step, in place.  step = noise step used by the
+ * force pass of this step (the caller increments it first, as
+ * marching_marching does).                                                */
+int orc_vv_step(const orc_params *p, int64_t step, int32_t n, double *x,
+                double *v, const double *m, const int32_t *pid,
+                const int32_t *sid, double *rho, double *press, double *f,
+                double *ct, double *act, double *vgt);
+
+/* statistic_compute_statistic: kinetic energy and momentum of fluid
+ * particles (naive summation, as the reference).                        */
+int orc_statistics(const orc_params *p, const double *v, const double *m,
+                   const int32_t *sid, int32_t n, double *ekin, double *mom);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

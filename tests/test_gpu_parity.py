@@ -1,0 +1,289 @@
+"""GPU parity of the HIP path against the FP64 oracle (run with -m gpu).
+
+Bars: bit-exact for the cell list, the neighbour sets and the noise
+stream; 1e-12 relative for density / pressure and for the forces (max over
+particles of |df_i| normalised by the rms force) in FP64 mode on identical
+inputs; trajectories after several steps are reported with a looser bar
+(1e-10) because the summation order differs from the reference's.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from mcf_amd import workloads as W
+from mcf_amd.sdpd import SDPD, SDPDError, pair_normals
+
+from gpu_helpers import (gpu_evaluate, oracle_evaluate, oracle_signature, rel_err,
+                         vec_err)
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+BOXES = {
+    "shear2d_sym": lambda: W.shear_wave_2d(20),
+    "shear2d_full": lambda: W.shear_wave_2d(17, symmetry=False),
+    "tait3d_sym_kt0": lambda: W.tait_box_3d(12, brownian=False),
+    "tait3d_sym_noise": lambda: W.tait_box_3d(13, kt=1e-3),
+    "tait3d_full_noise": lambda: W.tait_box_3d(11, kt=1e-3, symmetry=False),
+}
+
+
+def test_noise_stream_bitwise(oracle):
+    announce("noise stream: 20000 keys")
+    rng = np.random.default_rng(4)
+    lo = rng.integers(0, 2 ** 31, 20000).astype(np.uint32)
+    hi = (lo + rng.integers(1, 1000, 20000)).astype(np.uint32)
+    lo[:3] = [0, 0xFFFFFFFF, 7]
+    hi[:3] = [0, 0xFFFFFFFF, 7]
+    for seed, step in [(1, 0), (0xDEADBEEFCAFE, 12345), (7, (1 << 33) + 5)]:
+        g = pair_normals(lo, hi, seed, step)
+        ref = np.stack([oracle.pair_normals(seed, a, b, step) for a, b in zip(lo[:4000], hi[:4000])])
+        assert np.array_equal(g[:4000].view(np.uint32), ref.view(np.uint32))
+        assert np.isfinite(g).all()
+
+
+@pytest.mark.parametrize("name", ["shear2d_sym", "tait3d_sym_kt0", "tait3d_full_noise"])
+def test_cell_list_bitexact(oracle, name):
+    box = BOXES[name]()
+    announce(f"cell list: {name} n={box.n}")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.build_list()
+        nm, lhbx, lpdx = g.cells()
+    dim = box.prm.num_dim
+    ref_lhbx, ref_lpdx = oracle.rank(dim, box.x, np.array(box.prm.min_phys[:dim]),
+                                     np.array(box.prm.max_phys[:dim]), nm[:dim].astype(np.int32),
+                                     np.zeros(2 * dim, np.int32))
+    assert list(nm[:dim]) == box.prm.cell_counts()[:dim]
+    assert np.array_equal(lhbx, ref_lhbx)
+    assert np.array_equal(lpdx, ref_lpdx)
+
+
+def test_cell_list_faces_and_ties(oracle):
+    # particles exactly on cell faces and on the lower box face
+    box = W.tait_box_3d(12, brownian=False)
+    x = box.x.copy()
+    cs = 12.0 / 3
+    x[:40, 0] = cs * (np.arange(40) % 3)
+    x[40:60, 1] = 0.0
+    x[60:80, 2] = np.nextafter(12.0, 0.0)
+    box.x = x
+    announce("cell list: faces and ties")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.build_list()
+        nm, lhbx, lpdx = g.cells()
+    ref_lhbx, ref_lpdx = oracle.rank(3, x, np.zeros(3), np.full(3, 12.0), nm.astype(np.int32),
+                                     np.zeros(6, np.int32))
+    assert np.array_equal(lhbx, ref_lhbx) and np.array_equal(lpdx, ref_lpdx)
+
+
+@pytest.mark.parametrize("name", list(BOXES))
+def test_neighbor_sets_bitexact(oracle, name):
+    box = BOXES[name]()
+    announce(f"neighbour sets: {name} n={box.n}")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.build_list()
+        cnt, hs = g.neighbor_signature()
+    rc, rh = oracle_signature(oracle, box)
+    assert np.array_equal(cnt, rc)
+    assert np.array_equal(hs, rh)
+
+
+def test_neighbor_sets_perfect_lattice(oracle):
+    # 30 lattice neighbours sit exactly on r = 3 dx: strict cut-off
+    box = W.tait_box_3d(12, brownian=False)
+    box.x = W.lattice(3, 12, 1.0, 0.0, 0)
+    announce("neighbour sets: perfect lattice")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.build_list()
+        cnt, hs = g.neighbor_signature()
+    rc, rh = oracle_signature(oracle, box)
+    assert np.array_equal(cnt, rc) and np.array_equal(hs, rh)
+    assert (cnt == cnt[0]).all()
+
+
+@pytest.mark.parametrize("name", list(BOXES))
+def test_density_pressure_parity(oracle, name):
+    box = BOXES[name]()
+    announce(f"density: {name}")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.build_list()
+        g.compute_density()
+        rho, p = g.download_density()
+    ro, po = oracle.density(box.prm, box.x, box.m), None
+    po = oracle.pressure(box.prm, ro, box.m)
+    assert rel_err(rho, ro) < 1e-13
+    assert rel_err(p, po) < 1e-12
+
+
+@pytest.mark.parametrize("name", list(BOXES))
+@pytest.mark.parametrize("force_type", [3, 4])
+def test_force_parity(oracle, name, force_type):
+    box = BOXES[name]()
+    box.prm = box.prm.replace(rhs_force_type=force_type)
+    announce(f"forces: {name} type {force_type}")
+    g = gpu_evaluate(box, step=17)
+    f = g.download_force()
+    pairs, _ = g.counters()
+    g.close()
+    rho, p, fo, _ = oracle_evaluate(oracle, box, step=17)
+    assert vec_err(f, fo) < 1e-12
+    on = oracle.neighbor_pairs(box.prm, box.x)
+    assert pairs == (2 if box.prm.symmetry else 1) * len(on)
+
+
+def test_body_force_parity(oracle):
+    box = W.shear_wave_2d(16)
+    box.prm = box.prm.replace(body_force_type=1, body_force=[0.5, 0.0, 0.0])
+    announce("forces: body force")
+    g = gpu_evaluate(box)
+    f = g.download_force()
+    g.close()
+    _, _, fo, _ = oracle_evaluate(oracle, box)
+    assert vec_err(f, fo) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["shear2d_sym", "tait3d_sym_noise", "tait3d_full_noise"])
+def test_trajectory_parity(oracle, name):
+    box = BOXES[name]()
+    nsteps = 6
+    announce(f"trajectory: {name} {nsteps} steps")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.integrate_VV(nsteps)
+        x, v = g.download()
+        e, mom = g.compute_statistic()
+        rho, p = g.download_density()
+        f = g.download_force()
+        assert g.step == nsteps
+    st = oracle.State(box.prm, box.x, box.v, box.m, box.pid)
+    st.prologue(0)
+    for s in range(1, nsteps + 1):
+        st.vv_step(s)
+    L = np.array(box.prm.length[: box.prm.num_dim])
+    dx = x - st.x
+    dx -= L * np.round(dx / L)
+    assert np.abs(dx).max() < 1e-10 * L.max()
+    assert vec_err(v, st.v) < 1e-10
+    assert rel_err(rho, st.rho) < 1e-12
+    assert vec_err(f, st.f) < 1e-10
+    eo, mo = st.statistics()
+    assert e == pytest.approx(eo, rel=1e-10)
+    assert np.abs(mom - mo).max() < 1e-10 * (np.abs(box.m[:, None] * box.v).sum() + 1e-30)
+
+
+def test_step_api_matches_individual_calls(oracle):
+    # sdpd_step == kick, drift+wrap, build, density, forces, kick
+    box = BOXES["tait3d_sym_noise"]()
+    announce("step API vs individual calls")
+    with SDPD(box.prm) as a, SDPD(box.prm) as b:
+        for g in (a, b):
+            g.upload(box.x, box.v, box.m, box.pid)
+        a.integrate_VV(3)
+        b.evaluate()
+        for s in range(1, 4):
+            b.step = s
+            b.integrate_velocity(0.5)
+            b.integrate_position_and_adjust()
+            b.build_list()
+            b.compute_density()
+            b.compute_interaction()
+            b.integrate_velocity(0.5)
+        xa, va = a.download()
+        xb, vb = b.download()
+    assert np.array_equal(xa, xb)
+    assert np.array_equal(va, vb)
+
+
+def test_oldroyd_parity(oracle):
+    box = W.oldroyd_box_3d(12)
+    announce("Oldroyd-B forces, vgt, act")
+    rng = np.random.default_rng(9)
+    ct = np.tile(np.eye(3).reshape(-1), (box.n, 1))
+    a = rng.normal(0, 0.05, size=(box.n, 3, 3))
+    ct = ct + (a + a.transpose(0, 2, 1)).reshape(box.n, 9)
+    g = gpu_evaluate(box, step=2, ct=ct)
+    f = g.download_force()
+    ctg, vgt = g.download_tensors()
+    g.close()
+    rho, p, fo, vgto = oracle_evaluate(oracle, box, step=2, ct=ct)
+    assert vec_err(f, fo) < 1e-12
+    assert vec_err(vgt, vgto) < 1e-12
+    assert np.array_equal(ctg, ct)
+
+
+def test_oldroyd_trajectory(oracle):
+    box = W.oldroyd_box_3d(12)
+    announce("Oldroyd-B trajectory")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.integrate_VV(4)
+        x, v = g.download()
+        ct, vgt = g.download_tensors()
+    st = oracle.State(box.prm, box.x, box.v, box.m, box.pid)
+    st.prologue(0)
+    for s in range(1, 5):
+        st.vv_step(s)
+    assert vec_err(v, st.v) < 1e-10
+    assert vec_err(ct, st.ct) < 1e-10
+    assert vec_err(vgt, st.vgt) < 1e-9
+
+
+def test_run_to_run_bitwise():
+    box = BOXES["tait3d_sym_noise"]()
+    announce("determinism")
+    outs = []
+    for _ in range(2):
+        with SDPD(box.prm) as g:
+            g.upload(box.x, box.v, box.m, box.pid)
+            g.integrate_VV(3)
+            outs.append(g.download() + (g.download_force(),))
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
+def test_error_paths():
+    box = BOXES["shear2d_sym"]()
+    announce("error paths")
+    with SDPD(box.prm) as g:
+        with pytest.raises(SDPDError):
+            g.build_list()  # no particles yet
+        g.upload(box.x, box.v, box.m, box.pid)
+        with pytest.raises(SDPDError):
+            g.compute_density()  # no cell list yet
+        g.build_list()
+        with pytest.raises(SDPDError):
+            g.download_force()  # forces not current
+        bad = box.x.copy()
+        bad[0, 0] = 2.0
+        with pytest.raises(SDPDError):
+            g.upload(bad, box.v, box.m, box.pid)
+
+
+def test_full_size_tait_box_properties():
+    # 128^3 = 2,097,152 particles: size-independent properties
+    box = W.tait_box_3d(128)
+    announce(f"full size: n={box.n}")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.integrate_VV(2)
+        f = g.download_force()
+        rho, p = g.download_density()
+        pairs, _ = g.counters()
+        e, mom = g.compute_statistic()
+    mf = box.m[:, None] * f
+    assert np.abs(mf.sum(0)).max() < 1e-10 * np.abs(mf).sum()
+    nb = pairs / box.n
+    assert 100 < nb < 115
+    assert abs(rho.mean() - 360.0 / 359.0) < 0.01
+    assert np.isfinite(f).all()

@@ -27,6 +27,7 @@ struct sdpd_ctx {
   bool cells_valid = false;  // particles sorted for the current positions
   bool rho_valid = false;    // rho / p current for the current order
   bool f_valid = false;      // f (and act) current for the current order
+  bool vr_stale = false;     // mixed: FP32 velocity records older than v
   // phase timing: pairs of events recorded around each phase's launches
   bool timing = false;
   struct Mark {
@@ -91,6 +92,8 @@ int derive(const sdpd_params& p, DevParams& P, std::string& err) {
   if (!(p.dt > 0)) return err = "dt must be positive", SDPD_ERR_ARG;
   if (p.precision != SDPD_PRECISION_FP64 && p.precision != SDPD_PRECISION_MIXED)
     return err = "precision must be 0 (FP64) or 1 (mixed)", SDPD_ERR_ARG;
+  if (p.precision == SDPD_PRECISION_MIXED && !p.newtonian)
+    return err = "mixed precision is available for Newtonian flow only", SDPD_ERR_ARG;
 
   P.dim = p.num_dim;
   P.symmetry = p.symmetry ? 1 : 0;
@@ -157,14 +160,25 @@ int derive(const sdpd_params& p, DevParams& P, std::string& err) {
   P.kt = kt;
   P.dt = p.dt;
   P.sqrt_dt = std::sqrt(p.dt);
+  P.inv_sqrt_dt = 1.0 / P.sqrt_dt;
   P.tau = p.tau;
   P.G = p.n_p * p.kt_p;  // particles_compute_pressure_tensor.F90:94
   for (int d = 0; d < 3; ++d) P.bf[d] = p.body_force[d];
   P.seed = p.seed;
   P.step = 0;
+  P.mixed = p.precision == SDPD_PRECISION_MIXED ? 1 : 0;
   const float c2f = (float)P.cut2;
   P.cut2_lo_f32 = c2f * (1.0f - 2e-5f);
   P.cut2_hi_f32 = c2f * (1.0f + 2e-5f);
+  for (int d = 0; d < 3; ++d) P.csf[d] = (float)P.cs[d];
+  P.inv_hf = (float)(1.0 / P.h);
+  P.coeff = (float)P.coef;
+  P.coef_gradf = (float)P.coef_grad;
+  P.etaf = (float)P.eta;
+  P.eta_coeff = (float)P.eta_coef;
+  P.ktf = (float)P.kt;
+  P.inv_sqrt_dtf = (float)P.inv_sqrt_dt;
+  P.sqrt_2kt_etaf = (float)std::sqrt(2.0 * P.kt * P.eta);
   return SDPD_OK;
 }
 
@@ -176,10 +190,11 @@ int dalloc(sdpd_ctx* c, T** p, size_t count) {
 }
 
 void free_state(DeviceState& S) {
-  void* ptrs[] = {S.x,        S.v,        S.m,        S.pid,   S.perm,     S.x_alt,   S.v_alt,
-                  S.m_alt,    S.pid_alt,  S.perm_alt, S.f,     S.rho,      S.p,       S.ct,
-                  S.ct_alt,   S.act,      S.vgt,      S.key,   S.slot,     S.count,   S.start,
-                  S.unsorted, S.key_sorted, S.xr,     S.vr,    S.br,       S.partials, S.stats_out,
+  void* ptrs[] = {S.x,        S.v,        S.m,         S.pid,     S.perm,     S.x_alt,
+                  S.v_alt,    S.m_alt,    S.pid_alt,   S.perm_alt, S.f,       S.rho,
+                  S.p,        S.an,       S.bn,        S.ct,      S.ct_alt,   S.act,
+                  S.vgt,      S.key,      S.slot,      S.count,   S.start,    S.unsorted,
+                  S.key_sorted, S.xr,     S.vr,        S.pb,      S.partials, S.stats_out,
                   S.nbr_count, S.recheck, S.nbr_hash};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -206,6 +221,8 @@ int alloc_state(sdpd_ctx* c, int n) {
   ALLOC(f, dim * N);
   ALLOC(rho, N);
   ALLOC(p, N);
+  ALLOC(an, N);
+  ALLOC(bn, N);
   if (!P.newtonian) {
     ALLOC(ct, d2 * N);
     ALLOC(ct_alt, d2 * N);
@@ -221,7 +238,7 @@ int alloc_state(sdpd_ctx* c, int n) {
   if (c->prm.precision == SDPD_PRECISION_MIXED) {
     ALLOC(xr, N);
     ALLOC(vr, N);
-    ALLOC(br, N);
+    ALLOC(pb, N);
   }
   c->stats_nb = sdpd::stats_blocks(n);
   ALLOC(partials, (size_t)c->stats_nb * 4);
@@ -308,11 +325,12 @@ int run_density(sdpd_ctx* c) {
   c->P.step = c->step;
   {
     PhaseMark pm(c, 2);
-    sdpd::launch_density_f64(c->stream, c->P, c->S);
+    sdpd::launch_density(c->stream, c->P, c->S, c->P.mixed != 0);
   }
   int rc = check_launch(c, "density");
   if (rc) return rc;
   c->rho_valid = true;
+  c->vr_stale = false;
   return SDPD_OK;
 }
 
@@ -320,7 +338,11 @@ int run_forces(sdpd_ctx* c) {
   c->P.step = c->step;
   {
     PhaseMark pm(c, 3);
-    sdpd::launch_forces_f64(c->stream, c->P, c->S);
+    if (c->P.mixed && c->vr_stale) {
+      sdpd::launch_refresh_velocity_records(c->stream, c->P, c->S);
+      c->vr_stale = false;
+    }
+    sdpd::launch_forces(c->stream, c->P, c->S, c->P.mixed != 0);
   }
   int rc = check_launch(c, "forces");
   if (rc) return rc;
@@ -394,6 +416,8 @@ int sdpd_upload(sdpd_ctx* c, const double* x, const double* v, const double* m,
                 const int32_t* id, int32_t n) {
   if (!c) return SDPD_ERR_ARG;
   if (n <= 0 || !x || !v || !m || !id) return fail(c, SDPD_ERR_ARG, "sdpd_upload: bad arguments");
+  if (n >= (1 << 26))
+    return fail(c, SDPD_ERR_ARG, "sdpd_upload: at most 2^26 - 1 particles per context");
   const int dim = c->P.dim;
   for (int32_t i = 0; i < n; ++i) {
     if (id[2 * (size_t)i + 1] != 0)
@@ -485,6 +509,7 @@ int sdpd_eos(sdpd_ctx* c) {
   if (rc) return rc;
   if (!c->rho_valid) return fail(c, SDPD_ERR_STATE, "sdpd_eos: density is not current");
   sdpd::launch_eos(c->stream, c->P, c->S);
+  c->vr_stale = true;
   return check_launch(c, "eos");
 }
 
@@ -501,6 +526,7 @@ int sdpd_kick(sdpd_ctx* c, double lambda) {
   if (rc) return rc;
   if (!c->f_valid) return fail(c, SDPD_ERR_STATE, "sdpd_kick: forces are not current");
   sdpd::launch_kick(c->stream, c->P, c->S, lambda);
+  c->vr_stale = true;
   return check_launch(c, "kick");
 }
 
@@ -651,9 +677,9 @@ int sdpd_neighbor_signature(sdpd_ctx* c, int32_t mixed_decision, int32_t* count,
   if (rc) return rc;
   if (!c->cells_valid)
     return fail(c, SDPD_ERR_STATE, "sdpd_neighbor_signature: cells are not current");
-  if (mixed_decision)
-    return fail(c, SDPD_ERR_ARG, "sdpd_neighbor_signature: mixed decision not available");
-  sdpd::launch_signature_f64(c->stream, c->P, c->S);
+  if (mixed_decision && !c->P.mixed)
+    return fail(c, SDPD_ERR_ARG, "sdpd_neighbor_signature: mixed decision needs a mixed context");
+  sdpd::launch_signature(c->stream, c->P, c->S, mixed_decision != 0);
   if ((rc = check_launch(c, "signature"))) return rc;
   const int n = c->n;
   std::vector<int32_t> cnt(n), perm(n);

@@ -135,7 +135,8 @@ __global__ void k_place(DevParams P, const int32_t* __restrict__ key,
                         const int32_t* __restrict__ perm, const double* __restrict__ ct,
                         double* __restrict__ x2, double* __restrict__ v2, double* __restrict__ m2,
                         int32_t* __restrict__ pid2, int32_t* __restrict__ perm2,
-                        double* __restrict__ ct2, int32_t* __restrict__ key2) {
+                        double* __restrict__ ct2, int32_t* __restrict__ key2,
+                        float4* __restrict__ xr) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = P.n;
   if (i >= n) return;
@@ -156,6 +157,17 @@ __global__ void k_place(DevParams P, const int32_t* __restrict__ key,
   if (ct) {
     const int d2 = dim * dim;
     for (int q = 0; q < d2; ++q) ct2[(int64_t)q * n + p] = ct[(int64_t)q * n + i];
+  }
+  if (xr) {
+    // FP32 position relative to the lower corner of the own cell
+    int cx = k % P.nm[0];
+    int cy = (k / P.nm[0]) % P.nm[1];
+    int cz = k / (P.nm[0] * P.nm[1]);
+    float a0 = (float)(x[i] - (P.min_phys[0] + (double)cx * P.cs[0]));
+    float a1 = (float)(x[(int64_t)n + i] - (P.min_phys[1] + (double)cy * P.cs[1]));
+    float a2 = dim == 3 ? (float)(x[2 * (int64_t)n + i] - (P.min_phys[2] + (double)cz * P.cs[2]))
+                        : 0.f;
+    xr[p] = make_float4(a0, a1, a2, __uint_as_float((uint32_t)pid[i]));
   }
 }
 
@@ -192,7 +204,7 @@ void launch_cell_sort_after_keys(hipStream_t s, const DevParams& P, const Device
   hipLaunchKernelGGL(k_place, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, S.key, S.start,
                      S.unsorted, S.x, S.v, S.m, S.pid, S.perm, P.newtonian ? nullptr : S.ct,
                      S.x_alt, S.v_alt, S.m_alt, S.pid_alt, S.perm_alt,
-                     P.newtonian ? nullptr : S.ct_alt, S.key_sorted);
+                     P.newtonian ? nullptr : S.ct_alt, S.key_sorted, P.mixed ? S.xr : nullptr);
 }
 
 }  // namespace sdpd

@@ -28,13 +28,18 @@ struct DevParams {
   double cut_off, cut2;
   double h, coef, coef_grad, w0;
   double c2, rho_ref, gamma, p0;
-  double eta, eta_coef, kt, dt, sqrt_dt;
+  double eta, eta_coef, kt, dt, sqrt_dt, inv_sqrt_dt;
   double tau, G;
   double bf[3];
   uint64_t seed;
   int64_t step;
-  // mixed precision: relative band around cut2 resolved in FP64
+  int32_t mixed;          // FP32 pair arithmetic
+  // mixed precision: relative band around cut2 resolved in FP64, FP32
+  // copies of the constants and the cell edges
   float cut2_lo_f32, cut2_hi_f32;
+  float csf[3];
+  float inv_hf, coeff, coef_gradf;
+  float etaf, eta_coeff, ktf, inv_sqrt_dtf, sqrt_2kt_etaf;
 };
 
 // ---------------------------------------------------------------------

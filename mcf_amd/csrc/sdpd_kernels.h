@@ -18,14 +18,15 @@ struct DeviceState {
   int32_t *pid_alt, *perm_alt;
   double *f;             // acceleration (dim, n)
   double *rho, *p;       // density (number density for type 2), pressure
+  double *an, *bn;       // p / n^2 and 1 / n^2 of the last density pass
   // Oldroyd-B: conformation, its rate, velocity gradient (dim*dim, n)
   double *ct, *ct_alt, *act, *vgt;
   // cell list
   int32_t *key, *slot, *count, *start, *unsorted, *key_sorted;
   // mixed precision records (cell order)
   float4 *xr;            // position relative to the cell corner, w = pid bits
-  float4 *vr;            // velocity, w = A = p / n^2
-  float *br;             // B = 1 / n^2
+  float4 *vr;            // velocity, w = p / n^2
+  float2 *pb;            // (1 / n^2, p)
   // reductions and counters
   double *partials;      // stats partial sums (blocks x 8)
   double *stats_out;     // 8 doubles
@@ -41,10 +42,11 @@ void launch_fill_f64(hipStream_t s, double* a, double v, int64_t n);
 void launch_cell_keys(hipStream_t s, const DevParams& P, const DeviceState& S);
 void launch_cell_sort_after_keys(hipStream_t s, const DevParams& P, const DeviceState& S);
 
-void launch_density_f64(hipStream_t s, const DevParams& P, const DeviceState& S);
+void launch_density(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed);
 void launch_eos(hipStream_t s, const DevParams& P, const DeviceState& S);
-void launch_forces_f64(hipStream_t s, const DevParams& P, const DeviceState& S);
-void launch_signature_f64(hipStream_t s, const DevParams& P, const DeviceState& S);
+void launch_forces(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed);
+void launch_signature(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed);
+void launch_refresh_velocity_records(hipStream_t s, const DevParams& P, const DeviceState& S);
 
 void launch_kick(hipStream_t s, const DevParams& P, const DeviceState& S, double lambda);
 void launch_drift_wrap(hipStream_t s, const DevParams& P, const DeviceState& S);

@@ -287,3 +287,96 @@ def test_full_size_tait_box_properties():
     assert 100 < nb < 115
     assert abs(rho.mean() - 360.0 / 359.0) < 0.01
     assert np.isfinite(f).all()
+
+
+# ---------------------------------------------------------------------
+# mixed precision: FP32 pair arithmetic on cell-relative positions, FP64
+# state; same noise stream.  Bars: bit-exact neighbour sets, density to
+# 1e-6, forces to 1e-5 of the rms force against the FP64 path.
+# ---------------------------------------------------------------------
+MIXED_BOXES = {
+    "tait3d_mixed_noise": lambda: W.tait_box_3d(13, kt=1e-3),
+    "tait3d_mixed_full": lambda: W.tait_box_3d(11, kt=1e-3, symmetry=False),
+    "tait2d_mixed": lambda: W.shear_wave_2d(20),
+}
+
+
+@pytest.mark.parametrize("name", list(MIXED_BOXES))
+def test_mixed_neighbor_sets_bitexact(oracle, name):
+    box = W.mixed(MIXED_BOXES[name]())
+    announce(f"mixed neighbour sets: {name}")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.build_list()
+        cnt, hs = g.neighbor_signature(mixed_decision=True)
+    rc, rh = oracle_signature(oracle, box)
+    assert np.array_equal(cnt, rc) and np.array_equal(hs, rh)
+
+
+def test_mixed_neighbor_sets_perfect_lattice_ties(oracle):
+    # every on-sphere lattice neighbour falls in the FP64 re-decision band
+    box = W.mixed(W.tait_box_3d(12, brownian=False))
+    box.x = W.lattice(3, 12, 1.0, 0.0, 0)
+    announce("mixed neighbour sets: perfect lattice")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.build_list()
+        cnt, hs = g.neighbor_signature(mixed_decision=True)
+        _, rechecks = g.counters()
+    rc, rh = oracle_signature(oracle, box)
+    assert np.array_equal(cnt, rc) and np.array_equal(hs, rh)
+    assert rechecks > 0
+
+
+@pytest.mark.parametrize("name", list(MIXED_BOXES))
+@pytest.mark.parametrize("force_type", [3, 4])
+def test_mixed_vs_fp64(name, force_type):
+    box = MIXED_BOXES[name]()
+    box.prm = box.prm.replace(rhs_force_type=force_type)
+    mbox = W.mixed(box)
+    announce(f"mixed vs fp64: {name} type {force_type}")
+    g64 = gpu_evaluate(box, step=9)
+    gmx = gpu_evaluate(mbox, step=9)
+    rho64, p64 = g64.download_density()
+    rhomx, pmx = gmx.download_density()
+    f64 = g64.download_force()
+    fmx = gmx.download_force()
+    pairs64, _ = g64.counters()
+    pairsmx, _ = gmx.counters()
+    g64.close()
+    gmx.close()
+    assert pairs64 == pairsmx
+    assert rel_err(rhomx, rho64) < 1e-6
+    assert vec_err(fmx, f64) < 1e-5
+    mf = box.m[:, None] * fmx
+    assert np.abs(mf.sum(0)).max() < 1e-5 * np.abs(mf).sum()
+
+
+def test_mixed_trajectory_close_to_fp64():
+    box = W.tait_box_3d(13, kt=1e-3)
+    announce("mixed trajectory vs fp64")
+    out = []
+    for b in (box, W.mixed(box)):
+        with SDPD(b.prm) as g:
+            g.upload(b.x, b.v, b.m, b.pid)
+            g.integrate_VV(5)
+            out.append(g.download() + (g.download_force(),))
+    (x64, v64, f64), (xmx, vmx, fmx) = out
+    assert vec_err(vmx, v64) < 1e-5
+    assert vec_err(fmx, f64) < 1e-4
+    assert np.abs(xmx - x64).max() < 1e-8
+
+
+def test_mixed_full_size_properties():
+    box = W.mixed(W.tait_box_3d(128))
+    announce(f"mixed full size: n={box.n}")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.integrate_VV(2)
+        f = g.download_force()
+        pairs, rechecks = g.counters()
+    mf = box.m[:, None] * f
+    assert np.abs(mf.sum(0)).max() < 1e-6 * np.abs(mf).sum()
+    assert 100 < pairs / box.n < 115
+    assert rechecks < 1e-3 * pairs
+    assert np.isfinite(f).all()

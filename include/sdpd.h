@@ -132,19 +132,24 @@ int sdpd_step(sdpd_ctx *ctx, int32_t nsteps);
 int sdpd_set_step(sdpd_ctx *ctx, int64_t step);
 int sdpd_get_step(sdpd_ctx *ctx, int64_t *step);
 
-/* Cell list in upload indices, for checking against the reference's
- * ppm_util_rank3d output: lhbx(ncell+1) offsets, lpdx(n) upload indices,
- * nm[3] cells per dimension. */
+/* PPM's cell list for the current positions (technique_build_list ->
+ * ppm_util_rank3d on the particles in upload order, ppm_util_rank3d.F90:
+ * 395-531, symmetric layout without the ghost layer): nm[3] cells per
+ * dimension (INT(L / (1.0001 cut_off))), lhbx(ncell+1) 0-based offsets,
+ * lpdx(n) upload indices, ascending inside a cell.  The step itself sorts
+ * on a grid with half this cell edge. */
 int sdpd_get_cells(sdpd_ctx *ctx, int32_t *nm, int32_t *lhbx, int32_t *lpdx);
 
-/* Number of in-cutoff ordered pairs seen by the last force pass and the
- * number of candidate distance tests resolved in FP64 (mixed mode). */
+/* Number of in-cutoff ordered pairs found by the last density pass (the
+ * force pass of the same step evaluates exactly these) and the number of
+ * candidate tests that were re-decided in FP64 inside the cut-off band. */
 int sdpd_get_counters(sdpd_ctx *ctx, int64_t *pairs, int64_t *fp64_rechecks);
 
 /* Neighbour-set signature of the current cell list, in upload order: the
  * number of in-cutoff neighbours of each particle and the wrapping sum of
- * a 32-bit hash of (pid_i, pid_j) over them.  mixed_decision = 1 uses the
- * mixed-precision in/out decision (FP32 test, FP64 inside the band). */
+ * a 32-bit hash of (pid_i, pid_j) over them.  Both precision modes share
+ * one in/out decision (FP32 test, FP64 inside a band around the cut-off);
+ * mixed_decision is kept for callers that name the mode explicitly. */
 int sdpd_neighbor_signature(sdpd_ctx *ctx, int32_t mixed_decision,
                             int32_t *count, uint32_t *hash);
 

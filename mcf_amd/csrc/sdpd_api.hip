@@ -2,6 +2,7 @@
 // buffers, host <-> device copies and the velocity-Verlet step driver.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -109,32 +110,60 @@ int derive(const sdpd_params& p, DevParams& P, std::string& err) {
     for (int d = 0; d < p.num_dim; ++d)
       if (std::fabs(p.body_force[d]) > kMachineZero) P.body_force = 1;
   const double cell = p.cut_off * 1.0001;  // mcf.F90:617
-  int64_t ncell = 1;
+  int64_t ncell = 1, ncell_ref = 1;
+  double lmax = 0.0;
   for (int d = 0; d < 3; ++d) {
     if (d < p.num_dim) {
       double L = p.max_phys[d] - p.min_phys[d];
       if (!(L > 0)) return err = "max_phys must exceed min_phys", SDPD_ERR_ARG;
+      // PPM's grid (ppm_neighlist_clist.F90:370-376) and the sort grid with
+      // half its cell edge
       int nm = (int)(L / cell);
       if (nm < 1) nm = 1;
       if (nm < 3)
         return err = "the box must span at least 3 cells of 1.0001*cut_off per dimension",
                SDPD_ERR_ARG;
-      P.nm[d] = nm;
+      P.nm_ref[d] = nm;
+      P.nm[d] = 2 * nm;
       P.min_phys[d] = p.min_phys[d];
       P.length[d] = L;
-      P.rd[d] = (double)nm / L;
-      P.cs[d] = L / (double)nm;
+      P.rd_ref[d] = (double)nm / L;
+      P.rd[d] = (double)(2 * nm) / L;
+      P.cs[d] = L / (double)(2 * nm);
+      lmax = std::max(lmax, L);
     } else {
-      P.nm[d] = 1;
+      P.nm[d] = P.nm_ref[d] = 1;
       P.min_phys[d] = 0.0;
       P.length[d] = 1.0;
-      P.rd[d] = 1.0;
+      P.rd[d] = P.rd_ref[d] = 1.0;
       P.cs[d] = 1.0;
     }
     ncell *= P.nm[d];
+    ncell_ref *= P.nm_ref[d];
   }
   if (ncell > (int64_t)1 << 30) return err = "too many cells", SDPD_ERR_ARG;
   P.ncell = (int32_t)ncell;
+  P.ncell_ref = (int32_t)ncell_ref;
+  {
+    // Band of the FP32 candidate test.  Positions relative to the own cell
+    // corner and the stencil offsets (at most 3 cell edges) carry FP32
+    // errors below 2^-21 * 3 cs each, so a difference is off by at most
+    // dd = 2^-20 * 3 cs and r^2 by 2*sqrt(3)*r*dd + 4*2^-24*r^2.  The band
+    // is twice that bound relative to rc^2, and at least 2e-5.
+    double csmax = 0.0;
+    for (int d = 0; d < p.num_dim; ++d) csmax = std::max(csmax, P.cs[d]);
+    const double dd = std::ldexp(3.0 * csmax, -20);
+    double rel = 2.0 * (2.0 * std::sqrt(3.0) * dd / p.cut_off + 4.0 * std::ldexp(1.0, -24));
+    rel = std::max(rel, 2e-5);
+    const double c2 = p.cut_off * p.cut_off;
+    P.cut2_lo_f32 = (float)(c2 * std::max(0.0, 1.0 - rel)) * (1.0f - 1e-6f);
+    P.cut2_hi_f32 = (float)(c2 * (1.0 + rel)) * (1.0f + 1e-6f);
+    for (int d = 0; d < 3; ++d) {
+      P.lengthf[d] = (float)P.length[d];
+      P.csf[d] = (float)P.cs[d];
+    }
+    (void)lmax;
+  }
   P.cut_off = p.cut_off;
   P.cut2 = p.cut_off * p.cut_off;
   // kernel_new.F90:47-93
@@ -167,10 +196,6 @@ int derive(const sdpd_params& p, DevParams& P, std::string& err) {
   P.seed = p.seed;
   P.step = 0;
   P.mixed = p.precision == SDPD_PRECISION_MIXED ? 1 : 0;
-  const float c2f = (float)P.cut2;
-  P.cut2_lo_f32 = c2f * (1.0f - 2e-5f);
-  P.cut2_hi_f32 = c2f * (1.0f + 2e-5f);
-  for (int d = 0; d < 3; ++d) P.csf[d] = (float)P.cs[d];
   P.inv_hf = (float)(1.0 / P.h);
   P.coeff = (float)P.coef;
   P.coef_gradf = (float)P.coef_grad;
@@ -194,8 +219,10 @@ void free_state(DeviceState& S) {
                   S.v_alt,    S.m_alt,    S.pid_alt,   S.perm_alt, S.f,       S.rho,
                   S.p,        S.an,       S.bn,        S.ct,      S.ct_alt,   S.act,
                   S.vgt,      S.key,      S.slot,      S.count,   S.start,    S.unsorted,
-                  S.key_sorted, S.xr,     S.vr,        S.pb,      S.partials, S.stats_out,
-                  S.nbr_count, S.recheck, S.nbr_hash};
+                  S.key_sorted, S.xa,     S.vr,        S.qr,      S.partials, S.stats_out,
+                  S.nbr_count, S.recheck, S.nbr_hash, S.rkey,   S.rslot,    S.rcount,
+                  S.rstart,   S.runsorted, S.rlpdx,  S.nlist,   S.nbr_sig_count,
+                  S.row_nchunk, S.row_chunk_off, S.chunk_row, S.scan_tmp, S.rec64};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   std::memset(&S, 0, sizeof(S));
@@ -235,15 +262,28 @@ int alloc_state(sdpd_ctx* c, int n) {
   ALLOC(start, (size_t)P.ncell + 1);
   ALLOC(unsorted, N);
   ALLOC(key_sorted, N);
+  ALLOC(xa, N);
   if (c->prm.precision == SDPD_PRECISION_MIXED) {
-    ALLOC(xr, N);
     ALLOC(vr, N);
-    ALLOC(pb, N);
+    ALLOC(qr, N);
+  } else {
+    ALLOC(rec64, 8 * N);
+  }
+  {
+    const size_t nrows = (size_t)P.nm[1] * P.nm[2];
+    const size_t nchunk = (N + 63) / 64 + nrows;
+    const size_t ntile = ((size_t)std::max(P.ncell, (int32_t)nrows) + 4095) / 4096 + 1;
+    ALLOC(row_nchunk, nrows);
+    ALLOC(row_chunk_off, nrows + 1);
+    ALLOC(chunk_row, nchunk);
+    ALLOC(scan_tmp, std::max(ntile, (size_t)((P.ncell_ref + 4095) / 4096 + 1)));
   }
   c->stats_nb = sdpd::stats_blocks(n);
   ALLOC(partials, (size_t)c->stats_nb * 4);
   ALLOC(stats_out, 8);
   ALLOC(nbr_count, N);
+  ALLOC(nbr_sig_count, N);
+  ALLOC(nlist, (size_t)P.list_cap * N);
   ALLOC(recheck, N);
   ALLOC(nbr_hash, N);
 #undef ALLOC
@@ -338,8 +378,8 @@ int run_forces(sdpd_ctx* c) {
   c->P.step = c->step;
   {
     PhaseMark pm(c, 3);
-    if (c->P.mixed && c->vr_stale) {
-      sdpd::launch_refresh_velocity_records(c->stream, c->P, c->S);
+    if (c->vr_stale) {
+      sdpd::launch_refresh_records(c->stream, c->P, c->S);
       c->vr_stale = false;
     }
     sdpd::launch_forces(c->stream, c->P, c->S, c->P.mixed != 0);
@@ -416,8 +456,8 @@ int sdpd_upload(sdpd_ctx* c, const double* x, const double* v, const double* m,
                 const int32_t* id, int32_t n) {
   if (!c) return SDPD_ERR_ARG;
   if (n <= 0 || !x || !v || !m || !id) return fail(c, SDPD_ERR_ARG, "sdpd_upload: bad arguments");
-  if (n >= (1 << 26))
-    return fail(c, SDPD_ERR_ARG, "sdpd_upload: at most 2^26 - 1 particles per context");
+  if (n >= (1 << 25))
+    return fail(c, SDPD_ERR_ARG, "sdpd_upload: at most 2^25 - 1 particles per context");
   const int dim = c->P.dim;
   for (int32_t i = 0; i < n; ++i) {
     if (id[2 * (size_t)i + 1] != 0)
@@ -434,6 +474,19 @@ int sdpd_upload(sdpd_ctx* c, const double* x, const double* v, const double* m,
   free_state(c->S);
   c->n = 0;
   c->P.n = n;
+  {
+    // capacity of the stored neighbour lists: 1.6 times the neighbours a
+    // uniform fluid of this mean density has inside the cut-off, plus 32;
+    // a particle with more is handled by scanning again in the force pass
+    double vol = 1.0;
+    for (int d = 0; d < dim; ++d) vol *= c->P.length[d];
+    const double rc_ = c->P.cut_off;
+    const double ball = dim == 3 ? 4.0 / 3.0 * kPi * rc_ * rc_ * rc_ : kPi * rc_ * rc_;
+    const double expect = (double)n / vol * ball;
+    int cap = (int)std::ceil(1.6 * expect) + 32;
+    cap = (cap + 15) / 16 * 16;
+    c->P.list_cap = std::min(cap, 4096);
+  }
   int rc = alloc_state(c, n);
   if (rc) {
     free_state(c->S);
@@ -642,15 +695,29 @@ int sdpd_get_step(sdpd_ctx* c, int64_t* step) {
 int sdpd_get_cells(sdpd_ctx* c, int32_t* nm, int32_t* lhbx, int32_t* lpdx) {
   int rc = need_particles(c);
   if (rc) return rc;
-  if (!c->cells_valid) return fail(c, SDPD_ERR_STATE, "sdpd_get_cells: cells are not current");
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (nm)
-    for (int d = 0; d < 3; ++d) nm[d] = c->P.nm[d];
+    for (int d = 0; d < 3; ++d) nm[d] = c->P.nm_ref[d];
+  if (!lhbx && !lpdx) return SDPD_OK;
+  DeviceState& S = c->S;
+  const size_t N = (size_t)c->n;
+  if (!S.rkey) {
+#define RALLOC(ptr, cnt) \
+  if ((rc = dalloc(c, &S.ptr, (cnt))) != SDPD_OK) return rc;
+    RALLOC(rkey, N);
+    RALLOC(rslot, N);
+    RALLOC(rcount, (size_t)c->P.ncell_ref);
+    RALLOC(rstart, (size_t)c->P.ncell_ref + 1);
+    RALLOC(runsorted, N);
+    RALLOC(rlpdx, N);
+#undef RALLOC
+  }
+  sdpd::launch_reference_cells(c->stream, c->P, c->S);
+  if ((rc = check_launch(c, "reference cells"))) return rc;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (lhbx)
-    HIP_TRY(c, hipMemcpy(lhbx, c->S.start, ((size_t)c->P.ncell + 1) * sizeof(int32_t),
+    HIP_TRY(c, hipMemcpy(lhbx, S.rstart, ((size_t)c->P.ncell_ref + 1) * sizeof(int32_t),
                          hipMemcpyDeviceToHost));
-  if (lpdx)
-    HIP_TRY(c, hipMemcpy(lpdx, c->S.perm, (size_t)c->n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (lpdx) HIP_TRY(c, hipMemcpy(lpdx, S.rlpdx, N * sizeof(int32_t), hipMemcpyDeviceToHost));
   return SDPD_OK;
 }
 
@@ -677,15 +744,14 @@ int sdpd_neighbor_signature(sdpd_ctx* c, int32_t mixed_decision, int32_t* count,
   if (rc) return rc;
   if (!c->cells_valid)
     return fail(c, SDPD_ERR_STATE, "sdpd_neighbor_signature: cells are not current");
-  if (mixed_decision && !c->P.mixed)
-    return fail(c, SDPD_ERR_ARG, "sdpd_neighbor_signature: mixed decision needs a mixed context");
   sdpd::launch_signature(c->stream, c->P, c->S, mixed_decision != 0);
   if ((rc = check_launch(c, "signature"))) return rc;
   const int n = c->n;
   std::vector<int32_t> cnt(n), perm(n);
   std::vector<uint32_t> hs(n);
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  HIP_TRY(c, hipMemcpy(cnt.data(), c->S.nbr_count, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(cnt.data(), c->S.nbr_sig_count, n * sizeof(int32_t),
+                       hipMemcpyDeviceToHost));
   HIP_TRY(c, hipMemcpy(hs.data(), c->S.nbr_hash, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
   HIP_TRY(c, hipMemcpy(perm.data(), c->S.perm, n * sizeof(int32_t), hipMemcpyDeviceToHost));
   for (int i = 0; i < n; ++i) {

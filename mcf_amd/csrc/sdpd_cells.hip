@@ -66,58 +66,120 @@ __global__ void k_cell_key(DevParams P, const double* __restrict__ x, int32_t* _
   slot[i] = atomicAdd(&count[k], 1);
 }
 
-// Exclusive scan of count[0..ncell) into start[0..ncell], one workgroup of
-// 1024 threads walking the array in tiles of 4096 with a running carry.
-__global__ __launch_bounds__(1024) void k_scan_cells(const int32_t* __restrict__ count,
-                                                     int32_t* __restrict__ start, int32_t ncell) {
-  __shared__ int32_t wsum[16];
-  __shared__ int32_t carry_s;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wid = tid >> 6;
-  if (tid == 0) carry_s = 0;
+// Exclusive scan of count[0..m) into start[0..m] in three passes over
+// tiles of 4096 entries: tile sums, a scan of the tile sums in one
+// workgroup, and the tile scans with their offsets.
+constexpr int kScanTile = 4096;
+
+__device__ __forceinline__ int32_t block_exclusive_scan_1024(int32_t v, int32_t& total,
+                                                             int32_t* wsum) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int32_t incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int32_t t = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += t;
+  }
+  if (lane == 63) wsum[wid] = incl;
   __syncthreads();
-  for (int64_t base = 0; base < ncell; base += 4096) {
-    int32_t v[4];
-    int32_t local = 0;
+  if (wid == 0) {
+    int32_t s = lane < 16 ? wsum[lane] : 0;
+    int32_t si = s;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      int64_t idx = base + (int64_t)tid * 4 + q;
-      v[q] = idx < ncell ? count[idx] : 0;
-      local += v[q];
+    for (int off = 1; off < 16; off <<= 1) {
+      int32_t t = __shfl_up(si, off, 64);
+      if (lane >= off) si += t;
     }
-    // inclusive wave scan of local
-    int32_t incl = local;
+    if (lane < 16) wsum[lane] = si - s;
+    if (lane == 15) wsum[16] = si;
+  }
+  __syncthreads();
+  total = wsum[16];
+  int32_t r = wsum[wid] + incl - v;
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tile_sums(const int32_t* __restrict__ count,
+                                                         int32_t* __restrict__ sums, int32_t m) {
+  __shared__ int32_t wsum[17];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * 4;
+  int32_t local = 0;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      int32_t t = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += t;
-    }
-    if (lane == 63) wsum[wid] = incl;
+  for (int q = 0; q < 4; ++q)
+    if (base + q < m) local += count[base + q];
+  int32_t total;
+  (void)block_exclusive_scan_1024(local, total, wsum);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_sums(int32_t* __restrict__ sums, int32_t ntile,
+                                                    int32_t* __restrict__ out_total) {
+  __shared__ int32_t wsum[17];
+  __shared__ int32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int32_t b0 = 0; b0 < ntile; b0 += 1024) {
+    const int32_t idx = b0 + threadIdx.x;
+    const int32_t v = idx < ntile ? sums[idx] : 0;
+    int32_t total;
+    const int32_t ex = block_exclusive_scan_1024(v, total, wsum);
+    const int32_t c = carry;
+    if (idx < ntile) sums[idx] = c + ex;
     __syncthreads();
-    if (wid == 0) {
-      int32_t s = lane < 16 ? wsum[lane] : 0;
-      int32_t si = s;
-#pragma unroll
-      for (int off = 1; off < 16; off <<= 1) {
-        int32_t t = __shfl_up(si, off, 64);
-        if (lane >= off) si += t;
-      }
-      if (lane < 16) wsum[lane] = si - s;  // exclusive wave offsets
-    }
-    __syncthreads();
-    int32_t carry = carry_s;
-    int32_t excl = carry + wsum[wid] + incl - local;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      int64_t idx = base + (int64_t)tid * 4 + q;
-      if (idx < ncell) start[idx] = excl;
-      excl += v[q];
-    }
-    __syncthreads();
-    if (tid == 1023) carry_s = excl;
+    if (threadIdx.x == 0) carry = c + total;
     __syncthreads();
   }
-  if (tid == 0) start[ncell] = carry_s;
+  if (threadIdx.x == 0) *out_total = carry;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tiles(const int32_t* __restrict__ count,
+                                                     const int32_t* __restrict__ sums,
+                                                     int32_t* __restrict__ start, int32_t m) {
+  __shared__ int32_t wsum[17];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * 4;
+  int32_t v[4], local = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    v[q] = base + q < m ? count[base + q] : 0;
+    local += v[q];
+  }
+  int32_t total;
+  int32_t ex = block_exclusive_scan_1024(local, total, wsum) + sums[blockIdx.x];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (base + q < m) start[base + q] = ex;
+    ex += v[q];
+  }
+}
+
+// start[m] receives the total
+void launch_exclusive_scan(hipStream_t s, const int32_t* count, int32_t* start, int32_t m,
+                           int32_t* tile_sums) {
+  const int32_t ntile = (m + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(k_scan_tile_sums, dim3(ntile), dim3(1024), 0, s, count, tile_sums, m);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, tile_sums, ntile, start + m);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(1024), 0, s, count, tile_sums, start, m);
+}
+
+// Work map of the pair kernels: each (y, z) row of the sort grid is cut
+// into chunks of at most 64 consecutive particles; chunk c belongs to row
+// chunk_row[c] and is the (c - row_chunk_off[row])-th chunk of that row.
+__global__ void k_row_chunks(DevParams P, const int32_t* __restrict__ start,
+                             int32_t* __restrict__ row_nchunk) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nrows = P.nm[1] * P.nm[2];
+  if (r >= nrows) return;
+  int cnt = start[(r + 1) * P.nm[0]] - start[r * P.nm[0]];
+  row_nchunk[r] = (cnt + 63) / 64;
+}
+
+__global__ void k_chunk_map(DevParams P, const int32_t* __restrict__ row_chunk_off,
+                            int32_t* __restrict__ chunk_row) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nrows = P.nm[1] * P.nm[2];
+  if (r >= nrows) return;
+  for (int c = row_chunk_off[r]; c < row_chunk_off[r + 1]; ++c) chunk_row[c] = r;
 }
 
 __global__ void k_scatter(int n, const int32_t* __restrict__ key, const int32_t* __restrict__ slot,
@@ -136,7 +198,7 @@ __global__ void k_place(DevParams P, const int32_t* __restrict__ key,
                         double* __restrict__ x2, double* __restrict__ v2, double* __restrict__ m2,
                         int32_t* __restrict__ pid2, int32_t* __restrict__ perm2,
                         double* __restrict__ ct2, int32_t* __restrict__ key2,
-                        float4* __restrict__ xr) {
+                        float4* __restrict__ xa) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = P.n;
   if (i >= n) return;
@@ -158,17 +220,50 @@ __global__ void k_place(DevParams P, const int32_t* __restrict__ key,
     const int d2 = dim * dim;
     for (int q = 0; q < d2; ++q) ct2[(int64_t)q * n + p] = ct[(int64_t)q * n + i];
   }
-  if (xr) {
-    // FP32 position relative to the lower corner of the own cell
-    int cx = k % P.nm[0];
-    int cy = (k / P.nm[0]) % P.nm[1];
-    int cz = k / (P.nm[0] * P.nm[1]);
-    float a0 = (float)(x[i] - (P.min_phys[0] + (double)cx * P.cs[0]));
-    float a1 = (float)(x[(int64_t)n + i] - (P.min_phys[1] + (double)cy * P.cs[1]));
-    float a2 = dim == 3 ? (float)(x[2 * (int64_t)n + i] - (P.min_phys[2] + (double)cz * P.cs[2]))
-                        : 0.f;
-    xr[p] = make_float4(a0, a1, a2, __uint_as_float((uint32_t)pid[i]));
-  }
+  // FP32 position relative to the lower corner of the own cell, the cell's
+  // x index in w (records of the neighbour scan)
+  const int cx = k % P.nm[0];
+  const int cy = (k / P.nm[0]) % P.nm[1];
+  const int cz = k / (P.nm[0] * P.nm[1]);
+  xa[p] = make_float4((float)(x[i] - (P.min_phys[0] + (double)cx * P.cs[0])),
+                      (float)(x[(int64_t)n + i] - (P.min_phys[1] + (double)cy * P.cs[1])),
+                      dim == 3 ? (float)(x[2 * (int64_t)n + i] - (P.min_phys[2] + (double)cz * P.cs[2]))
+                               : 0.f,
+                      __int_as_float(cx));
+}
+
+// ---------------------------------------------------------------------
+// PPM's cell list on PPM's grid (cell edge >= 1.0001 cut_off), in upload
+// indices: the order inside a cell is ascending upload index, which is
+// what ppm_util_rank3d produces for particles stored in upload order.
+// ---------------------------------------------------------------------
+__global__ void k_ref_key(DevParams P, const double* __restrict__ x, int32_t* __restrict__ key,
+                          int32_t* __restrict__ slot, int32_t* __restrict__ count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const int64_t n = P.n;
+  int cx = cell_coord(x[i], P.min_phys[0], P.rd_ref[0], P.min_phys[0] + P.length[0], P.nm_ref[0]);
+  int cy = cell_coord(x[n + i], P.min_phys[1], P.rd_ref[1], P.min_phys[1] + P.length[1],
+                      P.nm_ref[1]);
+  int cz = P.dim == 3 ? cell_coord(x[2 * n + i], P.min_phys[2], P.rd_ref[2],
+                                   P.min_phys[2] + P.length[2], P.nm_ref[2])
+                      : 0;
+  int32_t k = cx + P.nm_ref[0] * (cy + P.nm_ref[1] * cz);
+  key[i] = k;
+  slot[i] = atomicAdd(&count[k], 1);
+}
+
+__global__ void k_ref_rank(int n, const int32_t* __restrict__ key,
+                           const int32_t* __restrict__ start,
+                           const int32_t* __restrict__ unsorted,
+                           const int32_t* __restrict__ perm, int32_t* __restrict__ lpdx) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t k = key[i];
+  int32_t s = start[k], e = start[k + 1];
+  int32_t me = perm[i], r = 0;
+  for (int32_t q = s; q < e; ++q) r += perm[unsorted[q]] < me ? 1 : 0;
+  lpdx[s + r] = me;
 }
 
 // ---------------------------------------------------------------------
@@ -198,13 +293,31 @@ void launch_cell_keys(hipStream_t s, const DevParams& P, const DeviceState& S) {
 
 void launch_cell_sort_after_keys(hipStream_t s, const DevParams& P, const DeviceState& S) {
   const int n = P.n;
-  hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, S.count, S.start, P.ncell);
+  launch_exclusive_scan(s, S.count, S.start, P.ncell, S.scan_tmp);
   hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, S.key, S.slot,
                      S.start, S.unsorted);
   hipLaunchKernelGGL(k_place, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, S.key, S.start,
                      S.unsorted, S.x, S.v, S.m, S.pid, S.perm, P.newtonian ? nullptr : S.ct,
                      S.x_alt, S.v_alt, S.m_alt, S.pid_alt, S.perm_alt,
-                     P.newtonian ? nullptr : S.ct_alt, S.key_sorted, P.mixed ? S.xr : nullptr);
+                     P.newtonian ? nullptr : S.ct_alt, S.key_sorted, S.xa);
+  const int nrows = P.nm[1] * P.nm[2];
+  hipLaunchKernelGGL(k_row_chunks, dim3(blocks_for(nrows, 256)), dim3(256), 0, s, P, S.start,
+                     S.row_nchunk);
+  launch_exclusive_scan(s, S.row_nchunk, S.row_chunk_off, nrows, S.scan_tmp);
+  hipLaunchKernelGGL(k_chunk_map, dim3(blocks_for(nrows, 256)), dim3(256), 0, s, P,
+                     S.row_chunk_off, S.chunk_row);
+}
+
+void launch_reference_cells(hipStream_t s, const DevParams& P, const DeviceState& S) {
+  const int n = P.n;
+  launch_fill_i32(s, S.rcount, 0, P.ncell_ref);
+  hipLaunchKernelGGL(k_ref_key, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, S.x, S.rkey,
+                     S.rslot, S.rcount);
+  launch_exclusive_scan(s, S.rcount, S.rstart, P.ncell_ref, S.scan_tmp);
+  hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, S.rkey, S.rslot,
+                     S.rstart, S.runsorted);
+  hipLaunchKernelGGL(k_ref_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, S.rkey, S.rstart,
+                     S.runsorted, S.perm, S.rlpdx);
 }
 
 }  // namespace sdpd

@@ -19,11 +19,17 @@ struct DevParams {
   int32_t noise;          // Brownian and kt > machine zero
   int32_t body_force;     // constant body force present
   int32_t n;              // particles
-  int32_t nm[3];          // cells per dimension (>= 3)
+  // sort / gather grid: half of PPM's cell edge, so the stencil is 5 cells
+  // wide per dimension (at least 6 cells per dimension)
+  int32_t nm[3];
   int32_t ncell;
+  // PPM's own grid, cell edge >= 1.0001 cut_off (at least 3 per dimension)
+  int32_t nm_ref[3];
+  int32_t ncell_ref;
   double min_phys[3];
   double length[3];
-  double rd[3];           // nm / length (PPM's rdx)
+  double rd[3];           // nm / length
+  double rd_ref[3];       // nm_ref / length (PPM's rdx)
   double cs[3];           // cell edge length / nm
   double cut_off, cut2;
   double h, coef, coef_grad, w0;
@@ -34,10 +40,13 @@ struct DevParams {
   uint64_t seed;
   int64_t step;
   int32_t mixed;          // FP32 pair arithmetic
-  // mixed precision: relative band around cut2 resolved in FP64, FP32
-  // copies of the constants and the cell edges
+  int32_t list_cap;       // stored neighbour entries per particle
+  // band around cut2 of the FP32 candidate test; inside it the decision is
+  // taken in FP64.  FP32 positions are relative to the own cell corner.
   float cut2_lo_f32, cut2_hi_f32;
-  float csf[3];
+  float lengthf[3];
+  float csf[3];           // FP32 cell edges of the sort grid
+  // FP32 copies of the constants (mixed precision)
   float inv_hf, coeff, coef_gradf;
   float etaf, eta_coeff, ktf, inv_sqrt_dtf, sqrt_2kt_etaf;
 };

@@ -24,15 +24,25 @@ struct DeviceState {
   // cell list
   int32_t *key, *slot, *count, *start, *unsorted, *key_sorted;
   // mixed precision records (cell order)
-  float4 *xr;            // position relative to the cell corner, w = pid bits
-  float4 *vr;            // velocity, w = p / n^2
-  float2 *pb;            // (1 / n^2, p)
+  float4 *xa;            // FP32 position relative to the own cell corner, w = cell x index
+  // work map of the pair kernels (chunks of <= 64 particles of one row)
+  int32_t *row_nchunk, *row_chunk_off, *chunk_row;
+  int32_t *scan_tmp;     // tile sums of the exclusive scans
+  // FP64 records of the force pass: x, y, z, p/n^2, vx, vy, vz, 1/n^2
+  double *rec64;
+  float4 *vr;            // velocity, w = pid bits
+  float4 *qr;            // (p / n^2, 1 / n^2, p, 0)
   // reductions and counters
   double *partials;      // stats partial sums (blocks x 8)
   double *stats_out;     // 8 doubles
-  int32_t *nbr_count;    // per-particle in-cutoff count of the last pass
+  int32_t *nbr_count;    // per-particle in-cutoff count of the last density pass
+  uint32_t *nlist;       // in-cutoff neighbour lists of the last density pass,
+                         // slot-major: entry q of particle i at [q * n + i]
+  int32_t *nbr_sig_count;  // neighbour counts of the last signature pass
   int32_t *recheck;      // per-particle FP64 re-decisions (mixed mode)
   uint32_t *nbr_hash;    // per-particle sum of pair hashes (checking)
+  // reference-grid cell list (built on request by sdpd_get_cells)
+  int32_t *rkey, *rslot, *rcount, *rstart, *runsorted, *rlpdx;
 };
 
 void launch_fill_i32(hipStream_t s, int32_t* a, int32_t v, int64_t n);
@@ -40,13 +50,19 @@ void launch_fill_f64(hipStream_t s, double* a, double v, int64_t n);
 // key + histogram (counts zeroed first), then scan, scatter and place into
 // the *_alt buffers; the caller swaps the ping-pong pointers afterwards
 void launch_cell_keys(hipStream_t s, const DevParams& P, const DeviceState& S);
+void launch_exclusive_scan(hipStream_t s, const int32_t* count, int32_t* start, int32_t m,
+                           int32_t* tile_sums);
 void launch_cell_sort_after_keys(hipStream_t s, const DevParams& P, const DeviceState& S);
+// PPM's cell list on PPM's grid, particles ordered by upload index inside a
+// cell (ppm_util_rank3d on the particles in upload order); fills rstart and
+// rlpdx without moving the state
+void launch_reference_cells(hipStream_t s, const DevParams& P, const DeviceState& S);
 
 void launch_density(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed);
 void launch_eos(hipStream_t s, const DevParams& P, const DeviceState& S);
 void launch_forces(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed);
 void launch_signature(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed);
-void launch_refresh_velocity_records(hipStream_t s, const DevParams& P, const DeviceState& S);
+void launch_refresh_records(hipStream_t s, const DevParams& P, const DeviceState& S);
 
 void launch_kick(hipStream_t s, const DevParams& P, const DeviceState& S, double lambda);
 void launch_drift_wrap(hipStream_t s, const DevParams& P, const DeviceState& S);

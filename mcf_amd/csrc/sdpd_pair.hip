@@ -7,204 +7,258 @@
 // pp_interaction_ff.inc, pp_vgt_ip.inc and rhs_force_ff_*).  The reference
 // visits each unordered pair once in a half stencil and adds the result to
 // both particles and to periodic ghost copies that are folded back later.
-// Here every particle gathers over the 27 (2D: 9) cells around its own
-// cell with the periodic image applied per stencil cell and writes only
-// its own sums: no atomics, no ghosts, identical results run to run.
+// Here every particle gathers over the cells around its own, with the
+// periodic image applied per stencil segment, and writes only its own
+// sums: no atomics, no ghosts, identical results run to run.
 //
-// Two phases per wavefront, so that the expensive pair body runs on full
-// lanes: (A) each lane scans its candidates (cells in fixed order, cell
-// members in cell order) and appends the in-cutoff ones to a private LDS
-// list; (B) once any lane's list is full, or all candidates are scanned,
-// every lane evaluates its list.  The order in which a particle adds its
-// pair terms is therefore the fixed scan order.
+// Work decomposition.  The sort grid has half of PPM's cell edge, so the
+// neighbours of a particle lie in the 5 x 5 x 5 (2D: 5 x 5) block of cells
+// around its own.  One wavefront takes a chunk of at most 64 consecutive
+// particles of one (y, z) row of cells, so all its lanes share the 25
+// (2D: 5) stencil rows.  For each stencil row the wave stages the union of
+// its lanes' candidate cells into LDS with coalesced loads -- laid out in
+// "virtual cell" order, where the cells seen across a periodic x face
+// follow the row with their image shift -- and every lane scans the five
+// cells around its own from LDS.
 //
-// Cut-off decision: FP64 mode forms r^2 exactly as the reference does
-// (pp_interaction.inc:37-46).  Mixed mode tests r^2 in FP32 on positions
-// relative to the cell corner and re-decides in FP64, with the reference's
-// image convention, every candidate within a relative band of 2e-5 around
-// rc^2 -- wider than the FP32 error bound -- so neighbour sets stay bit-exact.
+// Cut-off decision (both modes): r^2 in FP32 on positions relative to the
+// cell corner; a candidate within a band around rc^2 wider than the FP32
+// error bound is re-decided in FP64 exactly as the reference forms it
+// (pp_interaction.inc:37-46) with the reference's periodic image
+// convention.  Neighbour sets are therefore bit-exact in both modes.
+//
+// The density pass appends the accepted neighbours of every particle to a
+// per-lane LDS list, evaluates the density terms whenever the lists fill
+// up, and stores the lists slot-major in global memory; the force pass of
+// the same step (positions unchanged) walks the stored lists.
 #include "sdpd_kernels.h"
 
 namespace sdpd {
 
 constexpr int kWave = 64;
-constexpr int kBuf = 32;          // list entries per lane between flushes
-constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / kWave;
-constexpr int kCodeShift = 26;    // list entry = j | (stencil code << 26)
+constexpr int kBuf = 16;          // LDS list entries per lane between flushes
+constexpr int kTile = 224;        // staged candidate records per stencil row
+constexpr int kCodeShift = 25;    // list entry = j | (stencil code << 25)
 constexpr uint32_t kIndexMask = (1u << kCodeShift) - 1u;
 
-struct CellCoord {
-  int cx, cy, cz;
-};
-
-__device__ __forceinline__ CellCoord decode_key(const DevParams& P, int32_t k) {
-  CellCoord c;
-  c.cx = k % P.nm[0];
-  int r = k / P.nm[0];
-  c.cy = r % P.nm[1];
-  c.cz = r / P.nm[1];
-  return c;
+// Stencil code of a neighbour: offsets o in -2..2 per dimension,
+// code = (ox+2) + 5 (oy+2) + 25 (oz+2).
+__device__ __forceinline__ uint32_t make_code(int ox, int oy, int oz) {
+  return (uint32_t)((ox + 2) + 5 * (oy + 2) + 25 * (oz + 2));
 }
 
-// One stencil cell seen from a home cell: wrapped index, per-dimension
-// image mode (see wrap_cell) and the FP32 offset of its corner.
-struct CellInfo {
-  int nc;
+struct Offsets {
+  int ox, oy, oz;
+};
+__device__ __forceinline__ Offsets decode_code(uint32_t code) {
+  Offsets o;
+  o.ox = (int)(code % 5u) - 2;
+  o.oy = (int)((code / 5u) % 5u) - 2;
+  o.oz = (int)(code / 25u) - 2;
+  return o;
+}
+
+// Per-lane context of the home particle.
+struct Home {
+  int i;                // own index (valid for active lanes)
+  int ii;               // a safe index for every lane
+  bool active;
+  int cx, cy, cz;       // own cell
+  float4 a;             // FP32 position relative to the own cell corner
+  double x0, x1, x2;    // FP64 position
+};
+
+// Image modes of a neighbour at offsets o from the home cell.
+struct Modes {
   int m0, m1, m2;
-  float o0, o1, o2;
 };
-
 template <int DIM>
-__device__ __forceinline__ CellInfo cell_info(const DevParams& P, const CellCoord& c, int code) {
-  CellInfo ci;
-  int ox = code % 3 - 1;
-  int oy = (code / 3) % 3 - 1;
-  int oz = DIM == 3 ? code / 9 - 1 : 0;
-  int nx = wrap_cell(c.cx + ox, P.nm[0], P.symmetry, ci.m0);
-  int ny = wrap_cell(c.cy + oy, P.nm[1], P.symmetry, ci.m1);
-  int nz = 0;
-  ci.m2 = 0;
-  if (DIM == 3) nz = wrap_cell(c.cz + oz, P.nm[2], P.symmetry, ci.m2);
-  ci.nc = nx + P.nm[0] * (ny + P.nm[1] * nz);
-  ci.o0 = (float)ox * P.csf[0];
-  ci.o1 = (float)oy * P.csf[1];
-  ci.o2 = (float)oz * P.csf[2];
-  return ci;
+__device__ __forceinline__ Modes modes_of(const DevParams& P, const Home& h, const Offsets& o) {
+  Modes m;
+  (void)wrap_cell(h.cx + o.ox, P.nm[0], P.symmetry, m.m0);
+  (void)wrap_cell(h.cy + o.oy, P.nm[1], P.symmetry, m.m1);
+  m.m2 = 0;
+  if (DIM == 3) (void)wrap_cell(h.cz + o.oz, P.nm[2], P.symmetry, m.m2);
+  return m;
 }
 
-// Exact FP64 difference and squared distance for a candidate.
+// Exact FP64 difference and squared distance for a pair.
 template <int DIM>
 __device__ __forceinline__ double exact_d2(const DevParams& P, const double* __restrict__ x, int j,
-                                           double xi0, double xi1, double xi2, const CellInfo& ci,
-                                           double& d0, double& d1, double& d2) {
+                                           const Home& h, const Modes& m, double& d0, double& d1,
+                                           double& d2) {
   const int64_t n = P.n;
-  d0 = image_diff(xi0, x[j], P.length[0], ci.m0);
-  d1 = image_diff(xi1, x[n + j], P.length[1], ci.m1);
-  d2 = DIM == 3 ? image_diff(xi2, x[2 * n + j], P.length[2], ci.m2) : 0.0;
+  d0 = image_diff(h.x0, x[j], P.length[0], m.m0);
+  d1 = image_diff(h.x1, x[n + j], P.length[1], m.m1);
+  d2 = DIM == 3 ? image_diff(h.x2, x[2 * n + j], P.length[2], m.m2) : 0.0;
   return DIM == 3 ? dist2_exact(d0, d1, d2) : dist2_exact2(d0, d1);
 }
 
-// ---------------------------------------------------------------------
-// the two-phase gather driver
-// ---------------------------------------------------------------------
-// Pol provides:
-//   bool test(int j, const CellInfo&)   in-cutoff decision for candidate j
-//   void body(int j, const CellInfo&)   accumulate the pair term
-template <int DIM, class Pol>
-__device__ __forceinline__ void gather(const DevParams& P, const int32_t* __restrict__ start, int i,
-                                       bool active, const CellCoord& c, Pol& pol,
-                                       uint32_t* __restrict__ buf) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int ncode = DIM == 3 ? 27 : 9;
-  int code = -1, j = 0, e = 0;
-  bool done = !active;
-  CellInfo ci;
-  auto advance = [&]() {
-    while (j >= e) {
-      ++code;
-      if (code >= ncode) {
-        done = true;
-        return;
-      }
-      ci = cell_info<DIM>(P, c, code);
-      j = start[ci.nc];
-      e = start[ci.nc + 1];
-    }
-  };
-  if (!done) advance();
-  int cnt = 0;
-  while (true) {
-    // phase A: scan until a lane's list is full or every lane is done
-    while (true) {
-      if (__ballot(cnt >= kBuf) != 0ull) break;
-      if (__ballot(!done) == 0ull) break;
-      if (!done) {
-        if (j != i && pol.test(j, ci)) {
-          buf[cnt * kWave + lane] = (uint32_t)j | ((uint32_t)code << kCodeShift);
-          ++cnt;
-        }
-        ++j;
-        if (j >= e) advance();
-      }
-    }
-    // phase B: every lane evaluates its list
-    for (int q = 0; __ballot(q < cnt) != 0ull; ++q) {
-      if (q < cnt) {
-        uint32_t ent = buf[q * kWave + lane];
-        int jj = (int)(ent & kIndexMask);
-        CellInfo cj = cell_info<DIM>(P, c, (int)(ent >> kCodeShift));
-        pol.body(jj, cj);
-      }
-    }
-    cnt = 0;
-    if (__ballot(!done) == 0ull) break;
-  }
+// FP32 separation from cell-relative positions and the stencil offsets.
+template <int DIM>
+__device__ __forceinline__ void rel_diff(const DevParams& P, const Home& h, const float4& q,
+                                         const Offsets& o, float& d0, float& d1, float& d2) {
+  d0 = h.a.x - (q.x + (float)o.ox * P.csf[0]);
+  d1 = h.a.y - (q.y + (float)o.oy * P.csf[1]);
+  d2 = DIM == 3 ? h.a.z - (q.z + (float)o.oz * P.csf[2]) : 0.f;
+}
+
+// The chunk of the sort grid this wavefront works on; false when the wave
+// has none (the launch covers an upper bound of chunks).
+struct Chunk {
+  int row;              // cy + nm1 * cz
+  int p0, p1;           // particles [p0, p1)
+  int xmin, xmax;       // x range of the home cells
+};
+
+__device__ __forceinline__ bool load_chunk(const DevParams& P, const DeviceState& S, Chunk& ch) {
+  const int nrows = P.nm[1] * P.nm[2];
+  const int c = blockIdx.x;
+  if (c >= S.row_chunk_off[nrows]) return false;
+  ch.row = S.chunk_row[c];
+  const int k = c - S.row_chunk_off[ch.row];
+  const int rs = S.start[ch.row * P.nm[0]];
+  const int re = S.start[(ch.row + 1) * P.nm[0]];
+  ch.p0 = rs + kWave * k;
+  ch.p1 = min(ch.p0 + kWave, re);
+  ch.xmin = S.key_sorted[ch.p0] % P.nm[0];
+  ch.xmax = S.key_sorted[ch.p1 - 1] % P.nm[0];
+  return true;
+}
+
+template <int DIM>
+__device__ __forceinline__ Home load_home(const DevParams& P, const DeviceState& S,
+                                          const Chunk& ch) {
+  Home h;
+  const int lane = threadIdx.x;
+  h.i = ch.p0 + lane;
+  h.active = h.i < ch.p1;
+  h.ii = h.active ? h.i : ch.p0;
+  const int64_t n = P.n;
+  h.cx = S.key_sorted[h.ii] % P.nm[0];
+  h.cy = ch.row % P.nm[1];
+  h.cz = ch.row / P.nm[1];
+  h.a = S.xa[h.ii];
+  h.x0 = S.x[h.ii];
+  h.x1 = S.x[n + h.ii];
+  h.x2 = DIM == 3 ? S.x[2 * n + h.ii] : 0.0;
+  return h;
 }
 
 // ---------------------------------------------------------------------
-// cut-off decisions
+// the staged scan
 // ---------------------------------------------------------------------
-template <int DIM>
-struct DecideF64 {
-  const DevParams& P;
-  const double* __restrict__ x;
-  double xi0, xi1, xi2;
-  __device__ __forceinline__ bool operator()(int j, const CellInfo& ci) const {
-    double d0, d1, d2;
-    return exact_d2<DIM>(P, x, j, xi0, xi1, xi2, ci, d0, d1, d2) < P.cut2;
-  }
-};
-
-template <int DIM>
-struct DecideMixed {
-  const DevParams& P;
-  const double* __restrict__ x;
-  const float4* __restrict__ xr;
-  double xi0, xi1, xi2;
-  float a0, a1, a2;  // own position relative to the own cell corner
-  int rechecks;
-  __device__ __forceinline__ float d2f(int j, const CellInfo& ci) const {
-    float4 q = xr[j];
-    float d0 = a0 - (q.x + ci.o0);
-    float d1 = a1 - (q.y + ci.o1);
-    float s = d0 * d0 + d1 * d1;
-    if (DIM == 3) {
-      float d2 = a2 - (q.z + ci.o2);
-      s = s + d2 * d2;
+// One stencil row of a chunk is three contiguous pieces of the target row,
+// in virtual-cell order: [low wrap | main | high wrap].  The low wrap holds
+// the cells left of x = 0 seen through the periodic face (virtual cell =
+// physical - nm0), the high wrap those right of x = nm0 - 1 (virtual =
+// physical + nm0).  Returns the number of accepted neighbours.
+template <int DIM, class Pol>
+__device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState& S,
+                                           const Chunk& ch, const Home& h, Pol& pol,
+                                           float4* __restrict__ tile, uint32_t* __restrict__ buf,
+                                           int& rechecks, uint32_t* __restrict__ list) {
+  const int lane = threadIdx.x;
+  const int nm0 = P.nm[0];
+  const int32_t* __restrict__ start = S.start;
+  const int low_mode = P.symmetry ? 2 : 3;
+  const int64_t n = P.n;
+  int cnt = 0, stored = 0;
+  auto flush = [&]() {
+    for (int q = 0; __ballot(q < cnt) != 0ull; ++q) {
+      if (q < cnt) {
+        const uint32_t ent = buf[q * kWave + lane];
+        pol.body((int)(ent & kIndexMask), decode_code(ent >> kCodeShift));
+        if (list && stored + q < P.list_cap) list[(int64_t)(stored + q) * n + h.i] = ent;
+      }
     }
-    return s;
+    stored += cnt;
+    cnt = 0;
+  };
+  const int v_lo = ch.xmin - 2, v_hi = ch.xmax + 2;
+  const int nrow = DIM == 3 ? 25 : 5;
+#pragma unroll 1
+  for (int row = 0; row < nrow; ++row) {
+    const int dy = row % 5 - 2, dz = DIM == 3 ? row / 5 - 2 : 0;
+    int my, mz = 0;
+    const int ny = wrap_cell(h.cy + dy, P.nm[1], P.symmetry, my);
+    const int nz = DIM == 3 ? wrap_cell(h.cz + dz, P.nm[2], P.symmetry, mz) : 0;
+    const int rb = nm0 * (ny + P.nm[1] * nz);
+    // the three pieces (uniform across the wave)
+    const int ml = v_lo < 0 ? 0 : v_lo, mh = v_hi > nm0 - 1 ? nm0 - 1 : v_hi;
+    const int g1 = start[rb + ml];
+    const int len1 = start[rb + mh + 1] - g1;
+    const int g0 = v_lo < 0 ? start[rb + v_lo + nm0] : start[rb + nm0];
+    const int len0 = v_lo < 0 ? start[rb + nm0] - g0 : 0;
+    const int g2 = start[rb];
+    const int len2 = v_hi > nm0 - 1 ? start[rb + v_hi - nm0 + 1] - g2 : 0;
+    const int tot = len0 + len1 + len2;
+    const bool staged = tot <= kTile;
+    if (staged) {
+      for (int t = lane; t < tot; t += kWave) {
+        const int g = t < len0 ? g0 + t : (t < len0 + len1 ? g1 + (t - len0) : g2 + (t - len0 - len1));
+        tile[t] = S.xa[g];
+      }
+    }
+    __syncthreads();
+    // this lane's window: virtual cells cx-2 .. cx+2 as a tile range
+    int tb = 0, te = 0;
+    if (h.active) {
+      auto tile_of = [&](int v) -> int {  // first tile index of virtual cell v
+        if (v < 0) return start[rb + v + nm0] - g0;
+        if (v > nm0 - 1) return len0 + len1 + (start[rb + v - nm0] - g2);
+        return len0 + (start[rb + v] - g1);
+      };
+      tb = tile_of(h.cx - 2);
+      te = tile_of(h.cx + 3);
+    }
+    const float ay = h.a.y - (float)dy * P.csf[1];
+    const float az = h.a.z - (float)dz * P.csf[2];
+    for (int t = tb; __ballot(t < te) != 0ull; t += 2) {
+      if (__ballot(cnt > kBuf - 2) != 0ull) flush();
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int tt = t + u;
+        const bool live = tt < te;
+        const int piece = tt < len0 ? 0 : (tt < len0 + len1 ? 1 : 2);
+        const int g = piece == 0 ? g0 + tt : (piece == 1 ? g1 + (tt - len0) : g2 + (tt - len0 - len1));
+        float4 q;
+        if (staged)
+          q = tile[live ? tt : 0];
+        else
+          q = S.xa[live ? g : h.ii];
+        const int v = __float_as_int(q.w) + (piece == 0 ? -nm0 : (piece == 2 ? nm0 : 0));
+        const int ox = v - h.cx;
+        const bool ok = live && g != h.i;
+        const float d0 = h.a.x - (q.x + (float)ox * P.csf[0]);
+        const float d1 = ay - q.y;
+        float s2 = d0 * d0 + d1 * d1;
+        if (DIM == 3) {
+          const float d2 = az - q.z;
+          s2 = s2 + d2 * d2;
+        }
+        bool in = s2 < P.cut2_lo_f32;
+        if (ok && !in && s2 < P.cut2_hi_f32) {
+          // FP64 re-decision with the reference's image convention
+          ++rechecks;
+          Modes m;
+          m.m0 = piece == 0 ? low_mode : (piece == 2 ? 1 : 0);
+          m.m1 = my;
+          m.m2 = mz;
+          double e0, e1, e2;
+          in = exact_d2<DIM>(P, S.x, g, h, m, e0, e1, e2) < P.cut2;
+        }
+        if (ok && in) {
+          buf[cnt * kWave + lane] = (uint32_t)g | (make_code(ox, dy, dz) << kCodeShift);
+          ++cnt;
+        }
+      }
+    }
+    __syncthreads();
   }
-  __device__ __forceinline__ bool operator()(int j, const CellInfo& ci) {
-    float s = d2f(j, ci);
-    if (s < P.cut2_lo_f32) return true;
-    if (s >= P.cut2_hi_f32) return false;
-    ++rechecks;
-    double d0, d1, d2;
-    return exact_d2<DIM>(P, x, j, xi0, xi1, xi2, ci, d0, d1, d2) < P.cut2;
-  }
-};
-
-// ---------------------------------------------------------------------
-// K2: density + EOS
-// ---------------------------------------------------------------------
-template <int DIM>
-struct DensityF64 {
-  const DevParams& P;
-  DecideF64<DIM> dec;
-  const double* __restrict__ m;
-  double acc;
-  int cnt;
-  __device__ __forceinline__ bool test(int j, const CellInfo& ci) { return dec(j, ci); }
-  __device__ __forceinline__ void body(int j, const CellInfo& ci) {
-    double d0, d1, d2;
-    double r2 = exact_d2<DIM>(P, dec.x, j, dec.xi0, dec.xi1, dec.xi2, ci, d0, d1, d2);
-    double w, gw;
-    kernel_w_gradw(P, sqrt(r2), w, gw);
-    acc = acc + (P.density_type == 1 ? m[j] * w : w);
-    ++cnt;
-  }
-};
+  flush();
+  return stored;
+}
 
 // compensated FP32 accumulator (Kahan)
 struct KahanF {
@@ -218,78 +272,113 @@ struct KahanF {
   __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
+// ---------------------------------------------------------------------
+// K2: density + EOS
+// ---------------------------------------------------------------------
 template <int DIM>
-struct DensityMixed {
+struct DensityF64 {
   const DevParams& P;
-  DecideMixed<DIM> dec;
-  const double* __restrict__ m;
-  KahanF acc;
-  int cnt;
-  __device__ __forceinline__ bool test(int j, const CellInfo& ci) { return dec(j, ci); }
-  __device__ __forceinline__ void body(int j, const CellInfo& ci) {
-    float r = __builtin_sqrtf(dec.d2f(j, ci));
-    float w, gw;
-    kernel_w_gradw_f32(P.kernel_type, r * P.inv_hf, P.coeff, P.coef_gradf, w, gw);
-    acc.add(P.density_type == 1 ? (float)m[j] * w : w);
-    ++cnt;
+  const DeviceState& S;
+  const Home& h;
+  double acc;
+  __device__ __forceinline__ void body(int j, const Offsets& o) {
+    const Modes m = modes_of<DIM>(P, h, o);
+    double d0, d1, d2;
+    const double r2 = exact_d2<DIM>(P, S.x, j, h, m, d0, d1, d2);
+    double w, gw;
+    kernel_w_gradw(P, sqrt(r2), w, gw);
+    acc = acc + (P.density_type == 1 ? S.m[j] * w : w);
   }
 };
 
-template <int DIM, bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_density(DevParams P, DeviceState S) {
-  __shared__ uint32_t sbuf[kWaves][kBuf * kWave];
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  const bool active = i < P.n;
-  const int ii = active ? i : 0;
+template <int DIM>
+struct DensityMixed {
+  const DevParams& P;
+  const DeviceState& S;
+  const Home& h;
+  KahanF acc;
+  __device__ __forceinline__ void body(int j, const Offsets& o) {
+    float d0, d1, d2;
+    rel_diff<DIM>(P, h, S.xa[j], o, d0, d1, d2);
+    const float r = __builtin_sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+    float w, gw;
+    kernel_w_gradw_f32(P.kernel_type, r * P.inv_hf, P.coeff, P.coef_gradf, w, gw);
+    acc.add(P.density_type == 1 ? (float)S.m[j] * w : w);
+  }
+};
+
+// Records of the force pass, written once the density is known: FP64
+// (x, y, z, p/n^2, vx, vy, vz, 1/n^2) or FP32 (v, pid) and (p/n^2, 1/n^2, p).
+template <int DIM>
+__device__ __forceinline__ void write_force_records(const DevParams& P, const DeviceState& S, int i,
+                                                    double an, double bn, double press) {
   const int64_t n = P.n;
-  double xi0 = S.x[ii], xi1 = S.x[n + ii], xi2 = DIM == 3 ? S.x[2 * n + ii] : 0.0;
-  const double mi = S.m[ii];
-  CellCoord c = decode_key(P, S.key_sorted[ii]);
-  uint32_t* buf = sbuf[threadIdx.x / kWave];
+  if (P.mixed) {
+    S.vr[i] = make_float4((float)S.v[i], (float)S.v[n + i], DIM == 3 ? (float)S.v[2 * n + i] : 0.f,
+                          __uint_as_float((uint32_t)S.pid[i]));
+    S.qr[i] = make_float4((float)an, (float)bn, (float)press, 0.f);
+  } else {
+    double4* r = reinterpret_cast<double4*>(S.rec64 + 8 * (int64_t)i);
+    r[0] = make_double4(S.x[i], S.x[n + i], DIM == 3 ? S.x[2 * n + i] : 0.0, an);
+    r[1] = make_double4(S.v[i], S.v[n + i], DIM == 3 ? S.v[2 * n + i] : 0.0, bn);
+  }
+}
+
+template <int DIM, bool MIXED>
+__global__ __launch_bounds__(kWave) void k_density(DevParams P, DeviceState S) {
+  __shared__ float4 tile[kTile];
+  __shared__ uint32_t buf[kBuf * kWave];
+  Chunk ch;
+  if (!load_chunk(P, S, ch)) return;
+  const Home h = load_home<DIM>(P, S, ch);
+  const double mi = S.m[h.ii];
   double rho;
-  int cnt, rechecks = 0;
+  int rechecks = 0, cnt;
   if (MIXED) {
-    float4 a = S.xr[ii];
-    DensityMixed<DIM> pol{P, DecideMixed<DIM>{P, S.x, S.xr, xi0, xi1, xi2, a.x, a.y, a.z, 0}, S.m,
-                          KahanF{}, 0};
+    DensityMixed<DIM> pol{P, S, h, KahanF{}};
     pol.acc.add(P.density_type == 1 ? (float)(P.w0 * mi) : (float)P.w0);
-    gather<DIM>(P, S.start, i, active, c, pol, buf);
+    cnt = staged_scan<DIM>(P, S, ch, h, pol, tile, buf, rechecks, S.nlist);
     rho = pol.acc.value();
-    cnt = pol.cnt;
-    rechecks = pol.dec.rechecks;
   } else {
     // self term first (particles_compute_density.F90:369-558)
-    DensityF64<DIM> pol{P, DecideF64<DIM>{P, S.x, xi0, xi1, xi2}, S.m,
-                        P.density_type == 1 ? P.w0 * mi : P.w0, 0};
-    gather<DIM>(P, S.start, i, active, c, pol, buf);
+    DensityF64<DIM> pol{P, S, h, P.density_type == 1 ? P.w0 * mi : P.w0};
+    cnt = staged_scan<DIM>(P, S, ch, h, pol, tile, buf, rechecks, S.nlist);
     rho = pol.acc;
-    cnt = pol.cnt;
   }
-  if (!active) return;
+  if (!h.active) return;
+  const int i = h.i;
   const double press = eos_pressure(P, P.density_type == 2 ? rho * mi : rho);
   S.rho[i] = rho;
   S.p[i] = press;
   // per-particle factors of the pair forces: p/n^2 and 1/n^2
-  const double inv_n2 = 1.0 / (rho * rho);
-  S.an[i] = press / (rho * rho);
-  S.bn[i] = inv_n2;
+  const double an = press / (rho * rho);
+  const double bn = 1.0 / (rho * rho);
+  S.an[i] = an;
+  S.bn[i] = bn;
   S.nbr_count[i] = cnt;
   S.recheck[i] = rechecks;
-  if (MIXED) {
-    S.vr[i] = make_float4((float)S.v[i], (float)S.v[n + i], DIM == 3 ? (float)S.v[2 * n + i] : 0.f,
-                          (float)(press / (rho * rho)));
-    S.pb[i] = make_float2((float)inv_n2, (float)press);
-  }
+  write_force_records<DIM>(P, S, i, an, bn, press);
 }
 
 __global__ void k_eos(DevParams P, DeviceState S) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
-  double r = S.rho[i];
-  double press = eos_pressure(P, P.density_type == 2 ? r * S.m[i] : r);
+  const double r = S.rho[i];
+  const double press = eos_pressure(P, P.density_type == 2 ? r * S.m[i] : r);
   S.p[i] = press;
   S.an[i] = press / (r * r);
   S.bn[i] = 1.0 / (r * r);
+}
+
+// Force-pass records after the velocities or the pressure changed between
+// the density and the force pass (separate kick / EOS calls).
+__global__ void k_refresh_records(DevParams P, DeviceState S) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  if (P.dim == 3)
+    write_force_records<3>(P, S, i, S.an[i], S.bn[i], S.p[i]);
+  else
+    write_force_records<2>(P, S, i, S.an[i], S.bn[i], S.p[i]);
 }
 
 // ---------------------------------------------------------------------
@@ -325,29 +414,31 @@ __device__ __forceinline__ void noise_vector(const DevParams& P, uint32_t pidi, 
 template <int DIM, int FT>
 struct ForceF64 {
   const DevParams& P;
-  DecideF64<DIM> dec;
   const DeviceState& S;
+  const Home& h;
   double vi0, vi1, vi2, ai, bi, pi, inv_mi, ni;
   uint32_t pidi;
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
   double L[9];
   double pti[9];
-  int cnt = 0;
-  __device__ __forceinline__ bool test(int j, const CellInfo& ci) { return dec(j, ci); }
-  __device__ __forceinline__ void body(int j, const CellInfo& ci) {
+  __device__ __forceinline__ void body(int j, const Offsets& o) {
     const int64_t n = P.n;
-    double d0, d1, d2;
-    double r2 = exact_d2<DIM>(P, dec.x, j, dec.xi0, dec.xi1, dec.xi2, ci, d0, d1, d2);
-    double r = sqrt(r2);
+    const Modes m = modes_of<DIM>(P, h, o);
+    const double4* rj = reinterpret_cast<const double4*>(S.rec64 + 8 * (int64_t)j);
+    const double4 ra = rj[0], rb = rj[1];
+    const double d0 = image_diff(h.x0, ra.x, P.length[0], m.m0);
+    const double d1 = image_diff(h.x1, ra.y, P.length[1], m.m1);
+    const double d2 = DIM == 3 ? image_diff(h.x2, ra.z, P.length[2], m.m2) : 0.0;
+    const double r2 = DIM == 3 ? dist2_exact(d0, d1, d2) : dist2_exact2(d0, d1);
+    const double r = sqrt(r2);
     double w, gw;
     kernel_w_gradw(P, r, w, gw);
     const double ir = 1.0 / r;
-    double e0 = d0 * ir, e1 = d1 * ir, e2 = DIM == 3 ? d2 * ir : 0.0;
-    double vj0 = S.v[j], vj1 = S.v[n + j], vj2 = DIM == 3 ? S.v[2 * n + j] : 0.0;
-    double u0 = vi0 - vj0, u1 = vi1 - vj1, u2 = vi2 - vj2;
-    double bj = S.bn[j];
+    const double e0 = d0 * ir, e1 = d1 * ir, e2 = DIM == 3 ? d2 * ir : 0.0;
+    const double vj0 = rb.x, vj1 = rb.y, vj2 = rb.z;
+    const double u0 = vi0 - vj0, u1 = vi1 - vj1, u2 = vi2 - vj2;
+    const double bj = rb.w;
     double fi0, fi1, fi2;
-    ++cnt;
     if (FT == 5) {
       // pp_vgt_ip.inc:16-20
       const double du[3] = {vj0 - vi0, vj1 - vi1, vj2 - vi2};
@@ -387,7 +478,7 @@ struct ForceF64 {
       }
     } else if (FT == 3) {
       // rhs_force_ff_Newtonian_HuAdams.F90:140-237
-      const double f_c = (ai + S.an[j]) * gw;
+      const double f_c = (ai + ra.w) * gw;
       fi0 = 0.0 - f_c * e0 * inv_mi;
       fi1 = 0.0 - f_c * e1 * inv_mi;
       fi2 = DIM == 3 ? 0.0 - f_c * e2 * inv_mi : 0.0;
@@ -437,38 +528,34 @@ struct ForceF64 {
 template <int DIM, int FT>
 struct ForceMixed {
   const DevParams& P;
-  DecideMixed<DIM> dec;
   const DeviceState& S;
+  const Home& h;
   float vi0, vi1, vi2, ai, bi, pi;
   uint32_t pidi;
   KahanF f0, f1, f2;
-  int cnt = 0;
-  __device__ __forceinline__ bool test(int j, const CellInfo& ci) { return dec(j, ci); }
-  __device__ __forceinline__ void body(int j, const CellInfo& ci) {
-    float4 q = S.xr[j];
-    float d0 = dec.a0 - (q.x + ci.o0);
-    float d1 = dec.a1 - (q.y + ci.o1);
-    float d2 = DIM == 3 ? dec.a2 - (q.z + ci.o2) : 0.f;
-    float r2 = d0 * d0 + d1 * d1 + d2 * d2;
-    float r = __builtin_sqrtf(r2);
-    float ir = __builtin_amdgcn_rcpf(r);
+  __device__ __forceinline__ void body(int j, const Offsets& o) {
+    float d0, d1, d2;
+    rel_diff<DIM>(P, h, S.xa[j], o, d0, d1, d2);
+    const float4 vj = S.vr[j];
+    const float4 qj = S.qr[j];
+    const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    const float r = __builtin_sqrtf(r2);
+    const float ir = __builtin_amdgcn_rcpf(r);
     float w, gw;
     kernel_w_gradw_f32(P.kernel_type, r * P.inv_hf, P.coeff, P.coef_gradf, w, gw);
-    float e0 = d0 * ir, e1 = d1 * ir, e2 = d2 * ir;
-    float4 vj = S.vr[j];
-    float2 pbj = S.pb[j];
-    float u0 = vi0 - vj.x, u1 = vi1 - vj.y, u2 = vi2 - vj.z;
+    const float e0 = d0 * ir, e1 = d1 * ir, e2 = d2 * ir;
+    const float u0 = vi0 - vj.x, u1 = vi1 - vj.y, u2 = vi2 - vj.z;
+    const uint32_t pidj = __float_as_uint(vj.w);
     float fi0, fi1, fi2;
-    ++cnt;
     if (FT == 3) {
-      const float f_c = (ai + vj.w) * gw;
-      const float f_d = P.etaf * (bi + pbj.x) * gw * ir;
+      const float f_c = (ai + qj.x) * gw;
+      const float f_d = P.etaf * (bi + qj.y) * gw * ir;
       fi0 = f_d * u0 - f_c * e0;
       fi1 = f_d * u1 - f_c * e1;
       fi2 = f_d * u2 - f_c * e2;
       if (P.noise) {
         float w0, w1, w2;
-        noise_vector<DIM, float>(P, pidi, __float_as_uint(q.w), e0, e1, e2, w0, w1, w2);
+        noise_vector<DIM, float>(P, pidi, pidj, e0, e1, e2, w0, w1, w2);
         const float f_r = __builtin_sqrtf(-4.0f * P.ktf * f_d) * P.inv_sqrt_dtf;
         fi0 = fi0 + f_r * w0;
         fi1 = fi1 + f_r * w1;
@@ -476,15 +563,14 @@ struct ForceMixed {
       }
     } else {
       const float ev = e0 * u0 + e1 * u1 + e2 * u2;
-      const float pij = (pi + pbj.y) * 0.5f;
-      const float aij = (bi + pbj.x) * gw;
+      const float pij = (pi + qj.z) * 0.5f;
+      const float aij = (bi + qj.y) * gw;
       const float f_c = aij * pij;
-      float f_d = P.eta_coeff * P.etaf * aij * ev * ir;
+      const float f_d = P.eta_coeff * P.etaf * aij * ev * ir;
       float s = f_d - f_c;
       if (P.noise) {
-        const uint32_t pj_ = __float_as_uint(q.w);
         float g[4];
-        pair_normals(P.seed, pidi < pj_ ? pidi : pj_, pidi < pj_ ? pj_ : pidi, P.step, g);
+        pair_normals(P.seed, pidi < pidj ? pidi : pidj, pidi < pidj ? pidj : pidi, P.step, g);
         s = s + g[0] * P.sqrt_2kt_etaf * __builtin_sqrtf(-P.eta_coeff * aij * ir) * P.inv_sqrt_dtf;
       }
       fi0 = s * e0;
@@ -512,31 +598,48 @@ __device__ __forceinline__ void finish_force(const DevParams& P, const DeviceSta
   if (DIM == 3) S.f[2 * n + i] = f2;
 }
 
+// Walk the in-cutoff neighbours stored by the density pass of the same
+// step; a wave with a particle whose list overflowed scans again.
+template <int DIM, class Pol>
+__device__ __forceinline__ void for_each_listed(const DevParams& P, const DeviceState& S,
+                                                const Chunk& ch, const Home& h, Pol& pol,
+                                                float4* __restrict__ tile,
+                                                uint32_t* __restrict__ buf) {
+  const int cnt = h.active ? S.nbr_count[h.i] : 0;
+  if (__ballot(cnt > P.list_cap) == 0ull) {
+    const int64_t n = P.n;
+    for (int q = 0; __ballot(q < cnt) != 0ull; ++q) {
+      if (q < cnt) {
+        const uint32_t ent = S.nlist[(int64_t)q * n + h.i];
+        pol.body((int)(ent & kIndexMask), decode_code(ent >> kCodeShift));
+      }
+    }
+  } else {
+    int rechecks = 0;
+    (void)staged_scan<DIM>(P, S, ch, h, pol, tile, buf, rechecks, nullptr);
+  }
+}
+
 template <int DIM, int FT, bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_forces(DevParams P, DeviceState S) {
-  __shared__ uint32_t sbuf[kWaves][kBuf * kWave];
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  const bool active = i < P.n;
-  const int ii = active ? i : 0;
+__global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
+  __shared__ float4 tile[kTile];
+  __shared__ uint32_t buf[kBuf * kWave];
+  Chunk ch;
+  if (!load_chunk(P, S, ch)) return;
+  const Home h = load_home<DIM>(P, S, ch);
   const int64_t n = P.n;
-  double xi0 = S.x[ii], xi1 = S.x[n + ii], xi2 = DIM == 3 ? S.x[2 * n + ii] : 0.0;
-  CellCoord c = decode_key(P, S.key_sorted[ii]);
-  uint32_t* buf = sbuf[threadIdx.x / kWave];
+  const int ii = h.ii;
   const double inv_mi = 1.0 / S.m[ii];
   if (MIXED) {
-    float4 a = S.xr[ii];
-    float4 vi = S.vr[ii];
-    float2 pbi = S.pb[ii];
-    ForceMixed<DIM, FT> pol{P, DecideMixed<DIM>{P, S.x, S.xr, xi0, xi1, xi2, a.x, a.y, a.z, 0}, S,
-                            vi.x, vi.y, vi.z, vi.w, pbi.x, pbi.y, __float_as_uint(a.w)};
-    gather<DIM>(P, S.start, i, active, c, pol, buf);
-    if (!active) return;
-    finish_force<DIM>(P, S, i, pol.f0.value() * inv_mi, pol.f1.value() * inv_mi,
+    const float4 vi = S.vr[ii];
+    const float4 qi = S.qr[ii];
+    ForceMixed<DIM, FT> pol{P, S, h, vi.x, vi.y, vi.z, qi.x, qi.y, qi.z, __float_as_uint(vi.w)};
+    for_each_listed<DIM>(P, S, ch, h, pol, tile, buf);
+    if (!h.active) return;
+    finish_force<DIM>(P, S, h.i, pol.f0.value() * inv_mi, pol.f1.value() * inv_mi,
                       DIM == 3 ? pol.f2.value() * inv_mi : 0.0);
-    S.nbr_count[i] = pol.cnt;
-    S.recheck[i] = pol.dec.rechecks;
   } else {
-    ForceF64<DIM, FT> pol{P, DecideF64<DIM>{P, S.x, xi0, xi1, xi2}, S};
+    ForceF64<DIM, FT> pol{P, S, h};
     pol.vi0 = S.v[ii];
     pol.vi1 = S.v[n + ii];
     pol.vi2 = DIM == 3 ? S.v[2 * n + ii] : 0.0;
@@ -559,11 +662,10 @@ __global__ __launch_bounds__(kBlock) void k_forces(DevParams P, DeviceState S) {
           pol.pti[a + DIM * b] = val;
         }
     }
-    gather<DIM>(P, S.start, i, active, c, pol, buf);
-    if (!active) return;
+    for_each_listed<DIM>(P, S, ch, h, pol, tile, buf);
+    if (!h.active) return;
+    const int i = h.i;
     finish_force<DIM>(P, S, i, pol.f0, pol.f1, pol.f2);
-    S.nbr_count[i] = pol.cnt;
-    S.recheck[i] = 0;
     if (FT == 5) {
       // particles_compute_act.F90:140-155 on this particle's own ct and vgt
       const int d2n = DIM * DIM;
@@ -600,61 +702,41 @@ __device__ __forceinline__ uint32_t pair_hash(uint32_t a, uint32_t b) {
   return h;
 }
 
-template <class Dec>
 struct SignaturePol {
-  Dec dec;
   const int32_t* __restrict__ pid;
   uint32_t pidi;
-  int cnt;
   uint32_t hs;
-  __device__ __forceinline__ bool test(int j, const CellInfo& ci) { return dec(j, ci); }
-  __device__ __forceinline__ void body(int j, const CellInfo&) {
-    ++cnt;
+  __device__ __forceinline__ void body(int j, const Offsets&) {
     hs += pair_hash(pidi, (uint32_t)pid[j]);
   }
 };
 
-template <int DIM, bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_signature(DevParams P, DeviceState S) {
-  __shared__ uint32_t sbuf[kWaves][kBuf * kWave];
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  const bool active = i < P.n;
-  const int ii = active ? i : 0;
-  const int64_t n = P.n;
-  double xi0 = S.x[ii], xi1 = S.x[n + ii], xi2 = DIM == 3 ? S.x[2 * n + ii] : 0.0;
-  CellCoord c = decode_key(P, S.key_sorted[ii]);
-  uint32_t* buf = sbuf[threadIdx.x / kWave];
-  const uint32_t pidi = (uint32_t)S.pid[ii];
-  int cnt;
-  uint32_t hs;
+template <int DIM>
+__global__ __launch_bounds__(kWave) void k_signature(DevParams P, DeviceState S) {
+  __shared__ float4 tile[kTile];
+  __shared__ uint32_t buf[kBuf * kWave];
+  Chunk ch;
+  if (!load_chunk(P, S, ch)) return;
+  const Home h = load_home<DIM>(P, S, ch);
+  SignaturePol pol{S.pid, (uint32_t)S.pid[h.ii], 0u};
   int rechecks = 0;
-  if (MIXED) {
-    float4 a = S.xr[ii];
-    SignaturePol<DecideMixed<DIM>> pol{
-        DecideMixed<DIM>{P, S.x, S.xr, xi0, xi1, xi2, a.x, a.y, a.z, 0}, S.pid, pidi, 0, 0u};
-    gather<DIM>(P, S.start, i, active, c, pol, buf);
-    cnt = pol.cnt;
-    hs = pol.hs;
-    rechecks = pol.dec.rechecks;
-  } else {
-    SignaturePol<DecideF64<DIM>> pol{DecideF64<DIM>{P, S.x, xi0, xi1, xi2}, S.pid, pidi, 0, 0u};
-    gather<DIM>(P, S.start, i, active, c, pol, buf);
-    cnt = pol.cnt;
-    hs = pol.hs;
-  }
-  if (!active) return;
-  S.nbr_count[i] = cnt;
-  S.nbr_hash[i] = hs;
-  S.recheck[i] = rechecks;
+  const int cnt = staged_scan<DIM>(P, S, ch, h, pol, tile, buf, rechecks, nullptr);
+  if (!h.active) return;
+  S.nbr_hash[h.i] = pol.hs;
+  S.nbr_sig_count[h.i] = cnt;
+  S.recheck[h.i] = rechecks;
 }
 
 // ---------------------------------------------------------------------
-// launchers
+// launchers: one 64-lane workgroup per chunk; the grid covers the upper
+// bound of chunks (one partial chunk per row plus n / 64 full ones)
 // ---------------------------------------------------------------------
-static inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
+static inline int nchunks_max(const DevParams& P) {
+  return (P.n + kWave - 1) / kWave + P.nm[1] * P.nm[2];
+}
 
 void launch_density(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed) {
-  dim3 g(nblk(P.n)), b(kBlock);
+  dim3 g(nchunks_max(P)), b(kWave);
   if (P.dim == 3) {
     if (mixed)
       hipLaunchKernelGGL((k_density<3, true>), g, b, 0, s, P, S);
@@ -672,10 +754,14 @@ void launch_eos(hipStream_t s, const DevParams& P, const DeviceState& S) {
   hipLaunchKernelGGL(k_eos, dim3((P.n + 255) / 256), dim3(256), 0, s, P, S);
 }
 
+void launch_refresh_records(hipStream_t s, const DevParams& P, const DeviceState& S) {
+  hipLaunchKernelGGL(k_refresh_records, dim3((P.n + 255) / 256), dim3(256), 0, s, P, S);
+}
+
 template <int DIM>
 static void launch_forces_dim(hipStream_t s, const DevParams& P, const DeviceState& S,
                               bool mixed) {
-  dim3 g(nblk(P.n)), b(kBlock);
+  dim3 g(nchunks_max(P)), b(kWave);
   const int ft = P.newtonian ? P.force_type : 5;
   if (mixed) {
     if (ft == 3)
@@ -700,18 +786,12 @@ void launch_forces(hipStream_t s, const DevParams& P, const DeviceState& S, bool
 }
 
 void launch_signature(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed) {
-  dim3 g(nblk(P.n)), b(kBlock);
-  if (P.dim == 3) {
-    if (mixed)
-      hipLaunchKernelGGL((k_signature<3, true>), g, b, 0, s, P, S);
-    else
-      hipLaunchKernelGGL((k_signature<3, false>), g, b, 0, s, P, S);
-  } else {
-    if (mixed)
-      hipLaunchKernelGGL((k_signature<2, true>), g, b, 0, s, P, S);
-    else
-      hipLaunchKernelGGL((k_signature<2, false>), g, b, 0, s, P, S);
-  }
+  (void)mixed;  // both modes share the decision
+  dim3 g(nchunks_max(P)), b(kWave);
+  if (P.dim == 3)
+    hipLaunchKernelGGL(k_signature<3>, g, b, 0, s, P, S);
+  else
+    hipLaunchKernelGGL(k_signature<2>, g, b, 0, s, P, S);
 }
 
 }  // namespace sdpd

@@ -152,26 +152,8 @@ __global__ void k_pair_normals(const uint32_t* __restrict__ lo, const uint32_t* 
   for (int q = 0; q < 4; ++q) g[4 * (int64_t)i + q] = r[q];
 }
 
-// FP32 velocity / pressure records of the mixed path, refreshed when the
-// velocities changed after the density pass (a separate kick call).
-__global__ void k_refresh_vr(DevParams P, DeviceState S) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
-  const int64_t n = P.n;
-  float4 r = S.vr[i];
-  r.x = (float)S.v[i];
-  r.y = (float)S.v[n + i];
-  r.z = P.dim == 3 ? (float)S.v[2 * n + i] : 0.f;
-  r.w = (float)S.an[i];
-  S.vr[i] = r;
-  S.pb[i] = make_float2((float)S.bn[i], (float)S.p[i]);
-}
-
 static inline int nblk(int64_t n) { return (int)((n + 255) / 256); }
 
-void launch_refresh_velocity_records(hipStream_t s, const DevParams& P, const DeviceState& S) {
-  hipLaunchKernelGGL(k_refresh_vr, dim3(nblk(P.n)), dim3(256), 0, s, P, S);
-}
 
 int stats_blocks(int n) {
   int b = nblk(n);

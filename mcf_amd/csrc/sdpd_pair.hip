@@ -29,8 +29,9 @@
 //
 // The density pass appends the accepted neighbours of every particle to a
 // per-lane LDS list, evaluates the density terms whenever the lists fill
-// up, and stores the lists slot-major in global memory; the force pass of
-// the same step (positions unchanged) walks the stored lists.
+// up, and stores each particle's list as one row of list_cap entries in
+// global memory, in whole 32-byte groups; the force pass of the same step
+// (positions unchanged) walks the stored rows eight entries at a time.
 #include "sdpd_kernels.h"
 
 namespace sdpd {
@@ -162,18 +163,45 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
   const int nm0 = P.nm[0];
   const int32_t* __restrict__ start = S.start;
   const int low_mode = P.symmetry ? 2 : 3;
-  const int64_t n = P.n;
   int cnt = 0, stored = 0;
-  auto flush = [&]() {
-    for (int q = 0; __ballot(q < cnt) != 0ull; ++q) {
+  // buf holds cnt entries of this lane, the first ev of them evaluated
+  // already; the list row of particle i is list[i * list_cap + q] and is
+  // written in 32-byte groups of 8 entries so that every store fills
+  // whole sectors
+  int ev = 0;
+  uint32_t* __restrict__ row_out = list ? list + (int64_t)h.ii * P.list_cap : nullptr;
+  auto flush = [&](bool final) {
+    for (int q = ev; __ballot(q < cnt) != 0ull; ++q) {
       if (q < cnt) {
         const uint32_t ent = buf[q * kWave + lane];
         pol.body((int)(ent & kIndexMask), decode_code(ent >> kCodeShift));
-        if (list && stored + q < P.list_cap) list[(int64_t)(stored + q) * n + h.i] = ent;
       }
     }
-    stored += cnt;
-    cnt = 0;
+    const int w = final ? cnt : (cnt & ~7);
+    if (row_out) {
+      for (int q = 0; __ballot(q < w) != 0ull; q += 8) {
+        if (q < w && stored + q + 8 <= P.list_cap) {
+          uint4 lo, hi;
+          lo.x = buf[(q + 0) * kWave + lane];
+          lo.y = buf[(q + 1) * kWave + lane];
+          lo.z = buf[(q + 2) * kWave + lane];
+          lo.w = buf[(q + 3) * kWave + lane];
+          hi.x = buf[(q + 4) * kWave + lane];
+          hi.y = buf[(q + 5) * kWave + lane];
+          hi.z = buf[(q + 6) * kWave + lane];
+          hi.w = buf[(q + 7) * kWave + lane];
+          uint4* dst = reinterpret_cast<uint4*>(row_out + stored + q);
+          dst[0] = lo;
+          dst[1] = hi;
+        }
+      }
+    }
+    stored += w;
+    // keep the unwritten tail (fewer than 8 entries) at the front
+    const int rem = cnt - w;
+    for (int r = 0; r < rem; ++r) buf[r * kWave + lane] = buf[(w + r) * kWave + lane];
+    cnt = rem;
+    ev = rem;
   };
   const int v_lo = ch.xmin - 2, v_hi = ch.xmax + 2;
   const int nrow = DIM == 3 ? 25 : 5;
@@ -215,7 +243,7 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
     const float ay = h.a.y - (float)dy * P.csf[1];
     const float az = h.a.z - (float)dz * P.csf[2];
     for (int t = tb; __ballot(t < te) != 0ull; t += 2) {
-      if (__ballot(cnt > kBuf - 2) != 0ull) flush();
+      if (__ballot(cnt > kBuf - 2) != 0ull) flush(false);
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const int tt = t + u;
@@ -256,7 +284,7 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
     }
     __syncthreads();
   }
-  flush();
+  flush(true);
   return stored;
 }
 
@@ -607,11 +635,22 @@ __device__ __forceinline__ void for_each_listed(const DevParams& P, const Device
                                                 uint32_t* __restrict__ buf) {
   const int cnt = h.active ? S.nbr_count[h.i] : 0;
   if (__ballot(cnt > P.list_cap) == 0ull) {
-    const int64_t n = P.n;
-    for (int q = 0; __ballot(q < cnt) != 0ull; ++q) {
+    const uint32_t* __restrict__ row = S.nlist + (int64_t)h.ii * P.list_cap;
+    for (int q = 0; __ballot(q < cnt) != 0ull; q += 8) {
       if (q < cnt) {
-        const uint32_t ent = S.nlist[(int64_t)q * n + h.i];
-        pol.body((int)(ent & kIndexMask), decode_code(ent >> kCodeShift));
+        // eight entries per pair of 16-byte loads; entries past cnt are
+        // padding inside the row and are not evaluated
+        const uint4* src = reinterpret_cast<const uint4*>(row + q);
+        const uint4 lo = src[0], hi = src[1];
+        uint32_t e[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+        const int m = min(8, cnt - q);
+#pragma unroll 1
+        for (int u = 0; u < m; ++u) {
+          const uint32_t ent = e[0];
+#pragma unroll
+          for (int k = 0; k < 7; ++k) e[k] = e[k + 1];
+          pol.body((int)(ent & kIndexMask), decode_code(ent >> kCodeShift));
+        }
       }
     }
   } else {

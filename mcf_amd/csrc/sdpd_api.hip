@@ -283,7 +283,10 @@ int alloc_state(sdpd_ctx* c, int n) {
   ALLOC(stats_out, 8);
   ALLOC(nbr_count, N);
   ALLOC(nbr_sig_count, N);
-  ALLOC(nlist, (size_t)P.list_cap * N);
+  // neighbour lists: one block of list_cap x 64 entries per chunk of the
+  // sort grid (at most N / 64 + rows chunks, rounded up to 8)
+  ALLOC(nlist, (size_t)P.list_cap * 64 *
+                   (((N + 63) / 64 + (size_t)P.nm[1] * P.nm[2] + 7) / 8 * 8));
   ALLOC(recheck, N);
   ALLOC(nbr_hash, N);
 #undef ALLOC

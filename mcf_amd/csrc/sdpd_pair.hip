@@ -29,9 +29,9 @@
 //
 // The density pass appends the accepted neighbours of every particle to a
 // per-lane LDS list, evaluates the density terms whenever the lists fill
-// up, and stores each particle's list as one row of list_cap entries in
-// global memory, in whole 32-byte groups; the force pass of the same step
-// (positions unchanged) walks the stored rows eight entries at a time.
+// up, and stores the lists of a chunk as one block of groups of 8 entries
+// per lane: every store fills whole 32-byte sectors and the force pass of
+// the same step (positions unchanged) reads 2 KB contiguous per group.
 #include "sdpd_kernels.h"
 
 namespace sdpd {
@@ -107,15 +107,24 @@ __device__ __forceinline__ void rel_diff(const DevParams& P, const Home& h, cons
 // The chunk of the sort grid this wavefront works on; false when the wave
 // has none (the launch covers an upper bound of chunks).
 struct Chunk {
+  int c;                // chunk index
   int row;              // cy + nm1 * cz
   int p0, p1;           // particles [p0, p1)
   int xmin, xmax;       // x range of the home cells
 };
 
+// Workgroups are handed to the eight XCDs round-robin; chunk c of XCD
+// b % 8 is taken from a contiguous eighth of the chunk sequence, so each
+// XCD's L2 serves a compact slab of space.  The grid is a multiple of 8
+// and at least the number of chunks.
 __device__ __forceinline__ bool load_chunk(const DevParams& P, const DeviceState& S, Chunk& ch) {
   const int nrows = P.nm[1] * P.nm[2];
-  const int c = blockIdx.x;
-  if (c >= S.row_chunk_off[nrows]) return false;
+  const int total = S.row_chunk_off[nrows];
+  const int per = (total + 7) >> 3;
+  const int b = blockIdx.x;
+  const int c = (b & 7) * per + (b >> 3);
+  if ((b >> 3) >= per || c >= total) return false;
+  ch.c = c;
   ch.row = S.chunk_row[c];
   const int k = c - S.row_chunk_off[ch.row];
   const int rs = S.start[ch.row * P.nm[0]];
@@ -125,6 +134,15 @@ __device__ __forceinline__ bool load_chunk(const DevParams& P, const DeviceState
   ch.xmin = S.key_sorted[ch.p0] % P.nm[0];
   ch.xmax = S.key_sorted[ch.p1 - 1] % P.nm[0];
   return true;
+}
+
+// Neighbour list block of a chunk: list_cap / 8 groups, each 64 lanes x 8
+// entries; a lane's group g holds its entries 8g .. 8g+7.
+__device__ __forceinline__ uint32_t* list_block(const DevParams& P, uint32_t* list, const Chunk& ch) {
+  return list + (int64_t)ch.c * P.list_cap * kWave;
+}
+__device__ __forceinline__ int64_t group_offset(int q, int lane) {
+  return ((int64_t)(q >> 3) * kWave + lane) * 8;
 }
 
 template <int DIM>
@@ -165,11 +183,9 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
   const int low_mode = P.symmetry ? 2 : 3;
   int cnt = 0, stored = 0;
   // buf holds cnt entries of this lane, the first ev of them evaluated
-  // already; the list row of particle i is list[i * list_cap + q] and is
-  // written in 32-byte groups of 8 entries so that every store fills
-  // whole sectors
+  // already; entries reach the list block in whole groups of 8
   int ev = 0;
-  uint32_t* __restrict__ row_out = list ? list + (int64_t)h.ii * P.list_cap : nullptr;
+  uint32_t* __restrict__ blk = list ? list_block(P, list, ch) : nullptr;
   auto flush = [&](bool final) {
     for (int q = ev; __ballot(q < cnt) != 0ull; ++q) {
       if (q < cnt) {
@@ -178,7 +194,7 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
       }
     }
     const int w = final ? cnt : (cnt & ~7);
-    if (row_out) {
+    if (blk) {
       for (int q = 0; __ballot(q < w) != 0ull; q += 8) {
         if (q < w && stored + q + 8 <= P.list_cap) {
           uint4 lo, hi;
@@ -190,7 +206,7 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
           hi.y = buf[(q + 5) * kWave + lane];
           hi.z = buf[(q + 6) * kWave + lane];
           hi.w = buf[(q + 7) * kWave + lane];
-          uint4* dst = reinterpret_cast<uint4*>(row_out + stored + q);
+          uint4* dst = reinterpret_cast<uint4*>(blk + group_offset(stored + q, lane));
           dst[0] = lo;
           dst[1] = hi;
         }
@@ -635,12 +651,13 @@ __device__ __forceinline__ void for_each_listed(const DevParams& P, const Device
                                                 uint32_t* __restrict__ buf) {
   const int cnt = h.active ? S.nbr_count[h.i] : 0;
   if (__ballot(cnt > P.list_cap) == 0ull) {
-    const uint32_t* __restrict__ row = S.nlist + (int64_t)h.ii * P.list_cap;
+    const uint32_t* __restrict__ blk = list_block(P, S.nlist, ch);
+    const int lane = threadIdx.x;
     for (int q = 0; __ballot(q < cnt) != 0ull; q += 8) {
       if (q < cnt) {
         // eight entries per pair of 16-byte loads; entries past cnt are
         // padding inside the row and are not evaluated
-        const uint4* src = reinterpret_cast<const uint4*>(row + q);
+        const uint4* src = reinterpret_cast<const uint4*>(blk + group_offset(q, lane));
         const uint4 lo = src[0], hi = src[1];
         uint32_t e[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
         const int m = min(8, cnt - q);
@@ -771,7 +788,8 @@ __global__ __launch_bounds__(kWave) void k_signature(DevParams P, DeviceState S)
 // bound of chunks (one partial chunk per row plus n / 64 full ones)
 // ---------------------------------------------------------------------
 static inline int nchunks_max(const DevParams& P) {
-  return (P.n + kWave - 1) / kWave + P.nm[1] * P.nm[2];
+  const int m = (P.n + kWave - 1) / kWave + P.nm[1] * P.nm[2];
+  return (m + 7) / 8 * 8;
 }
 
 void launch_density(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed) {

@@ -1,7 +1,9 @@
-"""Build the HIP shared library libsdpd.so in-tree for gfx950.
+"""Build the HIP shared library libsdpd.so in-tree for gfx950, and the
+Fortran host mcf_amd/fortran/mcf_sdpd linked against it.
 
 Run as ``python -m mcf_amd.build`` (or through __graft_entry__.build()).
-hipcc cross-compiles for gfx950 without a GPU present.
+hipcc cross-compiles for gfx950 without a GPU present; the Fortran host is
+built with amdflang when it is installed.
 """
 from __future__ import annotations
 
@@ -14,6 +16,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libsdpd.so")
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_HOST = os.path.join(FORTRAN_DIR, "mcf_sdpd")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -38,12 +42,38 @@ def sources():
     return sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
 
 
+def amdflang():
+    for cand in ("/opt/rocm/bin/amdflang", shutil.which("amdflang")):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran_host(verbose: bool = False):
+    """Compile the iso_c_binding module and the host program; the binary
+    finds libsdpd.so next to its own directory through its rpath."""
+    fc = amdflang()
+    if fc is None:
+        if verbose:
+            print("amdflang not found: Fortran host not built", flush=True)
+        return None
+    srcs = [os.path.join(FORTRAN_DIR, f) for f in ("sdpd_c.f90", "mcf_sdpd.f90")]
+    cmd = [fc, "-O2", "-module-dir", FORTRAN_DIR, "-o", FORTRAN_HOST + ".tmp"] + srcs + \
+          ["-L", HERE, "-lsdpd", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(FORTRAN_HOST + ".tmp", FORTRAN_HOST)
+    return FORTRAN_HOST
+
+
 def build(verbose: bool = False) -> str:
     cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", LIB + ".tmp"] + sources()
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
+    build_fortran_host(verbose)
     return LIB
 
 

@@ -1,0 +1,396 @@
+!----------------------------------------------------------------------
+! mcf_sdpd.f90 -- Fortran host of the MI355X SDPD step.
+!
+! Usage:  mcf_sdpd [ctrl_file]          (default ctrl.mcf)
+!
+! Reads MCF's control file and the physics / io files it names
+! (key = value lines, '#' comments; io_read_ctrl.F90:157-357,
+! io_read_physics.F90), sets up the particles (a square / cubic lattice
+! at (i - 1/2) dx as particles_init_global_inter_square.F90:133-154, or
+! the read_external file of io_read_particles.F90:162-204) and marches
+! velocity-Verlet steps with the call order of marching_integrate_VV
+! (marching_integrate_VV.F90:303-1619), every particle operation going
+! through the C-ABI of include/sdpd.h:
+!
+!   prologue (marching_marching.F90:528-1123):
+!     build cells, density + EOS, forces (+ body force, act)
+!   each step:
+!     kick 1/2, drift + wrap, [ct 1/2], build cells, density + EOS,
+!     forces (+ body force, act), [ct 1/2], kick 1/2, statistics
+!
+! Keys beyond the reference's: precision = FP64 | mixed, fused_step = T
+! (whole steps through sdpd_step between statistics), lattice_jitter
+! (uniform jitter of the lattice in units of dx, seeded by random_seed).
+!----------------------------------------------------------------------
+MODULE mcf_sdpd_config
+  IMPLICIT NONE
+  INTEGER, PARAMETER :: MAX_KEYS = 512, KEY_LEN = 64, VAL_LEN = 256
+  TYPE config
+     INTEGER :: n = 0
+     CHARACTER(len=KEY_LEN) :: key(MAX_KEYS)
+     CHARACTER(len=VAL_LEN) :: val(MAX_KEYS)
+  END TYPE config
+
+CONTAINS
+
+  FUNCTION lower(s) RESULT(t)
+    CHARACTER(len=*), INTENT(IN) :: s
+    CHARACTER(len=LEN(s)) :: t
+    INTEGER :: k, c
+    t = s
+    DO k = 1, LEN(s)
+       c = IACHAR(s(k:k))
+       IF (c >= 65 .AND. c <= 90) t(k:k) = ACHAR(c + 32)
+    END DO
+  END FUNCTION lower
+
+  ! Appends the key = value lines of a file; returns stat /= 0 when the
+  ! file cannot be opened (io_read_ctrl.F90 aborts the same way).
+  SUBROUTINE config_read(cfg, path, stat)
+    TYPE(config), INTENT(INOUT) :: cfg
+    CHARACTER(len=*), INTENT(IN) :: path
+    INTEGER, INTENT(OUT) :: stat
+    CHARACTER(len=1024) :: line
+    INTEGER :: u, ios, eq, hash
+    OPEN(newunit=u, file=TRIM(path), status='old', action='read', iostat=stat)
+    IF (stat /= 0) RETURN
+    DO
+       READ(u, '(A)', iostat=ios) line
+       IF (ios /= 0) EXIT
+       hash = INDEX(line, '#')
+       IF (hash > 0) line(hash:) = ' '
+       eq = INDEX(line, '=')
+       IF (eq <= 1) CYCLE
+       IF (cfg%n >= MAX_KEYS) EXIT
+       cfg%n = cfg%n + 1
+       cfg%key(cfg%n) = lower(ADJUSTL(line(1:eq-1)))
+       cfg%val(cfg%n) = ADJUSTL(line(eq+1:))
+    END DO
+    CLOSE(u)
+    stat = 0
+  END SUBROUTINE config_read
+
+  ! Index of a key (the last occurrence wins), 0 when absent.
+  INTEGER FUNCTION config_find(cfg, key)
+    TYPE(config), INTENT(IN) :: cfg
+    CHARACTER(len=*), INTENT(IN) :: key
+    INTEGER :: k
+    config_find = 0
+    DO k = 1, cfg%n
+       IF (TRIM(cfg%key(k)) == lower(key)) config_find = k
+    END DO
+  END FUNCTION config_find
+
+  INTEGER FUNCTION get_int(cfg, key, default)
+    TYPE(config), INTENT(IN) :: cfg
+    CHARACTER(len=*), INTENT(IN) :: key
+    INTEGER, INTENT(IN) :: default
+    INTEGER :: k, ios
+    get_int = default
+    k = config_find(cfg, key)
+    IF (k > 0) THEN
+       READ(cfg%val(k), *, iostat=ios) get_int
+       IF (ios /= 0) get_int = default
+    END IF
+  END FUNCTION get_int
+
+  DOUBLE PRECISION FUNCTION get_real(cfg, key, default)
+    TYPE(config), INTENT(IN) :: cfg
+    CHARACTER(len=*), INTENT(IN) :: key
+    DOUBLE PRECISION, INTENT(IN) :: default
+    INTEGER :: k, ios
+    get_real = default
+    k = config_find(cfg, key)
+    IF (k > 0) THEN
+       READ(cfg%val(k), *, iostat=ios) get_real
+       IF (ios /= 0) get_real = default
+    END IF
+  END FUNCTION get_real
+
+  SUBROUTINE get_reals(cfg, key, n, vals)
+    TYPE(config), INTENT(IN) :: cfg
+    CHARACTER(len=*), INTENT(IN) :: key
+    INTEGER, INTENT(IN) :: n
+    DOUBLE PRECISION, INTENT(INOUT) :: vals(n)
+    INTEGER :: k, ios
+    DOUBLE PRECISION :: tmp(n)
+    k = config_find(cfg, key)
+    IF (k > 0) THEN
+       tmp = vals
+       READ(cfg%val(k), *, iostat=ios) tmp
+       IF (ios == 0) vals = tmp
+    END IF
+  END SUBROUTINE get_reals
+
+  SUBROUTINE get_ints(cfg, key, n, vals)
+    TYPE(config), INTENT(IN) :: cfg
+    CHARACTER(len=*), INTENT(IN) :: key
+    INTEGER, INTENT(IN) :: n
+    INTEGER, INTENT(INOUT) :: vals(n)
+    INTEGER :: k, ios
+    INTEGER :: tmp(n)
+    k = config_find(cfg, key)
+    IF (k > 0) THEN
+       tmp = vals
+       READ(cfg%val(k), *, iostat=ios) tmp
+       IF (ios == 0) vals = tmp
+    END IF
+  END SUBROUTINE get_ints
+
+  LOGICAL FUNCTION get_logical(cfg, key, default)
+    TYPE(config), INTENT(IN) :: cfg
+    CHARACTER(len=*), INTENT(IN) :: key
+    LOGICAL, INTENT(IN) :: default
+    CHARACTER(len=VAL_LEN) :: v
+    INTEGER :: k
+    get_logical = default
+    k = config_find(cfg, key)
+    IF (k > 0) THEN
+       v = lower(ADJUSTL(cfg%val(k)))
+       IF (v(1:1) == '.') v = v(2:)
+       IF (v(1:1) == 't') get_logical = .TRUE.
+       IF (v(1:1) == 'f') get_logical = .FALSE.
+    END IF
+  END FUNCTION get_logical
+
+  FUNCTION get_string(cfg, key, default) RESULT(s)
+    TYPE(config), INTENT(IN) :: cfg
+    CHARACTER(len=*), INTENT(IN) :: key, default
+    CHARACTER(len=VAL_LEN) :: s
+    INTEGER :: k
+    s = default
+    k = config_find(cfg, key)
+    IF (k > 0) s = cfg%val(k)
+  END FUNCTION get_string
+
+END MODULE mcf_sdpd_config
+
+
+PROGRAM mcf_sdpd
+  USE, INTRINSIC :: iso_c_binding
+  USE sdpd_c
+  USE mcf_sdpd_config
+  IMPLICIT NONE
+
+  TYPE(config) :: cfg
+  TYPE(sdpd_params), TARGET :: prm
+  TYPE(c_ptr) :: ctx
+  CHARACTER(len=VAL_LEN) :: ctrl_file, dir, physics_file, io_file, part_file, stat_file, out_file
+  INTEGER :: stat, dim, n, k, d, nsteps, step, step_start, stat_freq, out_freq, su, rc
+  INTEGER :: num_part(3), idx(3)
+  INTEGER(c_int32_t) :: rc32
+  LOGICAL :: read_external, fused, newtonian
+  DOUBLE PRECISION :: dx(3), jitter, mom(3), ekin, t0, t1, r(3)
+  DOUBLE PRECISION, ALLOCATABLE, TARGET :: x(:,:), v(:,:), m(:), rho(:)
+  INTEGER(c_int32_t), ALLOCATABLE :: id(:,:)
+  INTEGER :: seed_size
+  INTEGER, ALLOCATABLE :: seed_arr(:)
+  INTEGER(c_int64_t) :: cur_step
+
+  ctrl_file = 'ctrl.mcf'
+  IF (COMMAND_ARGUMENT_COUNT() >= 1) CALL GET_COMMAND_ARGUMENT(1, ctrl_file)
+  dir = ''
+  k = INDEX(ctrl_file, '/', back=.TRUE.)
+  IF (k > 0) dir = ctrl_file(1:k)
+
+  CALL config_read(cfg, ctrl_file, stat)
+  IF (stat /= 0) CALL die('cannot open control file '//TRIM(ctrl_file))
+  physics_file = get_string(cfg, 'mcf_physics_config_file', 'physics_config.mcf')
+  io_file = get_string(cfg, 'mcf_io_config_file', 'io_config.mcf')
+  CALL config_read(cfg, TRIM(dir)//TRIM(physics_file), stat)
+  IF (stat /= 0) CALL die('cannot open physics file '//TRIM(physics_file))
+  CALL config_read(cfg, TRIM(dir)//TRIM(io_file), stat)
+  IF (stat /= 0) CALL die('cannot open io file '//TRIM(io_file))
+
+  IF (get_int(cfg, 'integrate_type', 2) /= 2) &
+       CALL die('only integrate_type = 2 (velocity Verlet) is supported')
+
+  ! ---- parameters (physics_adjust_parameters.F90 conventions) ----
+  dim = get_int(cfg, 'num_dim', 3)
+  prm%num_dim = dim
+  prm%symmetry = MERGE(1, 0, get_logical(cfg, 'symmetry', .TRUE.))
+  prm%kernel_type = get_int(cfg, 'kernel_type', 1)
+  prm%rhs_density_type = get_int(cfg, 'rhs_density_type', 2)
+  prm%rhs_force_type = get_int(cfg, 'rhs_force_type', 3)
+  prm%state_equation_type = get_int(cfg, 'stateequation_type', 2)
+  newtonian = get_logical(cfg, 'newtonian', .TRUE.)
+  prm%newtonian = MERGE(1, 0, newtonian)
+  prm%brownian = MERGE(1, 0, get_logical(cfg, 'brownian', .FALSE.))
+  prm%body_force_type = get_int(cfg, 'body_force_type', 0)
+  prm%precision = SDPD_PRECISION_FP64
+  IF (lower(ADJUSTL(get_string(cfg, 'precision', 'fp64'))) == 'mixed') &
+       prm%precision = SDPD_PRECISION_MIXED
+  prm%min_phys = 0.0d0
+  prm%max_phys = 1.0d0
+  CALL get_reals(cfg, 'min_phys', dim, prm%min_phys(1:dim))
+  CALL get_reals(cfg, 'max_phys', dim, prm%max_phys(1:dim))
+  prm%cut_off = get_real(cfg, 'cut_off', 0.0d0)
+  prm%c = get_real(cfg, 'c', 10.0d0)
+  prm%rho = get_real(cfg, 'rho', 1.0d0)
+  prm%rho_ref = get_real(cfg, 'rho_ref', 1.0d0)
+  prm%gamma = get_real(cfg, 'gamma', 7.0d0)
+  prm%eta = get_real(cfg, 'eta', 0.1d0)
+  prm%eta_coef = get_real(cfg, 'eta_coef', 0.0d0)
+  prm%kt = get_real(cfg, 'kt', 0.0d0)
+  IF (prm%brownian == 0) prm%kt = 0.0d0   ! physics_adjust_parameters.F90:415-417
+  prm%dt = get_real(cfg, 'dt', -1.0d0)
+  prm%tau = get_real(cfg, 'tau', 0.0d0)
+  prm%n_p = get_real(cfg, 'n_p', 0.0d0)
+  prm%kt_p = get_real(cfg, 'kt_p', 0.0d0)
+  prm%body_force = 0.0d0
+  CALL get_reals(cfg, 'body_force', dim, prm%body_force(1:dim))
+  prm%seed = INT(MAX(get_int(cfg, 'random_seed', 0), 0), c_int64_t)
+  IF (prm%dt <= 0.0d0) CALL die('dt must be given (> 0)')
+
+  step_start = MAX(get_int(cfg, 'step_start', 0), 0)
+  nsteps = get_int(cfg, 'step_end', 0) - step_start
+  IF (nsteps < 0) CALL die('step_end must not be below step_start')
+  stat_freq = MAX(get_int(cfg, 'statistic_freq_step', 1), 1)
+  out_freq = get_int(cfg, 'output_particles_freq_step', -1)
+  stat_file = get_string(cfg, 'statistic_file', 'mcf_statistic.dat')
+  out_file = get_string(cfg, 'output_particles_file', 'mcf_particles')
+  read_external = get_logical(cfg, 'read_external', .FALSE.)
+  fused = get_logical(cfg, 'fused_step', .FALSE.)
+
+  ! ---- particles ----
+  IF (read_external) THEN
+     part_file = get_string(cfg, 'read_particles_file', 'mcf_restart_particles.dat')
+     CALL read_particles(TRIM(dir)//TRIM(part_file))
+  ELSE
+     num_part = 1
+     CALL get_ints(cfg, 'num_part', dim, num_part(1:dim))
+     jitter = get_real(cfg, 'lattice_jitter', 0.0d0)
+     n = PRODUCT(num_part(1:dim))
+     ALLOCATE(x(dim, n), v(dim, n), m(n), rho(n), id(2, n))
+     dx = 1.0d0
+     DO d = 1, dim
+        dx(d) = (prm%max_phys(d) - prm%min_phys(d)) / num_part(d)
+     END DO
+     CALL RANDOM_SEED(size=seed_size)
+     ALLOCATE(seed_arr(seed_size))
+     seed_arr = 12345 + INT(prm%seed)
+     CALL RANDOM_SEED(put=seed_arr)
+     DO k = 1, n
+        idx(1) = MOD(k - 1, num_part(1))
+        idx(2) = MOD((k - 1) / num_part(1), num_part(2))
+        idx(3) = (k - 1) / (num_part(1) * num_part(2))
+        CALL RANDOM_NUMBER(r)
+        DO d = 1, dim
+           x(d, k) = prm%min_phys(d) + (idx(d) + 0.5d0) * dx(d) + jitter * dx(d) * (2.0d0 * r(d) - 1.0d0)
+        END DO
+        m(k) = prm%rho * PRODUCT(dx(1:dim))
+        id(1, k) = k
+        id(2, k) = 0
+     END DO
+     v = 0.0d0
+     rho = prm%rho
+  END IF
+  WRITE(*, '(A,I0,A,I0,A,I0,A)') 'mcf_sdpd: ', n, ' particles, ', dim, 'D, ', nsteps, ' steps'
+
+  ! ---- set-up ----
+  rc = sdpd_create(prm, ctx)
+  IF (rc /= SDPD_OK) CALL die('sdpd_create failed')
+  CALL check(sdpd_upload(ctx, x, v, m, id, INT(n, c_int32_t)), 'sdpd_upload')
+  CALL check(sdpd_set_step(ctx, INT(step_start, c_int64_t)), 'sdpd_set_step')
+
+  OPEN(newunit=su, file=TRIM(stat_file), status='replace', action='write')
+
+  ! prologue: initial density, pressure and forces (marching_marching.F90:528-1123)
+  CALL check(sdpd_build_cells(ctx), 'sdpd_build_cells')
+  CALL check(sdpd_density(ctx), 'sdpd_density')
+  CALL check(sdpd_forces(ctx), 'sdpd_forces')
+  CALL write_statistic(step_start)
+
+  CALL CPU_TIME(t0)
+  step = step_start
+  DO WHILE (step < step_start + nsteps)
+     IF (fused) THEN
+        k = MIN(stat_freq - MOD(step - step_start, stat_freq), step_start + nsteps - step)
+        CALL check(sdpd_step(ctx, INT(k, c_int32_t)), 'sdpd_step')
+        step = step + k
+     ELSE
+        step = step + 1
+        CALL check(sdpd_set_step(ctx, INT(step, c_int64_t)), 'sdpd_set_step')
+        CALL vv_step()
+     END IF
+     IF (MOD(step - step_start, stat_freq) == 0) CALL write_statistic(step)
+     IF (out_freq > 0) THEN
+        IF (MOD(step - step_start, out_freq) == 0) CALL write_particles(step)
+     END IF
+  END DO
+  CALL check(sdpd_sync(ctx), 'sdpd_sync')
+  CALL CPU_TIME(t1)
+  CALL check(sdpd_get_step(ctx, cur_step), 'sdpd_get_step')
+  CLOSE(su)
+  CALL write_particles(step)
+  WRITE(*, '(A,I0,A,ES12.4,A)') 'mcf_sdpd: reached step ', cur_step, ' in ', t1 - t0, ' s of host time'
+  rc32 = sdpd_destroy(ctx)
+
+CONTAINS
+
+  ! One velocity-Verlet step in the order of marching_integrate_VV.F90.
+  SUBROUTINE vv_step()
+    CALL check(sdpd_kick(ctx, 0.5d0), 'sdpd_kick')                    ! :303
+    CALL check(sdpd_drift_wrap(ctx), 'sdpd_drift_wrap')               ! :321, :814
+    IF (.NOT. newtonian) CALL check(sdpd_conformation(ctx, 0.5d0), 'sdpd_conformation')  ! :413
+    CALL check(sdpd_build_cells(ctx), 'sdpd_build_cells')             ! :968-971
+    CALL check(sdpd_density(ctx), 'sdpd_density')                     ! :1002, :1226
+    CALL check(sdpd_forces(ctx), 'sdpd_forces')                       ! :1279, :1374, :1496
+    IF (.NOT. newtonian) CALL check(sdpd_conformation(ctx, 0.5d0), 'sdpd_conformation')  ! :1511
+    CALL check(sdpd_kick(ctx, 0.5d0), 'sdpd_kick')                    ! :1619
+  END SUBROUTINE vv_step
+
+  SUBROUTINE write_statistic(st)
+    INTEGER, INTENT(IN) :: st
+    CALL check(sdpd_stats(ctx, mom, ekin), 'sdpd_stats')
+    WRITE(su, '(I10,1X,ES16.8,1X,ES16.8,3(1X,ES16.8))') st, st * prm%dt, ekin, mom(1:dim)
+  END SUBROUTINE write_statistic
+
+  ! Particle file in the read_external layout (x v rho m pid sid).
+  SUBROUTINE write_particles(st)
+    INTEGER, INTENT(IN) :: st
+    CHARACTER(len=VAL_LEN+16) :: name
+    INTEGER :: u, i
+    CALL check(sdpd_download(ctx, c_loc(x), c_loc(v), c_loc(rho), c_null_ptr, c_null_ptr), &
+         'sdpd_download')
+    WRITE(name, '(A,I8.8,A)') TRIM(out_file), st, '.dat'
+    OPEN(newunit=u, file=TRIM(name), status='replace', action='write')
+    WRITE(u, '(I0)') n
+    DO i = 1, n
+       WRITE(u, '(*(ES25.17,1X))', advance='no') x(:, i), v(:, i), rho(i), m(i)
+       WRITE(u, '(I0,1X,I0)') id(1, i), id(2, i)
+    END DO
+    CLOSE(u)
+  END SUBROUTINE write_particles
+
+  SUBROUTINE read_particles(path)
+    CHARACTER(len=*), INTENT(IN) :: path
+    INTEGER :: u, i, ios
+    DOUBLE PRECISION :: pidr, sidr
+    OPEN(newunit=u, file=path, status='old', action='read', iostat=ios)
+    IF (ios /= 0) CALL die('cannot open particle file '//path)
+    READ(u, *, iostat=ios) n
+    IF (ios /= 0 .OR. n <= 0) CALL die('reading particles has problem at line 1')
+    ALLOCATE(x(dim, n), v(dim, n), m(n), rho(n), id(2, n))
+    DO i = 1, n
+       READ(u, *, iostat=ios) x(:, i), v(:, i), rho(i), m(i), pidr, sidr
+       IF (ios /= 0) CALL die('actual number of lines is not equal to number given !')
+       id(1, i) = NINT(pidr)
+       id(2, i) = NINT(sidr)
+    END DO
+    CLOSE(u)
+  END SUBROUTINE read_particles
+
+  SUBROUTINE check(code, what)
+    INTEGER(c_int), INTENT(IN) :: code
+    CHARACTER(len=*), INTENT(IN) :: what
+    IF (code /= SDPD_OK) CALL die(what//': '//sdpd_last_error(ctx))
+  END SUBROUTINE check
+
+  SUBROUTINE die(msg)
+    CHARACTER(len=*), INTENT(IN) :: msg
+    WRITE(*, '(A)') 'mcf_sdpd: '//msg
+    ERROR STOP 1
+  END SUBROUTINE die
+
+END PROGRAM mcf_sdpd

@@ -177,6 +177,7 @@ int derive(const sdpd_params& p, DevParams& P, std::string& err) {
                             : 105.0 / (16.0 * kPi * P.h * P.h * P.h);
   }
   P.coef_grad = -P.coef / P.h;
+  P.inv_h = 1.0 / P.h;
   P.w0 = p.kernel_type == 1 ? P.coef * (243.0 + -192.0 + 15.0) : P.coef;
   P.c2 = p.c * p.c;
   P.rho_ref = p.rho_ref;
@@ -190,6 +191,7 @@ int derive(const sdpd_params& p, DevParams& P, std::string& err) {
   P.dt = p.dt;
   P.sqrt_dt = std::sqrt(p.dt);
   P.inv_sqrt_dt = 1.0 / P.sqrt_dt;
+  P.sqrt_2kt_eta = std::sqrt(2.0 * P.kt * P.eta);
   P.tau = p.tau;
   P.G = p.n_p * p.kt_p;  // particles_compute_pressure_tensor.F90:94
   for (int d = 0; d < 3; ++d) P.bf[d] = p.body_force[d];

@@ -32,9 +32,9 @@ struct DevParams {
   double rd_ref[3];       // nm_ref / length (PPM's rdx)
   double cs[3];           // cell edge length / nm
   double cut_off, cut2;
-  double h, coef, coef_grad, w0;
+  double h, inv_h, coef, coef_grad, w0;
   double c2, rho_ref, gamma, p0;
-  double eta, eta_coef, kt, dt, sqrt_dt, inv_sqrt_dt;
+  double eta, eta_coef, kt, dt, sqrt_dt, inv_sqrt_dt, sqrt_2kt_eta;
   double tau, G;
   double bf[3];
   uint64_t seed;
@@ -55,66 +55,62 @@ struct DevParams {
 // smoothing kernel (kernel_kernel_quintic_spline.F90:160-215,
 // kernel_kernel_Lucy.F90), same operation order as the reference
 // ---------------------------------------------------------------------
-__device__ __forceinline__ void kernel_w_gradw(const DevParams& P, double r,
-                                               double& w, double& gw) {
-  double s = r / P.h;
+// W and dW/dr at s = r/h without branches: with t_k = max(k - s, 0) the
+// piecewise quintic spline is coef (t3^5 - 6 t2^5 + 15 t1^5) and its
+// derivative coef_grad (5 t3^4 - 30 t2^4 + 75 t1^4), coef_grad = -coef/h
+// (kernel_kernel_quintic_spline.F90:1-221); Lucy: coef (1 + 3s) t1^3 and
+// coef_grad 12 s t1^2 (kernel_kernel_Lucy.F90:1-174).  Zero from s = 3
+// (quintic) or s = 1 (Lucy) on.
+__device__ __forceinline__ void kernel_eval(const DevParams& P, double s, double& w,
+                                            double& gw) {
   if (P.kernel_type == 1) {
-    double s1 = 1.0 - s, s2 = 2.0 - s, s3 = 3.0 - s;
-    double s1_4 = (s1 * s1) * (s1 * s1);
-    double s2_4 = (s2 * s2) * (s2 * s2);
-    double s3_4 = (s3 * s3) * (s3 * s3);
-    double s1_5 = 15.0 * s1_4 * s1;
-    double s2_5 = -6.0 * s2_4 * s2;
-    double s3_5 = s3_4 * s3;
-    double g1 = 75.0 * s1_4, g2 = -30.0 * s2_4, g3 = 5.0 * s3_4;
-    if (s < 1.0) {
-      w = P.coef * (s3_5 + s2_5 + s1_5);
-      gw = P.coef_grad * (g3 + g2 + g1);
-    } else if (s < 2.0) {
-      w = P.coef * (s3_5 + s2_5);
-      gw = P.coef_grad * (g3 + g2);
-    } else if (s < 3.0) {
-      w = P.coef * s3_5;
-      gw = P.coef_grad * g3;
-    } else {
-      w = 0.0;
-      gw = 0.0;
-    }
+    const double t1 = fmax(1.0 - s, 0.0), t2 = fmax(2.0 - s, 0.0), t3 = fmax(3.0 - s, 0.0);
+    const double q1 = (t1 * t1) * (t1 * t1);
+    const double q2 = (t2 * t2) * (t2 * t2);
+    const double q3 = (t3 * t3) * (t3 * t3);
+    w = P.coef * (q3 * t3 - 6.0 * (q2 * t2) + 15.0 * (q1 * t1));
+    gw = P.coef_grad * (5.0 * q3 - 30.0 * q2 + 75.0 * q1);
   } else {
-    if (s <= 1.0) {
-      double s1 = 1.0 - s;
-      w = P.coef * (1.0 + 3.0 * s) * (s1 * s1 * s1);
-      gw = P.coef_grad * 12.0 * s * (s1 * s1);
-    } else {
-      w = 0.0;
-      gw = 0.0;
-    }
+    const double t = fmax(1.0 - s, 0.0);
+    w = P.coef * (1.0 + 3.0 * s) * (t * t * t);
+    gw = P.coef_grad * 12.0 * s * (t * t);
   }
 }
 
 __device__ __forceinline__ void kernel_w_gradw_f32(int kernel_type, float s, float coef,
                                                    float coef_grad, float& w, float& gw) {
   if (kernel_type == 1) {
-    float s1 = 1.0f - s, s2 = 2.0f - s, s3 = 3.0f - s;
-    float s1_4 = (s1 * s1) * (s1 * s1);
-    float s2_4 = (s2 * s2) * (s2 * s2);
-    float s3_4 = (s3 * s3) * (s3 * s3);
-    float wv = s3_4 * s3, gv = 5.0f * s3_4;
-    if (s < 2.0f) {
-      wv = fmaf(-6.0f * s2_4, s2, wv);
-      gv = fmaf(-30.0f, s2_4, gv);
-    }
-    if (s < 1.0f) {
-      wv = fmaf(15.0f * s1_4, s1, wv);
-      gv = fmaf(75.0f, s1_4, gv);
-    }
-    w = coef * wv;
-    gw = coef_grad * gv;
+    const float t1 = fmaxf(1.0f - s, 0.0f), t2 = fmaxf(2.0f - s, 0.0f), t3 = fmaxf(3.0f - s, 0.0f);
+    const float q1 = (t1 * t1) * (t1 * t1);
+    const float q2 = (t2 * t2) * (t2 * t2);
+    const float q3 = (t3 * t3) * (t3 * t3);
+    w = coef * (q3 * t3 - 6.0f * (q2 * t2) + 15.0f * (q1 * t1));
+    gw = coef_grad * (5.0f * q3 - 30.0f * q2 + 75.0f * q1);
   } else {
-    float s1 = 1.0f - s;
-    w = coef * (1.0f + 3.0f * s) * (s1 * s1 * s1);
-    gw = coef_grad * 12.0f * s * (s1 * s1);
+    const float t = fmaxf(1.0f - s, 0.0f);
+    w = coef * (1.0f + 3.0f * s) * (t * t * t);
+    gw = coef_grad * 12.0f * s * (t * t);
   }
+}
+
+// 1/sqrt(x) for normal x > 0: the hardware estimate refined by two Newton
+// steps y <- y + y (1/2 - x y^2 / 2), within an ulp of the rounded value.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  double e = fma(-hx * y, y, 0.5);
+  y = fma(y, e, y);
+  e = fma(-hx * y, y, 0.5);
+  return fma(y, e, y);
+}
+
+// sqrt(x) for x >= 0 through rsqrt_nr (0 for x = 0).
+__device__ __forceinline__ double sqrt_nr(double x) { return x > 0.0 ? x * rsqrt_nr(x) : 0.0; }
+
+// Periodic image of a neighbour in virtual cell v (outside 0..nm-1 across
+// a face): the amount added to x_i - x_j.
+__device__ __forceinline__ double image_shift(int v, int nm, double L) {
+  return v < 0 ? L : (v >= nm ? -L : 0.0);
 }
 
 // ---------------------------------------------------------------------

@@ -73,16 +73,6 @@ struct Home {
 struct Modes {
   int m0, m1, m2;
 };
-template <int DIM>
-__device__ __forceinline__ Modes modes_of(const DevParams& P, const Home& h, const Offsets& o) {
-  Modes m;
-  (void)wrap_cell(h.cx + o.ox, P.nm[0], P.symmetry, m.m0);
-  (void)wrap_cell(h.cy + o.oy, P.nm[1], P.symmetry, m.m1);
-  m.m2 = 0;
-  if (DIM == 3) (void)wrap_cell(h.cz + o.oz, P.nm[2], P.symmetry, m.m2);
-  return m;
-}
-
 // Exact FP64 difference and squared distance for a pair.
 template <int DIM>
 __device__ __forceinline__ double exact_d2(const DevParams& P, const double* __restrict__ x, int j,
@@ -319,6 +309,20 @@ struct KahanF {
 // ---------------------------------------------------------------------
 // K2: density + EOS
 // ---------------------------------------------------------------------
+// FP64 separation of a listed pair: x_i - x_j plus the periodic image
+// shift of the neighbour's virtual cell.  The in/out decision was taken
+// with the reference's own formula; the value here differs from it by at
+// most an ulp of the box length.
+template <int DIM>
+__device__ __forceinline__ double pair_d2(const DevParams& P, const Home& h, const Offsets& o,
+                                          double xj0, double xj1, double xj2, double& d0,
+                                          double& d1, double& d2) {
+  d0 = (h.x0 - xj0) + image_shift(h.cx + o.ox, P.nm[0], P.length[0]);
+  d1 = (h.x1 - xj1) + image_shift(h.cy + o.oy, P.nm[1], P.length[1]);
+  d2 = DIM == 3 ? (h.x2 - xj2) + image_shift(h.cz + o.oz, P.nm[2], P.length[2]) : 0.0;
+  return DIM == 3 ? fma(d2, d2, fma(d1, d1, d0 * d0)) : fma(d1, d1, d0 * d0);
+}
+
 template <int DIM>
 struct DensityF64 {
   const DevParams& P;
@@ -326,11 +330,12 @@ struct DensityF64 {
   const Home& h;
   double acc;
   __device__ __forceinline__ void body(int j, const Offsets& o) {
-    const Modes m = modes_of<DIM>(P, h, o);
+    const int64_t n = P.n;
     double d0, d1, d2;
-    const double r2 = exact_d2<DIM>(P, S.x, j, h, m, d0, d1, d2);
+    const double r2 = pair_d2<DIM>(P, h, o, S.x[j], S.x[n + j], DIM == 3 ? S.x[2 * n + j] : 0.0,
+                                   d0, d1, d2);
     double w, gw;
-    kernel_w_gradw(P, sqrt(r2), w, gw);
+    kernel_eval(P, sqrt_nr(r2) * P.inv_h, w, gw);
     acc = acc + (P.density_type == 1 ? S.m[j] * w : w);
   }
 };
@@ -344,7 +349,7 @@ struct DensityMixed {
   __device__ __forceinline__ void body(int j, const Offsets& o) {
     float d0, d1, d2;
     rel_diff<DIM>(P, h, S.xa[j], o, d0, d1, d2);
-    const float r = __builtin_sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+    const float r = __builtin_amdgcn_sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
     float w, gw;
     kernel_w_gradw_f32(P.kernel_type, r * P.inv_hf, P.coeff, P.coef_gradf, w, gw);
     acc.add(P.density_type == 1 ? (float)S.m[j] * w : w);
@@ -454,118 +459,96 @@ __device__ __forceinline__ void noise_vector(const DevParams& P, uint32_t pidi, 
   }
 }
 
-// force types: 3 Hu-Adams, 4 Hu-Adams angular, 5 Oldroyd-B (non-Newtonian Hu-Adams)
+// force types: 3 Hu-Adams, 4 Hu-Adams angular, 5 Oldroyd-B (non-Newtonian
+// Hu-Adams).  The bodies sum m_i times the acceleration; finish_force
+// divides by m_i once.
 template <int DIM, int FT>
 struct ForceF64 {
   const DevParams& P;
   const DeviceState& S;
   const Home& h;
-  double vi0, vi1, vi2, ai, bi, pi, inv_mi, ni;
+  double vi0, vi1, vi2, ai, bi, pi, inv_ni;
   uint32_t pidi;
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
   double L[9];
   double pti[9];
   __device__ __forceinline__ void body(int j, const Offsets& o) {
     const int64_t n = P.n;
-    const Modes m = modes_of<DIM>(P, h, o);
     const double4* rj = reinterpret_cast<const double4*>(S.rec64 + 8 * (int64_t)j);
     const double4 ra = rj[0], rb = rj[1];
-    const double d0 = image_diff(h.x0, ra.x, P.length[0], m.m0);
-    const double d1 = image_diff(h.x1, ra.y, P.length[1], m.m1);
-    const double d2 = DIM == 3 ? image_diff(h.x2, ra.z, P.length[2], m.m2) : 0.0;
-    const double r2 = DIM == 3 ? dist2_exact(d0, d1, d2) : dist2_exact2(d0, d1);
-    const double r = sqrt(r2);
+    double d0, d1, d2;
+    const double r2 = pair_d2<DIM>(P, h, o, ra.x, ra.y, ra.z, d0, d1, d2);
+    const double ir = rsqrt_nr(r2);
+    const double r = r2 * ir;
     double w, gw;
-    kernel_w_gradw(P, r, w, gw);
-    const double ir = 1.0 / r;
-    const double e0 = d0 * ir, e1 = d1 * ir, e2 = DIM == 3 ? d2 * ir : 0.0;
-    const double vj0 = rb.x, vj1 = rb.y, vj2 = rb.z;
-    const double u0 = vi0 - vj0, u1 = vi1 - vj1, u2 = vi2 - vj2;
+    kernel_eval(P, r * P.inv_h, w, gw);
+    const double u0 = vi0 - rb.x, u1 = vi1 - rb.y, u2 = DIM == 3 ? vi2 - rb.z : 0.0;
     const double bj = rb.w;
-    double fi0, fi1, fi2;
-    if (FT == 5) {
-      // pp_vgt_ip.inc:16-20
-      const double du[3] = {vj0 - vi0, vj1 - vi1, vj2 - vi2};
-      const double dx[3] = {d0, d1, d2};
-      const double gfac = gw / ni * ir;
-#pragma unroll
-      for (int b = 0; b < DIM; ++b)
-#pragma unroll
-        for (int a = 0; a < DIM; ++a) L[a + DIM * b] = L[a + DIM * b] + du[a] * dx[b] * gfac;
-      // rhs_force_ff_non_Newtonian_HuAdams.F90:105-131 with
-      // Pi = (p + G) I - G c (particles_compute_pressure_tensor.F90:114-133)
-      const double pj = S.p[j];
-      const double ev[3] = {e0, e1, e2};
-      double fc[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-      for (int jj = 0; jj < DIM; ++jj)
-#pragma unroll
-        for (int a = 0; a < DIM; ++a) {
-          double ptj = -P.G * S.ct[(int64_t)(a + DIM * jj) * n + j];
-          if (a == jj) ptj = ptj + pj + P.G;
-          fc[jj] = fc[jj] + pti[a + DIM * jj] * ev[a] * bi + ptj * ev[a] * bj;
-        }
-      fi0 = 0.0 - fc[0] * gw * inv_mi;
-      fi1 = 0.0 - fc[1] * gw * inv_mi;
-      fi2 = DIM == 3 ? 0.0 - fc[2] * gw * inv_mi : 0.0;
-      const double f_d = P.eta * (bi + bj) * gw * ir;
-      fi0 = fi0 + f_d * u0 * inv_mi;
-      fi1 = fi1 + f_d * u1 * inv_mi;
-      if (DIM == 3) fi2 = fi2 + f_d * u2 * inv_mi;
-      if (P.noise) {
-        double w0, w1, w2;
-        noise_vector<DIM, double>(P, pidi, (uint32_t)S.pid[j], e0, e1, e2, w0, w1, w2);
-        const double f_r = sqrt(-4.0 * P.kt * f_d) * P.inv_sqrt_dt;
-        fi0 = fi0 + f_r * w0 * inv_mi;
-        fi1 = fi1 + f_r * w1 * inv_mi;
-        if (DIM == 3) fi2 = fi2 + f_r * w2 * inv_mi;
-      }
-    } else if (FT == 3) {
-      // rhs_force_ff_Newtonian_HuAdams.F90:140-237
-      const double f_c = (ai + ra.w) * gw;
-      fi0 = 0.0 - f_c * e0 * inv_mi;
-      fi1 = 0.0 - f_c * e1 * inv_mi;
-      fi2 = DIM == 3 ? 0.0 - f_c * e2 * inv_mi : 0.0;
-      const double f_d = P.eta * (bi + bj) * gw * ir;
-      fi0 = fi0 + f_d * u0 * inv_mi;
-      fi1 = fi1 + f_d * u1 * inv_mi;
-      if (DIM == 3) fi2 = fi2 + f_d * u2 * inv_mi;
-      if (P.noise) {
-        double w0, w1, w2;
-        noise_vector<DIM, double>(P, pidi, (uint32_t)S.pid[j], e0, e1, e2, w0, w1, w2);
-        const double f_r = sqrt(-4.0 * P.kt * f_d) * P.inv_sqrt_dt;
-        fi0 = fi0 + f_r * w0 * inv_mi;
-        fi1 = fi1 + f_r * w1 * inv_mi;
-        if (DIM == 3) fi2 = fi2 + f_r * w2 * inv_mi;
-      }
-    } else {
-      // rhs_force_ff_Newtonian_HuAdams_angular.F90:166-229
-      double ev = e0 * u0 + e1 * u1;
-      if (DIM == 3) ev = ev + e2 * u2;
-      const double pij = (pi + S.p[j]) * 0.5;
+    double a0, a1, a2;
+    if (FT == 4) {
+      // rhs_force_ff_Newtonian_HuAdams_angular.F90:166-229: everything
+      // along e = d / r
+      const double du = DIM == 3 ? fma(d2, u2, fma(d1, u1, d0 * u0)) : fma(d1, u1, d0 * u0);
       const double aij = (bi + bj) * gw;
-      const double f_c = aij * pij;
-      fi0 = 0.0 - f_c * e0 * inv_mi;
-      fi1 = 0.0 - f_c * e1 * inv_mi;
-      fi2 = DIM == 3 ? 0.0 - f_c * e2 * inv_mi : 0.0;
-      const double f_d = P.eta_coef * P.eta * aij * ev * ir;
-      fi0 = fi0 + f_d * e0 * inv_mi;
-      fi1 = fi1 + f_d * e1 * inv_mi;
-      if (DIM == 3) fi2 = fi2 + f_d * e2 * inv_mi;
+      const double pij = (pi + S.p[j]) * 0.5;
+      double sc = P.eta_coef * P.eta * aij * du * (ir * ir) - aij * pij;
       if (P.noise) {
         const uint32_t pj_ = (uint32_t)S.pid[j];
         float g[4];
         pair_normals(P.seed, pidi < pj_ ? pidi : pj_, pidi < pj_ ? pj_ : pidi, P.step, g);
-        const double f_r = (double)g[0] * sqrt(2.0 * P.kt * P.eta) *
-                           sqrt(-P.eta_coef * aij * ir) * P.inv_sqrt_dt;
-        fi0 = fi0 + f_r * e0 * inv_mi;
-        fi1 = fi1 + f_r * e1 * inv_mi;
-        if (DIM == 3) fi2 = fi2 + f_r * e2 * inv_mi;
+        sc = sc + (double)g[0] * P.sqrt_2kt_eta * sqrt_nr(-P.eta_coef * aij * ir) * P.inv_sqrt_dt;
+      }
+      sc = sc * ir;
+      a0 = sc * d0;
+      a1 = sc * d1;
+      a2 = sc * d2;
+    } else {
+      const double fd = P.eta * (bi + bj) * gw * ir;
+      if (FT == 3) {
+        // rhs_force_ff_Newtonian_HuAdams.F90:140-237
+        const double fcr = (ai + ra.w) * gw * ir;
+        a0 = fd * u0 - fcr * d0;
+        a1 = fd * u1 - fcr * d1;
+        a2 = fd * u2 - fcr * d2;
+      } else {
+        // rhs_force_ff_non_Newtonian_HuAdams.F90:105-131 with
+        // Pi = (p + G) I - G c (particles_compute_pressure_tensor.F90:114-133),
+        // and the velocity gradient of pp_vgt_ip.inc:16-20
+        const double du[3] = {-u0, -u1, -u2};
+        const double dx[3] = {d0, d1, d2};
+        const double gfac = gw * inv_ni * ir;
+#pragma unroll
+        for (int b = 0; b < DIM; ++b)
+#pragma unroll
+          for (int a = 0; a < DIM; ++a) L[a + DIM * b] = fma(du[a] * dx[b], gfac, L[a + DIM * b]);
+        const double pj = S.p[j];
+        double fc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int jj = 0; jj < DIM; ++jj)
+#pragma unroll
+          for (int a = 0; a < DIM; ++a) {
+            double ptj = -P.G * S.ct[(int64_t)(a + DIM * jj) * n + j];
+            if (a == jj) ptj = ptj + pj + P.G;
+            fc[jj] = fc[jj] + (pti[a + DIM * jj] * bi + ptj * bj) * dx[a];
+          }
+        const double gir = gw * ir;
+        a0 = fd * u0 - fc[0] * gir;
+        a1 = fd * u1 - fc[1] * gir;
+        a2 = DIM == 3 ? fd * u2 - fc[2] * gir : 0.0;
+      }
+      if (P.noise) {
+        double w0, w1, w2;
+        noise_vector<DIM, double>(P, pidi, (uint32_t)S.pid[j], d0 * ir, d1 * ir, d2 * ir, w0, w1,
+                                  w2);
+        const double fr = sqrt_nr(-4.0 * P.kt * fd) * P.inv_sqrt_dt;
+        a0 = fma(fr, w0, a0);
+        a1 = fma(fr, w1, a1);
+        a2 = fma(fr, w2, a2);
       }
     }
-    f0 = f0 + fi0;
-    f1 = f1 + fi1;
-    if (DIM == 3) f2 = f2 + fi2;
+    f0 = f0 + a0;
+    f1 = f1 + a1;
+    if (DIM == 3) f2 = f2 + a2;
   }
 };
 
@@ -583,43 +566,44 @@ struct ForceMixed {
     const float4 vj = S.vr[j];
     const float4 qj = S.qr[j];
     const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
-    const float r = __builtin_sqrtf(r2);
-    const float ir = __builtin_amdgcn_rcpf(r);
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * ir;
     float w, gw;
     kernel_w_gradw_f32(P.kernel_type, r * P.inv_hf, P.coeff, P.coef_gradf, w, gw);
-    const float e0 = d0 * ir, e1 = d1 * ir, e2 = d2 * ir;
     const float u0 = vi0 - vj.x, u1 = vi1 - vj.y, u2 = vi2 - vj.z;
     const uint32_t pidj = __float_as_uint(vj.w);
     float fi0, fi1, fi2;
     if (FT == 3) {
-      const float f_c = (ai + qj.x) * gw;
+      const float fcr = (ai + qj.x) * gw * ir;
       const float f_d = P.etaf * (bi + qj.y) * gw * ir;
-      fi0 = f_d * u0 - f_c * e0;
-      fi1 = f_d * u1 - f_c * e1;
-      fi2 = f_d * u2 - f_c * e2;
+      fi0 = f_d * u0 - fcr * d0;
+      fi1 = f_d * u1 - fcr * d1;
+      fi2 = f_d * u2 - fcr * d2;
       if (P.noise) {
         float w0, w1, w2;
-        noise_vector<DIM, float>(P, pidi, pidj, e0, e1, e2, w0, w1, w2);
-        const float f_r = __builtin_sqrtf(-4.0f * P.ktf * f_d) * P.inv_sqrt_dtf;
+        noise_vector<DIM, float>(P, pidi, pidj, d0 * ir, d1 * ir, d2 * ir, w0, w1, w2);
+        const float f_r = __builtin_amdgcn_sqrtf(-4.0f * P.ktf * f_d) * P.inv_sqrt_dtf;
         fi0 = fi0 + f_r * w0;
         fi1 = fi1 + f_r * w1;
         fi2 = fi2 + f_r * w2;
       }
     } else {
-      const float ev = e0 * u0 + e1 * u1 + e2 * u2;
+      const float ev = (d0 * u0 + d1 * u1 + d2 * u2) * ir;
       const float pij = (pi + qj.z) * 0.5f;
       const float aij = (bi + qj.y) * gw;
       const float f_c = aij * pij;
       const float f_d = P.eta_coeff * P.etaf * aij * ev * ir;
-      float s = f_d - f_c;
+      float sc = f_d - f_c;
       if (P.noise) {
         float g[4];
         pair_normals(P.seed, pidi < pidj ? pidi : pidj, pidi < pidj ? pidj : pidi, P.step, g);
-        s = s + g[0] * P.sqrt_2kt_etaf * __builtin_sqrtf(-P.eta_coeff * aij * ir) * P.inv_sqrt_dtf;
+        sc = sc + g[0] * P.sqrt_2kt_etaf * __builtin_amdgcn_sqrtf(-P.eta_coeff * aij * ir) *
+                      P.inv_sqrt_dtf;
       }
-      fi0 = s * e0;
-      fi1 = s * e1;
-      fi2 = s * e2;
+      sc = sc * ir;
+      fi0 = sc * d0;
+      fi1 = sc * d1;
+      fi2 = sc * d2;
     }
     f0.add(fi0);
     f1.add(fi1);
@@ -702,8 +686,7 @@ __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
     pol.ai = S.an[ii];
     pol.bi = S.bn[ii];
     pol.pi = S.p[ii];
-    pol.ni = S.rho[ii];
-    pol.inv_mi = inv_mi;
+    pol.inv_ni = 1.0 / S.rho[ii];
     pol.pidi = (uint32_t)S.pid[ii];
 #pragma unroll
     for (int q = 0; q < 9; ++q) pol.L[q] = 0.0;
@@ -721,7 +704,7 @@ __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
     for_each_listed<DIM>(P, S, ch, h, pol, tile, buf);
     if (!h.active) return;
     const int i = h.i;
-    finish_force<DIM>(P, S, i, pol.f0, pol.f1, pol.f2);
+    finish_force<DIM>(P, S, i, pol.f0 * inv_mi, pol.f1 * inv_mi, pol.f2 * inv_mi);
     if (FT == 5) {
       // particles_compute_act.F90:140-155 on this particle's own ct and vgt
       const int d2n = DIM * DIM;

@@ -145,14 +145,15 @@ int derive(const sdpd_params& p, DevParams& P, std::string& err) {
   P.ncell = (int32_t)ncell;
   P.ncell_ref = (int32_t)ncell_ref;
   {
-    // Band of the FP32 candidate test.  Positions relative to the own cell
-    // corner and the stencil offsets (at most 3 cell edges) carry FP32
-    // errors below 2^-21 * 3 cs each, so a difference is off by at most
-    // dd = 2^-20 * 3 cs and r^2 by 2*sqrt(3)*r*dd + 4*2^-24*r^2.  The band
-    // is twice that bound relative to rc^2, and at least 2e-5.
-    double csmax = 0.0;
-    for (int d = 0; d < p.num_dim; ++d) csmax = std::max(csmax, P.cs[d]);
-    const double dd = std::ldexp(3.0 * csmax, -20);
+    // Band of the FP32 candidate test.  The scan compares positions
+    // relative to the corner of a chunk's first candidate cell in x (at
+    // most L + 5 cs in size) and to a neighbouring cell corner in y, z;
+    // with six FP32 roundings per difference each difference is off by at
+    // most dd = 2^-21 (L + 5 cs), and r^2 by 2 sqrt(3) r dd + 4 2^-24 r^2.
+    // The band is twice that bound relative to rc^2, and at least 2e-5.
+    double mmax = 0.0;
+    for (int d = 0; d < p.num_dim; ++d) mmax = std::max(mmax, P.length[d] + 5.0 * P.cs[d]);
+    const double dd = std::ldexp(mmax, -21);
     double rel = 2.0 * (2.0 * std::sqrt(3.0) * dd / p.cut_off + 4.0 * std::ldexp(1.0, -24));
     rel = std::max(rel, 2e-5);
     const double c2 = p.cut_off * p.cut_off;

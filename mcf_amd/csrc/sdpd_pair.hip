@@ -211,6 +211,13 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
   };
   const int v_lo = ch.xmin - 2, v_hi = ch.xmax + 2;
   const int nrow = DIM == 3 ? 25 : 5;
+  // Positions in the tile are relative to the corner of virtual cell v_lo
+  // in x (and to the candidate's own cell in y, z); the tile's .w holds
+  // the list entry with the virtual cell (mod 128) in place of the code,
+  // so that entry + kadd is the entry with the stencil code.
+  const int crel = h.cx - v_lo;
+  const float ax = h.a.x + (float)crel * P.csf[0];
+  const uint32_t self_w = (uint32_t)h.ii | ((uint32_t)crel << kCodeShift);
 #pragma unroll 1
   for (int row = 0; row < nrow; ++row) {
     const int dy = row % 5 - 2, dz = DIM == 3 ? row / 5 - 2 : 0;
@@ -227,68 +234,72 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
     const int g2 = start[rb];
     const int len2 = v_hi > nm0 - 1 ? start[rb + v_hi - nm0 + 1] - g2 : 0;
     const int tot = len0 + len1 + len2;
-    const bool staged = tot <= kTile;
-    if (staged) {
-      for (int t = lane; t < tot; t += kWave) {
-        const int g = t < len0 ? g0 + t : (t < len0 + len1 ? g1 + (t - len0) : g2 + (t - len0 - len1));
-        tile[t] = S.xa[g];
-      }
-    }
-    __syncthreads();
-    // this lane's window: virtual cells cx-2 .. cx+2 as a tile range
+    // this lane's window: virtual cells cx-2 .. cx+2 as a range of the
+    // row sequence
     int tb = 0, te = 0;
     if (h.active) {
-      auto tile_of = [&](int v) -> int {  // first tile index of virtual cell v
+      auto seq_of = [&](int v) -> int {  // first sequence index of virtual cell v
         if (v < 0) return start[rb + v + nm0] - g0;
         if (v > nm0 - 1) return len0 + len1 + (start[rb + v - nm0] - g2);
         return len0 + (start[rb + v] - g1);
       };
-      tb = tile_of(h.cx - 2);
-      te = tile_of(h.cx + 3);
+      tb = seq_of(h.cx - 2);
+      te = seq_of(h.cx + 3);
     }
     const float ay = h.a.y - (float)dy * P.csf[1];
-    const float az = h.a.z - (float)dz * P.csf[2];
-    for (int t = tb; __ballot(t < te) != 0ull; t += 2) {
-      if (__ballot(cnt > kBuf - 2) != 0ull) flush(false);
+    const float az = DIM == 3 ? h.a.z - (float)dz * P.csf[2] : 0.f;
+    const uint32_t kadd = (uint32_t)(2 - crel + 5 * (dy + 2) + 25 * (dz + 2)) << kCodeShift;
+    // the row in batches of at most kTile records (one batch in practice)
+    for (int base = 0; base < tot; base += kTile) {
+      const int bend = min(tot, base + kTile);
+      for (int t = base + lane; t < bend; t += kWave) {
+        const int piece = t < len0 ? 0 : (t < len0 + len1 ? 1 : 2);
+        const int g = piece == 0 ? g0 + t : (piece == 1 ? g1 + (t - len0) : g2 + (t - len0 - len1));
+        const float4 q = S.xa[g];
+        const int vrel = __float_as_int(q.w) + (piece == 0 ? -nm0 : (piece == 2 ? nm0 : 0)) - v_lo;
+        tile[t - base] = make_float4(fmaf((float)vrel, P.csf[0], q.x), q.y, q.z,
+                                     __uint_as_float((uint32_t)g | ((uint32_t)vrel << kCodeShift)));
+      }
+      __syncthreads();
+      const int lb = max(tb, base) - base, le = min(te, bend) - base;
+      for (int t = lb; __ballot(t < le) != 0ull; t += 2) {
+        if (__ballot(cnt > kBuf - 2) != 0ull) flush(false);
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int tt = t + u;
-        const bool live = tt < te;
-        const int piece = tt < len0 ? 0 : (tt < len0 + len1 ? 1 : 2);
-        const int g = piece == 0 ? g0 + tt : (piece == 1 ? g1 + (tt - len0) : g2 + (tt - len0 - len1));
-        float4 q;
-        if (staged)
-          q = tile[live ? tt : 0];
-        else
-          q = S.xa[live ? g : h.ii];
-        const int v = __float_as_int(q.w) + (piece == 0 ? -nm0 : (piece == 2 ? nm0 : 0));
-        const int ox = v - h.cx;
-        const bool ok = live && g != h.i;
-        const float d0 = h.a.x - (q.x + (float)ox * P.csf[0]);
-        const float d1 = ay - q.y;
-        float s2 = d0 * d0 + d1 * d1;
-        if (DIM == 3) {
-          const float d2 = az - q.z;
-          s2 = s2 + d2 * d2;
-        }
-        bool in = s2 < P.cut2_lo_f32;
-        if (ok && !in && s2 < P.cut2_hi_f32) {
-          // FP64 re-decision with the reference's image convention
-          ++rechecks;
-          Modes m;
-          m.m0 = piece == 0 ? low_mode : (piece == 2 ? 1 : 0);
-          m.m1 = my;
-          m.m2 = mz;
-          double e0, e1, e2;
-          in = exact_d2<DIM>(P, S.x, g, h, m, e0, e1, e2) < P.cut2;
-        }
-        if (ok && in) {
-          buf[cnt * kWave + lane] = (uint32_t)g | (make_code(ox, dy, dz) << kCodeShift);
-          ++cnt;
+        for (int u = 0; u < 2; ++u) {
+          const int tt = t + u;
+          const bool live = tt < le;
+          const float4 q = tile[live ? tt : 0];
+          const uint32_t wv = __float_as_uint(q.w);
+          const float d0 = ax - q.x;
+          const float d1 = ay - q.y;
+          float s2 = fmaf(d1, d1, d0 * d0);
+          if (DIM == 3) {
+            const float d2 = az - q.z;
+            s2 = fmaf(d2, d2, s2);
+          }
+          const bool ok = live && wv != self_w;
+          const uint32_t ent = wv + kadd;
+          bool in = s2 < P.cut2_lo_f32;
+          if (ok && !in && s2 < P.cut2_hi_f32) {
+            // FP64 re-decision with the reference's image convention
+            ++rechecks;
+            const Offsets o = decode_code(ent >> kCodeShift);
+            const int v = h.cx + o.ox;
+            Modes m;
+            m.m0 = v < 0 ? low_mode : (v > nm0 - 1 ? 1 : 0);
+            m.m1 = my;
+            m.m2 = mz;
+            double e0, e1, e2;
+            in = exact_d2<DIM>(P, S.x, (int)(ent & kIndexMask), h, m, e0, e1, e2) < P.cut2;
+          }
+          if (ok && in) {
+            buf[cnt * kWave + lane] = ent;
+            ++cnt;
+          }
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
   }
   flush(true);
   return stored;

@@ -155,13 +155,55 @@ __device__ __forceinline__ Home load_home(const DevParams& P, const DeviceState&
 }
 
 // ---------------------------------------------------------------------
+// stencil rows of a chunk
+// ---------------------------------------------------------------------
+// One stencil row (dy, dz) of a chunk is three contiguous pieces of the
+// target row of cells, in virtual-cell order: [low wrap | main | high
+// wrap].  The low wrap holds the cells left of x = 0 seen through the
+// periodic face (virtual cell = physical - nm0), the high wrap those right
+// of x = nm0 - 1 (virtual = physical + nm0).  Sequence index t runs over
+// the three pieces in that order.
+struct RowPieces {
+  int rb;                 // first cell of the target row
+  int my, mz;             // image modes of the row in y and z
+  int g0, len0, g1, len1, g2, len2;
+  int tot;
+};
+
+template <int DIM>
+__device__ __forceinline__ RowPieces row_pieces(const DevParams& P, const DeviceState& S,
+                                                const Chunk& ch, const Home& h, int row) {
+  RowPieces rp;
+  const int nm0 = P.nm[0];
+  const int32_t* __restrict__ start = S.start;
+  const int dy = row % 5 - 2, dz = DIM == 3 ? row / 5 - 2 : 0;
+  rp.mz = 0;
+  const int ny = wrap_cell(h.cy + dy, P.nm[1], P.symmetry, rp.my);
+  const int nz = DIM == 3 ? wrap_cell(h.cz + dz, P.nm[2], P.symmetry, rp.mz) : 0;
+  rp.rb = nm0 * (ny + P.nm[1] * nz);
+  const int v_lo = ch.xmin - 2, v_hi = ch.xmax + 2;
+  const int ml = v_lo < 0 ? 0 : v_lo, mh = v_hi > nm0 - 1 ? nm0 - 1 : v_hi;
+  rp.g1 = start[rp.rb + ml];
+  rp.len1 = start[rp.rb + mh + 1] - rp.g1;
+  rp.g0 = v_lo < 0 ? start[rp.rb + v_lo + nm0] : start[rp.rb + nm0];
+  rp.len0 = v_lo < 0 ? start[rp.rb + nm0] - rp.g0 : 0;
+  rp.g2 = start[rp.rb];
+  rp.len2 = v_hi > nm0 - 1 ? start[rp.rb + v_hi - nm0 + 1] - rp.g2 : 0;
+  rp.tot = rp.len0 + rp.len1 + rp.len2;
+  return rp;
+}
+
+// particle index of sequence position t
+__device__ __forceinline__ int seq_particle(const RowPieces& rp, int t) {
+  return t < rp.len0 ? rp.g0 + t
+                     : (t < rp.len0 + rp.len1 ? rp.g1 + (t - rp.len0) : rp.g2 + (t - rp.len0 - rp.len1));
+}
+
+// ---------------------------------------------------------------------
 // the staged scan
 // ---------------------------------------------------------------------
-// One stencil row of a chunk is three contiguous pieces of the target row,
-// in virtual-cell order: [low wrap | main | high wrap].  The low wrap holds
-// the cells left of x = 0 seen through the periodic face (virtual cell =
-// physical - nm0), the high wrap those right of x = nm0 - 1 (virtual =
-// physical + nm0).  Returns the number of accepted neighbours.
+// Candidate scan over the stencil rows of a chunk; returns the number of
+// accepted neighbours.
 template <int DIM, class Pol>
 __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState& S,
                                            const Chunk& ch, const Home& h, Pol& pol,
@@ -209,7 +251,7 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
     cnt = rem;
     ev = rem;
   };
-  const int v_lo = ch.xmin - 2, v_hi = ch.xmax + 2;
+  const int v_lo = ch.xmin - 2;
   const int nrow = DIM == 3 ? 25 : 5;
   // Positions in the tile are relative to the corner of virtual cell v_lo
   // in x (and to the candidate's own cell in y, z); the tile's .w holds
@@ -221,19 +263,10 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
 #pragma unroll 1
   for (int row = 0; row < nrow; ++row) {
     const int dy = row % 5 - 2, dz = DIM == 3 ? row / 5 - 2 : 0;
-    int my, mz = 0;
-    const int ny = wrap_cell(h.cy + dy, P.nm[1], P.symmetry, my);
-    const int nz = DIM == 3 ? wrap_cell(h.cz + dz, P.nm[2], P.symmetry, mz) : 0;
-    const int rb = nm0 * (ny + P.nm[1] * nz);
-    // the three pieces (uniform across the wave)
-    const int ml = v_lo < 0 ? 0 : v_lo, mh = v_hi > nm0 - 1 ? nm0 - 1 : v_hi;
-    const int g1 = start[rb + ml];
-    const int len1 = start[rb + mh + 1] - g1;
-    const int g0 = v_lo < 0 ? start[rb + v_lo + nm0] : start[rb + nm0];
-    const int len0 = v_lo < 0 ? start[rb + nm0] - g0 : 0;
-    const int g2 = start[rb];
-    const int len2 = v_hi > nm0 - 1 ? start[rb + v_hi - nm0 + 1] - g2 : 0;
-    const int tot = len0 + len1 + len2;
+    const RowPieces rp = row_pieces<DIM>(P, S, ch, h, row);
+    const int my = rp.my, mz = rp.mz, rb = rp.rb;
+    const int g0 = rp.g0, len0 = rp.len0, g1 = rp.g1, len1 = rp.len1, g2 = rp.g2;
+    const int tot = rp.tot;
     // this lane's window: virtual cells cx-2 .. cx+2 as a range of the
     // row sequence
     int tb = 0, te = 0;
@@ -254,7 +287,7 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
       const int bend = min(tot, base + kTile);
       for (int t = base + lane; t < bend; t += kWave) {
         const int piece = t < len0 ? 0 : (t < len0 + len1 ? 1 : 2);
-        const int g = piece == 0 ? g0 + t : (piece == 1 ? g1 + (t - len0) : g2 + (t - len0 - len1));
+        const int g = seq_particle(rp, t);
         const float4 q = S.xa[g];
         const int vrel = __float_as_int(q.w) + (piece == 0 ? -nm0 : (piece == 2 ? nm0 : 0)) - v_lo;
         tile[t - base] = make_float4(fmaf((float)vrel, P.csf[0], q.x), q.y, q.z,
@@ -548,13 +581,26 @@ struct ForceF64 {
         a2 = DIM == 3 ? fd * u2 - fc[2] * gir : 0.0;
       }
       if (P.noise) {
-        double w0, w1, w2;
-        noise_vector<DIM, double>(P, pidi, (uint32_t)S.pid[j], d0 * ir, d1 * ir, d2 * ir, w0, w1,
-                                  w2);
+        // f_r W e with W e = s (ca g + cb (g.e) e), e = d / r, s = +1 when
+        // pid_i < pid_j (noise_vector), written on d to save operations
+        const uint32_t pidj = (uint32_t)S.pid[j];
+        float g[4];
+        pair_normals(P.seed, pidi < pidj ? pidi : pidj, pidi < pidj ? pidj : pidi, P.step, g);
         const double fr = sqrt_nr(-4.0 * P.kt * fd) * P.inv_sqrt_dt;
-        a0 = fma(fr, w0, a0);
-        a1 = fma(fr, w1, a1);
-        a2 = fma(fr, w2, a2);
+        const double sfr = pidi < pidj ? fr : -fr;
+        const double ca = 0.70710678118654752440;
+        const double g0 = (double)g[0], g1 = (double)g[1];
+        if (DIM == 3) {
+          const double cb = 0.81649658092772603273 - 0.70710678118654752440;
+          const double g2 = (double)g[2];
+          const double k = cb * fma(g2, d2, fma(g1, d1, g0 * d0)) * (ir * ir);
+          a0 = fma(sfr, fma(k, d0, ca * g0), a0);
+          a1 = fma(sfr, fma(k, d1, ca * g1), a1);
+          a2 = fma(sfr, fma(k, d2, ca * g2), a2);
+        } else {
+          a0 = fma(sfr, ca * g0, a0);
+          a1 = fma(sfr, ca * g1, a1);
+        }
       }
     }
     f0 = f0 + a0;
@@ -572,10 +618,13 @@ struct ForceMixed {
   uint32_t pidi;
   KahanF f0, f1, f2;
   __device__ __forceinline__ void body(int j, const Offsets& o) {
+    body_rec(S.xa[j], S.vr[j], S.qr[j], o);
+  }
+  // xj: cell-relative position; vj: (v, pid); qj: (p/n^2, 1/n^2, p)
+  __device__ __forceinline__ void body_rec(const float4& xj, const float4& vj, const float4& qj,
+                                           const Offsets& o) {
     float d0, d1, d2;
-    rel_diff<DIM>(P, h, S.xa[j], o, d0, d1, d2);
-    const float4 vj = S.vr[j];
-    const float4 qj = S.qr[j];
+    rel_diff<DIM>(P, h, xj, o, d0, d1, d2);
     const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
     const float ir = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * ir;

@@ -29,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 # Stencil-model algorithmic bytes per particle and launch (27 neighbour-cell
 # records per particle): FP64 force record 68 B -> 27*68 + f 24 = 1860 B;
 # density record 24 B -> 27*24 + n, p 16 = 664 B + self 8 = 672 B.
@@ -63,6 +64,18 @@ def cpu_baseline(seconds_hint: float = 12.0) -> dict:
         "sample": f"FP64 oracle, same physics on a 32^3 = {box.n} particle box, {steps} VV steps "
                   f"in {dt:.1f} s on one core of {os.uname().nodename}",
     }
+
+
+def measured_traffic(n_side: int, precision: str):
+    """HBM bytes per pair-force launch from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy calibration), for
+    the workload it was measured on; None otherwise."""
+    if n_side != 128 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as fh:
+        data = json.load(fh)
+    entry = data.get(f"tait128_{precision}_forces")
+    return None if entry is None else float(entry["bytes"])
 
 
 def copy_probe(nbytes: int = 1 << 30, reps: int = 5) -> int:
@@ -160,7 +173,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": force_gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.n_side, args.precision),
             "bytes_per_particle": FORCE_BYTES[args.precision],
             "launch_ms": t_force * 1e3,
             "density_gbs": DENSITY_BYTES[args.precision] * box.n / t_dens / 1e9,

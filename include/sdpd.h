@@ -78,6 +78,22 @@ int sdpd_upload(sdpd_ctx *ctx, const double *x, const double *v,
  * after sdpd_upload is the identity (particles_init_partial_inter.F90:145-149). */
 int sdpd_upload_ct(sdpd_ctx *ctx, const double *ct);
 
+/* Batched ensemble.  The reference runs one box per job; a batched context
+ * holds nrep independent periodic boxes of different sizes that share the
+ * physics of the params (whose min_phys / max_phys are then unused), so
+ * that one launch of each kernel advances all of them.  Each replica is a
+ * periodic box of its own: no particle sees another replica's particles.
+ * box_lo, box_hi: (3, nrep), 3 columns per replica whatever num_dim.  Any
+ * uploaded particles are discarded. */
+int sdpd_set_replicas(sdpd_ctx *ctx, int32_t nrep, const double *box_lo,
+                      const double *box_hi);
+
+/* sdpd_upload for a batched context: rep(n) is each particle's replica
+ * (0 .. nrep-1); particles must lie in their replica's box. */
+int sdpd_upload_replicas(sdpd_ctx *ctx, const double *x, const double *v,
+                         const double *m, const int32_t *id, const int32_t *rep,
+                         int32_t n);
+
 /* Replaces technique_build_list (technique_build_list.F90:1-127 ->
  * ppm_neighlist_clist + ppm_util_rank3d + MkNeighIdx): sorts particles
  * into cells with a stable counting sort. */
@@ -114,6 +130,11 @@ int sdpd_conformation(sdpd_ctx *ctx, double lambda);
 /* Replaces statistic_compute_statistic (statistic_compute_statistic.F90:230-283):
  * kinetic energy and momentum of fluid particles, FP64 compensated sums. */
 int sdpd_stats(sdpd_ctx *ctx, double *mom /*[3]*/, double *ekin);
+
+/* statistic_compute_statistic (statistic_compute_statistic.F90:230-283)
+ * for every replica: ekin(nrep), mom(3, nrep); compensated sums in a fixed
+ * order. */
+int sdpd_stats_replicas(sdpd_ctx *ctx, double *mom, double *ekin);
 
 /* Copies state back in upload order.  Any pointer may be NULL.
  * x, v, f: (num_dim, n); rho, p: n; ct, vgt: (num_dim*num_dim, n). */

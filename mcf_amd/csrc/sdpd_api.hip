@@ -38,6 +38,11 @@ struct sdpd_ctx {
   std::vector<Mark> marks;
   std::vector<hipEvent_t> event_pool;
   std::string err;
+  // boxes: one (params' min_phys / max_phys) or a batch of replicas
+  std::vector<double> rep_lo, rep_hi;  // (3, nrep) when batched
+  std::vector<sdpd::RepGeom> geo;
+  std::vector<int32_t> row_cell, row_len, row_rep;
+  bool sorted_once = false;
 };
 
 namespace {
@@ -144,27 +149,11 @@ int derive(const sdpd_params& p, DevParams& P, std::string& err) {
   if (ncell > (int64_t)1 << 30) return err = "too many cells", SDPD_ERR_ARG;
   P.ncell = (int32_t)ncell;
   P.ncell_ref = (int32_t)ncell_ref;
-  {
-    // Band of the FP32 candidate test.  The scan compares positions
-    // relative to the corner of a chunk's first candidate cell in x (at
-    // most L + 5 cs in size) and to a neighbouring cell corner in y, z;
-    // with six FP32 roundings per difference each difference is off by at
-    // most dd = 2^-21 (L + 5 cs), and r^2 by 2 sqrt(3) r dd + 4 2^-24 r^2.
-    // The band is twice that bound relative to rc^2, and at least 2e-5.
-    double mmax = 0.0;
-    for (int d = 0; d < p.num_dim; ++d) mmax = std::max(mmax, P.length[d] + 5.0 * P.cs[d]);
-    const double dd = std::ldexp(mmax, -21);
-    double rel = 2.0 * (2.0 * std::sqrt(3.0) * dd / p.cut_off + 4.0 * std::ldexp(1.0, -24));
-    rel = std::max(rel, 2e-5);
-    const double c2 = p.cut_off * p.cut_off;
-    P.cut2_lo_f32 = (float)(c2 * std::max(0.0, 1.0 - rel)) * (1.0f - 1e-6f);
-    P.cut2_hi_f32 = (float)(c2 * (1.0 + rel)) * (1.0f + 1e-6f);
-    for (int d = 0; d < 3; ++d) {
-      P.lengthf[d] = (float)P.length[d];
-      P.csf[d] = (float)P.cs[d];
-    }
-    (void)lmax;
+  for (int d = 0; d < 3; ++d) {
+    P.lengthf[d] = (float)P.length[d];
+    P.csf[d] = (float)P.cs[d];
   }
+  (void)lmax;
   P.cut_off = p.cut_off;
   P.cut2 = p.cut_off * p.cut_off;
   // kernel_new.F90:47-93
@@ -210,6 +199,77 @@ int derive(const sdpd_params& p, DevParams& P, std::string& err) {
   return SDPD_OK;
 }
 
+// Geometry tables of the context's boxes: the single box of the params,
+// or the replicas of sdpd_set_replicas.  Sets P.nrep, P.ncell, P.nrows and
+// the FP32 cut-off band, which is sized for the largest box.
+int build_geometry(sdpd_ctx* c) {
+  DevParams& P = c->P;
+  const int dim = P.dim;
+  const bool batched = !c->rep_lo.empty();
+  const int nrep = batched ? (int)(c->rep_lo.size() / 3) : 1;
+  const double cell = c->prm.cut_off * 1.0001;  // mcf.F90:617
+  c->geo.assign(nrep, sdpd::RepGeom{});
+  c->row_cell.clear();
+  c->row_len.clear();
+  c->row_rep.clear();
+  int64_t cells = 0, rows = 0;
+  double mmax = 0.0;
+  for (int q = 0; q < nrep; ++q) {
+    sdpd::RepGeom& g = c->geo[q];
+    for (int d = 0; d < 3; ++d) {
+      if (d < dim) {
+        const double lo = batched ? c->rep_lo[3 * (size_t)q + d] : P.min_phys[d];
+        const double L = batched ? c->rep_hi[3 * (size_t)q + d] - lo : P.length[d];
+        if (!(L > 0)) return fail(c, SDPD_ERR_ARG, "replica box with non-positive edge");
+        const int nm_ref = (int)(L / cell);
+        if (nm_ref < 3)
+          return fail(c, SDPD_ERR_ARG,
+                      "every box must span at least 3 cells of 1.0001*cut_off per dimension");
+        g.nm[d] = 2 * nm_ref;
+        g.lo[d] = lo;
+        g.L[d] = L;
+        g.rd[d] = (double)g.nm[d] / L;
+        g.cs[d] = L / (double)g.nm[d];
+        mmax = std::max(mmax, L + 5.0 * g.cs[d]);
+      } else {
+        g.nm[d] = 1;
+        g.lo[d] = 0.0;
+        g.L[d] = 1.0;
+        g.rd[d] = 1.0;
+        g.cs[d] = 1.0;
+      }
+      g.csf[d] = (float)g.cs[d];
+    }
+    g.cell_base = (int32_t)cells;
+    g.row_base = (int32_t)rows;
+    const int nr = g.nm[1] * g.nm[2];
+    for (int r = 0; r < nr; ++r) {
+      c->row_cell.push_back((int32_t)(cells + (int64_t)r * g.nm[0]));
+      c->row_len.push_back(g.nm[0]);
+      c->row_rep.push_back(q);
+    }
+    cells += (int64_t)g.nm[0] * g.nm[1] * g.nm[2];
+    rows += nr;
+    if (cells > ((int64_t)1 << 30)) return fail(c, SDPD_ERR_ARG, "too many cells");
+  }
+  P.nrep = nrep;
+  P.ncell = (int32_t)cells;
+  P.nrows = (int32_t)rows;
+  // Band of the FP32 candidate test.  The scan compares positions relative
+  // to the corner of a chunk's first candidate cell in x (at most L + 5 cs
+  // in size) and to a neighbouring cell corner in y, z; with six FP32
+  // roundings per difference each difference is off by at most
+  // dd = 2^-21 (L + 5 cs), and r^2 by 2 sqrt(3) r dd + 4 2^-24 r^2.  The
+  // band is twice that bound relative to rc^2, and at least 2e-5.
+  const double dd = std::ldexp(mmax, -21);
+  double rel = 2.0 * (2.0 * std::sqrt(3.0) * dd / P.cut_off + 4.0 * std::ldexp(1.0, -24));
+  rel = std::max(rel, 2e-5);
+  const double c2 = P.cut_off * P.cut_off;
+  P.cut2_lo_f32 = (float)(c2 * std::max(0.0, 1.0 - rel)) * (1.0f - 1e-6f);
+  P.cut2_hi_f32 = (float)(c2 * (1.0 + rel)) * (1.0f + 1e-6f);
+  return SDPD_OK;
+}
+
 template <typename T>
 int dalloc(sdpd_ctx* c, T** p, size_t count) {
   if (count == 0) count = 1;
@@ -225,7 +285,9 @@ void free_state(DeviceState& S) {
                   S.key_sorted, S.xa,     S.vr,        S.qr,      S.partials, S.stats_out,
                   S.nbr_count, S.recheck, S.nbr_hash, S.rkey,   S.rslot,    S.rcount,
                   S.rstart,   S.runsorted, S.rlpdx,  S.nlist,   S.nbr_sig_count,
-                  S.row_nchunk, S.row_chunk_off, S.chunk_row, S.scan_tmp, S.rec64};
+                  S.row_nchunk, S.row_chunk_off, S.chunk_row, S.scan_tmp, S.rec64,
+                  (void*)S.geo, S.rep,      S.rep_alt,  S.row_cell, S.row_len,  S.row_rep,
+                  S.rep_stats};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   std::memset(&S, 0, sizeof(S));
@@ -273,7 +335,27 @@ int alloc_state(sdpd_ctx* c, int n) {
     ALLOC(rec64, 8 * N);
   }
   {
-    const size_t nrows = (size_t)P.nm[1] * P.nm[2];
+    // replica tables (static for the context's lifetime)
+    sdpd::RepGeom* geo = nullptr;
+    if ((rc = dalloc(c, &geo, c->geo.size())) != SDPD_OK) return rc;
+    S.geo = geo;
+    HIP_TRY(c, hipMemcpy(geo, c->geo.data(), c->geo.size() * sizeof(sdpd::RepGeom),
+                         hipMemcpyHostToDevice));
+    ALLOC(rep, N);
+    ALLOC(rep_alt, N);
+    ALLOC(row_cell, c->row_cell.size());
+    ALLOC(row_len, c->row_len.size());
+    ALLOC(row_rep, c->row_rep.size());
+    HIP_TRY(c, hipMemcpy(S.row_cell, c->row_cell.data(), c->row_cell.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(S.row_len, c->row_len.data(), c->row_len.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(S.row_rep, c->row_rep.data(), c->row_rep.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
+    ALLOC(rep_stats, 4 * (size_t)P.nrep);
+  }
+  {
+    const size_t nrows = (size_t)P.nrows;
     const size_t nchunk = (N + 63) / 64 + nrows;
     const size_t ntile = ((size_t)std::max(P.ncell, (int32_t)nrows) + 4095) / 4096 + 1;
     ALLOC(row_nchunk, nrows);
@@ -288,8 +370,7 @@ int alloc_state(sdpd_ctx* c, int n) {
   ALLOC(nbr_sig_count, N);
   // neighbour lists: one block of list_cap x 64 entries per chunk of the
   // sort grid (at most N / 64 + rows chunks, rounded up to 8)
-  ALLOC(nlist, (size_t)P.list_cap * 64 *
-                   (((N + 63) / 64 + (size_t)P.nm[1] * P.nm[2] + 7) / 8 * 8));
+  ALLOC(nlist, (size_t)P.list_cap * 64 * (((N + 63) / 64 + (size_t)P.nrows + 7) / 8 * 8));
   ALLOC(recheck, N);
   ALLOC(nbr_hash, N);
 #undef ALLOC
@@ -302,6 +383,7 @@ void swap_sorted(DeviceState& S) {
   std::swap(S.m, S.m_alt);
   std::swap(S.pid, S.pid_alt);
   std::swap(S.perm, S.perm_alt);
+  std::swap(S.rep, S.rep_alt);
   std::swap(S.ct, S.ct_alt);
 }
 
@@ -362,6 +444,7 @@ int run_sort(sdpd_ctx* c, bool keys_done) {
   if (rc) return rc;
   swap_sorted(c->S);
   c->cells_valid = true;
+  c->sorted_once = true;
   c->rho_valid = false;
   c->f_valid = false;
   return SDPD_OK;
@@ -428,6 +511,11 @@ int sdpd_create(const sdpd_params* params, sdpd_ctx** out) {
     delete c;
     return rc;
   }
+  if ((rc = build_geometry(c)) != SDPD_OK) {
+    std::fprintf(stderr, "sdpd_create : %s\n", c->err.c_str());
+    delete c;
+    return rc;
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
     std::fprintf(stderr, "sdpd_create : no HIP device available\n");
@@ -458,22 +546,30 @@ int sdpd_destroy(sdpd_ctx* c) {
 
 const char* sdpd_last_error(const sdpd_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
-int sdpd_upload(sdpd_ctx* c, const double* x, const double* v, const double* m,
-                const int32_t* id, int32_t n) {
-  if (!c) return SDPD_ERR_ARG;
-  if (n <= 0 || !x || !v || !m || !id) return fail(c, SDPD_ERR_ARG, "sdpd_upload: bad arguments");
+}  // extern "C"
+
+namespace {
+
+int upload_impl(sdpd_ctx* c, const double* x, const double* v, const double* m,
+                const int32_t* id, const int32_t* rep, int32_t n, const char* who) {
+  const std::string w(who);
+  if (n <= 0 || !x || !v || !m || !id) return fail(c, SDPD_ERR_ARG, w + ": bad arguments");
   if (n >= (1 << 25))
-    return fail(c, SDPD_ERR_ARG, "sdpd_upload: at most 2^25 - 1 particles per context");
+    return fail(c, SDPD_ERR_ARG, w + ": at most 2^25 - 1 particles per context");
   const int dim = c->P.dim;
+  const int nrep = c->P.nrep;
   for (int32_t i = 0; i < n; ++i) {
     if (id[2 * (size_t)i + 1] != 0)
-      return fail(c, SDPD_ERR_ARG, "sdpd_upload: only fluid particles (sid = 0) are supported");
+      return fail(c, SDPD_ERR_ARG, w + ": only fluid particles (sid = 0) are supported");
     if (!(m[i] > kMachineZero))
-      return fail(c, SDPD_ERR_PHYSICS, "sdpd_upload: particle mass must be positive");
+      return fail(c, SDPD_ERR_PHYSICS, w + ": particle mass must be positive");
+    const int q = rep ? rep[i] : 0;
+    if (q < 0 || q >= nrep) return fail(c, SDPD_ERR_ARG, w + ": replica index out of range");
+    const sdpd::RepGeom& g = c->geo[q];
     for (int d = 0; d < dim; ++d) {
       double xv = x[(size_t)d + (size_t)dim * i];
-      if (!(xv >= c->P.min_phys[d] && xv < c->P.min_phys[d] + c->P.length[d]))
-        return fail(c, SDPD_ERR_ARG, "sdpd_upload: particle outside [min_phys, max_phys)");
+      if (!(xv >= g.lo[d] && xv < g.lo[d] + g.L[d]))
+        return fail(c, SDPD_ERR_ARG, w + ": particle outside its box [min, max)");
     }
   }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -484,8 +580,12 @@ int sdpd_upload(sdpd_ctx* c, const double* x, const double* v, const double* m,
     // capacity of the stored neighbour lists: 1.6 times the neighbours a
     // uniform fluid of this mean density has inside the cut-off, plus 32;
     // a particle with more is handled by scanning again in the force pass
-    double vol = 1.0;
-    for (int d = 0; d < dim; ++d) vol *= c->P.length[d];
+    double vol = 0.0;
+    for (const sdpd::RepGeom& g : c->geo) {
+      double vq = 1.0;
+      for (int d = 0; d < dim; ++d) vq *= g.L[d];
+      vol += vq;
+    }
     const double rc_ = c->P.cut_off;
     const double ball = dim == 3 ? 4.0 / 3.0 * kPi * rc_ * rc_ * rc_ : kPi * rc_ * rc_;
     const double expect = (double)n / vol * ball;
@@ -513,6 +613,10 @@ int sdpd_upload(sdpd_ctx* c, const double* x, const double* v, const double* m,
   HIP_TRY(c, hipMemcpy(c->S.m, m, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->S.pid, pid.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->S.perm, perm.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
+  if (rep)
+    HIP_TRY(c, hipMemcpy(c->S.rep, rep, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
+  else
+    sdpd::launch_fill_i32(c->stream, c->S.rep, 0, n);
   sdpd::launch_fill_f64(c->stream, c->S.f, 0.0, (int64_t)dim * n);
   sdpd::launch_fill_f64(c->stream, c->S.rho, 0.0, n);
   sdpd::launch_fill_f64(c->stream, c->S.p, 0.0, n);
@@ -531,6 +635,69 @@ int sdpd_upload(sdpd_ctx* c, const double* x, const double* v, const double* m,
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   c->n = n;
   c->cells_valid = c->rho_valid = c->f_valid = false;
+  c->sorted_once = false;
+  return SDPD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sdpd_upload(sdpd_ctx* c, const double* x, const double* v, const double* m,
+                const int32_t* id, int32_t n) {
+  if (!c) return SDPD_ERR_ARG;
+  if (c->P.nrep != 1 || !c->rep_lo.empty())
+    return fail(c, SDPD_ERR_STATE, "sdpd_upload: batched context, use sdpd_upload_replicas");
+  return upload_impl(c, x, v, m, id, nullptr, n, "sdpd_upload");
+}
+
+int sdpd_set_replicas(sdpd_ctx* c, int32_t nrep, const double* box_lo, const double* box_hi) {
+  if (!c) return SDPD_ERR_ARG;
+  if (nrep < 1 || !box_lo || !box_hi)
+    return fail(c, SDPD_ERR_ARG, "sdpd_set_replicas: bad arguments");
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  free_state(c->S);
+  c->n = 0;
+  c->cells_valid = c->rho_valid = c->f_valid = false;
+  c->rep_lo.assign(box_lo, box_lo + 3 * (size_t)nrep);
+  c->rep_hi.assign(box_hi, box_hi + 3 * (size_t)nrep);
+  int rc = build_geometry(c);
+  if (rc) {
+    c->rep_lo.clear();
+    c->rep_hi.clear();
+    (void)build_geometry(c);
+  }
+  return rc;
+}
+
+int sdpd_upload_replicas(sdpd_ctx* c, const double* x, const double* v, const double* m,
+                         const int32_t* id, const int32_t* rep, int32_t n) {
+  if (!c) return SDPD_ERR_ARG;
+  if (!rep) return fail(c, SDPD_ERR_ARG, "sdpd_upload_replicas: null replica array");
+  return upload_impl(c, x, v, m, id, rep, n, "sdpd_upload_replicas");
+}
+
+int sdpd_stats_replicas(sdpd_ctx* c, double* mom, double* ekin) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  if (!mom || !ekin) return fail(c, SDPD_ERR_ARG, "sdpd_stats_replicas: null output");
+  // the replicas' particle ranges come from the last sort
+  if (!c->sorted_once && (rc = run_sort(c, false))) return rc;
+  {
+    PhaseMark pm(c, 4);
+    sdpd::launch_replica_stats(c->stream, c->P, c->S);
+  }
+  rc = check_launch(c, "replica stats");
+  if (rc) return rc;
+  const int nrep = c->P.nrep;
+  std::vector<double> out(4 * (size_t)nrep);
+  HIP_TRY(c, hipMemcpyAsync(out.data(), c->S.rep_stats, out.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (int q = 0; q < nrep; ++q) {
+    ekin[q] = out[4 * (size_t)q];
+    for (int d = 0; d < 3; ++d) mom[3 * (size_t)q + d] = d < c->P.dim ? out[4 * (size_t)q + 1 + d] : 0.0;
+  }
   return SDPD_OK;
 }
 
@@ -701,6 +868,8 @@ int sdpd_get_step(sdpd_ctx* c, int64_t* step) {
 int sdpd_get_cells(sdpd_ctx* c, int32_t* nm, int32_t* lhbx, int32_t* lpdx) {
   int rc = need_particles(c);
   if (rc) return rc;
+  if (c->P.nrep != 1 || !c->rep_lo.empty())
+    return fail(c, SDPD_ERR_STATE, "sdpd_get_cells: one box only (batched context)");
   if (nm)
     for (int d = 0; d < 3; ++d) nm[d] = c->P.nm_ref[d];
   if (!lhbx && !lpdx) return SDPD_OK;

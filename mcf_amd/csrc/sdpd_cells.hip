@@ -33,35 +33,15 @@ __global__ void k_fill_f64(double* __restrict__ a, double v, int64_t n) {
   for (; i < n; i += stride) a[i] = v;
 }
 
-__device__ __forceinline__ int cell_coord(double xv, double x0, double rd, double xmax, int nm) {
-  int k = (int)floor((xv - x0) * rd);
-  // ppm_util_rank3d.F90:433-459 with no lower ghost layer
-  if (xv >= xmax && k < nm) k = nm;
-  if (xv < xmax && k >= nm) k = nm - 1;
-  // particles are wrapped into [min, max) before every sort; clamp so a
-  // stray value can never address outside the cell arrays
-  if (k < 0) k = 0;
-  if (k >= nm) k = nm - 1;
-  return k;
-}
-
-__device__ __forceinline__ int32_t cell_key_of(const DevParams& P, double x0, double x1,
-                                               double x2) {
-  int cx = cell_coord(x0, P.min_phys[0], P.rd[0], P.min_phys[0] + P.length[0], P.nm[0]);
-  int cy = cell_coord(x1, P.min_phys[1], P.rd[1], P.min_phys[1] + P.length[1], P.nm[1]);
-  int cz = P.dim == 3
-               ? cell_coord(x2, P.min_phys[2], P.rd[2], P.min_phys[2] + P.length[2], P.nm[2])
-               : 0;
-  return cx + P.nm[0] * (cy + P.nm[1] * cz);
-}
-
-__global__ void k_cell_key(DevParams P, const double* __restrict__ x, int32_t* __restrict__ key,
-                           int32_t* __restrict__ slot, int32_t* __restrict__ count) {
+__global__ void k_cell_key(DevParams P, const double* __restrict__ x,
+                           const int32_t* __restrict__ rep, const RepGeom* __restrict__ geo,
+                           int32_t* __restrict__ key, int32_t* __restrict__ slot,
+                           int32_t* __restrict__ count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
-  const int n = P.n;
-  double x2 = P.dim == 3 ? x[2 * (int64_t)n + i] : 0.0;
-  int32_t k = cell_key_of(P, x[i], x[(int64_t)n + i], x2);
+  const int64_t n = P.n;
+  const RepGeom& g = geo[rep[i]];
+  int32_t k = cell_key_geo(g, P.dim, x[i], x[n + i], P.dim == 3 ? x[2 * n + i] : 0.0);
   key[i] = k;
   slot[i] = atomicAdd(&count[k], 1);
 }
@@ -166,19 +146,20 @@ void launch_exclusive_scan(hipStream_t s, const int32_t* count, int32_t* start, 
 // into chunks of at most 64 consecutive particles; chunk c belongs to row
 // chunk_row[c] and is the (c - row_chunk_off[row])-th chunk of that row.
 __global__ void k_row_chunks(DevParams P, const int32_t* __restrict__ start,
+                             const int32_t* __restrict__ row_cell,
+                             const int32_t* __restrict__ row_len,
                              int32_t* __restrict__ row_nchunk) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nrows = P.nm[1] * P.nm[2];
-  if (r >= nrows) return;
-  int cnt = start[(r + 1) * P.nm[0]] - start[r * P.nm[0]];
+  if (r >= P.nrows) return;
+  const int c0 = row_cell[r];
+  int cnt = start[c0 + row_len[r]] - start[c0];
   row_nchunk[r] = (cnt + 63) / 64;
 }
 
 __global__ void k_chunk_map(DevParams P, const int32_t* __restrict__ row_chunk_off,
                             int32_t* __restrict__ chunk_row) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nrows = P.nm[1] * P.nm[2];
-  if (r >= nrows) return;
+  if (r >= P.nrows) return;
   for (int c = row_chunk_off[r]; c < row_chunk_off[r + 1]; ++c) chunk_row[c] = r;
 }
 
@@ -194,11 +175,12 @@ __global__ void k_place(DevParams P, const int32_t* __restrict__ key,
                         const int32_t* __restrict__ start, const int32_t* __restrict__ unsorted,
                         const double* __restrict__ x, const double* __restrict__ v,
                         const double* __restrict__ m, const int32_t* __restrict__ pid,
-                        const int32_t* __restrict__ perm, const double* __restrict__ ct,
+                        const int32_t* __restrict__ perm, const int32_t* __restrict__ rep,
+                        const double* __restrict__ ct, const RepGeom* __restrict__ geo,
                         double* __restrict__ x2, double* __restrict__ v2, double* __restrict__ m2,
                         int32_t* __restrict__ pid2, int32_t* __restrict__ perm2,
-                        double* __restrict__ ct2, int32_t* __restrict__ key2,
-                        float4* __restrict__ xa) {
+                        int32_t* __restrict__ rep2, double* __restrict__ ct2,
+                        int32_t* __restrict__ key2, float4* __restrict__ xa) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = P.n;
   if (i >= n) return;
@@ -215,6 +197,8 @@ __global__ void k_place(DevParams P, const int32_t* __restrict__ key,
   m2[p] = m[i];
   pid2[p] = pid[i];
   perm2[p] = perm[i];
+  const int q = rep[i];
+  rep2[p] = q;
   key2[p] = k;
   if (ct) {
     const int d2 = dim * dim;
@@ -222,12 +206,14 @@ __global__ void k_place(DevParams P, const int32_t* __restrict__ key,
   }
   // FP32 position relative to the lower corner of the own cell, the cell's
   // x index in w (records of the neighbour scan)
-  const int cx = k % P.nm[0];
-  const int cy = (k / P.nm[0]) % P.nm[1];
-  const int cz = k / (P.nm[0] * P.nm[1]);
-  xa[p] = make_float4((float)(x[i] - (P.min_phys[0] + (double)cx * P.cs[0])),
-                      (float)(x[(int64_t)n + i] - (P.min_phys[1] + (double)cy * P.cs[1])),
-                      dim == 3 ? (float)(x[2 * (int64_t)n + i] - (P.min_phys[2] + (double)cz * P.cs[2]))
+  const RepGeom& g = geo[q];
+  const int kl = k - g.cell_base;
+  const int cx = kl % g.nm[0];
+  const int cy = (kl / g.nm[0]) % g.nm[1];
+  const int cz = kl / (g.nm[0] * g.nm[1]);
+  xa[p] = make_float4((float)(x[i] - (g.lo[0] + (double)cx * g.cs[0])),
+                      (float)(x[(int64_t)n + i] - (g.lo[1] + (double)cy * g.cs[1])),
+                      dim == 3 ? (float)(x[2 * (int64_t)n + i] - (g.lo[2] + (double)cz * g.cs[2]))
                                : 0.f,
                       __int_as_float(cx));
 }
@@ -287,8 +273,8 @@ void launch_fill_f64(hipStream_t s, double* a, double v, int64_t n) {
 
 void launch_cell_keys(hipStream_t s, const DevParams& P, const DeviceState& S) {
   launch_fill_i32(s, S.count, 0, P.ncell);
-  hipLaunchKernelGGL(k_cell_key, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, P, S.x, S.key,
-                     S.slot, S.count);
+  hipLaunchKernelGGL(k_cell_key, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, P, S.x, S.rep,
+                     S.geo, S.key, S.slot, S.count);
 }
 
 void launch_cell_sort_after_keys(hipStream_t s, const DevParams& P, const DeviceState& S) {
@@ -297,12 +283,12 @@ void launch_cell_sort_after_keys(hipStream_t s, const DevParams& P, const Device
   hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, S.key, S.slot,
                      S.start, S.unsorted);
   hipLaunchKernelGGL(k_place, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, S.key, S.start,
-                     S.unsorted, S.x, S.v, S.m, S.pid, S.perm, P.newtonian ? nullptr : S.ct,
-                     S.x_alt, S.v_alt, S.m_alt, S.pid_alt, S.perm_alt,
-                     P.newtonian ? nullptr : S.ct_alt, S.key_sorted, S.xa);
-  const int nrows = P.nm[1] * P.nm[2];
+                     S.unsorted, S.x, S.v, S.m, S.pid, S.perm, S.rep,
+                     P.newtonian ? nullptr : S.ct, S.geo, S.x_alt, S.v_alt, S.m_alt, S.pid_alt,
+                     S.perm_alt, S.rep_alt, P.newtonian ? nullptr : S.ct_alt, S.key_sorted, S.xa);
+  const int nrows = P.nrows;
   hipLaunchKernelGGL(k_row_chunks, dim3(blocks_for(nrows, 256)), dim3(256), 0, s, P, S.start,
-                     S.row_nchunk);
+                     S.row_cell, S.row_len, S.row_nchunk);
   launch_exclusive_scan(s, S.row_nchunk, S.row_chunk_off, nrows, S.scan_tmp);
   hipLaunchKernelGGL(k_chunk_map, dim3(blocks_for(nrows, 256)), dim3(256), 0, s, P,
                      S.row_chunk_off, S.chunk_row);

@@ -21,8 +21,9 @@ struct DevParams {
   int32_t n;              // particles
   // sort / gather grid: half of PPM's cell edge, so the stencil is 5 cells
   // wide per dimension (at least 6 cells per dimension)
+  // (single box: nm, length, ... below; batches: RepGeom per replica)
   int32_t nm[3];
-  int32_t ncell;
+  int32_t ncell;          // cells of all replicas
   // PPM's own grid, cell edge >= 1.0001 cut_off (at least 3 per dimension)
   int32_t nm_ref[3];
   int32_t ncell_ref;
@@ -41,6 +42,8 @@ struct DevParams {
   int64_t step;
   int32_t mixed;          // FP32 pair arithmetic
   int32_t list_cap;       // stored neighbour entries per particle
+  int32_t nrep;           // boxes (replicas) in the context
+  int32_t nrows;          // rows of the sort grids of all replicas
   // band around cut2 of the FP32 candidate test; inside it the decision is
   // taken in FP64.  FP32 positions are relative to the own cell corner.
   float cut2_lo_f32, cut2_hi_f32;
@@ -50,6 +53,46 @@ struct DevParams {
   float inv_hf, coeff, coef_gradf;
   float etaf, eta_coeff, ktf, inv_sqrt_dtf, sqrt_2kt_etaf;
 };
+
+// Geometry of one periodic box of a context (a context holds one box or a
+// batch of independent replicas of different sizes sharing the physics).
+// Cells of all replicas are numbered globally: replica q owns cells
+// [cell_base, cell_base + nm0 nm1 nm2) and rows (lines of cells along x)
+// [row_base, row_base + nm1 nm2).
+struct RepGeom {
+  int32_t nm[3];          // sort-grid cells per dimension
+  int32_t cell_base;
+  int32_t row_base;
+  int32_t pad;
+  double lo[3];           // lower corner
+  double L[3];            // edge lengths
+  double rd[3];           // nm / L
+  double cs[3];           // cell edges
+  float csf[3];
+  float pad2;
+};
+
+// cell index along one dimension: PPM's FLOOR((x - x0) * rdx) with its
+// upper-face corrections (ppm_util_rank3d.F90:433-459), clamped
+__device__ __forceinline__ int cell_coord(double xv, double x0, double rd, double xmax, int nm) {
+  int k = (int)floor((xv - x0) * rd);
+  if (xv >= xmax && k < nm) k = nm;
+  if (xv < xmax && k >= nm) k = nm - 1;
+  // particles are wrapped into [lo, lo + L) before every sort; clamp so a
+  // stray value can never address outside the cell arrays
+  if (k < 0) k = 0;
+  if (k >= nm) k = nm - 1;
+  return k;
+}
+
+// global cell key of a position in replica geometry g
+__device__ __forceinline__ int32_t cell_key_geo(const RepGeom& g, int dim, double x0, double x1,
+                                                double x2) {
+  const int cx = cell_coord(x0, g.lo[0], g.rd[0], g.lo[0] + g.L[0], g.nm[0]);
+  const int cy = cell_coord(x1, g.lo[1], g.rd[1], g.lo[1] + g.L[1], g.nm[1]);
+  const int cz = dim == 3 ? cell_coord(x2, g.lo[2], g.rd[2], g.lo[2] + g.L[2], g.nm[2]) : 0;
+  return g.cell_base + cx + g.nm[0] * (cy + g.nm[1] * cz);
+}
 
 // ---------------------------------------------------------------------
 // smoothing kernel (kernel_kernel_quintic_spline.F90:160-215,

@@ -37,10 +37,17 @@ struct DeviceState {
   double *stats_out;     // 8 doubles
   int32_t *nbr_count;    // per-particle in-cutoff count of the last density pass
   uint32_t *nlist;       // in-cutoff neighbour lists of the last density pass,
-                         // slot-major: entry q of particle i at [q * n + i]
+                         // one block of groups of 8 entries per chunk
   int32_t *nbr_sig_count;  // neighbour counts of the last signature pass
   int32_t *recheck;      // per-particle FP64 re-decisions (mixed mode)
   uint32_t *nbr_hash;    // per-particle sum of pair hashes (checking)
+  // replicas: geometry table, replica of each particle (cell order, with
+  // its ping-pong copy), and per row of the sort grids its first cell,
+  // its length (cells) and its replica
+  const RepGeom* geo;
+  int32_t *rep, *rep_alt;
+  int32_t *row_cell, *row_len, *row_rep;
+  double* rep_stats;     // per replica (E_k, P_x, P_y, P_z)
   // reference-grid cell list (built on request by sdpd_get_cells)
   int32_t *rkey, *rslot, *rcount, *rstart, *runsorted, *rlpdx;
 };
@@ -70,6 +77,7 @@ void launch_conformation(hipStream_t s, const DevParams& P, const DeviceState& S
 void launch_boundary(hipStream_t s, const DevParams& P, const DeviceState& S, int closing,
                      int opening);
 void launch_stats(hipStream_t s, const DevParams& P, const DeviceState& S, int nblocks);
+void launch_replica_stats(hipStream_t s, const DevParams& P, const DeviceState& S);
 void launch_pair_normals(hipStream_t s, const uint32_t* lo, const uint32_t* hi, int count,
                          uint64_t seed, int64_t step, float* g);
 

@@ -59,14 +59,23 @@ __device__ __forceinline__ Offsets decode_code(uint32_t code) {
   return o;
 }
 
+// Geometry of the replica a chunk belongs to (uniform across the wave).
+struct Geo {
+  int nm0, nm1, nm2;    // sort-grid cells per dimension
+  int cell_base, row_base;
+  double L0, L1, L2;    // edge lengths
+  float cs0, cs1, cs2;  // FP32 cell edges
+};
+
 // Per-lane context of the home particle.
 struct Home {
   int i;                // own index (valid for active lanes)
   int ii;               // a safe index for every lane
   bool active;
-  int cx, cy, cz;       // own cell
+  int cx, cy, cz;       // own cell (replica-local)
   float4 a;             // FP32 position relative to the own cell corner
   double x0, x1, x2;    // FP64 position
+  Geo G;
 };
 
 // Image modes of a neighbour at offsets o from the home cell.
@@ -79,9 +88,9 @@ __device__ __forceinline__ double exact_d2(const DevParams& P, const double* __r
                                            const Home& h, const Modes& m, double& d0, double& d1,
                                            double& d2) {
   const int64_t n = P.n;
-  d0 = image_diff(h.x0, x[j], P.length[0], m.m0);
-  d1 = image_diff(h.x1, x[n + j], P.length[1], m.m1);
-  d2 = DIM == 3 ? image_diff(h.x2, x[2 * n + j], P.length[2], m.m2) : 0.0;
+  d0 = image_diff(h.x0, x[j], h.G.L0, m.m0);
+  d1 = image_diff(h.x1, x[n + j], h.G.L1, m.m1);
+  d2 = DIM == 3 ? image_diff(h.x2, x[2 * n + j], h.G.L2, m.m2) : 0.0;
   return DIM == 3 ? dist2_exact(d0, d1, d2) : dist2_exact2(d0, d1);
 }
 
@@ -89,18 +98,19 @@ __device__ __forceinline__ double exact_d2(const DevParams& P, const double* __r
 template <int DIM>
 __device__ __forceinline__ void rel_diff(const DevParams& P, const Home& h, const float4& q,
                                          const Offsets& o, float& d0, float& d1, float& d2) {
-  d0 = h.a.x - (q.x + (float)o.ox * P.csf[0]);
-  d1 = h.a.y - (q.y + (float)o.oy * P.csf[1]);
-  d2 = DIM == 3 ? h.a.z - (q.z + (float)o.oz * P.csf[2]) : 0.f;
+  d0 = h.a.x - (q.x + (float)o.ox * h.G.cs0);
+  d1 = h.a.y - (q.y + (float)o.oy * h.G.cs1);
+  d2 = DIM == 3 ? h.a.z - (q.z + (float)o.oz * h.G.cs2) : 0.f;
 }
 
 // The chunk of the sort grid this wavefront works on; false when the wave
 // has none (the launch covers an upper bound of chunks).
 struct Chunk {
   int c;                // chunk index
-  int row;              // cy + nm1 * cz
+  int row;              // global row; replica-local row = cy + nm1 * cz
   int p0, p1;           // particles [p0, p1)
   int xmin, xmax;       // x range of the home cells
+  Geo G;
 };
 
 // Workgroups are handed to the eight XCDs round-robin; chunk c of XCD
@@ -108,8 +118,7 @@ struct Chunk {
 // XCD's L2 serves a compact slab of space.  The grid is a multiple of 8
 // and at least the number of chunks.
 __device__ __forceinline__ bool load_chunk(const DevParams& P, const DeviceState& S, Chunk& ch) {
-  const int nrows = P.nm[1] * P.nm[2];
-  const int total = S.row_chunk_off[nrows];
+  const int total = S.row_chunk_off[P.nrows];
   const int per = (total + 7) >> 3;
   const int b = blockIdx.x;
   const int c = (b & 7) * per + (b >> 3);
@@ -117,12 +126,25 @@ __device__ __forceinline__ bool load_chunk(const DevParams& P, const DeviceState
   ch.c = c;
   ch.row = S.chunk_row[c];
   const int k = c - S.row_chunk_off[ch.row];
-  const int rs = S.start[ch.row * P.nm[0]];
-  const int re = S.start[(ch.row + 1) * P.nm[0]];
+  const RepGeom& g = S.geo[S.row_rep[ch.row]];
+  ch.G.nm0 = g.nm[0];
+  ch.G.nm1 = g.nm[1];
+  ch.G.nm2 = g.nm[2];
+  ch.G.cell_base = g.cell_base;
+  ch.G.row_base = g.row_base;
+  ch.G.L0 = g.L[0];
+  ch.G.L1 = g.L[1];
+  ch.G.L2 = g.L[2];
+  ch.G.cs0 = g.csf[0];
+  ch.G.cs1 = g.csf[1];
+  ch.G.cs2 = g.csf[2];
+  const int c0 = S.row_cell[ch.row];
+  const int rs = S.start[c0];
+  const int re = S.start[c0 + ch.G.nm0];
   ch.p0 = rs + kWave * k;
   ch.p1 = min(ch.p0 + kWave, re);
-  ch.xmin = S.key_sorted[ch.p0] % P.nm[0];
-  ch.xmax = S.key_sorted[ch.p1 - 1] % P.nm[0];
+  ch.xmin = (S.key_sorted[ch.p0] - ch.G.cell_base) % ch.G.nm0;
+  ch.xmax = (S.key_sorted[ch.p1 - 1] - ch.G.cell_base) % ch.G.nm0;
   return true;
 }
 
@@ -144,9 +166,11 @@ __device__ __forceinline__ Home load_home(const DevParams& P, const DeviceState&
   h.active = h.i < ch.p1;
   h.ii = h.active ? h.i : ch.p0;
   const int64_t n = P.n;
-  h.cx = S.key_sorted[h.ii] % P.nm[0];
-  h.cy = ch.row % P.nm[1];
-  h.cz = ch.row / P.nm[1];
+  h.G = ch.G;
+  h.cx = (S.key_sorted[h.ii] - ch.G.cell_base) % ch.G.nm0;
+  const int lr = ch.row - ch.G.row_base;
+  h.cy = lr % ch.G.nm1;
+  h.cz = lr / ch.G.nm1;
   h.a = S.xa[h.ii];
   h.x0 = S.x[h.ii];
   h.x1 = S.x[n + h.ii];
@@ -174,13 +198,13 @@ template <int DIM>
 __device__ __forceinline__ RowPieces row_pieces(const DevParams& P, const DeviceState& S,
                                                 const Chunk& ch, const Home& h, int row) {
   RowPieces rp;
-  const int nm0 = P.nm[0];
+  const int nm0 = h.G.nm0;
   const int32_t* __restrict__ start = S.start;
   const int dy = row % 5 - 2, dz = DIM == 3 ? row / 5 - 2 : 0;
   rp.mz = 0;
-  const int ny = wrap_cell(h.cy + dy, P.nm[1], P.symmetry, rp.my);
-  const int nz = DIM == 3 ? wrap_cell(h.cz + dz, P.nm[2], P.symmetry, rp.mz) : 0;
-  rp.rb = nm0 * (ny + P.nm[1] * nz);
+  const int ny = wrap_cell(h.cy + dy, h.G.nm1, P.symmetry, rp.my);
+  const int nz = DIM == 3 ? wrap_cell(h.cz + dz, h.G.nm2, P.symmetry, rp.mz) : 0;
+  rp.rb = h.G.cell_base + nm0 * (ny + h.G.nm1 * nz);
   const int v_lo = ch.xmin - 2, v_hi = ch.xmax + 2;
   const int ml = v_lo < 0 ? 0 : v_lo, mh = v_hi > nm0 - 1 ? nm0 - 1 : v_hi;
   rp.g1 = start[rp.rb + ml];
@@ -210,7 +234,7 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
                                            float4* __restrict__ tile, uint32_t* __restrict__ buf,
                                            int& rechecks, uint32_t* __restrict__ list) {
   const int lane = threadIdx.x;
-  const int nm0 = P.nm[0];
+  const int nm0 = h.G.nm0;
   const int32_t* __restrict__ start = S.start;
   const int low_mode = P.symmetry ? 2 : 3;
   int cnt = 0, stored = 0;
@@ -258,7 +282,7 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
   // the list entry with the virtual cell (mod 128) in place of the code,
   // so that entry + kadd is the entry with the stencil code.
   const int crel = h.cx - v_lo;
-  const float ax = h.a.x + (float)crel * P.csf[0];
+  const float ax = h.a.x + (float)crel * h.G.cs0;
   const uint32_t self_w = (uint32_t)h.ii | ((uint32_t)crel << kCodeShift);
 #pragma unroll 1
   for (int row = 0; row < nrow; ++row) {
@@ -279,8 +303,8 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
       tb = seq_of(h.cx - 2);
       te = seq_of(h.cx + 3);
     }
-    const float ay = h.a.y - (float)dy * P.csf[1];
-    const float az = DIM == 3 ? h.a.z - (float)dz * P.csf[2] : 0.f;
+    const float ay = h.a.y - (float)dy * h.G.cs1;
+    const float az = DIM == 3 ? h.a.z - (float)dz * h.G.cs2 : 0.f;
     const uint32_t kadd = (uint32_t)(2 - crel + 5 * (dy + 2) + 25 * (dz + 2)) << kCodeShift;
     // the row in batches of at most kTile records (one batch in practice)
     for (int base = 0; base < tot; base += kTile) {
@@ -290,7 +314,7 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
         const int g = seq_particle(rp, t);
         const float4 q = S.xa[g];
         const int vrel = __float_as_int(q.w) + (piece == 0 ? -nm0 : (piece == 2 ? nm0 : 0)) - v_lo;
-        tile[t - base] = make_float4(fmaf((float)vrel, P.csf[0], q.x), q.y, q.z,
+        tile[t - base] = make_float4(fmaf((float)vrel, h.G.cs0, q.x), q.y, q.z,
                                      __uint_as_float((uint32_t)g | ((uint32_t)vrel << kCodeShift)));
       }
       __syncthreads();
@@ -361,9 +385,9 @@ template <int DIM>
 __device__ __forceinline__ double pair_d2(const DevParams& P, const Home& h, const Offsets& o,
                                           double xj0, double xj1, double xj2, double& d0,
                                           double& d1, double& d2) {
-  d0 = (h.x0 - xj0) + image_shift(h.cx + o.ox, P.nm[0], P.length[0]);
-  d1 = (h.x1 - xj1) + image_shift(h.cy + o.oy, P.nm[1], P.length[1]);
-  d2 = DIM == 3 ? (h.x2 - xj2) + image_shift(h.cz + o.oz, P.nm[2], P.length[2]) : 0.0;
+  d0 = (h.x0 - xj0) + image_shift(h.cx + o.ox, h.G.nm0, h.G.L0);
+  d1 = (h.x1 - xj1) + image_shift(h.cy + o.oy, h.G.nm1, h.G.L1);
+  d2 = DIM == 3 ? (h.x2 - xj2) + image_shift(h.cz + o.oz, h.G.nm2, h.G.L2) : 0.0;
   return DIM == 3 ? fma(d2, d2, fma(d1, d1, d0 * d0)) : fma(d1, d1, d0 * d0);
 }
 
@@ -831,7 +855,7 @@ __global__ __launch_bounds__(kWave) void k_signature(DevParams P, DeviceState S)
 // bound of chunks (one partial chunk per row plus n / 64 full ones)
 // ---------------------------------------------------------------------
 static inline int nchunks_max(const DevParams& P) {
-  const int m = (P.n + kWave - 1) / kWave + P.nm[1] * P.nm[2];
+  const int m = (P.n + kWave - 1) / kWave + P.nrows;
   return (m + 7) / 8 * 8;
 }
 

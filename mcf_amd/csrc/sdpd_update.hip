@@ -30,9 +30,10 @@ __global__ void k_drift_wrap(DevParams P, DeviceState S) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const int64_t n = P.n;
+  const RepGeom& g = S.geo[S.rep[i]];
   for (int d = 0; d < P.dim; ++d) {
     double xv = S.x[d * n + i] + S.v[d * n + i] * P.dt;
-    S.x[d * n + i] = wrap_coord(xv, P.min_phys[d], P.length[d]);
+    S.x[d * n + i] = wrap_coord(xv, g.lo[d], g.L[d]);
   }
 }
 
@@ -45,15 +46,6 @@ __global__ void k_conformation(DevParams P, DeviceState S, double lambda) {
     S.ct[q * n + i] = S.ct[q * n + i] + S.act[q * n + i] * P.dt * lambda;
 }
 
-__device__ __forceinline__ int cell_coord_u(double xv, double x0, double rd, double xmax, int nm) {
-  int k = (int)floor((xv - x0) * rd);
-  if (xv >= xmax && k < nm) k = nm;
-  if (xv < xmax && k >= nm) k = nm - 1;
-  if (k < 0) k = 0;
-  if (k >= nm) k = nm - 1;
-  return k;
-}
-
 // Step boundary: the closing half kick of the previous step, then the
 // opening half kick, drift, Oldroyd-B half update and periodic wrap of the
 // next one, then the cell key and cell histogram of the next sort.
@@ -64,6 +56,7 @@ __global__ __launch_bounds__(256) void k_boundary(DevParams P, DeviceState S, in
   if (i >= P.n) return;
   const int64_t n = P.n;
   const int dim = P.dim;
+  const RepGeom& g = S.geo[S.rep[i]];
   double xv[3] = {0.0, 0.0, 0.0};
   for (int d = 0; d < dim; ++d) {
     double vv = S.v[d * n + i];
@@ -72,7 +65,7 @@ __global__ __launch_bounds__(256) void k_boundary(DevParams P, DeviceState S, in
     if (opening) vv = vv + 0.5 * ff * P.dt;
     S.v[d * n + i] = vv;
     double x = S.x[d * n + i];
-    if (opening) x = wrap_coord(x + vv * P.dt, P.min_phys[d], P.length[d]);
+    if (opening) x = wrap_coord(x + vv * P.dt, g.lo[d], g.L[d]);
     S.x[d * n + i] = x;
     xv[d] = x;
   }
@@ -86,12 +79,7 @@ __global__ __launch_bounds__(256) void k_boundary(DevParams P, DeviceState S, in
       S.ct[q * n + i] = c;
     }
   }
-  int cx = cell_coord_u(xv[0], P.min_phys[0], P.rd[0], P.min_phys[0] + P.length[0], P.nm[0]);
-  int cy = cell_coord_u(xv[1], P.min_phys[1], P.rd[1], P.min_phys[1] + P.length[1], P.nm[1]);
-  int cz = dim == 3
-               ? cell_coord_u(xv[2], P.min_phys[2], P.rd[2], P.min_phys[2] + P.length[2], P.nm[2])
-               : 0;
-  int32_t k = cx + P.nm[0] * (cy + P.nm[1] * cz);
+  int32_t k = cell_key_geo(g, dim, xv[0], xv[1], xv[2]);
   S.key[i] = k;
   S.slot[i] = atomicAdd(&S.count[k], 1);
 }
@@ -143,6 +131,36 @@ __global__ void k_stats_final(int nblocks, DeviceState S) {
   }
 }
 
+// Per-replica (E_k, P) with one workgroup per replica over its particles,
+// which are contiguous in cell order: [start[cell_base_q], start[cell_base_q+1]).
+__global__ __launch_bounds__(256) void k_replica_stats(DevParams P, DeviceState S) {
+  __shared__ double sh[4][256];
+  const int q = blockIdx.x;
+  const int64_t n = P.n;
+  const int dim = P.dim;
+  const int c0 = S.geo[q].cell_base;
+  const int c1 = q + 1 < P.nrep ? S.geo[q + 1].cell_base : P.ncell;
+  const int p0 = S.start[c0], p1 = S.start[c1];
+  KSum acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  for (int i = p0 + threadIdx.x; i < p1; i += 256) {
+    double mi = S.m[i];
+    double v2 = 0.0;
+    for (int d = 0; d < dim; ++d) {
+      double vv = S.v[d * n + i];
+      v2 = v2 + vv * vv;
+      kadd(acc[1 + d], mi * vv);
+    }
+    kadd(acc[0], 0.5 * mi * v2);
+  }
+  for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] = acc[k].s + acc[k].c;
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    KSum t = {0, 0};
+    for (int k = 0; k < 256; ++k) kadd(t, sh[threadIdx.x][k]);
+    S.rep_stats[(int64_t)q * 4 + threadIdx.x] = t.s + t.c;
+  }
+}
+
 __global__ void k_pair_normals(const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
                                int count, uint64_t seed, int64_t step, float* __restrict__ g) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -181,6 +199,10 @@ void launch_boundary(hipStream_t s, const DevParams& P, const DeviceState& S, in
 void launch_stats(hipStream_t s, const DevParams& P, const DeviceState& S, int nblocks) {
   hipLaunchKernelGGL(k_stats_partial, dim3(nblocks), dim3(256), 0, s, P, S);
   hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, s, nblocks, S);
+}
+
+void launch_replica_stats(hipStream_t s, const DevParams& P, const DeviceState& S) {
+  hipLaunchKernelGGL(k_replica_stats, dim3(P.nrep), dim3(256), 0, s, P, S);
 }
 
 void launch_pair_normals(hipStream_t s, const uint32_t* lo, const uint32_t* hi, int count,

@@ -66,6 +66,31 @@ MODULE sdpd_c
        INTEGER(c_int32_t), VALUE        :: n
      END FUNCTION sdpd_upload
 
+     INTEGER(c_int) FUNCTION sdpd_set_replicas(ctx, nrep, box_lo, box_hi) &
+          BIND(C, name="sdpd_set_replicas")
+       IMPORT :: c_int, c_ptr, c_double, c_int32_t
+       TYPE(c_ptr), VALUE         :: ctx
+       INTEGER(c_int32_t), VALUE  :: nrep
+       REAL(c_double), INTENT(IN) :: box_lo(3, *), box_hi(3, *)
+     END FUNCTION sdpd_set_replicas
+
+     INTEGER(c_int) FUNCTION sdpd_upload_replicas(ctx, x, v, m, id, rep, n) &
+          BIND(C, name="sdpd_upload_replicas")
+       IMPORT :: c_int, c_ptr, c_double, c_int32_t
+       TYPE(c_ptr), VALUE               :: ctx
+       REAL(c_double), INTENT(IN)       :: x(*), v(*), m(*)
+       INTEGER(c_int32_t), INTENT(IN)   :: id(*), rep(*)
+       INTEGER(c_int32_t), VALUE        :: n
+     END FUNCTION sdpd_upload_replicas
+
+     INTEGER(c_int) FUNCTION sdpd_stats_replicas(ctx, mom, ekin) &
+          BIND(C, name="sdpd_stats_replicas")
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE          :: ctx
+       REAL(c_double), INTENT(OUT) :: mom(3, *)
+       REAL(c_double), INTENT(OUT) :: ekin(*)
+     END FUNCTION sdpd_stats_replicas
+
      INTEGER(c_int) FUNCTION sdpd_upload_ct(ctx, ct) BIND(C, name="sdpd_upload_ct")
        IMPORT :: c_int, c_ptr, c_double
        TYPE(c_ptr), VALUE         :: ctx

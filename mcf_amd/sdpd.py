@@ -80,6 +80,8 @@ _FP = C.POINTER(C.c_float)
 SYMBOLS = [
     ("sdpd_create", C.c_int, [C.POINTER(CParams), C.POINTER(C.c_void_p)]),
     ("sdpd_upload", C.c_int, [_P, _DP, _DP, _DP, _IP, C.c_int32]),
+    ("sdpd_set_replicas", C.c_int, [_P, C.c_int32, _DP, _DP]),
+    ("sdpd_upload_replicas", C.c_int, [_P, _DP, _DP, _DP, _IP, _IP, C.c_int32]),
     ("sdpd_upload_ct", C.c_int, [_P, _DP]),
     ("sdpd_build_cells", C.c_int, [_P]),
     ("sdpd_density", C.c_int, [_P]),
@@ -89,6 +91,7 @@ SYMBOLS = [
     ("sdpd_drift_wrap", C.c_int, [_P]),
     ("sdpd_conformation", C.c_int, [_P, C.c_double]),
     ("sdpd_stats", C.c_int, [_P, _DP, _DP]),
+    ("sdpd_stats_replicas", C.c_int, [_P, _DP, _DP]),
     ("sdpd_download", C.c_int, [_P, _DP, _DP, _DP, _DP, _DP]),
     ("sdpd_download_tensors", C.c_int, [_P, _DP, _DP]),
     ("sdpd_step", C.c_int, [_P, C.c_int32]),
@@ -214,6 +217,38 @@ class SDPD:
             ids[:, 1] = sid
         self._check(self.lib.sdpd_upload(self.h, _dp(x), _dp(v), _dp(m), _ip(ids), n), "sdpd_upload")
         self.n = n
+
+    def set_replicas(self, box_lo, box_hi) -> None:
+        """Batched ensemble of independent periodic boxes; box_lo, box_hi:
+        (nrep, 3) lower and upper corners."""
+        lo = np.ascontiguousarray(box_lo, np.float64).reshape(-1, 3)
+        hi = np.ascontiguousarray(box_hi, np.float64).reshape(-1, 3)
+        if lo.shape != hi.shape:
+            raise ValueError("box_lo and box_hi differ in shape")
+        self._check(self.lib.sdpd_set_replicas(self.h, lo.shape[0], _dp(lo), _dp(hi)),
+                    "sdpd_set_replicas")
+        self.nrep = lo.shape[0]
+        self.n = 0
+
+    def upload_replicas(self, x, v, m, pid, rep) -> None:
+        x = np.ascontiguousarray(x, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        m = np.ascontiguousarray(m, np.float64)
+        rep = np.ascontiguousarray(rep, np.int32)
+        n = x.shape[0]
+        ids = np.zeros((n, 2), np.int32)
+        ids[:, 0] = pid
+        self._check(self.lib.sdpd_upload_replicas(self.h, _dp(x), _dp(v), _dp(m), _ip(ids), _ip(rep), n),
+                    "sdpd_upload_replicas")
+        self.n = n
+
+    def compute_statistic_replicas(self):
+        """(ekin[nrep], mom[nrep, dim]) of every replica."""
+        nrep = getattr(self, "nrep", 1)
+        mom = np.zeros((nrep, 3))
+        ek = np.zeros(nrep)
+        self._check(self.lib.sdpd_stats_replicas(self.h, _dp(mom), _dp(ek)), "sdpd_stats_replicas")
+        return ek, mom[:, : self.prm.num_dim]
 
     def upload_ct(self, ct) -> None:
         ct = np.ascontiguousarray(ct, np.float64)

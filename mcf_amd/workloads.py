@@ -17,6 +17,11 @@ tait_box_3d    3D periodic box of 128^3 particles, dx = 1 (m = 1),
                cut_off 3, Tait EOS gamma 7, c = 10, eta = 0.1,
                Maxwellian velocities at kT = 1e-3, dt = 5e-5, Brownian.
 oldroyd_box_3d Oldroyd-B variant of tait_box_3d (tau 0.1, eta_p 0.05).
+oldroyd_ensemble
+               batch of independent periodic Oldroyd-B boxes: particle
+               counts drawn log-uniformly in [2000, 60000], each a cubic
+               lattice of edge round(N^(1/3)) dx (N is then that cube),
+               tau 0.1, eta_p 0.05, kT 1e-3.
 """
 from __future__ import annotations
 
@@ -114,6 +119,77 @@ def oldroyd_box_3d(n_side: int = 24, precision: int = PRECISION_FP64, kt: float 
     box.prm = box.prm.replace(Newtonian=False, tau=tau, n_p=1.0, kt_p=eta_p / tau)
     box.name = "oldroyd_box_3d"
     return box
+
+
+class Ensemble:
+    """A batched workload: parameters shared by all replicas, one periodic
+    box per replica ((nrep, 3) corners), particles concatenated with their
+    replica index.  Particle ids are unique across the batch, so replicas
+    draw independent thermal noise."""
+
+    def __init__(self, name, prm, box_lo, box_hi, x, v, m, pid, rep):
+        self.name = name
+        self.prm = prm
+        self.box_lo = np.ascontiguousarray(box_lo, np.float64)
+        self.box_hi = np.ascontiguousarray(box_hi, np.float64)
+        self.x = np.ascontiguousarray(x, np.float64)
+        self.v = np.ascontiguousarray(v, np.float64)
+        self.m = np.ascontiguousarray(m, np.float64)
+        self.pid = np.ascontiguousarray(pid, np.int32)
+        self.rep = np.ascontiguousarray(rep, np.int32)
+
+    @property
+    def n(self) -> int:
+        return self.x.shape[0]
+
+    @property
+    def nrep(self) -> int:
+        return self.box_lo.shape[0]
+
+    def replica(self, q: int) -> Box:
+        """Replica q as a single-box workload (same ids and state)."""
+        sel = self.rep == q
+        lo = list(self.box_lo[q]) + [0.0] * (3 - self.box_lo.shape[1])
+        hi = list(self.box_hi[q]) + [0.0] * (3 - self.box_hi.shape[1])
+        prm = self.prm.replace(min_phys=lo, max_phys=hi)
+        return Box(f"{self.name}[{q}]", prm, self.x[sel], self.v[sel], self.m[sel], self.pid[sel])
+
+
+def ensemble_of_cubes(sides, prm: SDPDParams, kt: float, seed: int = 1, name="ensemble") -> Ensemble:
+    """Replicas that are jittered cubic lattices of the given edges (dx 1,
+    m = rho) at the origin, Maxwellian velocities at kT with each replica's
+    mean momentum removed."""
+    xs, vs, reps = [], [], []
+    lo = np.zeros((len(sides), 3))
+    hi = np.zeros((len(sides), 3))
+    rng = np.random.default_rng(seed + 104729)
+    for q, side in enumerate(sides):
+        side = int(side)
+        hi[q] = side
+        pr = prm.replace(min_phys=[0.0, 0.0, 0.0], max_phys=[float(side)] * 3)
+        x = wrap(lattice(3, side, 1.0, 0.05, seed + q), pr)
+        sigma = math.sqrt(kt / prm.rho) if kt > 0 else 0.0
+        v = rng.normal(0.0, 1.0, size=x.shape) * sigma
+        v -= v.mean(axis=0, keepdims=True)
+        xs.append(x)
+        vs.append(v)
+        reps.append(np.full(x.shape[0], q, np.int32))
+    x = np.concatenate(xs)
+    v = np.concatenate(vs)
+    rep = np.concatenate(reps)
+    m = np.full(x.shape[0], prm.rho)
+    pid = np.arange(1, x.shape[0] + 1, dtype=np.int32)
+    return Ensemble(name, prm, lo, hi, x, v, m, pid, rep)
+
+
+def oldroyd_ensemble(nrep: int = 256, nmin: int = 2000, nmax: int = 60000, seed: int = 1,
+                     precision: int = PRECISION_FP64, kt: float = 1e-3) -> Ensemble:
+    rng = np.random.default_rng(seed)
+    counts = np.round(np.exp(rng.uniform(math.log(nmin), math.log(nmax), size=nrep)))
+    # edge round(N^(1/3)), at least 10 dx (3 cells of 1.0001 * cut_off)
+    sides = np.maximum(np.round(np.cbrt(counts)), 10).astype(int)
+    prm = oldroyd_box_3d(10, precision, kt).prm
+    return ensemble_of_cubes(sides, prm, kt, seed, name="oldroyd_ensemble")
 
 
 def mixed(box: Box) -> Box:

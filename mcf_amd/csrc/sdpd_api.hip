@@ -98,8 +98,6 @@ int derive(const sdpd_params& p, DevParams& P, std::string& err) {
   if (!(p.dt > 0)) return err = "dt must be positive", SDPD_ERR_ARG;
   if (p.precision != SDPD_PRECISION_FP64 && p.precision != SDPD_PRECISION_MIXED)
     return err = "precision must be 0 (FP64) or 1 (mixed)", SDPD_ERR_ARG;
-  if (p.precision == SDPD_PRECISION_MIXED && !p.newtonian)
-    return err = "mixed precision is available for Newtonian flow only", SDPD_ERR_ARG;
 
   P.dim = p.num_dim;
   P.symmetry = p.symmetry ? 1 : 0;
@@ -287,7 +285,7 @@ void free_state(DeviceState& S) {
                   S.rstart,   S.runsorted, S.rlpdx,  S.nlist,   S.nbr_sig_count,
                   S.row_nchunk, S.row_chunk_off, S.chunk_row, S.scan_tmp, S.rec64,
                   (void*)S.geo, S.rep,      S.rep_alt,  S.row_cell, S.row_len,  S.row_rep,
-                  S.rep_stats};
+                  S.rep_stats, S.qr2};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   std::memset(&S, 0, sizeof(S));
@@ -331,6 +329,7 @@ int alloc_state(sdpd_ctx* c, int n) {
   if (c->prm.precision == SDPD_PRECISION_MIXED) {
     ALLOC(vr, N);
     ALLOC(qr, N);
+    if (!P.newtonian) ALLOC(qr2, N);
   } else {
     ALLOC(rec64, 8 * N);
   }
@@ -714,6 +713,7 @@ int sdpd_upload_ct(sdpd_ctx* c, const double* ct) {
   for (int i = 0; i < n; ++i)
     for (int q = 0; q < d2; ++q) tmp[(size_t)q * n + i] = ct[(size_t)q + (size_t)d2 * perm[i]];
   HIP_TRY(c, hipMemcpy(c->S.ct, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->vr_stale = true;  // mixed records carry Pi / n^2
   return SDPD_OK;
 }
 
@@ -770,6 +770,7 @@ int sdpd_conformation(sdpd_ctx* c, double lambda) {
   if (c->P.newtonian) return fail(c, SDPD_ERR_STATE, "sdpd_conformation: Newtonian context");
   if (!c->f_valid) return fail(c, SDPD_ERR_STATE, "sdpd_conformation: act is not current");
   sdpd::launch_conformation(c->stream, c->P, c->S, lambda);
+  c->vr_stale = true;  // mixed records carry Pi / n^2
   return check_launch(c, "conformation");
 }
 

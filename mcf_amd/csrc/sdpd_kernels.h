@@ -31,7 +31,8 @@ struct DeviceState {
   // FP64 records of the force pass: x, y, z, p/n^2, vx, vy, vz, 1/n^2
   double *rec64;
   float4 *vr;            // velocity, w = pid bits
-  float4 *qr;            // (p / n^2, 1 / n^2, p, 0)
+  float4 *qr;            // (p / n^2, 1 / n^2, p, 0); Oldroyd-B: (1/n^2, Pxx, Pyy, Pzz)
+  float4 *qr2;           // Oldroyd-B: (Pxy, Pxz, Pyz, 0), P = Pi / n^2
   // reductions and counters
   double *partials;      // stats partial sums (blocks x 8)
   double *stats_out;     // 8 doubles

@@ -425,7 +425,10 @@ struct DensityMixed {
 };
 
 // Records of the force pass, written once the density is known: FP64
-// (x, y, z, p/n^2, vx, vy, vz, 1/n^2) or FP32 (v, pid) and (p/n^2, 1/n^2, p).
+// (x, y, z, p/n^2, vx, vy, vz, 1/n^2) or FP32 (v, pid) and (p/n^2, 1/n^2, p);
+// Oldroyd-B in FP32: (1/n^2, Pxx, Pyy, Pzz) and (Pxy, Pxz, Pyz, 0) with
+// P = Pi / n^2, Pi = (p + G) I - G c (particles_compute_pressure_tensor.F90:
+// 114-133; c is symmetric).
 template <int DIM>
 __device__ __forceinline__ void write_force_records(const DevParams& P, const DeviceState& S, int i,
                                                     double an, double bn, double press) {
@@ -433,7 +436,21 @@ __device__ __forceinline__ void write_force_records(const DevParams& P, const De
   if (P.mixed) {
     S.vr[i] = make_float4((float)S.v[i], (float)S.v[n + i], DIM == 3 ? (float)S.v[2 * n + i] : 0.f,
                           __uint_as_float((uint32_t)S.pid[i]));
-    S.qr[i] = make_float4((float)an, (float)bn, (float)press, 0.f);
+    if (P.newtonian) {
+      S.qr[i] = make_float4((float)an, (float)bn, (float)press, 0.f);
+    } else {
+      auto pi = [&](int a, int b) {
+        const double c = S.ct[(int64_t)(a + DIM * b) * n + i];
+        return ((a == b ? press + P.G : 0.0) - P.G * c) * bn;
+      };
+      if (DIM == 3) {
+        S.qr[i] = make_float4((float)bn, (float)pi(0, 0), (float)pi(1, 1), (float)pi(2, 2));
+        S.qr2[i] = make_float4((float)pi(0, 1), (float)pi(0, 2), (float)pi(1, 2), 0.f);
+      } else {
+        S.qr[i] = make_float4((float)bn, (float)pi(0, 0), (float)pi(1, 1), 0.f);
+        S.qr2[i] = make_float4((float)pi(0, 1), 0.f, 0.f, 0.f);
+      }
+    }
   } else {
     double4* r = reinterpret_cast<double4*>(S.rec64 + 8 * (int64_t)i);
     r[0] = make_double4(S.x[i], S.x[n + i], DIM == 3 ? S.x[2 * n + i] : 0.0, an);
@@ -641,12 +658,16 @@ struct ForceMixed {
   float vi0, vi1, vi2, ai, bi, pi;
   uint32_t pidi;
   KahanF f0, f1, f2;
+  // Oldroyd-B: own Pi / n^2 (xx, yy, zz, xy, xz, yz), 1 / n, velocity gradient
+  float pb[6];
+  float inv_ni;
+  KahanF L[DIM * DIM];
   __device__ __forceinline__ void body(int j, const Offsets& o) {
-    body_rec(S.xa[j], S.vr[j], S.qr[j], o);
+    body_rec(j, S.xa[j], S.vr[j], S.qr[j], o);
   }
   // xj: cell-relative position; vj: (v, pid); qj: (p/n^2, 1/n^2, p)
-  __device__ __forceinline__ void body_rec(const float4& xj, const float4& vj, const float4& qj,
-                                           const Offsets& o) {
+  __device__ __forceinline__ void body_rec(int jidx, const float4& xj, const float4& vj,
+                                           const float4& qj, const Offsets& o) {
     float d0, d1, d2;
     rel_diff<DIM>(P, h, xj, o, d0, d1, d2);
     const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
@@ -657,7 +678,43 @@ struct ForceMixed {
     const float u0 = vi0 - vj.x, u1 = vi1 - vj.y, u2 = vi2 - vj.z;
     const uint32_t pidj = __float_as_uint(vj.w);
     float fi0, fi1, fi2;
-    if (FT == 3) {
+    if (FT == 5) {
+      // rhs_force_ff_non_Newtonian_HuAdams.F90:105-131 with the records'
+      // Pi / n^2, and the velocity gradient of pp_vgt_ip.inc:16-20
+      const float4 q2 = S.qr2[jidx];
+      const float sxx = pb[0] + qj.y, syy = pb[1] + qj.z, szz = pb[2] + qj.w;
+      const float sxy = pb[3] + q2.x, sxz = pb[4] + q2.y, syz = pb[5] + q2.z;
+      float c0, c1, c2;
+      if (DIM == 3) {
+        c0 = sxx * d0 + sxy * d1 + sxz * d2;
+        c1 = sxy * d0 + syy * d1 + syz * d2;
+        c2 = sxz * d0 + syz * d1 + szz * d2;
+      } else {
+        c0 = sxx * d0 + sxy * d1;
+        c1 = sxy * d0 + syy * d1;
+        c2 = 0.f;
+      }
+      const float gir = gw * ir;
+      const float f_d = P.etaf * (bi + qj.x) * gir;
+      fi0 = f_d * u0 - c0 * gir;
+      fi1 = f_d * u1 - c1 * gir;
+      fi2 = f_d * u2 - c2 * gir;
+      const float gfac = gir * inv_ni;
+      const float du[3] = {-u0, -u1, -u2};
+      const float dx[3] = {d0, d1, d2};
+#pragma unroll
+      for (int b = 0; b < DIM; ++b)
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) L[a + DIM * b].add(du[a] * dx[b] * gfac);
+      if (P.noise) {
+        float w0, w1, w2;
+        noise_vector<DIM, float>(P, pidi, pidj, d0 * ir, d1 * ir, d2 * ir, w0, w1, w2);
+        const float f_r = __builtin_amdgcn_sqrtf(-4.0f * P.ktf * f_d) * P.inv_sqrt_dtf;
+        fi0 = fi0 + f_r * w0;
+        fi1 = fi1 + f_r * w1;
+        fi2 = fi2 + f_r * w2;
+      }
+    } else if (FT == 3) {
       const float fcr = (ai + qj.x) * gw * ir;
       const float f_d = P.etaf * (bi + qj.y) * gw * ir;
       fi0 = f_d * u0 - fcr * d0;
@@ -744,6 +801,32 @@ __device__ __forceinline__ void for_each_listed(const DevParams& P, const Device
   }
 }
 
+// particles_compute_act.F90:140-155 on particle i's own ct and velocity
+// gradient L: stores vgt = L and act = c L^T + L c - (c - I) / tau.
+template <int DIM>
+__device__ __forceinline__ void conformation_rate(const DevParams& P, const DeviceState& S, int i,
+                                                  const double* L) {
+  const int64_t n = P.n;
+  const int d2n = DIM * DIM;
+  double cc[9];
+#pragma unroll
+  for (int q = 0; q < d2n; ++q) {
+    cc[q] = S.ct[(int64_t)q * n + i];
+    S.vgt[(int64_t)q * n + i] = L[q];
+  }
+#pragma unroll
+  for (int b = 0; b < DIM; ++b)
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k)
+        t = t + cc[a + DIM * k] * L[b + DIM * k] + L[a + DIM * k] * cc[k + DIM * b];
+      double id = a == b ? 1.0 : 0.0;
+      S.act[(int64_t)(a + DIM * b) * n + i] = t - (cc[a + DIM * b] - id) / P.tau;
+    }
+}
+
 template <int DIM, int FT, bool MIXED>
 __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
   __shared__ float4 tile[kTile];
@@ -758,10 +841,27 @@ __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
     const float4 vi = S.vr[ii];
     const float4 qi = S.qr[ii];
     ForceMixed<DIM, FT> pol{P, S, h, vi.x, vi.y, vi.z, qi.x, qi.y, qi.z, __float_as_uint(vi.w)};
+    if (FT == 5) {
+      const float4 q2 = S.qr2[ii];
+      pol.bi = qi.x;
+      pol.pb[0] = qi.y;
+      pol.pb[1] = qi.z;
+      pol.pb[2] = qi.w;
+      pol.pb[3] = q2.x;
+      pol.pb[4] = q2.y;
+      pol.pb[5] = q2.z;
+      pol.inv_ni = (float)(1.0 / S.rho[ii]);
+    }
     for_each_listed<DIM>(P, S, ch, h, pol, tile, buf);
     if (!h.active) return;
     finish_force<DIM>(P, S, h.i, pol.f0.value() * inv_mi, pol.f1.value() * inv_mi,
                       DIM == 3 ? pol.f2.value() * inv_mi : 0.0);
+    if (FT == 5) {
+      double Ld[DIM * DIM];
+#pragma unroll
+      for (int q = 0; q < DIM * DIM; ++q) Ld[q] = pol.L[q].value();
+      conformation_rate<DIM>(P, S, h.i, Ld);
+    }
   } else {
     ForceF64<DIM, FT> pol{P, S, h};
     pol.vi0 = S.v[ii];
@@ -789,27 +889,7 @@ __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
     if (!h.active) return;
     const int i = h.i;
     finish_force<DIM>(P, S, i, pol.f0 * inv_mi, pol.f1 * inv_mi, pol.f2 * inv_mi);
-    if (FT == 5) {
-      // particles_compute_act.F90:140-155 on this particle's own ct and vgt
-      const int d2n = DIM * DIM;
-      double cc[9];
-#pragma unroll
-      for (int q = 0; q < d2n; ++q) {
-        cc[q] = S.ct[(int64_t)q * n + i];
-        S.vgt[(int64_t)q * n + i] = pol.L[q];
-      }
-#pragma unroll
-      for (int b = 0; b < DIM; ++b)
-#pragma unroll
-        for (int a = 0; a < DIM; ++a) {
-          double t = 0.0;
-#pragma unroll
-          for (int k = 0; k < DIM; ++k)
-            t = t + cc[a + DIM * k] * pol.L[b + DIM * k] + pol.L[a + DIM * k] * cc[k + DIM * b];
-          double id = a == b ? 1.0 : 0.0;
-          S.act[(int64_t)(a + DIM * b) * n + i] = t - (cc[a + DIM * b] - id) / P.tau;
-        }
-    }
+    if (FT == 5) conformation_rate<DIM>(P, S, i, pol.L);
   }
 }
 
@@ -890,8 +970,10 @@ static void launch_forces_dim(hipStream_t s, const DevParams& P, const DeviceSta
   if (mixed) {
     if (ft == 3)
       hipLaunchKernelGGL((k_forces<DIM, 3, true>), g, b, 0, s, P, S);
-    else
+    else if (ft == 4)
       hipLaunchKernelGGL((k_forces<DIM, 4, true>), g, b, 0, s, P, S);
+    else
+      hipLaunchKernelGGL((k_forces<DIM, 5, true>), g, b, 0, s, P, S);
   } else {
     if (ft == 3)
       hipLaunchKernelGGL((k_forces<DIM, 3, false>), g, b, 0, s, P, S);

@@ -392,3 +392,43 @@ def test_mixed_full_size_properties():
     # the jittered lattice piles ~30 neighbours near r = 3 dx: < 1 FP64 re-decision per particle
     assert rechecks < 0.01 * pairs
     assert np.isfinite(f).all()
+
+
+def _perturbed_ct(n, seed=9):
+    rng = np.random.default_rng(seed)
+    ct = np.tile(np.eye(3).reshape(-1), (n, 1))
+    a = rng.normal(0, 0.05, size=(n, 3, 3))
+    return ct + (a + a.transpose(0, 2, 1)).reshape(n, 9)
+
+
+def test_mixed_oldroyd_vs_fp64():
+    # Pi / n^2 records and the velocity gradient in FP32 (Kahan sums)
+    box = W.oldroyd_box_3d(12)
+    announce("mixed Oldroyd-B vs fp64")
+    ct = _perturbed_ct(box.n)
+    g = gpu_evaluate(box, step=2, ct=ct)
+    f64 = g.download_force()
+    _, vgt64 = g.download_tensors()
+    g.close()
+    g = gpu_evaluate(W.mixed(box), step=2, ct=ct)
+    fmx = g.download_force()
+    _, vgtmx = g.download_tensors()
+    g.close()
+    assert vec_err(fmx, f64) < 1e-5
+    assert vec_err(vgtmx, vgt64) < 1e-5
+
+
+def test_mixed_oldroyd_trajectory_close_to_fp64():
+    box = W.oldroyd_box_3d(12)
+    announce("mixed Oldroyd-B trajectory")
+    out = []
+    for b in (box, W.mixed(box)):
+        with SDPD(b.prm) as g:
+            g.upload(b.x, b.v, b.m, b.pid)
+            g.integrate_VV(4)
+            x, v = g.download()
+            ct, vgt = g.download_tensors()
+        out.append((v, ct))
+    assert vec_err(out[1][0], out[0][0]) < 1e-5
+    assert vec_err(out[1][1] - np.tile(np.eye(3).reshape(-1), (box.n, 1)),
+                   out[0][1] - np.tile(np.eye(3).reshape(-1), (box.n, 1))) < 1e-4

@@ -47,7 +47,7 @@ def run_single(box, steps):
     return x, v, f, rho
 
 
-@pytest.mark.parametrize("kind,precision", [("tait", 0), ("tait", 1), ("oldroyd", 0)])
+@pytest.mark.parametrize("kind,precision", [("tait", 0), ("tait", 1), ("oldroyd", 0), ("oldroyd", 1)])
 def test_batch_equals_single_boxes(kind, precision):
     ens = small_ensemble(kind, precision=precision)
     announce(f"batch vs single boxes: {kind} precision={precision} n={ens.n}")

@@ -149,6 +149,11 @@ int sdpd_download_tensors(sdpd_ctx *ctx, double *ct, double *vgt);
  * unless sdpd_forces has run since. */
 int sdpd_step(sdpd_ctx *ctx, int32_t nsteps);
 
+/* sdpd_step runs pairs of steps through a captured HIP graph (default on,
+ * and off while phase timing is enabled); enable = 0 launches every kernel
+ * directly.  Results are identical either way. */
+int sdpd_set_graphs(sdpd_ctx *ctx, int32_t enable);
+
 /* Noise step counter (the reference's marching step). */
 int sdpd_set_step(sdpd_ctx *ctx, int64_t step);
 int sdpd_get_step(sdpd_ctx *ctx, int64_t *step);

@@ -43,6 +43,11 @@ struct sdpd_ctx {
   std::vector<sdpd::RepGeom> geo;
   std::vector<int32_t> row_cell, row_len, row_rep;
   bool sorted_once = false;
+  // captured two-step graphs of sdpd_step, one per parity of the sort's
+  // ping-pong buffers (kernel arguments hold the buffer pointers)
+  bool use_graphs = true;
+  int parity = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -275,6 +280,14 @@ int dalloc(sdpd_ctx* c, T** p, size_t count) {
   return SDPD_OK;
 }
 
+void drop_graphs(sdpd_ctx* c) {
+  for (hipGraphExec_t& g : c->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
 void free_state(DeviceState& S) {
   void* ptrs[] = {S.x,        S.v,        S.m,         S.pid,     S.perm,     S.x_alt,
                   S.v_alt,    S.m_alt,    S.pid_alt,   S.perm_alt, S.f,       S.rho,
@@ -285,7 +298,7 @@ void free_state(DeviceState& S) {
                   S.rstart,   S.runsorted, S.rlpdx,  S.nlist,   S.nbr_sig_count,
                   S.row_nchunk, S.row_chunk_off, S.chunk_row, S.scan_tmp, S.rec64,
                   (void*)S.geo, S.rep,      S.rep_alt,  S.row_cell, S.row_len,  S.row_rep,
-                  S.rep_stats, S.qr2};
+                  S.rep_stats, S.qr2,     S.step_dev};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   std::memset(&S, 0, sizeof(S));
@@ -352,6 +365,7 @@ int alloc_state(sdpd_ctx* c, int n) {
     HIP_TRY(c, hipMemcpy(S.row_rep, c->row_rep.data(), c->row_rep.size() * sizeof(int32_t),
                          hipMemcpyHostToDevice));
     ALLOC(rep_stats, 4 * (size_t)P.nrep);
+    ALLOC(step_dev, 1);
   }
   {
     const size_t nrows = (size_t)P.nrows;
@@ -376,7 +390,9 @@ int alloc_state(sdpd_ctx* c, int n) {
   return SDPD_OK;
 }
 
-void swap_sorted(DeviceState& S) {
+void swap_sorted(sdpd_ctx* c) {
+  DeviceState& S = c->S;
+  c->parity ^= 1;
   std::swap(S.x, S.x_alt);
   std::swap(S.v, S.v_alt);
   std::swap(S.m, S.m_alt);
@@ -441,7 +457,7 @@ int run_sort(sdpd_ctx* c, bool keys_done) {
   }
   int rc = check_launch(c, "cell sort");
   if (rc) return rc;
-  swap_sorted(c->S);
+  swap_sorted(c);
   c->cells_valid = true;
   c->sorted_once = true;
   c->rho_valid = false;
@@ -475,6 +491,52 @@ int run_forces(sdpd_ctx* c) {
   int rc = check_launch(c, "forces");
   if (rc) return rc;
   c->f_valid = true;
+  return SDPD_OK;
+}
+
+// One velocity-Verlet step after the prologue: advance the marching step
+// (host and device copies), step boundary (closing kick of the previous
+// step if asked, opening kick, drift, ct half update, wrap, cell keys),
+// sort, density + EOS, forces.
+int step_body(sdpd_ctx* c, int closing) {
+  c->step += 1;
+  c->P.step = c->step;
+  sdpd::launch_advance_step(c->stream, c->S);
+  {
+    PhaseMark pm(c, 0);
+    sdpd::launch_boundary(c->stream, c->P, c->S, closing, 1);
+  }
+  int rc = check_launch(c, "step boundary");
+  if (rc) return rc;
+  c->cells_valid = false;
+  if ((rc = run_sort(c, true))) return rc;
+  if ((rc = run_density(c))) return rc;
+  return run_forces(c);
+}
+
+// Records two closing-kick steps on the context stream into a graph.
+// Nothing runs during capture: the host step counter is restored, and the
+// two sorts leave the ping-pong parity where it was.
+int capture_two_steps(sdpd_ctx* c, hipGraphExec_t* out) {
+  const int64_t step0 = c->step;
+  HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = step_body(c, 1);
+  if (!rc) rc = step_body(c, 1);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  c->step = step0;
+  c->P.step = step0;
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(c, SDPD_ERR_DEVICE, std::string("step graph capture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    *out = nullptr;
+    return fail(c, SDPD_ERR_DEVICE, std::string("step graph instantiate: ") + hipGetErrorString(e));
+  }
   return SDPD_OK;
 }
 
@@ -537,6 +599,7 @@ int sdpd_destroy(sdpd_ctx* c) {
   release_events(c);
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
   c->event_pool.clear();
+  drop_graphs(c);
   free_state(c->S);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -572,7 +635,9 @@ int upload_impl(sdpd_ctx* c, const double* x, const double* v, const double* m,
     }
   }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);
   free_state(c->S);
+  c->parity = 0;
   c->n = 0;
   c->P.n = n;
   {
@@ -616,6 +681,7 @@ int upload_impl(sdpd_ctx* c, const double* x, const double* v, const double* m,
     HIP_TRY(c, hipMemcpy(c->S.rep, rep, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
   else
     sdpd::launch_fill_i32(c->stream, c->S.rep, 0, n);
+  sdpd::launch_set_step(c->stream, c->S, c->step);
   sdpd::launch_fill_f64(c->stream, c->S.f, 0.0, (int64_t)dim * n);
   sdpd::launch_fill_f64(c->stream, c->S.rho, 0.0, n);
   sdpd::launch_fill_f64(c->stream, c->S.p, 0.0, n);
@@ -655,7 +721,9 @@ int sdpd_set_replicas(sdpd_ctx* c, int32_t nrep, const double* box_lo, const dou
   if (nrep < 1 || !box_lo || !box_hi)
     return fail(c, SDPD_ERR_ARG, "sdpd_set_replicas: bad arguments");
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);
   free_state(c->S);
+  c->parity = 0;
   c->n = 0;
   c->cells_valid = c->rho_valid = c->f_valid = false;
   c->rep_lo.assign(box_lo, box_lo + 3 * (size_t)nrep);
@@ -831,20 +899,24 @@ int sdpd_step(sdpd_ctx* c, int32_t nsteps) {
     if ((rc = run_density(c))) return rc;
     if ((rc = run_forces(c))) return rc;
   }
-  for (int32_t s = 0; s < nsteps; ++s) {
-    c->step += 1;
-    c->P.step = c->step;
-    {
-      PhaseMark pm(c, 0);
-      sdpd::launch_boundary(c->stream, c->P, c->S, s > 0 ? 1 : 0, 1);
+  if (nsteps == 0) return SDPD_OK;
+  // the first step opens without a closing kick (the previous call ended
+  // with it); every later one starts with the closing kick of the one
+  // before, two at a time through a captured graph when possible
+  if ((rc = step_body(c, 0))) return rc;
+  int32_t left = nsteps - 1;
+  if (c->use_graphs && !c->timing && left >= 2) {
+    hipGraphExec_t& g = c->graph[c->parity];
+    if (!g && (rc = capture_two_steps(c, &g))) return rc;
+    while (left >= 2) {
+      HIP_TRY(c, hipGraphLaunch(g, c->stream));
+      c->step += 2;
+      left -= 2;
     }
-    if ((rc = check_launch(c, "step boundary"))) return rc;
-    c->cells_valid = false;
-    if ((rc = run_sort(c, true))) return rc;
-    if ((rc = run_density(c))) return rc;
-    if ((rc = run_forces(c))) return rc;
   }
-  if (nsteps > 0) {
+  for (; left > 0; --left)
+    if ((rc = step_body(c, 1))) return rc;
+  {
     // closing half kick (and conformation half update) of the last step
     PhaseMark pm(c, 0);
     if (!c->P.newtonian) sdpd::launch_conformation(c->stream, c->P, c->S, 0.5);
@@ -854,9 +926,23 @@ int sdpd_step(sdpd_ctx* c, int32_t nsteps) {
   return SDPD_OK;
 }
 
+int sdpd_set_graphs(sdpd_ctx* c, int32_t enable) {
+  if (!c) return SDPD_ERR_ARG;
+  c->use_graphs = enable != 0;
+  if (!c->use_graphs) {
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+  }
+  return SDPD_OK;
+}
+
 int sdpd_set_step(sdpd_ctx* c, int64_t step) {
   if (!c) return SDPD_ERR_ARG;
   c->step = step;
+  if (c->S.step_dev) {
+    sdpd::launch_set_step(c->stream, c->S, step);
+    return check_launch(c, "sdpd_set_step");
+  }
   return SDPD_OK;
 }
 

@@ -523,11 +523,12 @@ __global__ void k_refresh_records(DevParams P, DeviceState S) {
 // vector with the covariance of the reference's symmetric traceless
 // dW . e_ij (I/2 + e e^T/6 in 3D, I/2 in 2D), oriented by the lower pid.
 template <int DIM, typename T>
-__device__ __forceinline__ void noise_vector(const DevParams& P, uint32_t pidi, uint32_t pidj,
-                                             T e0, T e1, T e2, T& w0, T& w1, T& w2) {
+__device__ __forceinline__ void noise_vector(const DevParams& P, int64_t step, uint32_t pidi,
+                                             uint32_t pidj, T e0, T e1, T e2, T& w0, T& w1,
+                                             T& w2) {
   uint32_t lo = pidi < pidj ? pidi : pidj, hi = pidi < pidj ? pidj : pidi;
   float g[4];
-  pair_normals(P.seed, lo, hi, P.step, g);
+  pair_normals(P.seed, lo, hi, step, g);
   const T sgn = pidi < pidj ? T(1) : T(-1);
   const T ca = T(0.70710678118654752440);
   if (DIM == 3) {
@@ -554,6 +555,7 @@ struct ForceF64 {
   const Home& h;
   double vi0, vi1, vi2, ai, bi, pi, inv_ni;
   uint32_t pidi;
+  int64_t step;            // noise step (the context's marching step)
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
   double L[9];
   double pti[9];
@@ -580,7 +582,7 @@ struct ForceF64 {
       if (P.noise) {
         const uint32_t pj_ = (uint32_t)S.pid[j];
         float g[4];
-        pair_normals(P.seed, pidi < pj_ ? pidi : pj_, pidi < pj_ ? pj_ : pidi, P.step, g);
+        pair_normals(P.seed, pidi < pj_ ? pidi : pj_, pidi < pj_ ? pj_ : pidi, step, g);
         sc = sc + (double)g[0] * P.sqrt_2kt_eta * sqrt_nr(-P.eta_coef * aij * ir) * P.inv_sqrt_dt;
       }
       sc = sc * ir;
@@ -626,7 +628,7 @@ struct ForceF64 {
         // pid_i < pid_j (noise_vector), written on d to save operations
         const uint32_t pidj = (uint32_t)S.pid[j];
         float g[4];
-        pair_normals(P.seed, pidi < pidj ? pidi : pidj, pidi < pidj ? pidj : pidi, P.step, g);
+        pair_normals(P.seed, pidi < pidj ? pidi : pidj, pidi < pidj ? pidj : pidi, step, g);
         const double fr = sqrt_nr(-4.0 * P.kt * fd) * P.inv_sqrt_dt;
         const double sfr = pidi < pidj ? fr : -fr;
         const double ca = 0.70710678118654752440;
@@ -657,6 +659,7 @@ struct ForceMixed {
   const Home& h;
   float vi0, vi1, vi2, ai, bi, pi;
   uint32_t pidi;
+  int64_t step;            // noise step (the context's marching step)
   KahanF f0, f1, f2;
   // Oldroyd-B: own Pi / n^2 (xx, yy, zz, xy, xz, yz), 1 / n, velocity gradient
   float pb[6];
@@ -708,7 +711,7 @@ struct ForceMixed {
         for (int a = 0; a < DIM; ++a) L[a + DIM * b].add(du[a] * dx[b] * gfac);
       if (P.noise) {
         float w0, w1, w2;
-        noise_vector<DIM, float>(P, pidi, pidj, d0 * ir, d1 * ir, d2 * ir, w0, w1, w2);
+        noise_vector<DIM, float>(P, step, pidi, pidj, d0 * ir, d1 * ir, d2 * ir, w0, w1, w2);
         const float f_r = __builtin_amdgcn_sqrtf(-4.0f * P.ktf * f_d) * P.inv_sqrt_dtf;
         fi0 = fi0 + f_r * w0;
         fi1 = fi1 + f_r * w1;
@@ -722,7 +725,7 @@ struct ForceMixed {
       fi2 = f_d * u2 - fcr * d2;
       if (P.noise) {
         float w0, w1, w2;
-        noise_vector<DIM, float>(P, pidi, pidj, d0 * ir, d1 * ir, d2 * ir, w0, w1, w2);
+        noise_vector<DIM, float>(P, step, pidi, pidj, d0 * ir, d1 * ir, d2 * ir, w0, w1, w2);
         const float f_r = __builtin_amdgcn_sqrtf(-4.0f * P.ktf * f_d) * P.inv_sqrt_dtf;
         fi0 = fi0 + f_r * w0;
         fi1 = fi1 + f_r * w1;
@@ -737,7 +740,7 @@ struct ForceMixed {
       float sc = f_d - f_c;
       if (P.noise) {
         float g[4];
-        pair_normals(P.seed, pidi < pidj ? pidi : pidj, pidi < pidj ? pidj : pidi, P.step, g);
+        pair_normals(P.seed, pidi < pidj ? pidi : pidj, pidi < pidj ? pidj : pidi, step, g);
         sc = sc + g[0] * P.sqrt_2kt_etaf * __builtin_amdgcn_sqrtf(-P.eta_coeff * aij * ir) *
                       P.inv_sqrt_dtf;
       }
@@ -840,7 +843,8 @@ __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
   if (MIXED) {
     const float4 vi = S.vr[ii];
     const float4 qi = S.qr[ii];
-    ForceMixed<DIM, FT> pol{P, S, h, vi.x, vi.y, vi.z, qi.x, qi.y, qi.z, __float_as_uint(vi.w)};
+    ForceMixed<DIM, FT> pol{P, S, h, vi.x, vi.y, vi.z, qi.x, qi.y, qi.z, __float_as_uint(vi.w),
+                            S.step_dev[0]};
     if (FT == 5) {
       const float4 q2 = S.qr2[ii];
       pol.bi = qi.x;
@@ -872,6 +876,7 @@ __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
     pol.pi = S.p[ii];
     pol.inv_ni = 1.0 / S.rho[ii];
     pol.pidi = (uint32_t)S.pid[ii];
+    pol.step = S.step_dev[0];
 #pragma unroll
     for (int q = 0; q < 9; ++q) pol.L[q] = 0.0;
     if (FT == 5) {

@@ -161,6 +161,14 @@ __global__ __launch_bounds__(256) void k_replica_stats(DevParams P, DeviceState 
   }
 }
 
+__global__ void k_set_step(int64_t* __restrict__ step_dev, int64_t step) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) step_dev[0] = step;
+}
+
+__global__ void k_advance_step(int64_t* __restrict__ step_dev) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) step_dev[0] = step_dev[0] + 1;
+}
+
 __global__ void k_pair_normals(const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
                                int count, uint64_t seed, int64_t step, float* __restrict__ g) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -203,6 +211,14 @@ void launch_stats(hipStream_t s, const DevParams& P, const DeviceState& S, int n
 
 void launch_replica_stats(hipStream_t s, const DevParams& P, const DeviceState& S) {
   hipLaunchKernelGGL(k_replica_stats, dim3(P.nrep), dim3(256), 0, s, P, S);
+}
+
+void launch_set_step(hipStream_t s, const DeviceState& S, int64_t step) {
+  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, S.step_dev, step);
+}
+
+void launch_advance_step(hipStream_t s, const DeviceState& S) {
+  hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, s, S.step_dev);
 }
 
 void launch_pair_normals(hipStream_t s, const uint32_t* lo, const uint32_t* hi, int count,

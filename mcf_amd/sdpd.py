@@ -95,6 +95,7 @@ SYMBOLS = [
     ("sdpd_download", C.c_int, [_P, _DP, _DP, _DP, _DP, _DP]),
     ("sdpd_download_tensors", C.c_int, [_P, _DP, _DP]),
     ("sdpd_step", C.c_int, [_P, C.c_int32]),
+    ("sdpd_set_graphs", C.c_int, [_P, C.c_int32]),
     ("sdpd_set_step", C.c_int, [_P, C.c_int64]),
     ("sdpd_get_step", C.c_int, [_P, _LP]),
     ("sdpd_get_cells", C.c_int, [_P, _IP, _IP, _IP]),
@@ -307,6 +308,11 @@ class SDPD:
         e = C.c_double()
         self._check(self.lib.sdpd_stats(self.h, _dp(mom), C.byref(e)), "statistic_compute_statistic")
         return e.value, mom[: self.prm.num_dim]
+
+    def set_graphs(self, enable: bool) -> None:
+        """Run sdpd_step's steps through captured HIP graphs (default) or
+        launch every kernel directly."""
+        self._check(self.lib.sdpd_set_graphs(self.h, 1 if enable else 0), "sdpd_set_graphs")
 
     def integrate_VV(self, nsteps: int = 1) -> None:
         self._check(self.lib.sdpd_step(self.h, int(nsteps)), "marching_integrate_VV")

@@ -432,3 +432,28 @@ def test_mixed_oldroyd_trajectory_close_to_fp64():
     assert vec_err(out[1][0], out[0][0]) < 1e-5
     assert vec_err(out[1][1] - np.tile(np.eye(3).reshape(-1), (box.n, 1)),
                    out[0][1] - np.tile(np.eye(3).reshape(-1), (box.n, 1))) < 1e-4
+
+
+@pytest.mark.parametrize("variant", ["tait_fp64", "oldroyd_mixed"])
+def test_graph_steps_match_direct_launches(variant):
+    # sdpd_step replays two-step graphs; every replay must equal direct
+    # launches, also after eager work in between (downloads, a re-sort that
+    # flips the ping-pong parity, separate calls)
+    box = BOXES["tait3d_sym_noise"]() if variant == "tait_fp64" else W.mixed(W.oldroyd_box_3d(12))
+    announce(f"graph replays vs direct launches: {variant}")
+    with SDPD(box.prm) as a, SDPD(box.prm) as b:
+        b.set_graphs(False)
+        for g in (a, b):
+            g.upload(box.x, box.v, box.m, box.pid)
+        for chunk in (5, 4, 7, 3):
+            a.integrate_VV(chunk)
+            b.integrate_VV(chunk)
+            xa, va = a.download()
+            xb, vb = b.download()
+            assert np.array_equal(xa, xb) and np.array_equal(va, vb), chunk
+            assert a.step == b.step
+            if chunk == 4:
+                for g in (a, b):
+                    g.build_list()
+                    g.compute_density()
+                    g.compute_interaction()

@@ -154,6 +154,24 @@ int sdpd_step(sdpd_ctx *ctx, int32_t nsteps);
  * directly.  Results are identical either way. */
 int sdpd_set_graphs(sdpd_ctx *ctx, int32_t enable);
 
+/* Launch shapes of the pair kernels.  No counterpart in the reference
+ * (its loops have no tunable shape).  density_shape picks the candidate
+ * scan's LDS tile and per-lane list buffer: 0 = 224 records / 16 entries
+ * (default), 1 = 128 / 16, 2 = 224 / 12, 3 = 128 / 12; chunk_order 1 lets
+ * each XCD work through a contiguous eighth of the chunks (default), 0
+ * hands chunk b to workgroup b.  Every choice gives identical results. */
+typedef struct sdpd_tuning {
+  int32_t density_shape;
+  int32_t chunk_order;
+} sdpd_tuning;
+int sdpd_set_tuning(sdpd_ctx *ctx, const sdpd_tuning *t);
+int sdpd_get_tuning(sdpd_ctx *ctx, sdpd_tuning *t);
+
+/* Autotuner: times reps density + force passes at the current positions
+ * for every launch shape (the simulation does not advance), applies the
+ * fastest and returns it with its time per pass pair (ms). */
+int sdpd_autotune(sdpd_ctx *ctx, int32_t reps, sdpd_tuning *best, double *ms_best);
+
 /* Noise step counter (the reference's marching step). */
 int sdpd_set_step(sdpd_ctx *ctx, int64_t step);
 int sdpd_get_step(sdpd_ctx *ctx, int64_t *step);

@@ -191,6 +191,8 @@ int derive(const sdpd_params& p, DevParams& P, std::string& err) {
   P.seed = p.seed;
   P.step = 0;
   P.mixed = p.precision == SDPD_PRECISION_MIXED ? 1 : 0;
+  P.density_shape = 0;
+  P.xcd_order = 1;
   P.inv_hf = (float)(1.0 / P.h);
   P.coeff = (float)P.coef;
   P.coef_gradf = (float)P.coef_grad;
@@ -933,6 +935,77 @@ int sdpd_set_graphs(sdpd_ctx* c, int32_t enable) {
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     drop_graphs(c);
   }
+  return SDPD_OK;
+}
+
+int sdpd_set_tuning(sdpd_ctx* c, const sdpd_tuning* t) {
+  if (!c || !t) return SDPD_ERR_ARG;
+  if (t->density_shape < 0 || t->density_shape > 3 || (t->chunk_order != 0 && t->chunk_order != 1))
+    return fail(c, SDPD_ERR_ARG, "sdpd_set_tuning: density_shape 0..3, chunk_order 0 or 1");
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);  // kernel arguments change
+  c->P.density_shape = t->density_shape;
+  c->P.xcd_order = t->chunk_order;
+  return SDPD_OK;
+}
+
+int sdpd_get_tuning(sdpd_ctx* c, sdpd_tuning* t) {
+  if (!c || !t) return SDPD_ERR_ARG;
+  t->density_shape = c->P.density_shape;
+  t->chunk_order = c->P.xcd_order;
+  return SDPD_OK;
+}
+
+int sdpd_autotune(sdpd_ctx* c, int32_t reps, sdpd_tuning* best, double* ms_best) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  if (reps < 1) return fail(c, SDPD_ERR_ARG, "sdpd_autotune: reps < 1");
+  if (!c->cells_valid && (rc = run_sort(c, false))) return rc;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+  const bool timing = c->timing;
+  c->timing = false;
+  hipEvent_t a = nullptr, b = nullptr;
+  HIP_TRY(c, hipEventCreate(&a));
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return fail(c, SDPD_ERR_DEVICE, "sdpd_autotune: event creation failed");
+  }
+  sdpd_tuning pick{0, 1};
+  float tbest = -1.f;
+  for (int order = 1; order >= 0 && rc == SDPD_OK; --order)
+    for (int shape = 0; shape < 4 && rc == SDPD_OK; ++shape) {
+      c->P.density_shape = shape;
+      c->P.xcd_order = order;
+      // one untimed pass, then reps timed density + force passes at the
+      // current positions (the state is recomputed identically each time)
+      if ((rc = run_density(c)) || (rc = run_forces(c))) break;
+      (void)hipEventRecord(a, c->stream);
+      for (int r = 0; r < reps && rc == SDPD_OK; ++r)
+        if (!(rc = run_density(c))) rc = run_forces(c);
+      (void)hipEventRecord(b, c->stream);
+      if (rc) break;
+      if (hipEventSynchronize(b) != hipSuccess) {
+        rc = fail(c, SDPD_ERR_DEVICE, "sdpd_autotune: event wait failed");
+        break;
+      }
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, a, b);
+      if (tbest < 0.f || ms < tbest) {
+        tbest = ms;
+        pick.density_shape = shape;
+        pick.chunk_order = order;
+      }
+    }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  c->timing = timing;
+  if (rc) return rc;
+  c->P.density_shape = pick.density_shape;
+  c->P.xcd_order = pick.chunk_order;
+  if ((rc = run_density(c)) || (rc = run_forces(c))) return rc;
+  if (best) *best = pick;
+  if (ms_best) *ms_best = (double)tbest / reps;
   return SDPD_OK;
 }
 

@@ -44,6 +44,10 @@ struct DevParams {
   int32_t list_cap;       // stored neighbour entries per particle
   int32_t nrep;           // boxes (replicas) in the context
   int32_t nrows;          // rows of the sort grids of all replicas
+  // tuned shapes: density scan (0: tile 224, buffer 16; 1: 128, 16;
+  // 2: 224, 12; 3: 128, 12) and chunk order (1: XCD-contiguous)
+  int32_t density_shape;
+  int32_t xcd_order;
   // band around cut2 of the FP32 candidate test; inside it the decision is
   // taken in FP64.  FP32 positions are relative to the own cell corner.
   float cut2_lo_f32, cut2_hi_f32;

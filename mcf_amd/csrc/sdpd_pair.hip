@@ -37,8 +37,10 @@
 namespace sdpd {
 
 constexpr int kWave = 64;
-constexpr int kBuf = 16;          // LDS list entries per lane between flushes
-constexpr int kTile = 224;        // staged candidate records per stencil row
+// Defaults of the two tunable shapes of the candidate scan (the density
+// kernel is instantiated for the autotuner's other choices too):
+constexpr int kBuf = 16;          // LDS list entries per lane between flushes (>= 10)
+constexpr int kTile = 224;        // staged candidate records per stencil row batch
 constexpr int kCodeShift = 25;    // list entry = j | (stencil code << 25)
 constexpr uint32_t kIndexMask = (1u << kCodeShift) - 1u;
 
@@ -113,16 +115,21 @@ struct Chunk {
   Geo G;
 };
 
-// Workgroups are handed to the eight XCDs round-robin; chunk c of XCD
-// b % 8 is taken from a contiguous eighth of the chunk sequence, so each
-// XCD's L2 serves a compact slab of space.  The grid is a multiple of 8
+// Workgroups are handed to the eight XCDs round-robin.  With xcd_order,
+// workgroup b takes chunk (b % 8) ceil(chunks / 8) + b / 8, i.e. each XCD
+// works through a contiguous eighth of the chunk sequence and its L2 serves
+// a compact slab of space; otherwise chunk b.  The grid is a multiple of 8
 // and at least the number of chunks.
 __device__ __forceinline__ bool load_chunk(const DevParams& P, const DeviceState& S, Chunk& ch) {
   const int total = S.row_chunk_off[P.nrows];
-  const int per = (total + 7) >> 3;
   const int b = blockIdx.x;
-  const int c = (b & 7) * per + (b >> 3);
-  if ((b >> 3) >= per || c >= total) return false;
+  int c = b;
+  if (P.xcd_order) {
+    const int per = (total + 7) >> 3;
+    c = (b & 7) * per + (b >> 3);
+    if ((b >> 3) >= per) return false;
+  }
+  if (c >= total) return false;
   ch.c = c;
   ch.row = S.chunk_row[c];
   const int k = c - S.row_chunk_off[ch.row];
@@ -228,7 +235,7 @@ __device__ __forceinline__ int seq_particle(const RowPieces& rp, int t) {
 // ---------------------------------------------------------------------
 // Candidate scan over the stencil rows of a chunk; returns the number of
 // accepted neighbours.
-template <int DIM, class Pol>
+template <int DIM, class Pol, int TILE = kTile, int BUF = kBuf>
 __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState& S,
                                            const Chunk& ch, const Home& h, Pol& pol,
                                            float4* __restrict__ tile, uint32_t* __restrict__ buf,
@@ -306,9 +313,9 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
     const float ay = h.a.y - (float)dy * h.G.cs1;
     const float az = DIM == 3 ? h.a.z - (float)dz * h.G.cs2 : 0.f;
     const uint32_t kadd = (uint32_t)(2 - crel + 5 * (dy + 2) + 25 * (dz + 2)) << kCodeShift;
-    // the row in batches of at most kTile records (one batch in practice)
-    for (int base = 0; base < tot; base += kTile) {
-      const int bend = min(tot, base + kTile);
+    // the row in batches of at most TILE records (one batch in practice)
+    for (int base = 0; base < tot; base += TILE) {
+      const int bend = min(tot, base + TILE);
       for (int t = base + lane; t < bend; t += kWave) {
         const int piece = t < len0 ? 0 : (t < len0 + len1 ? 1 : 2);
         const int g = seq_particle(rp, t);
@@ -320,7 +327,7 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
       __syncthreads();
       const int lb = max(tb, base) - base, le = min(te, bend) - base;
       for (int t = lb; __ballot(t < le) != 0ull; t += 2) {
-        if (__ballot(cnt > kBuf - 2) != 0ull) flush(false);
+        if (__ballot(cnt > BUF - 2) != 0ull) flush(false);
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
           const int tt = t + u;
@@ -458,10 +465,10 @@ __device__ __forceinline__ void write_force_records(const DevParams& P, const De
   }
 }
 
-template <int DIM, bool MIXED>
+template <int DIM, bool MIXED, int TILE, int BUF>
 __global__ __launch_bounds__(kWave) void k_density(DevParams P, DeviceState S) {
-  __shared__ float4 tile[kTile];
-  __shared__ uint32_t buf[kBuf * kWave];
+  __shared__ float4 tile[TILE];
+  __shared__ uint32_t buf[BUF * kWave];
   Chunk ch;
   if (!load_chunk(P, S, ch)) return;
   const Home h = load_home<DIM>(P, S, ch);
@@ -471,12 +478,14 @@ __global__ __launch_bounds__(kWave) void k_density(DevParams P, DeviceState S) {
   if (MIXED) {
     DensityMixed<DIM> pol{P, S, h, KahanF{}};
     pol.acc.add(P.density_type == 1 ? (float)(P.w0 * mi) : (float)P.w0);
-    cnt = staged_scan<DIM>(P, S, ch, h, pol, tile, buf, rechecks, S.nlist);
+    cnt = staged_scan<DIM, DensityMixed<DIM>, TILE, BUF>(P, S, ch, h, pol, tile, buf, rechecks,
+                                                          S.nlist);
     rho = pol.acc.value();
   } else {
     // self term first (particles_compute_density.F90:369-558)
     DensityF64<DIM> pol{P, S, h, P.density_type == 1 ? P.w0 * mi : P.w0};
-    cnt = staged_scan<DIM>(P, S, ch, h, pol, tile, buf, rechecks, S.nlist);
+    cnt = staged_scan<DIM, DensityF64<DIM>, TILE, BUF>(P, S, ch, h, pol, tile, buf, rechecks,
+                                                        S.nlist);
     rho = pol.acc;
   }
   if (!h.active) return;
@@ -944,18 +953,35 @@ static inline int nchunks_max(const DevParams& P) {
   return (m + 7) / 8 * 8;
 }
 
-void launch_density(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed) {
+template <int DIM, bool MIXED>
+static void launch_density_shape(hipStream_t s, const DevParams& P, const DeviceState& S) {
   dim3 g(nchunks_max(P)), b(kWave);
+  switch (P.density_shape) {
+    case 1:
+      hipLaunchKernelGGL((k_density<DIM, MIXED, 128, 16>), g, b, 0, s, P, S);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_density<DIM, MIXED, 224, 12>), g, b, 0, s, P, S);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_density<DIM, MIXED, 128, 12>), g, b, 0, s, P, S);
+      break;
+    default:
+      hipLaunchKernelGGL((k_density<DIM, MIXED, kTile, kBuf>), g, b, 0, s, P, S);
+  }
+}
+
+void launch_density(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed) {
   if (P.dim == 3) {
     if (mixed)
-      hipLaunchKernelGGL((k_density<3, true>), g, b, 0, s, P, S);
+      launch_density_shape<3, true>(s, P, S);
     else
-      hipLaunchKernelGGL((k_density<3, false>), g, b, 0, s, P, S);
+      launch_density_shape<3, false>(s, P, S);
   } else {
     if (mixed)
-      hipLaunchKernelGGL((k_density<2, true>), g, b, 0, s, P, S);
+      launch_density_shape<2, true>(s, P, S);
     else
-      hipLaunchKernelGGL((k_density<2, false>), g, b, 0, s, P, S);
+      launch_density_shape<2, false>(s, P, S);
   }
 }
 

@@ -70,6 +70,11 @@ class CParams(C.Structure):
     ]
 
 
+class CTuning(C.Structure):
+    """struct sdpd_tuning of include/sdpd.h."""
+    _fields_ = [("density_shape", C.c_int32), ("chunk_order", C.c_int32)]
+
+
 # (name, restype, argtypes) of every symbol include/sdpd.h declares
 _P = C.c_void_p
 _DP = C.POINTER(C.c_double)
@@ -96,6 +101,9 @@ SYMBOLS = [
     ("sdpd_download_tensors", C.c_int, [_P, _DP, _DP]),
     ("sdpd_step", C.c_int, [_P, C.c_int32]),
     ("sdpd_set_graphs", C.c_int, [_P, C.c_int32]),
+    ("sdpd_set_tuning", C.c_int, [_P, C.POINTER(CTuning)]),
+    ("sdpd_get_tuning", C.c_int, [_P, C.POINTER(CTuning)]),
+    ("sdpd_autotune", C.c_int, [_P, C.c_int32, C.POINTER(CTuning), _DP]),
     ("sdpd_set_step", C.c_int, [_P, C.c_int64]),
     ("sdpd_get_step", C.c_int, [_P, _LP]),
     ("sdpd_get_cells", C.c_int, [_P, _IP, _IP, _IP]),
@@ -313,6 +321,24 @@ class SDPD:
         """Run sdpd_step's steps through captured HIP graphs (default) or
         launch every kernel directly."""
         self._check(self.lib.sdpd_set_graphs(self.h, 1 if enable else 0), "sdpd_set_graphs")
+
+    def tuning(self):
+        """(density_shape, chunk_order) in use."""
+        t = CTuning()
+        self._check(self.lib.sdpd_get_tuning(self.h, C.byref(t)), "sdpd_get_tuning")
+        return int(t.density_shape), int(t.chunk_order)
+
+    def set_tuning(self, density_shape: int, chunk_order: int) -> None:
+        t = CTuning(int(density_shape), int(chunk_order))
+        self._check(self.lib.sdpd_set_tuning(self.h, C.byref(t)), "sdpd_set_tuning")
+
+    def autotune(self, reps: int = 3):
+        """Time every launch shape on the current state, apply the fastest;
+        returns ((density_shape, chunk_order), ms per density + force pass)."""
+        t = CTuning()
+        ms = C.c_double()
+        self._check(self.lib.sdpd_autotune(self.h, int(reps), C.byref(t), C.byref(ms)), "sdpd_autotune")
+        return (int(t.density_shape), int(t.chunk_order)), ms.value
 
     def integrate_VV(self, nsteps: int = 1) -> None:
         self._check(self.lib.sdpd_step(self.h, int(nsteps)), "marching_integrate_VV")

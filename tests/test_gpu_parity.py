@@ -457,3 +457,28 @@ def test_graph_steps_match_direct_launches(variant):
                     g.build_list()
                     g.compute_density()
                     g.compute_interaction()
+
+
+def test_launch_shapes_give_identical_results():
+    box = BOXES["tait3d_sym_noise"]()
+    announce("launch shapes and autotuner")
+    outs = {}
+    for prec_box in (box, W.mixed(box)):
+        with SDPD(prec_box.prm) as g:
+            g.upload(prec_box.x, prec_box.v, prec_box.m, prec_box.pid)
+            g.step = 3
+            g.build_list()
+            ref = None
+            for shape in range(4):
+                for order in (1, 0):
+                    g.set_tuning(shape, order)
+                    g.compute_density()
+                    g.compute_interaction()
+                    out = (g.download_density()[0], g.download_force())
+                    if ref is None:
+                        ref = out
+                    assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1])
+            (shape, order), ms = g.autotune(2)
+            assert g.tuning() == (shape, order) and ms > 0
+            assert np.array_equal(g.download_force(), ref[1])
+            outs[prec_box.prm.precision] = ref

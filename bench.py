@@ -6,7 +6,9 @@ dx = 1, quintic kernel with cut-off 3 dx, gamma 7, c = 10, eta = 0.1,
 Maxwellian velocities and thermal noise at kT = 1e-3, dt = 5e-5, in plain
 FP64 (--precision fp64).  One step = one full velocity-Verlet step of every
 particle: step boundary (kicks, drift, wrap, cell keys), cell sort,
-density + EOS, pair forces with noise.
+density + EOS, pair forces with noise.  --workload ensemble runs a batch of
+256 independent Oldroyd-B boxes (mixed precision) in one context instead.
+The launch shapes come from the autotuner (cached per workload class).
 
 Prints one JSON line.  value = particle-steps/s of the whole job with the
 state resident in HBM.  roofline = the pair-force kernel's stencil-model
@@ -34,7 +36,9 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 # records per particle): FP64 force record 68 B -> 27*68 + f 24 = 1860 B;
 # density record 24 B -> 27*24 + n, p 16 = 664 B + self 8 = 672 B.
 FORCE_BYTES = {"fp64": 1860.0, "mixed": 996.0}
-DENSITY_BYTES = {"fp64": 672.0, "mixed": 340.0}
+# Oldroyd-B adds Pi to the force record: 6 components, FP64 48 B / FP32 24 B
+# (FP64 record 68 + 48 = 116 B, mixed 36 + 24 = 60 B), plus f 24 B.
+OB_FORCE_BYTES = {"fp64": 27 * 116.0 + 24.0, "mixed": 27 * 60.0 + 24.0}
 
 
 def log(msg: str) -> None:
@@ -98,8 +102,14 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["tait128", "ensemble"], default="tait128",
+                    help="tait128: the 3D Tait box (size from --n-side); ensemble: a batch of "
+                         "independent Oldroyd-B boxes (count from --nrep)")
     ap.add_argument("--n-side", type=int, default=128)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--nrep", type=int, default=256)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
+                    help="default: fp64 for tait128, mixed for ensemble")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 1 GiB (calibrates the HBM byte counters)")
@@ -110,17 +120,42 @@ def main() -> int:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU design is described but not built", file=sys.stderr)
         return 2
+    if args.precision is None:
+        args.precision = "fp64" if args.workload == "tait128" else "mixed"
 
     from mcf_amd import PRECISION_FP64, PRECISION_MIXED
+    from mcf_amd import autotune as A
     from mcf_amd import workloads as W
     from mcf_amd.sdpd import SDPD
 
     prec = PRECISION_FP64 if args.precision == "fp64" else PRECISION_MIXED
-    box = W.tait_box_3d(args.n_side, precision=prec, kt=1e-3)
-    log(f"bench: {box.name} n={box.n} precision={args.precision} warmup={args.warmup} steps={args.steps}")
+    if args.workload == "tait128":
+        box = W.tait_box_3d(args.n_side, precision=prec, kt=1e-3)
+        nrep, volume = 1, float(args.n_side) ** 3
+        workload = (f"3D periodic Tait box {args.n_side}^3 = {box.n} particles, dx 1, cut-off 3, "
+                    f"gamma 7, c 10, eta 0.1, kT 1e-3, dt 5e-5, {args.precision}")
+        force_bytes = FORCE_BYTES[args.precision]
+    else:
+        box = W.oldroyd_ensemble(args.nrep, precision=prec)
+        nrep = box.nrep
+        volume = float(np.prod(box.box_hi - box.box_lo, axis=1).sum())
+        workload = (f"batch of {nrep} periodic Oldroyd-B boxes (N log-uniform in [2000, 60000], "
+                    f"edge round(N^(1/3)) dx), {box.n} particles in total, tau 0.1, eta_p 0.05, "
+                    f"eta 0.1, kT 1e-3, dt 5e-5, {args.precision}")
+        force_bytes = OB_FORCE_BYTES[args.precision]
+    log(f"bench: {box.name} n={box.n} nrep={nrep} precision={args.precision} "
+        f"warmup={args.warmup} steps={args.steps}")
     g = SDPD(box.prm)
-    g.upload(box.x, box.v, box.m, box.pid)
-    g.integrate_VV(args.warmup)
+    if nrep > 1 or args.workload == "ensemble":
+        g.set_replicas(box.box_lo, box.box_hi)
+        g.upload_replicas(box.x, box.v, box.m, box.pid, box.rep)
+    else:
+        g.upload(box.x, box.v, box.m, box.pid)
+    g.integrate_VV(max(args.warmup, 1))
+    tuned = None
+    if not args.no_autotune:
+        tuned = A.tune(g, box.prm, box.n, volume, nrep)
+        log(f"bench: launch shape {tuned['shape']} ({'cached' if tuned['cached'] else 'tuned'})")
     g.sync()
     log("bench: warmup done")
     t0 = time.perf_counter()
@@ -129,7 +164,7 @@ def main() -> int:
     t1 = time.perf_counter()
     wall = t1 - t0
     pairs_ordered, _ = g.counters()
-    unordered = pairs_ordered / 2.0 if box.prm.symmetry else pairs_ordered / 2.0
+    unordered = pairs_ordered / 2.0
     log(f"bench: timed {args.steps} steps in {wall:.3f} s")
 
     # per-phase kernel times from HIP events on the context stream
@@ -140,8 +175,7 @@ def main() -> int:
     g.set_timing(False)
     g.close()
     t_force = ph["forces"][0] / max(ph["forces"][1], 1) / 1e3
-    t_dens = ph["density"][0] / max(ph["density"][1], 1) / 1e3
-    force_gbs = FORCE_BYTES[args.precision] * box.n / t_force / 1e9
+    force_gbs = force_bytes * box.n / t_force / 1e9
     log("bench: phases " + json.dumps({k: round(v[0] / max(v[1], 1), 4) for k, v in ph.items()}))
 
     out = {
@@ -156,14 +190,15 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32 pairs / f64 state",
-        "data": "synthetic (seeded jittered lattice, Maxwellian velocities)",
+        "data": "synthetic (seeded jittered lattices, Maxwellian velocities)",
         "config": {
-            "workload": f"3D periodic Tait box {args.n_side}^3 = {box.n} particles, dx 1, cut-off 3, "
-                        f"gamma 7, c 10, eta 0.1, kT 1e-3, dt 5e-5, {args.precision}",
+            "workload": workload,
             "particles": box.n,
+            "replicas": nrep,
             "precision": args.precision,
+            "launch_shape": None if tuned is None else tuned["shape"],
         },
-        "pair_interactions_per_s": unordered * args.steps / wall / 1.0,
+        "pair_interactions_per_s": unordered * args.steps / wall,
         "pairs_per_step": unordered,
         "phase_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in ph.items()},
         "roofline": {
@@ -173,14 +208,11 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": force_gbs / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.n_side, args.precision),
-            "bytes_per_particle": FORCE_BYTES[args.precision],
+            "traffic": measured_traffic(args.n_side, args.precision) if args.workload == "tait128" else None,
+            "bytes_per_particle": force_bytes,
             "launch_ms": t_force * 1e3,
-            "density_gbs": DENSITY_BYTES[args.precision] * box.n / t_dens / 1e9,
         },
     }
-    # per-step pair count is per evaluation; the whole-job rate uses it per step
-    out["pair_interactions_per_s"] = unordered / (wall / args.steps) / 1.0 if wall > 0 else 0.0
     if not args.no_cpu_baseline:
         log("bench: cpu baseline (oracle, one core)")
         out["cpu_baseline"] = cpu_baseline()

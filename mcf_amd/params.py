@@ -51,6 +51,13 @@ class SDPDParams:
     kt_p: float = 0.0
     body_force: List[float] = field(default_factory=lambda: [0.0, 0.0, 0.0])
     random_seed: int = 1
+    # Lees-Edwards faces (boundary config): le_dim = the dimension whose two
+    # faces are Lees-Edwards, 1-based like the reference's bcdef faces
+    # (2 = y, 3 = z; 0 = none), le_v_lower / le_v_upper = the face
+    # velocities shear_v0(:, 2 le_dim - 1) and shear_v0(:, 2 le_dim)
+    le_dim: int = 0
+    le_v_lower: List[float] = field(default_factory=lambda: [0.0, 0.0, 0.0])
+    le_v_upper: List[float] = field(default_factory=lambda: [0.0, 0.0, 0.0])
     # device-path options
     precision: int = PRECISION_FP64
 
@@ -58,6 +65,8 @@ class SDPDParams:
         self.min_phys = [float(v) for v in list(self.min_phys) + [0.0] * (3 - len(self.min_phys))][:3]
         self.max_phys = [float(v) for v in list(self.max_phys) + [1.0] * (3 - len(self.max_phys))][:3]
         self.body_force = [float(v) for v in list(self.body_force) + [0.0] * (3 - len(self.body_force))][:3]
+        self.le_v_lower = [float(v) for v in list(self.le_v_lower) + [0.0] * (3 - len(self.le_v_lower))][:3]
+        self.le_v_upper = [float(v) for v in list(self.le_v_upper) + [0.0] * (3 - len(self.le_v_upper))][:3]
 
     @property
     def kt_effective(self) -> float:

@@ -47,6 +47,11 @@ class OrcParams(C.Structure):
         ("kt_p", C.c_double),
         ("body_force", C.c_double * 3),
         ("seed", C.c_uint64),
+        ("le_dim", C.c_int32),
+        ("pad1", C.c_int32),
+        ("le_vlo", C.c_double * 3),
+        ("le_vhi", C.c_double * 3),
+        ("le_time", C.c_double),
     ]
 
 
@@ -89,8 +94,9 @@ def lib():
     return _lib
 
 
-def to_orc(prm) -> OrcParams:
-    """Convert an SDPDParams-like object into the oracle's record."""
+def to_orc(prm, le_time: float = 0.0) -> OrcParams:
+    """Convert an SDPDParams-like object into the oracle's record; le_time
+    is the time the Lees-Edwards shear length is evaluated at."""
     o = OrcParams()
     o.num_dim = int(prm.num_dim)
     o.symmetry = 1 if prm.symmetry else 0
@@ -118,6 +124,11 @@ def to_orc(prm) -> OrcParams:
     o.n_p = float(prm.n_p)
     o.kt_p = float(prm.kt_p)
     o.seed = int(prm.random_seed) & 0xFFFFFFFFFFFFFFFF
+    o.le_dim = int(getattr(prm, "le_dim", 0))
+    for d in range(3):
+        o.le_vlo[d] = float(getattr(prm, "le_v_lower", [0.0] * 3)[d])
+        o.le_vhi[d] = float(getattr(prm, "le_v_upper", [0.0] * 3)[d])
+    o.le_time = float(le_time)
     return o
 
 
@@ -161,34 +172,35 @@ def rank(dim, x, xmin, xmax, nm, ngl):
     return lhbx, lpdx[: cnt.value]
 
 
-def ghosts(prm, x):
+def ghosts(prm, x, le_time=0.0):
     x = _f64(x)
     n, dim = x.shape
     cap = 8 * n + 16
     xg = np.zeros((cap, dim))
     owner = np.zeros(cap, np.int32)
     ng = C.c_int32()
-    _check(lib().orc_ghost_get(C.byref(to_orc(prm)), x, n, xg, owner, cap, C.byref(ng)), "ghost_get")
+    _check(lib().orc_ghost_get(C.byref(to_orc(prm, le_time)), x, n, xg, owner, cap, C.byref(ng)),
+           "ghost_get")
     return xg[: ng.value].copy(), owner[: ng.value].copy()
 
 
-def neighbor_pairs(prm, x, brute=False):
+def neighbor_pairs(prm, x, brute=False, le_time=0.0):
     x = _f64(x)
     n = x.shape[0]
     cap = max(1024, n * 200)
     pairs = np.zeros(2 * cap, np.int32)
     np_ = C.c_int64()
     fn = lib().orc_brute_pairs if brute else lib().orc_neighbor_pairs
-    _check(fn(C.byref(to_orc(prm)), x, n, pairs, cap, C.byref(np_)), "pairs")
+    _check(fn(C.byref(to_orc(prm, le_time)), x, n, pairs, cap, C.byref(np_)), "pairs")
     return pairs[: 2 * np_.value].reshape(-1, 2).copy()
 
 
-def density(prm, x, m, sid=None):
+def density(prm, x, m, sid=None, le_time=0.0):
     x = _f64(x)
     n = x.shape[0]
     sid = np.zeros(n, np.int32) if sid is None else _i32(sid)
     rho = np.zeros(n)
-    _check(lib().orc_density(C.byref(to_orc(prm)), x, _f64(m), sid, n, rho), "density")
+    _check(lib().orc_density(C.byref(to_orc(prm, le_time)), x, _f64(m), sid, n, rho), "density")
     return rho
 
 
@@ -199,7 +211,7 @@ def pressure(prm, rho, m):
     return p
 
 
-def interaction(prm, step, x, v, rho, press, m, pid, sid=None, ct=None):
+def interaction(prm, step, x, v, rho, press, m, pid, sid=None, ct=None, le_time=0.0):
     x = _f64(x)
     n, dim = x.shape
     sid = np.zeros(n, np.int32) if sid is None else _i32(sid)
@@ -209,7 +221,7 @@ def interaction(prm, step, x, v, rho, press, m, pid, sid=None, ct=None):
     if ct is not None:
         ct = _f64(ct)
         ctp = ct.ctypes.data_as(C.c_void_p)
-    _check(lib().orc_interaction(C.byref(to_orc(prm)), int(step), x, _f64(v), _f64(rho), _f64(press),
+    _check(lib().orc_interaction(C.byref(to_orc(prm, le_time)), int(step), x, _f64(v), _f64(rho), _f64(press),
                                  _f64(m), _i32(pid), sid, ctp, n, f,
                                  vgt.ctypes.data_as(C.c_void_p)), "interaction")
     return f, vgt
@@ -256,6 +268,7 @@ class State:
         self.ct = np.array(ct, np.float64, copy=True).reshape(n, d2)
         self.act = np.zeros((n, d2))
         self.vgt = np.zeros((n, d2))
+        self.time = 0.0  # time_current (accumulated by repeated addition of dt)
 
     def _args(self):
         vp = C.c_void_p
@@ -263,14 +276,20 @@ class State:
                 self.vgt.ctypes.data_as(vp))
 
     def prologue(self, step=0):
+        # marching_marching.F90:437: the boundary is updated at the current
+        # time before the initial evaluation
         n = self.x.shape[0]
-        _check(lib().orc_prologue(C.byref(to_orc(self.prm)), int(step), n, self.x, self.v, self.m,
-                                  self.pid, self.sid, *self._args()), "prologue")
+        _check(lib().orc_prologue(C.byref(to_orc(self.prm, self.time)), int(step), n, self.x, self.v,
+                                  self.m, self.pid, self.sid, *self._args()), "prologue")
 
     def vv_step(self, step):
+        # marching_marching.F90:1392-1404 advances time_current before the
+        # step; marching_integrate_VV.F90:784-786 evaluates the shear length
+        # at time_current + dt
         n = self.x.shape[0]
-        _check(lib().orc_vv_step(C.byref(to_orc(self.prm)), int(step), n, self.x, self.v, self.m,
-                                 self.pid, self.sid, *self._args()), "vv_step")
+        self.time = self.time + self.prm.dt
+        _check(lib().orc_vv_step(C.byref(to_orc(self.prm, self.time + self.prm.dt)), int(step), n,
+                                 self.x, self.v, self.m, self.pid, self.sid, *self._args()), "vv_step")
 
     def statistics(self):
         n = self.x.shape[0]

@@ -230,6 +230,19 @@ static int ghost_count_bound(const orc_params *p, int32_t n) {
   return 8 * n + 16; /* at most 7 images per particle in 3D (3 in 2D) */
 }
 
+/* Fortran MODULO(a, p) for reals: the exact remainder with the sign of p
+ * (computed through fmod, which is exact, as gfortran does). */
+static double f_modulo(double a, double p) {
+  double r = fmod(a, p);
+  if (r != 0.0 && ((r < 0.0) != (p < 0.0))) r = r + p;
+  return r;
+}
+
+/* index of the Lees-Edwards dimension (0-based), -1 when none */
+static int le_index(const orc_params *p) {
+  return p->le_dim >= 1 && p->le_dim <= p->num_dim ? p->le_dim - 1 : -1;
+}
+
 int orc_ghost_get(const orc_params *p, const double *x, int32_t n, double *xg,
                   int32_t *owner, int32_t cap, int32_t *nghost) {
   const int dim = p->num_dim;
@@ -242,7 +255,49 @@ int orc_ghost_get(const orc_params *p, const double *x, int32_t n, double *xg,
   if (!xt || !ig) { free(xt); free(ig); return -1; }
   memcpy(xt, x, sizeof(double) * (size_t)dim * n);
   for (int32_t i = 0; i < n; ++i) ig[i] = i;
+  const int gle = le_index(p);
+  if (gle >= 0) {
+    /* Lees-Edwards images first (ppm_map_part_ghost_get.F90:777-793,
+     * ppm_bcdef_le_y.inc): across the lower face the copy moves up by L
+     * and the other coordinates by MODULO(x + shear_length(:,upper), L);
+     * without symmetry also across the upper face, moved down by L and by
+     * MODULO(x + shear_length(:,lower), L), shear_length(:,lower) = -S. */
+    const int d = gle;
+    const double L = p->max_phys[d] - p->min_phys[d];
+    int32_t nplus = total;
+    double lo_f = p->min_phys[d], lo_i = p->min_phys[d] + g;
+    for (int32_t i = 0; i < nplus; ++i) {
+      double xv = xt[d + (size_t)dim * i];
+      if (xv >= lo_f && xv < lo_i) {
+        for (int e = 0; e < dim; ++e) {
+          double Le = p->max_phys[e] - p->min_phys[e];
+          double sl = (p->le_vhi[e] - p->le_vlo[e]) * p->le_time;
+          xt[e + (size_t)dim * total] =
+              e == d ? xv + L : f_modulo(xt[e + (size_t)dim * i] + sl, Le);
+        }
+        ig[total] = ig[i];
+        ++total;
+      }
+    }
+    if (!p->symmetry) {
+      double hi_f = p->max_phys[d], hi_i = p->max_phys[d] - g;
+      for (int32_t i = 0; i < nplus; ++i) {
+        double xv = xt[d + (size_t)dim * i];
+        if (xv > hi_i && xv <= hi_f) {
+          for (int e = 0; e < dim; ++e) {
+            double Le = p->max_phys[e] - p->min_phys[e];
+            double sl = (p->le_vlo[e] - p->le_vhi[e]) * p->le_time;
+            xt[e + (size_t)dim * total] =
+                e == d ? xv - L : f_modulo(xt[e + (size_t)dim * i] + sl, Le);
+          }
+          ig[total] = ig[i];
+          ++total;
+        }
+      }
+    }
+  }
   for (int d = 0; d < dim; ++d) {
+    if (d == gle) continue;
     double L = p->max_phys[d] - p->min_phys[d];
     int32_t nplus = total;
     double lo_f = p->min_phys[d], lo_i = p->min_phys[d] + g;
@@ -491,9 +546,24 @@ int orc_brute_pairs(const orc_params *p, const double *x, int32_t n,
       for (int img = 0; img < nimg; ++img) {
         int s[3] = {img % 3 - 1, (img / 3) % 3 - 1, dim == 3 ? img / 9 - 1 : 0};
         double d2 = 0.0;
+        const int gle = le_index(p);
         for (int d = 0; d < dim; ++d) {
           double L = p->max_phys[d] - p->min_phys[d];
           double xi = x[d + (size_t)dim * i], xj = x[d + (size_t)dim * j];
+          if (gle >= 0 && d != gle && s[gle] != 0) {
+            /* across a Lees-Edwards face the image of the shifted particle
+             * (j above: j's upper copy; below: i's upper copy with
+             * symmetry, j's lower copy without) is moved in the other
+             * coordinates by MODULO(x + shear_length, L) first */
+            double up = (p->le_vhi[d] - p->le_vlo[d]) * p->le_time;
+            double dn = (p->le_vlo[d] - p->le_vhi[d]) * p->le_time;
+            if (s[gle] > 0)
+              xj = f_modulo(xj + up, L);
+            else if (p->symmetry)
+              xi = f_modulo(xi + up, L);
+            else
+              xj = f_modulo(xj + dn, L);
+          }
           double r;
           if (s[d] == 0) {
             r = xi - xj;
@@ -894,6 +964,24 @@ static int interaction_on_cells(const orc_params *p, const orc_cells *c,
     ma[i] = m[o];
     ia[i] = pid[o];
   }
+  {
+    /* Lees-Edwards images move with the face they crossed
+     * (particles_set_boundary_ghost_velocity.F90:175-240) */
+    const int gle = le_index(p);
+    if (gle >= 0)
+      for (int32_t i = c->nreal; i < na; ++i) {
+        double xg = c->xall[gle + (size_t)dim * i];
+        if (xg < p->min_phys[gle]) {
+          for (int k = 0; k < dim; ++k)
+            if (k != gle)
+              va[k + (size_t)dim * i] = va[k + (size_t)dim * i] + p->le_vlo[k] - p->le_vhi[k];
+        } else if (xg >= p->max_phys[gle]) {
+          for (int k = 0; k < dim; ++k)
+            if (k != gle)
+              va[k + (size_t)dim * i] = va[k + (size_t)dim * i] + p->le_vhi[k] - p->le_vlo[k];
+        }
+      }
+  }
   if (!p->newtonian) {
     /* particles_compute_pressure_tensor.F90:114-133, G = n_p kt_p */
     double G = p->n_p * p->kt_p;
@@ -990,14 +1078,45 @@ static void integrate_position(const orc_params *p, int32_t n, double *x,
         x[d + (size_t)dim * i] = x[d + (size_t)dim * i] + v[d + (size_t)dim * i] * p->dt;
 }
 
-static void adjust_periodic(const orc_params *p, int32_t n, double *x) {
+/* particles_adjust_particles.F90:137-389, periodic and Lees-Edwards
+ * faces: crossing a Lees-Edwards face moves the other coordinates by
+ * MODULO(x - shear_length, L) and adds the face velocity jump, and ends
+ * the particle's check (EXIT). */
+static void adjust_particles(const orc_params *p, int32_t n, double *x, double *v) {
   const int dim = p->num_dim;
+  const int gle = le_index(p);
   for (int32_t i = 0; i < n; ++i)
     for (int d = 0; d < dim; ++d) {
       double L = p->max_phys[d] - p->min_phys[d];
       double *xv = x + d + (size_t)dim * i;
-      if (*xv < p->min_phys[d]) *xv = *xv + L;
-      if (*xv >= p->max_phys[d]) *xv = *xv - L;
+      if (*xv < p->min_phys[d]) {
+        if (d == gle) {
+          *xv = *xv + L;
+          for (int k = 0; k < dim; ++k)
+            if (k != d) {
+              double Lk = p->max_phys[k] - p->min_phys[k];
+              double sl = (p->le_vlo[k] - p->le_vhi[k]) * p->le_time; /* shear_length(k, lower) */
+              x[k + (size_t)dim * i] = f_modulo(x[k + (size_t)dim * i] - sl, Lk);
+              v[k + (size_t)dim * i] = v[k + (size_t)dim * i] + (p->le_vhi[k] - p->le_vlo[k]);
+            }
+          break;
+        }
+        *xv = *xv + L;
+      }
+      if (*xv >= p->max_phys[d]) {
+        if (d == gle) {
+          *xv = *xv - L;
+          for (int k = 0; k < dim; ++k)
+            if (k != d) {
+              double Lk = p->max_phys[k] - p->min_phys[k];
+              double sl = (p->le_vhi[k] - p->le_vlo[k]) * p->le_time; /* shear_length(k, upper) */
+              x[k + (size_t)dim * i] = f_modulo(x[k + (size_t)dim * i] - sl, Lk);
+              v[k + (size_t)dim * i] = v[k + (size_t)dim * i] + (p->le_vlo[k] - p->le_vhi[k]);
+            }
+          break;
+        }
+        *xv = *xv - L;
+      }
     }
 }
 
@@ -1040,7 +1159,7 @@ int orc_vv_step(const orc_params *p, int64_t step, int32_t n, double *x,
   integrate_velocity(p, n, v, f, sid, 0.5);
   integrate_position(p, n, x, v, sid);
   if (!p->newtonian) integrate_ct(p, n, ct, act, 0.5);
-  adjust_periodic(p, n, x);
+  adjust_particles(p, n, x, v);
   int rc = evaluate(p, step, n, x, v, m, pid, rho, press, f, ct, act, vgt);
   if (rc) return rc;
   if (!p->newtonian) integrate_ct(p, n, ct, act, 0.5);

@@ -39,6 +39,17 @@ typedef struct orc_params {
   double tau, n_p, kt_p;
   double body_force[3];
   uint64_t seed;
+  /* Lees-Edwards faces: le_dim = 0 none, 2 (y) or 3 (z) 1-based as the
+   * reference's bcdef index; le_vlo / le_vhi = velocities of the lower and
+   * upper face (shear_v0(:, 2i-1), shear_v0(:, 2i)); le_time = the time the
+   * shear length is evaluated at: shear_length(:, upper) = (vhi - vlo) *
+   * le_time (boundary_update_boundary.F90:102-121).  min_phys must be 0 in
+   * every dimension (the reference's MODULO wraps into [0, L)). */
+  int32_t le_dim;
+  int32_t pad1;
+  double le_vlo[3];
+  double le_vhi[3];
+  double le_time;
 } orc_params;
 
 /* smoothing kernel W(r), W'(r) */

@@ -149,6 +149,21 @@ int sdpd_download_tensors(sdpd_ctx *ctx, double *ct, double *vgt);
  * unless sdpd_forces has run since. */
 int sdpd_step(sdpd_ctx *ctx, int32_t nsteps);
 
+/* Lees-Edwards faces (the reference's bcdef = LE on both faces of one
+ * dimension, boundary_update_boundary.F90:102-121): le_dim = 2 (y) or 3
+ * (z), 1-based as the reference's face pairs, 0 switches them off;
+ * v_lower / v_upper[3] = the face velocities shear_v0(:, 2 le_dim - 1) and
+ * shear_v0(:, 2 le_dim).  The faces may only shear along x.  Images across
+ * the faces are shifted along x by the shear length (v_upper - v_lower) t
+ * and carry the velocity jump (ppm_map_part_ghost_get.F90:777-793,
+ * particles_set_boundary_ghost_velocity.F90:175-240); particles crossing
+ * them are wrapped the same way (particles_adjust_particles.F90:228-257,
+ * 352-381).  t = time_current + dt of the step (the initial evaluation:
+ * time_current), time_current accumulating dt from step 0.  One box only,
+ * with min_phys = 0. */
+int sdpd_set_lees_edwards(sdpd_ctx *ctx, int32_t le_dim, const double *v_lower,
+                          const double *v_upper);
+
 /* sdpd_step runs pairs of steps through a captured HIP graph (default on,
  * and off while phase timing is enabled); enable = 0 launches every kernel
  * directly.  Results are identical either way. */

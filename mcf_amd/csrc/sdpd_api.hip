@@ -68,6 +68,18 @@ int fail(sdpd_ctx* c, int code, const std::string& msg) {
                   std::string(#call) + ": " + hipGetErrorString(e_));              \
   } while (0)
 
+// time_current after `step` marching steps, accumulated by repeated
+// addition of dt as marching_marching.F90:1393 does, and the time at which
+// that step evaluates the Lees-Edwards shear length (time_current + dt,
+// marching_integrate_VV.F90:784-786; the initial evaluation uses
+// time_current, marching_marching.F90:437)
+void marching_time(int64_t step, double dt, double* t, double* t_le) {
+  double tt = 0.0;
+  for (int64_t s = 0; s < step; ++s) tt = tt + dt;
+  *t = tt;
+  *t_le = step > 0 ? tt + dt : tt;
+}
+
 int check_launch(sdpd_ctx* c, const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
@@ -193,6 +205,7 @@ int derive(const sdpd_params& p, DevParams& P, std::string& err) {
   P.mixed = p.precision == SDPD_PRECISION_MIXED ? 1 : 0;
   P.density_shape = 0;
   P.xcd_order = 1;
+  P.le_g = -1;
   P.inv_hf = (float)(1.0 / P.h);
   P.coeff = (float)P.coef;
   P.coef_gradf = (float)P.coef_grad;
@@ -300,7 +313,7 @@ void free_state(DeviceState& S) {
                   S.rstart,   S.runsorted, S.rlpdx,  S.nlist,   S.nbr_sig_count,
                   S.row_nchunk, S.row_chunk_off, S.chunk_row, S.scan_tmp, S.rec64,
                   (void*)S.geo, S.rep,      S.rep_alt,  S.row_cell, S.row_len,  S.row_rep,
-                  S.rep_stats, S.qr2,     S.step_dev};
+                  S.rep_stats, S.qr2,     S.step_dev, S.time_dev};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   std::memset(&S, 0, sizeof(S));
@@ -368,6 +381,7 @@ int alloc_state(sdpd_ctx* c, int n) {
                          hipMemcpyHostToDevice));
     ALLOC(rep_stats, 4 * (size_t)P.nrep);
     ALLOC(step_dev, 1);
+    ALLOC(time_dev, 2);
   }
   {
     const size_t nrows = (size_t)P.nrows;
@@ -503,7 +517,7 @@ int run_forces(sdpd_ctx* c) {
 int step_body(sdpd_ctx* c, int closing) {
   c->step += 1;
   c->P.step = c->step;
-  sdpd::launch_advance_step(c->stream, c->S);
+  sdpd::launch_advance_step(c->stream, c->P, c->S);
   {
     PhaseMark pm(c, 0);
     sdpd::launch_boundary(c->stream, c->P, c->S, closing, 1);
@@ -683,7 +697,11 @@ int upload_impl(sdpd_ctx* c, const double* x, const double* v, const double* m,
     HIP_TRY(c, hipMemcpy(c->S.rep, rep, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
   else
     sdpd::launch_fill_i32(c->stream, c->S.rep, 0, n);
-  sdpd::launch_set_step(c->stream, c->S, c->step);
+  {
+    double t, tle;
+    marching_time(c->step, c->P.dt, &t, &tle);
+    sdpd::launch_set_step(c->stream, c->S, c->step, t, tle);
+  }
   sdpd::launch_fill_f64(c->stream, c->S.f, 0.0, (int64_t)dim * n);
   sdpd::launch_fill_f64(c->stream, c->S.rho, 0.0, n);
   sdpd::launch_fill_f64(c->stream, c->S.p, 0.0, n);
@@ -722,6 +740,8 @@ int sdpd_set_replicas(sdpd_ctx* c, int32_t nrep, const double* box_lo, const dou
   if (!c) return SDPD_ERR_ARG;
   if (nrep < 1 || !box_lo || !box_hi)
     return fail(c, SDPD_ERR_ARG, "sdpd_set_replicas: bad arguments");
+  if (c->P.le_g >= 0)
+    return fail(c, SDPD_ERR_STATE, "sdpd_set_replicas: Lees-Edwards faces are for one box only");
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   drop_graphs(c);
   free_state(c->S);
@@ -928,6 +948,36 @@ int sdpd_step(sdpd_ctx* c, int32_t nsteps) {
   return SDPD_OK;
 }
 
+int sdpd_set_lees_edwards(sdpd_ctx* c, int32_t le_dim, const double* v_lower,
+                          const double* v_upper) {
+  if (!c) return SDPD_ERR_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);  // kernel arguments change
+  if (le_dim == 0) {
+    c->P.le_g = -1;
+    return SDPD_OK;
+  }
+  if (!v_lower || !v_upper) return fail(c, SDPD_ERR_ARG, "sdpd_set_lees_edwards: null velocity");
+  const int dim = c->P.dim;
+  if (le_dim < 2 || le_dim > dim)
+    return fail(c, SDPD_ERR_ARG, "sdpd_set_lees_edwards: le_dim must be 2 (y) or 3 (z)");
+  if (c->P.nrep != 1 || !c->rep_lo.empty())
+    return fail(c, SDPD_ERR_STATE, "sdpd_set_lees_edwards: one box only (batched context)");
+  for (int d = 0; d < dim; ++d)
+    if (c->P.min_phys[d] != 0.0)
+      return fail(c, SDPD_ERR_ARG, "sdpd_set_lees_edwards: min_phys must be 0 (MODULO wraps into [0, L))");
+  for (int k = 1; k < dim; ++k)
+    if (k != le_dim - 1 && v_upper[k] - v_lower[k] != 0.0)
+      return fail(c, SDPD_ERR_ARG,
+                  "sdpd_set_lees_edwards: the faces may only shear along x (rows of cells)");
+  c->P.le_g = le_dim - 1;
+  c->P.le_vlo = v_lower[0];
+  c->P.le_vhi = v_upper[0];
+  c->P.le_u = v_upper[0] - v_lower[0];
+  c->P.le_uf = (float)c->P.le_u;
+  return SDPD_OK;
+}
+
 int sdpd_set_graphs(sdpd_ctx* c, int32_t enable) {
   if (!c) return SDPD_ERR_ARG;
   c->use_graphs = enable != 0;
@@ -1013,7 +1063,9 @@ int sdpd_set_step(sdpd_ctx* c, int64_t step) {
   if (!c) return SDPD_ERR_ARG;
   c->step = step;
   if (c->S.step_dev) {
-    sdpd::launch_set_step(c->stream, c->S, step);
+    double t, tle;
+    marching_time(step, c->P.dt, &t, &tle);
+    sdpd::launch_set_step(c->stream, c->S, step, t, tle);
     return check_launch(c, "sdpd_set_step");
   }
   return SDPD_OK;

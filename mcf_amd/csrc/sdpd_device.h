@@ -48,6 +48,11 @@ struct DevParams {
   // 2: 224, 12; 3: 128, 12) and chunk order (1: XCD-contiguous)
   int32_t density_shape;
   int32_t xcd_order;
+  // Lees-Edwards faces: le_g = 0-based dimension (1 or 2) whose faces shear
+  // along x, -1 when none; face velocities (x components) and their jump
+  int32_t le_g;
+  double le_vlo, le_vhi, le_u;  // le_u = le_vhi - le_vlo
+  float le_uf;
   // band around cut2 of the FP32 candidate test; inside it the decision is
   // taken in FP64.  FP32 positions are relative to the own cell corner.
   float cut2_lo_f32, cut2_hi_f32;
@@ -87,6 +92,14 @@ __device__ __forceinline__ int cell_coord(double xv, double x0, double rd, doubl
   if (k < 0) k = 0;
   if (k >= nm) k = nm - 1;
   return k;
+}
+
+// Fortran MODULO(a, p) for reals: the exact remainder (fmod) carrying the
+// sign of p.
+__device__ __forceinline__ double f_modulo(double a, double p) {
+  double r = fmod(a, p);
+  if (r != 0.0 && ((r < 0.0) != (p < 0.0))) r = r + p;
+  return r;
 }
 
 // global cell key of a position in replica geometry g

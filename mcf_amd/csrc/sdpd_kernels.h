@@ -50,6 +50,7 @@ struct DeviceState {
   int32_t *row_cell, *row_len, *row_rep;
   double* rep_stats;     // per replica (E_k, P_x, P_y, P_z)
   int64_t* step_dev;     // marching step read by the force pass (noise key)
+  double* time_dev;      // [0] time_current, [1] time of the Lees-Edwards shear length
   // reference-grid cell list (built on request by sdpd_get_cells)
   int32_t *rkey, *rslot, *rcount, *rstart, *runsorted, *rlpdx;
 };
@@ -82,8 +83,9 @@ void launch_stats(hipStream_t s, const DevParams& P, const DeviceState& S, int n
 void launch_replica_stats(hipStream_t s, const DevParams& P, const DeviceState& S);
 // the device copy of the marching step: set, or advance by one (a
 // captured step graph advances it itself)
-void launch_set_step(hipStream_t s, const DeviceState& S, int64_t step);
-void launch_advance_step(hipStream_t s, const DeviceState& S);
+void launch_set_step(hipStream_t s, const DeviceState& S, int64_t step, double time,
+                     double le_time);
+void launch_advance_step(hipStream_t s, const DevParams& P, const DeviceState& S);
 void launch_pair_normals(hipStream_t s, const uint32_t* lo, const uint32_t* hi, int count,
                          uint64_t seed, int64_t step, float* g);
 

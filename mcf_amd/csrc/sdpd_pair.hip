@@ -66,7 +66,22 @@ struct Geo {
   int nm0, nm1, nm2;    // sort-grid cells per dimension
   int cell_base, row_base;
   double L0, L1, L2;    // edge lengths
+  double csd0;          // FP64 cell edge along x
   float cs0, cs1, cs2;  // FP32 cell edges
+};
+
+// Lees-Edwards view of a lane (P.le_g >= 0).  Rows across the upper face
+// (sigma 0) see the neighbours' upper copies, shifted along x by the shear
+// length S_up = (v_upper - v_lower) t; rows across the lower face (sigma 1)
+// by S_down = -S_up.  In the frame shifted by sp = MODULO(S, Lx) the lane
+// sits in cell wc, dxf = (cx - wc) cs - sp is its FP32 offset from its own
+// cell corner in that frame, and wmin / wmax bound wc over the wave.
+struct LeLane {
+  double t;             // time of the shear length
+  double sp[2];
+  int wc[2];
+  float dxf[2];
+  int wmin[2], wmax[2];
 };
 
 // Per-lane context of the home particle.
@@ -78,7 +93,22 @@ struct Home {
   float4 a;             // FP32 position relative to the own cell corner
   double x0, x1, x2;    // FP64 position
   Geo G;
+  LeLane le;
 };
+
+// Lees-Edwards sign of the stencil row at offsets (dy, dz): 0 across the
+// upper face of the shearing dimension, 1 across the lower face, -1 none
+__device__ __forceinline__ int le_sigma(const DevParams& P, const Home& h, int dy, int dz) {
+  if (P.le_g == 1) {
+    const int c = h.cy + dy;
+    return c >= h.G.nm1 ? 0 : (c < 0 ? 1 : -1);
+  }
+  if (P.le_g == 2) {
+    const int c = h.cz + dz;
+    return c >= h.G.nm2 ? 0 : (c < 0 ? 1 : -1);
+  }
+  return -1;
+}
 
 // Image modes of a neighbour at offsets o from the home cell.
 struct Modes {
@@ -100,7 +130,13 @@ __device__ __forceinline__ double exact_d2(const DevParams& P, const double* __r
 template <int DIM>
 __device__ __forceinline__ void rel_diff(const DevParams& P, const Home& h, const float4& q,
                                          const Offsets& o, float& d0, float& d1, float& d2) {
-  d0 = h.a.x - (q.x + (float)o.ox * h.G.cs0);
+  float ax = h.a.x;
+  if (P.le_g >= 0) {
+    // across a Lees-Edwards face ox counts from the sheared window centre
+    const int sg = le_sigma(P, h, o.oy, o.oz);
+    if (sg >= 0) ax = ax + h.le.dxf[sg];
+  }
+  d0 = ax - (q.x + (float)o.ox * h.G.cs0);
   d1 = h.a.y - (q.y + (float)o.oy * h.G.cs1);
   d2 = DIM == 3 ? h.a.z - (q.z + (float)o.oz * h.G.cs2) : 0.f;
 }
@@ -142,6 +178,7 @@ __device__ __forceinline__ bool load_chunk(const DevParams& P, const DeviceState
   ch.G.L0 = g.L[0];
   ch.G.L1 = g.L[1];
   ch.G.L2 = g.L[2];
+  ch.G.csd0 = g.cs[0];
   ch.G.cs0 = g.csf[0];
   ch.G.cs1 = g.csf[1];
   ch.G.cs2 = g.csf[2];
@@ -182,24 +219,55 @@ __device__ __forceinline__ Home load_home(const DevParams& P, const DeviceState&
   h.x0 = S.x[h.ii];
   h.x1 = S.x[n + h.ii];
   h.x2 = DIM == 3 ? S.x[2 * n + h.ii] : 0.0;
+  if (P.le_g >= 0) {
+    // boxes with Lees-Edwards faces start at 0 (the reference's MODULO)
+    h.le.t = S.time_dev[1];
+    const double cs = ch.G.csd0;
+#pragma unroll
+    for (int sg = 0; sg < 2; ++sg) {
+      const double sl = sg == 0 ? (P.le_vhi - P.le_vlo) * h.le.t : (P.le_vlo - P.le_vhi) * h.le.t;
+      const double sp = f_modulo(sl, ch.G.L0);
+      const int wc = (int)floor((h.x0 - sp) / cs);
+      h.le.sp[sg] = sp;
+      h.le.wc[sg] = wc;
+      h.le.dxf[sg] = (float)((double)(h.cx - wc) * cs - sp);
+      int mn = h.active ? wc : INT_MAX, mx = h.active ? wc : INT_MIN;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        mn = min(mn, __shfl_xor(mn, off));
+        mx = max(mx, __shfl_xor(mx, off));
+      }
+      h.le.wmin[sg] = mn;
+      h.le.wmax[sg] = mx;
+    }
+  }
   return h;
 }
 
 // ---------------------------------------------------------------------
 // stencil rows of a chunk
 // ---------------------------------------------------------------------
-// One stencil row (dy, dz) of a chunk is three contiguous pieces of the
-// target row of cells, in virtual-cell order: [low wrap | main | high
-// wrap].  The low wrap holds the cells left of x = 0 seen through the
-// periodic face (virtual cell = physical - nm0), the high wrap those right
-// of x = nm0 - 1 (virtual = physical + nm0).  Sequence index t runs over
-// the three pieces in that order.
+// One stencil row (dy, dz) of a chunk covers the virtual cells
+// [vlo, vhi] of the target row of cells: virtual cell v is physical cell
+// v - m nm0 seen through m periodic x faces.  Lanes look at the five cells
+// around their own (home cell, or the Lees-Edwards window centre), so the
+// range is [xmin - 2, xmax + 2] (or the window centres' extremes +-2) and
+// splits into at most three pieces of one period each, contiguous in
+// memory: piece k spans virtual [a_k, a_k+1) of period m0 + k.  Sequence
+// index t runs over the pieces in that order.
 struct RowPieces {
   int rb;                 // first cell of the target row
   int my, mz;             // image modes of the row in y and z
+  int sigma;              // Lees-Edwards sign of the row (-1 none)
+  int vlo, a1, a2, aend;  // piece bounds in virtual cells
+  int m0;                 // period of piece 0
   int g0, len0, g1, len1, g2, len2;
   int tot;
 };
+
+__device__ __forceinline__ int floor_div(int a, int b) {
+  return a >= 0 ? a / b : -((-a + b - 1) / b);
+}
 
 template <int DIM>
 __device__ __forceinline__ RowPieces row_pieces(const DevParams& P, const DeviceState& S,
@@ -212,22 +280,73 @@ __device__ __forceinline__ RowPieces row_pieces(const DevParams& P, const Device
   const int ny = wrap_cell(h.cy + dy, h.G.nm1, P.symmetry, rp.my);
   const int nz = DIM == 3 ? wrap_cell(h.cz + dz, h.G.nm2, P.symmetry, rp.mz) : 0;
   rp.rb = h.G.cell_base + nm0 * (ny + h.G.nm1 * nz);
-  const int v_lo = ch.xmin - 2, v_hi = ch.xmax + 2;
-  const int ml = v_lo < 0 ? 0 : v_lo, mh = v_hi > nm0 - 1 ? nm0 - 1 : v_hi;
-  rp.g1 = start[rp.rb + ml];
-  rp.len1 = start[rp.rb + mh + 1] - rp.g1;
-  rp.g0 = v_lo < 0 ? start[rp.rb + v_lo + nm0] : start[rp.rb + nm0];
-  rp.len0 = v_lo < 0 ? start[rp.rb + nm0] - rp.g0 : 0;
-  rp.g2 = start[rp.rb];
-  rp.len2 = v_hi > nm0 - 1 ? start[rp.rb + v_hi - nm0 + 1] - rp.g2 : 0;
+  rp.sigma = P.le_g >= 0 ? le_sigma(P, h, dy, dz) : -1;
+  int vhi;
+  if (rp.sigma < 0) {
+    rp.vlo = ch.xmin - 2;
+    vhi = ch.xmax + 2;
+  } else {
+    rp.vlo = h.le.wmin[rp.sigma] - 2;
+    vhi = h.le.wmax[rp.sigma] + 2;
+  }
+  rp.aend = vhi + 1;
+  rp.m0 = floor_div(rp.vlo, nm0);
+  rp.a1 = min(rp.aend, (rp.m0 + 1) * nm0);
+  rp.a2 = min(rp.aend, (rp.m0 + 2) * nm0);
+  const int b0 = rp.m0 * nm0, b1 = b0 + nm0, b2 = b1 + nm0;
+  rp.g0 = start[rp.rb + rp.vlo - b0];
+  rp.len0 = start[rp.rb + rp.a1 - b0] - rp.g0;
+  rp.g1 = rp.a2 > rp.a1 ? start[rp.rb + rp.a1 - b1] : rp.g0;
+  rp.len1 = rp.a2 > rp.a1 ? start[rp.rb + rp.a2 - b1] - rp.g1 : 0;
+  rp.g2 = rp.aend > rp.a2 ? start[rp.rb + rp.a2 - b2] : rp.g0;
+  rp.len2 = rp.aend > rp.a2 ? start[rp.rb + rp.aend - b2] - rp.g2 : 0;
   rp.tot = rp.len0 + rp.len1 + rp.len2;
   return rp;
+}
+
+// sequence index of the first particle of virtual cell v (v in [vlo, aend])
+__device__ __forceinline__ int seq_of(const RowPieces& rp, const int32_t* __restrict__ start,
+                                      int nm0, int v) {
+  if (v >= rp.aend) return rp.tot;
+  const int b0 = rp.m0 * nm0;
+  if (v < rp.a1) return start[rp.rb + v - b0] - rp.g0;
+  if (v < rp.a2) return rp.len0 + (start[rp.rb + v - b0 - nm0] - rp.g1);
+  return rp.len0 + rp.len1 + (start[rp.rb + v - b0 - 2 * nm0] - rp.g2);
 }
 
 // particle index of sequence position t
 __device__ __forceinline__ int seq_particle(const RowPieces& rp, int t) {
   return t < rp.len0 ? rp.g0 + t
                      : (t < rp.len0 + rp.len1 ? rp.g1 + (t - rp.len0) : rp.g2 + (t - rp.len0 - rp.len1));
+}
+
+// Exact FP64 squared distance of a pair across a Lees-Edwards face, as the
+// reference forms it from its images (ppm_bcdef_le_y.inc, then the
+// periodic cascade): the shifted copy is MODULO(x + shear_length, Lx)
+// along x (j's upper copy for sigma 0; for sigma 1 i's upper copy with
+// symmetry, j's lower copy without), whose periodic image nearest the
+// other particle is the one inside the cut-off.
+template <int DIM>
+__device__ __forceinline__ double exact_d2_le(const DevParams& P, const double* __restrict__ x,
+                                              int j, const Home& h, int sigma, int my, int mz) {
+  const int64_t n = P.n;
+  const double L0 = h.G.L0;
+  const double su = (P.le_vhi - P.le_vlo) * h.le.t, sd = (P.le_vlo - P.le_vhi) * h.le.t;
+  double xi = h.x0, xj = x[j];
+  if (sigma == 0)
+    xj = f_modulo(xj + su, L0);
+  else if (P.symmetry)
+    xi = f_modulo(xi + su, L0);
+  else
+    xj = f_modulo(xj + sd, L0);
+  const double raw = xi - xj;
+  const int mx = raw > 0.5 * L0 ? 1 : (raw < -0.5 * L0 ? (P.symmetry ? 2 : 3) : 0);
+  const int mg = sigma == 0 ? 1 : (P.symmetry ? 2 : 3);
+  const double d0 = image_diff(xi, xj, L0, mx);
+  const double d1 = image_diff(h.x1, x[n + j], h.G.L1, P.le_g == 1 ? mg : my);
+  if (DIM == 2) return dist2_exact2(d0, d1);
+  const double d2 = image_diff(h.x2, x[2 * n + j], h.G.L2, P.le_g == 2 ? mg : mz);
+  return dist2_exact(d0, d1, d2);
 }
 
 // ---------------------------------------------------------------------
@@ -282,33 +401,31 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
     cnt = rem;
     ev = rem;
   };
-  const int v_lo = ch.xmin - 2;
   const int nrow = DIM == 3 ? 25 : 5;
-  // Positions in the tile are relative to the corner of virtual cell v_lo
-  // in x (and to the candidate's own cell in y, z); the tile's .w holds
-  // the list entry with the virtual cell (mod 128) in place of the code,
-  // so that entry + kadd is the entry with the stencil code.
-  const int crel = h.cx - v_lo;
-  const float ax = h.a.x + (float)crel * h.G.cs0;
-  const uint32_t self_w = (uint32_t)h.ii | ((uint32_t)crel << kCodeShift);
 #pragma unroll 1
   for (int row = 0; row < nrow; ++row) {
     const int dy = row % 5 - 2, dz = DIM == 3 ? row / 5 - 2 : 0;
     const RowPieces rp = row_pieces<DIM>(P, S, ch, h, row);
-    const int my = rp.my, mz = rp.mz, rb = rp.rb;
-    const int g0 = rp.g0, len0 = rp.len0, g1 = rp.g1, len1 = rp.len1, g2 = rp.g2;
+    const int my = rp.my, mz = rp.mz;
     const int tot = rp.tot;
-    // this lane's window: virtual cells cx-2 .. cx+2 as a range of the
+    const int v_lo = rp.vlo;
+    // Positions in the tile are relative to the corner of virtual cell
+    // v_lo in x (and to the candidate's own cell in y, z); the tile's .w
+    // holds the list entry with the virtual cell (mod 128) in place of the
+    // code, so that entry + kadd is the entry with the stencil code (the
+    // offset from the lane's window centre).  Across a Lees-Edwards face
+    // the window centre and the lane's x come from the sheared frame.
+    const bool le_row = rp.sigma >= 0;
+    const int wc = le_row ? h.le.wc[rp.sigma] : h.cx;
+    const int crel = wc - v_lo;
+    const float ax = (le_row ? h.a.x + h.le.dxf[rp.sigma] : h.a.x) + (float)crel * h.G.cs0;
+    const uint32_t self_w = le_row ? 0xFFFFFFFFu : ((uint32_t)h.ii | ((uint32_t)crel << kCodeShift));
+    // this lane's window: virtual cells wc-2 .. wc+2 as a range of the
     // row sequence
     int tb = 0, te = 0;
     if (h.active) {
-      auto seq_of = [&](int v) -> int {  // first sequence index of virtual cell v
-        if (v < 0) return start[rb + v + nm0] - g0;
-        if (v > nm0 - 1) return len0 + len1 + (start[rb + v - nm0] - g2);
-        return len0 + (start[rb + v] - g1);
-      };
-      tb = seq_of(h.cx - 2);
-      te = seq_of(h.cx + 3);
+      tb = seq_of(rp, start, nm0, wc - 2);
+      te = seq_of(rp, start, nm0, wc + 3);
     }
     const float ay = h.a.y - (float)dy * h.G.cs1;
     const float az = DIM == 3 ? h.a.z - (float)dz * h.G.cs2 : 0.f;
@@ -317,10 +434,10 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
     for (int base = 0; base < tot; base += TILE) {
       const int bend = min(tot, base + TILE);
       for (int t = base + lane; t < bend; t += kWave) {
-        const int piece = t < len0 ? 0 : (t < len0 + len1 ? 1 : 2);
+        const int piece = t < rp.len0 ? 0 : (t < rp.len0 + rp.len1 ? 1 : 2);
         const int g = seq_particle(rp, t);
         const float4 q = S.xa[g];
-        const int vrel = __float_as_int(q.w) + (piece == 0 ? -nm0 : (piece == 2 ? nm0 : 0)) - v_lo;
+        const int vrel = __float_as_int(q.w) + (rp.m0 + piece) * nm0 - v_lo;
         tile[t - base] = make_float4(fmaf((float)vrel, h.G.cs0, q.x), q.y, q.z,
                                      __uint_as_float((uint32_t)g | ((uint32_t)vrel << kCodeShift)));
       }
@@ -347,14 +464,19 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
           if (ok && !in && s2 < P.cut2_hi_f32) {
             // FP64 re-decision with the reference's image convention
             ++rechecks;
-            const Offsets o = decode_code(ent >> kCodeShift);
-            const int v = h.cx + o.ox;
-            Modes m;
-            m.m0 = v < 0 ? low_mode : (v > nm0 - 1 ? 1 : 0);
-            m.m1 = my;
-            m.m2 = mz;
-            double e0, e1, e2;
-            in = exact_d2<DIM>(P, S.x, (int)(ent & kIndexMask), h, m, e0, e1, e2) < P.cut2;
+            const int j = (int)(ent & kIndexMask);
+            if (le_row) {
+              in = exact_d2_le<DIM>(P, S.x, j, h, rp.sigma, my, mz) < P.cut2;
+            } else {
+              const Offsets o = decode_code(ent >> kCodeShift);
+              const int v = h.cx + o.ox;
+              Modes m;
+              m.m0 = v < 0 ? low_mode : (v > nm0 - 1 ? 1 : 0);
+              m.m1 = my;
+              m.m2 = mz;
+              double e0, e1, e2;
+              in = exact_d2<DIM>(P, S.x, j, h, m, e0, e1, e2) < P.cut2;
+            }
           }
           if (ok && in) {
             buf[cnt * kWave + lane] = ent;
@@ -384,6 +506,20 @@ struct KahanF {
 // ---------------------------------------------------------------------
 // K2: density + EOS
 // ---------------------------------------------------------------------
+// Velocity of a neighbour's Lees-Edwards copy relative to its own: + the
+// face velocity jump across the upper face, - across the lower one
+// (particles_set_boundary_ghost_velocity.F90:175-240).
+__device__ __forceinline__ double le_dv(const DevParams& P, const Home& h, const Offsets& o) {
+  if (P.le_g < 0) return 0.0;
+  const int sg = le_sigma(P, h, o.oy, o.oz);
+  return sg == 0 ? P.le_u : (sg == 1 ? -P.le_u : 0.0);
+}
+__device__ __forceinline__ float le_dvf(const DevParams& P, const Home& h, const Offsets& o) {
+  if (P.le_g < 0) return 0.f;
+  const int sg = le_sigma(P, h, o.oy, o.oz);
+  return sg == 0 ? P.le_uf : (sg == 1 ? -P.le_uf : 0.f);
+}
+
 // FP64 separation of a listed pair: x_i - x_j plus the periodic image
 // shift of the neighbour's virtual cell.  The in/out decision was taken
 // with the reference's own formula; the value here differs from it by at
@@ -393,6 +529,15 @@ __device__ __forceinline__ double pair_d2(const DevParams& P, const Home& h, con
                                           double xj0, double xj1, double xj2, double& d0,
                                           double& d1, double& d2) {
   d0 = (h.x0 - xj0) + image_shift(h.cx + o.ox, h.G.nm0, h.G.L0);
+  if (P.le_g >= 0) {
+    // across a Lees-Edwards face: the sheared copy, nearest periodic image
+    const int sg = le_sigma(P, h, o.oy, o.oz);
+    if (sg >= 0) {
+      // x_i - x_j - S' lies in (-2 Lx, Lx): nearest period
+      const double raw = (h.x0 - xj0) - h.le.sp[sg];
+      d0 = raw - h.G.L0 * rint(raw / h.G.L0);
+    }
+  }
   d1 = (h.x1 - xj1) + image_shift(h.cy + o.oy, h.G.nm1, h.G.L1);
   d2 = DIM == 3 ? (h.x2 - xj2) + image_shift(h.cz + o.oz, h.G.nm2, h.G.L2) : 0.0;
   return DIM == 3 ? fma(d2, d2, fma(d1, d1, d0 * d0)) : fma(d1, d1, d0 * d0);
@@ -578,7 +723,7 @@ struct ForceF64 {
     const double r = r2 * ir;
     double w, gw;
     kernel_eval(P, r * P.inv_h, w, gw);
-    const double u0 = vi0 - rb.x, u1 = vi1 - rb.y, u2 = DIM == 3 ? vi2 - rb.z : 0.0;
+    const double u0 = vi0 - (rb.x + le_dv(P, h, o)), u1 = vi1 - rb.y, u2 = DIM == 3 ? vi2 - rb.z : 0.0;
     const double bj = rb.w;
     double a0, a1, a2;
     if (FT == 4) {
@@ -687,7 +832,7 @@ struct ForceMixed {
     const float r = r2 * ir;
     float w, gw;
     kernel_w_gradw_f32(P.kernel_type, r * P.inv_hf, P.coeff, P.coef_gradf, w, gw);
-    const float u0 = vi0 - vj.x, u1 = vi1 - vj.y, u2 = vi2 - vj.z;
+    const float u0 = vi0 - (vj.x + le_dvf(P, h, o)), u1 = vi1 - vj.y, u2 = vi2 - vj.z;
     const uint32_t pidj = __float_as_uint(vj.w);
     float fi0, fi1, fi2;
     if (FT == 5) {

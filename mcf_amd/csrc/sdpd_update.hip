@@ -20,10 +20,37 @@ __global__ void k_kick(DevParams P, DeviceState S, double lambda) {
     S.v[d * n + i] = S.v[d * n + i] + lambda * S.f[d * n + i] * P.dt;
 }
 
-__device__ __forceinline__ double wrap_coord(double x, double lo, double L) {
-  if (x < lo) x = x + L;
-  if (x >= lo + L) x = x - L;
-  return x;
+// Periodic and Lees-Edwards faces after a drift (particles_adjust_particles
+// .F90:137-389): dimensions in order; crossing a Lees-Edwards face moves x
+// by MODULO(x - shear_length, L) (the box starts at 0 then), adds the face
+// velocity jump, and ends the particle's check.
+template <int DIM>
+__device__ __forceinline__ void adjust_particle(const DevParams& P, const RepGeom& g, double* xv,
+                                                double* vv, double le_time) {
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const double lo = g.lo[d], L = g.L[d];
+    if (xv[d] < lo) {
+      if (d == P.le_g) {
+        xv[d] = xv[d] + L;
+        const double sl = (P.le_vlo - P.le_vhi) * le_time;  // shear_length(x, lower)
+        xv[0] = f_modulo(xv[0] - sl, g.L[0]);
+        vv[0] = vv[0] + (P.le_vhi - P.le_vlo);
+        return;
+      }
+      xv[d] = xv[d] + L;
+    }
+    if (xv[d] >= lo + L) {
+      if (d == P.le_g) {
+        xv[d] = xv[d] - L;
+        const double sl = (P.le_vhi - P.le_vlo) * le_time;  // shear_length(x, upper)
+        xv[0] = f_modulo(xv[0] - sl, g.L[0]);
+        vv[0] = vv[0] + (P.le_vlo - P.le_vhi);
+        return;
+      }
+      xv[d] = xv[d] - L;
+    }
+  }
 }
 
 __global__ void k_drift_wrap(DevParams P, DeviceState S) {
@@ -31,9 +58,18 @@ __global__ void k_drift_wrap(DevParams P, DeviceState S) {
   if (i >= P.n) return;
   const int64_t n = P.n;
   const RepGeom& g = S.geo[S.rep[i]];
+  double xv[3] = {0.0, 0.0, 0.0}, vv[3] = {0.0, 0.0, 0.0};
   for (int d = 0; d < P.dim; ++d) {
-    double xv = S.x[d * n + i] + S.v[d * n + i] * P.dt;
-    S.x[d * n + i] = wrap_coord(xv, g.lo[d], g.L[d]);
+    vv[d] = S.v[d * n + i];
+    xv[d] = S.x[d * n + i] + vv[d] * P.dt;
+  }
+  if (P.dim == 3)
+    adjust_particle<3>(P, g, xv, vv, S.time_dev[1]);
+  else
+    adjust_particle<2>(P, g, xv, vv, S.time_dev[1]);
+  for (int d = 0; d < P.dim; ++d) {
+    S.x[d * n + i] = xv[d];
+    S.v[d * n + i] = vv[d];
   }
 }
 
@@ -57,17 +93,26 @@ __global__ __launch_bounds__(256) void k_boundary(DevParams P, DeviceState S, in
   const int64_t n = P.n;
   const int dim = P.dim;
   const RepGeom& g = S.geo[S.rep[i]];
-  double xv[3] = {0.0, 0.0, 0.0};
+  double xv[3] = {0.0, 0.0, 0.0}, vw[3] = {0.0, 0.0, 0.0};
   for (int d = 0; d < dim; ++d) {
     double vv = S.v[d * n + i];
     double ff = S.f[d * n + i];
     if (closing) vv = vv + 0.5 * ff * P.dt;
     if (opening) vv = vv + 0.5 * ff * P.dt;
-    S.v[d * n + i] = vv;
+    vw[d] = vv;
     double x = S.x[d * n + i];
-    if (opening) x = wrap_coord(x + vv * P.dt, g.lo[d], g.L[d]);
-    S.x[d * n + i] = x;
+    if (opening) x = x + vv * P.dt;
     xv[d] = x;
+  }
+  if (opening) {
+    if (dim == 3)
+      adjust_particle<3>(P, g, xv, vw, S.time_dev[1]);
+    else
+      adjust_particle<2>(P, g, xv, vw, S.time_dev[1]);
+  }
+  for (int d = 0; d < dim; ++d) {
+    S.v[d * n + i] = vw[d];
+    S.x[d * n + i] = xv[d];
   }
   if (!P.newtonian) {
     const int d2 = dim * dim;
@@ -161,12 +206,26 @@ __global__ __launch_bounds__(256) void k_replica_stats(DevParams P, DeviceState 
   }
 }
 
-__global__ void k_set_step(int64_t* __restrict__ step_dev, int64_t step) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) step_dev[0] = step;
+__global__ void k_set_step(int64_t* __restrict__ step_dev, double* __restrict__ time_dev,
+                           int64_t step, double time, double le_time) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    step_dev[0] = step;
+    time_dev[0] = time;
+    time_dev[1] = le_time;
+  }
 }
 
-__global__ void k_advance_step(int64_t* __restrict__ step_dev) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) step_dev[0] = step_dev[0] + 1;
+// One marching step: step + 1, time_current + dt (marching_marching.F90:
+// 1392-1393), and the Lees-Edwards shear length is evaluated at
+// time_current + dt (marching_integrate_VV.F90:784-786).
+__global__ void k_advance_step(int64_t* __restrict__ step_dev, double* __restrict__ time_dev,
+                               double dt) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    step_dev[0] = step_dev[0] + 1;
+    const double t = time_dev[0] + dt;
+    time_dev[0] = t;
+    time_dev[1] = t + dt;
+  }
 }
 
 __global__ void k_pair_normals(const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
@@ -213,12 +272,14 @@ void launch_replica_stats(hipStream_t s, const DevParams& P, const DeviceState& 
   hipLaunchKernelGGL(k_replica_stats, dim3(P.nrep), dim3(256), 0, s, P, S);
 }
 
-void launch_set_step(hipStream_t s, const DeviceState& S, int64_t step) {
-  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, S.step_dev, step);
+void launch_set_step(hipStream_t s, const DeviceState& S, int64_t step, double time,
+                     double le_time) {
+  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, S.step_dev, S.time_dev, step, time,
+                     le_time);
 }
 
-void launch_advance_step(hipStream_t s, const DeviceState& S) {
-  hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, s, S.step_dev);
+void launch_advance_step(hipStream_t s, const DevParams& P, const DeviceState& S) {
+  hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, s, S.step_dev, S.time_dev, P.dt);
 }
 
 void launch_pair_normals(hipStream_t s, const uint32_t* lo, const uint32_t* hi, int count,

@@ -186,6 +186,9 @@ PROGRAM mcf_sdpd
   INTEGER :: seed_size
   INTEGER, ALLOCATABLE :: seed_arr(:)
   INTEGER(c_int64_t) :: cur_step
+  INTEGER :: bcdef(6), le_dim
+  DOUBLE PRECISION :: shear_in(12)
+  REAL(c_double) :: v_lo(3), v_hi(3)
 
   ctrl_file = 'ctrl.mcf'
   IF (COMMAND_ARGUMENT_COUNT() >= 1) CALL GET_COMMAND_ARGUMENT(1, ctrl_file)
@@ -287,9 +290,45 @@ PROGRAM mcf_sdpd
   END IF
   WRITE(*, '(A,I0,A,I0,A,I0,A)') 'mcf_sdpd: ', n, ' particles, ', dim, 'D, ', nsteps, ' steps'
 
+  ! Lees-Edwards faces: bcdef = 10 on both faces of y or z
+  ! (io_read_physics.F90:1358-1400 reads bcdef and shear_v)
+  bcdef = 1
+  CALL get_ints(cfg, 'bcdef', 2 * dim, bcdef(1:2 * dim))
+  shear_in = 0.0d0
+  CALL get_reals(cfg, 'shear_v', 2 * dim * (dim - 1), shear_in(1:2 * dim * (dim - 1)))
+  IF (bcdef(1) == 10 .OR. bcdef(2) == 10) CALL die('Lees-Edwards faces of x are not supported')
+  le_dim = 0
+  DO d = 1, 2 * dim
+     IF (bcdef(d) /= 1 .AND. bcdef(d) /= 10) CALL die('only periodic (1) and Lees-Edwards (10) faces')
+  END DO
+  DO d = 2, dim
+     IF (bcdef(2 * d - 1) == 10 .AND. bcdef(2 * d) == 10) THEN
+        IF (le_dim /= 0) CALL die('Lees-Edwards faces on one dimension only')
+        le_dim = d
+     ELSE IF (bcdef(2 * d - 1) == 10 .OR. bcdef(2 * d) == 10) THEN
+        CALL die('bcdef = 10 (Lees-Edwards) must be set on both faces of a dimension')
+     END IF
+  END DO
+
   ! ---- set-up ----
   rc = sdpd_create(prm, ctx)
   IF (rc /= SDPD_OK) CALL die('sdpd_create failed')
+  IF (le_dim > 0) THEN
+     ! shear_v lists, per face, the velocity components along the face;
+     ! x is the first of them for the y and z faces
+     d = le_dim
+     v_lo = 0.0d0
+     v_hi = 0.0d0
+     v_lo(1) = shear_in((2 * d - 2) * (dim - 1) + 1)
+     v_hi(1) = shear_in((2 * d - 1) * (dim - 1) + 1)
+     IF (dim == 3) THEN
+        ! the other component along the face (z for y faces, y for z
+        ! faces); the library rejects it unless it is zero
+        v_lo(5 - d) = shear_in((2 * d - 2) * 2 + 2)
+        v_hi(5 - d) = shear_in((2 * d - 1) * 2 + 2)
+     END IF
+     CALL check(sdpd_set_lees_edwards(ctx, INT(d, c_int32_t), v_lo, v_hi), 'sdpd_set_lees_edwards')
+  END IF
   CALL check(sdpd_upload(ctx, x, v, m, id, INT(n, c_int32_t)), 'sdpd_upload')
   CALL check(sdpd_set_step(ctx, INT(step_start, c_int64_t)), 'sdpd_set_step')
 

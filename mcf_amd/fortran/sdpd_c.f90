@@ -91,6 +91,14 @@ MODULE sdpd_c
        REAL(c_double), INTENT(OUT) :: ekin(*)
      END FUNCTION sdpd_stats_replicas
 
+     INTEGER(c_int) FUNCTION sdpd_set_lees_edwards(ctx, le_dim, v_lower, v_upper) &
+          BIND(C, name="sdpd_set_lees_edwards")
+       IMPORT :: c_int, c_ptr, c_double, c_int32_t
+       TYPE(c_ptr), VALUE         :: ctx
+       INTEGER(c_int32_t), VALUE  :: le_dim
+       REAL(c_double), INTENT(IN) :: v_lower(3), v_upper(3)
+     END FUNCTION sdpd_set_lees_edwards
+
      INTEGER(c_int) FUNCTION sdpd_upload_ct(ctx, ct) BIND(C, name="sdpd_upload_ct")
        IMPORT :: c_int, c_ptr, c_double
        TYPE(c_ptr), VALUE         :: ctx

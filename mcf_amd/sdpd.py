@@ -100,6 +100,7 @@ SYMBOLS = [
     ("sdpd_download", C.c_int, [_P, _DP, _DP, _DP, _DP, _DP]),
     ("sdpd_download_tensors", C.c_int, [_P, _DP, _DP]),
     ("sdpd_step", C.c_int, [_P, C.c_int32]),
+    ("sdpd_set_lees_edwards", C.c_int, [_P, C.c_int32, _DP, _DP]),
     ("sdpd_set_graphs", C.c_int, [_P, C.c_int32]),
     ("sdpd_set_tuning", C.c_int, [_P, C.POINTER(CTuning)]),
     ("sdpd_get_tuning", C.c_int, [_P, C.POINTER(CTuning)]),
@@ -190,6 +191,8 @@ class SDPD:
             raise SDPDError(f"sdpd_create failed (stat_info={rc}); see stderr")
         self.h = h
         self.n = 0
+        if getattr(prm, "le_dim", 0):
+            self.set_lees_edwards(prm.le_dim, prm.le_v_lower, prm.le_v_upper)
 
     # -- lifetime -------------------------------------------------------
     def close(self) -> None:
@@ -316,6 +319,14 @@ class SDPD:
         e = C.c_double()
         self._check(self.lib.sdpd_stats(self.h, _dp(mom), C.byref(e)), "statistic_compute_statistic")
         return e.value, mom[: self.prm.num_dim]
+
+    def set_lees_edwards(self, le_dim: int, v_lower, v_upper) -> None:
+        """Lees-Edwards faces on dimension le_dim (2 = y, 3 = z; 0 = off)
+        moving with v_lower / v_upper (3-vectors, x components only)."""
+        lo = np.ascontiguousarray(list(v_lower) + [0.0] * (3 - len(v_lower)), np.float64)
+        hi = np.ascontiguousarray(list(v_upper) + [0.0] * (3 - len(v_upper)), np.float64)
+        self._check(self.lib.sdpd_set_lees_edwards(self.h, int(le_dim), _dp(lo), _dp(hi)),
+                    "sdpd_set_lees_edwards")
 
     def set_graphs(self, enable: bool) -> None:
         """Run sdpd_step's steps through captured HIP graphs (default) or

@@ -21,7 +21,7 @@ def announce(msg):
     print(msg, flush=True)
 
 
-def write_configs(tmp, box, nsteps, fused=False, precision="fp64", stat_freq=2):
+def write_configs(tmp, box, nsteps, fused=False, precision="fp64", stat_freq=2, extra=""):
     prm = box.prm
     d = prm.num_dim
     W.write_external(os.path.join(tmp, "particles.dat"), box)
@@ -57,7 +57,7 @@ gamma = {prm.gamma!r}
 eta = {prm.eta!r}
 eta_coef = {prm.eta_coef!r}
 kt = {prm.kt!r}
-""")
+{extra}""")
     with open(os.path.join(tmp, "io_config.mcf"), "w") as fh:
         fh.write(f"""read_particles_file = particles.dat
 statistic_file = stat.dat
@@ -123,3 +123,58 @@ def test_host_matches_python_binding(tmp_path, fused):
     stat = np.loadtxt(str(tmp_path / "stat.dat"), ndmin=2)
     assert list(stat[:, 0]) == [0, 2, 4]
     assert stat[-1, 2] == pytest.approx(ek, rel=1e-7)
+
+
+def shear_v_line(d, le_dim, v_lo, v_hi):
+    """The reference's shear_v key (io_read_physics.F90:1378-1402): per face,
+    its velocity components along the face, faces x1, x2, y1, y2[, z1, z2]."""
+    vals = []
+    for face in range(2 * d):
+        along = [c for c in range(d) if c != face // 2]
+        vel = [0.0] * 3
+        if face // 2 == le_dim - 1:
+            vel = v_lo if face % 2 == 0 else v_hi
+        vals += [vel[c] for c in along]
+    return "shear_v = " + ", ".join(repr(float(a)) for a in vals)
+
+
+def le_extra(d, le_dim, v_lo, v_hi):
+    bc = [1] * (2 * d)
+    bc[2 * le_dim - 2] = bc[2 * le_dim - 1] = 10
+    return "bcdef = " + ", ".join(map(str, bc)) + "\n" + shear_v_line(d, le_dim, v_lo, v_hi) + "\n"
+
+
+@needs_host
+@pytest.mark.parametrize("bc,msg", [("10, 1, 1, 1, 1, 1", "faces of x"),
+                                    ("1, 1, 10, 1, 1, 1", "both faces"),
+                                    ("1, 1, 10, 10, 10, 10", "one dimension"),
+                                    ("1, 1, 7, 7, 1, 1", "only periodic")])
+def test_host_rejects_bad_faces(tmp_path, bc, msg):
+    box = W.tait_box_3d(10, kt=1e-3)
+    ctrl = write_configs(str(tmp_path), box, 2, extra=f"bcdef = {bc}\n")
+    r = run_host(ctrl, str(tmp_path))
+    assert r.returncode != 0 and msg in r.stdout, r.stdout
+
+
+@pytest.mark.gpu
+@needs_host
+@pytest.mark.parametrize("le_dim,fused", [(2, False), (3, True)])
+def test_host_lees_edwards_matches_python_binding(tmp_path, le_dim, fused):
+    box = W.tait_box_3d(12, kt=1e-3)
+    u = 3.0
+    v_lo, v_hi = [-u, 0.0, 0.0], [u, 0.0, 0.0]
+    box.prm = box.prm.replace(dt=box.prm.dt * 100)
+    v = box.v.copy()
+    v[:, 0] += 2 * u / 12.0 * (box.x[:, le_dim - 1] - 6.0)
+    box.v = v
+    nsteps = 4
+    announce(f"Fortran host with Lees-Edwards faces on dim {le_dim}, fused={fused}")
+    ctrl = write_configs(str(tmp_path), box, nsteps, fused=fused, extra=le_extra(3, le_dim, v_lo, v_hi))
+    r = run_host(ctrl, str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    x, vh = read_parts(str(tmp_path / f"parts{nsteps:08d}.dat"), 3)
+    with SDPD(box.prm.replace(le_dim=le_dim, le_v_lower=v_lo, le_v_upper=v_hi)) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.integrate_VV(nsteps)
+        xr, vr = g.download()
+    assert np.array_equal(x, xr) and np.array_equal(vh, vr)

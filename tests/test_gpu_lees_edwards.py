@@ -1,0 +1,169 @@
+"""Lees-Edwards faces on the GPU against the oracle: neighbour sets
+bit-exact across the sheared faces, density / forces within the FP64 bar,
+trajectories with particles crossing the faces, mixed against FP64."""
+import numpy as np
+import pytest
+
+from mcf_amd import workloads as W
+from mcf_amd.sdpd import SDPD, SDPDError, pair_hash
+from tests.gpu_helpers import rel_err, vec_err
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print(msg, flush=True)
+
+
+def marching_time(step, dt):
+    t = 0.0
+    for _ in range(step):
+        t = t + dt
+    return t + dt if step > 0 else t
+
+
+def le_box(kind="y3d", sym=True, n_side=12, kt=1e-3):
+    if kind == "y2d":
+        box = W.shear_wave_2d(24, brownian=True, kt=1e-4, symmetry=sym)
+        L = 1.0
+        le_dim = 2
+        u = 0.5 * L / 2
+    else:
+        box = W.tait_box_3d(n_side, kt=kt, symmetry=sym)
+        L = float(n_side)
+        le_dim = 2 if kind == "y3d" else 3
+        u = 0.5 * L / 2
+    # large dt so that the shear length moves by cells between the steps
+    box.prm = box.prm.replace(le_dim=le_dim, le_v_lower=[-u, 0, 0], le_v_upper=[u, 0, 0],
+                              dt=box.prm.dt * 400)
+    return box
+
+
+def oracle_signature_le(O, box, t):
+    pairs = O.neighbor_pairs(box.prm, box.x, le_time=t)
+    if box.prm.symmetry:
+        a, b = pairs[:, 0], pairs[:, 1]
+        aa, bb = np.concatenate([a, b]), np.concatenate([b, a])
+    else:
+        aa, bb = pairs[:, 0], pairs[:, 1]
+    cnt = np.bincount(aa, minlength=box.n).astype(np.int32)
+    h = pair_hash(box.pid[aa], box.pid[bb]).astype(np.uint64)
+    hs = np.zeros(box.n, np.uint64)
+    np.add.at(hs, aa, h)
+    return cnt, (hs & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+
+
+CASES = [("y3d", True), ("y3d", False), ("z3d", True), ("y2d", True)]
+
+
+@pytest.mark.parametrize("kind,sym", CASES)
+@pytest.mark.parametrize("step", [0, 3, 77])
+def test_le_neighbor_sets(oracle, kind, sym, step):
+    box = le_box(kind, sym)
+    t = marching_time(step, box.prm.dt)
+    announce(f"LE neighbour sets: {kind} sym={sym} step={step} t={t}")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.step = step
+        g.build_list()
+        cnt, hs = g.neighbor_signature()
+    rc, rh = oracle_signature_le(oracle, box, t)
+    assert np.array_equal(cnt, rc) and np.array_equal(hs, rh)
+
+
+@pytest.mark.parametrize("kind,sym", CASES)
+@pytest.mark.parametrize("force_type", [3, 4])
+def test_le_density_forces(oracle, kind, sym, force_type):
+    box = le_box(kind, sym)
+    box.prm = box.prm.replace(rhs_force_type=force_type)
+    step = 5
+    t = marching_time(step, box.prm.dt)
+    rng = np.random.default_rng(4)
+    L = box.prm.max_phys[1]
+    # a linear shear profile plus noise
+    v = box.v.copy()
+    v[:, 0] += 0.5 * (box.x[:, 1] - L / 2)
+    v += rng.normal(0, 0.01, size=v.shape)
+    announce(f"LE density / forces: {kind} sym={sym} type {force_type}")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, v, box.m, box.pid)
+        g.step = step
+        g.build_list()
+        g.compute_density()
+        g.compute_interaction()
+        rho, p = g.download_density()
+        f = g.download_force()
+    ro = oracle.density(box.prm, box.x, box.m, le_time=t)
+    po = oracle.pressure(box.prm, ro, box.m)
+    fo, _ = oracle.interaction(box.prm, step, box.x, v, ro, po, box.m, box.pid, le_time=t)
+    assert rel_err(rho, ro) < 1e-13
+    assert vec_err(f, fo) < 1e-12
+
+
+@pytest.mark.parametrize("kind,sym", [("y3d", True), ("y3d", False), ("z3d", True)])
+def test_le_trajectory_with_crossings(oracle, kind, sym):
+    box = le_box(kind, sym, kt=1e-3)
+    L = box.prm.max_phys[1]
+    g_dim = box.prm.le_dim - 1
+    v = box.v.copy()
+    v[:, 0] += 0.5 * (box.x[:, g_dim] - L / 2)
+    # a few particles cross the sheared faces in the first steps
+    dt = box.prm.dt
+    top = np.argsort(box.x[:, g_dim])[-6:]
+    bot = np.argsort(box.x[:, g_dim])[:6]
+    v[top, g_dim] = (L - box.x[top, g_dim] + 0.05) / dt
+    v[bot, g_dim] = -(box.x[bot, g_dim] + 0.05) / dt
+    announce(f"LE trajectory: {kind} sym={sym}")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, v, box.m, box.pid)
+        g.integrate_VV(6)
+        x, vg = g.download()
+    st = oracle.State(box.prm, box.x, v, box.m, box.pid)
+    st.prologue(0)
+    for s in range(1, 7):
+        st.vv_step(s)
+    dx = np.abs(x - st.x)
+    dx = np.minimum(dx, L - dx)
+    assert dx.max() < 1e-10 * L
+    assert vec_err(vg, st.v) < 1e-10
+
+
+def test_le_mixed_close_to_fp64():
+    box = le_box("y3d", True, n_side=13)
+    L = box.prm.max_phys[1]
+    v = box.v.copy()
+    v[:, 0] += 0.5 * (box.x[:, 1] - L / 2)
+    announce("LE mixed vs fp64")
+    out = []
+    for b in (box, W.mixed(box)):
+        with SDPD(b.prm) as g:
+            g.upload(b.x, v, b.m, b.pid)
+            g.step = 9
+            g.build_list()
+            g.compute_density()
+            g.compute_interaction()
+            out.append((g.download_density()[0], g.download_force(), g.counters()[0]))
+    assert out[0][2] == out[1][2]
+    assert rel_err(out[1][0], out[0][0]) < 1e-6
+    assert vec_err(out[1][1], out[0][1]) < 1e-5
+
+
+def test_le_errors():
+    box = le_box("y3d", True)
+    announce("LE error paths")
+    with SDPD(box.prm.replace(le_dim=0)) as g:
+        with pytest.raises(SDPDError):
+            g.set_lees_edwards(1, [0, 0, 0], [1, 0, 0])       # x faces
+        with pytest.raises(SDPDError):
+            g.set_lees_edwards(2, [0, 0, 0], [0, 0, 1])       # shear along z
+    shifted = box.prm.replace(min_phys=[1.0, 0, 0], max_phys=[13.0, 12, 12])
+    with pytest.raises(SDPDError):
+        SDPD(shifted)
+    ens = W.ensemble_of_cubes((10, 11), box.prm.replace(le_dim=0), kt=0.0)
+    with SDPD(ens.prm) as g:
+        g.set_replicas(ens.box_lo, ens.box_hi)
+        with pytest.raises(SDPDError):
+            g.set_lees_edwards(2, [-1, 0, 0], [1, 0, 0])
+    with SDPD(box.prm) as g:
+        with pytest.raises(SDPDError):
+            g.set_replicas(ens.box_lo, ens.box_hi)

@@ -53,7 +53,9 @@ struct sdpd_ctx {
 namespace {
 
 constexpr double kPi = 3.141592653589793238;
-constexpr double kMachineZero = 4.440892098500626e-16;
+// mcf_machine_zero as the reference sets it at start-up:
+// 10^(INT(LOG10(EPSILON(1d0))) + 1) = 1e-14 (technique_new.F90:361-363)
+constexpr double kMachineZero = 1e-14;
 
 int fail(sdpd_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;

@@ -915,9 +915,9 @@ __device__ __forceinline__ void finish_force(const DevParams& P, const DeviceSta
   const int64_t n = P.n;
   if (P.body_force) {
     // particles_apply_body_force.F90:125-193
-    if (fabs(P.bf[0]) > 4.440892098500626e-16) f0 = f0 + P.bf[0];
-    if (fabs(P.bf[1]) > 4.440892098500626e-16) f1 = f1 + P.bf[1];
-    if (DIM == 3 && fabs(P.bf[2]) > 4.440892098500626e-16) f2 = f2 + P.bf[2];
+    if (fabs(P.bf[0]) > 1e-14) f0 = f0 + P.bf[0];
+    if (fabs(P.bf[1]) > 1e-14) f1 = f1 + P.bf[1];
+    if (DIM == 3 && fabs(P.bf[2]) > 1e-14) f2 = f2 + P.bf[2];
   }
   S.f[i] = f0;
   S.f[n + i] = f1;

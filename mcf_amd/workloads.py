@@ -192,6 +192,187 @@ def oldroyd_ensemble(nrep: int = 256, nmin: int = 2000, nmax: int = 60000, seed:
     return ensemble_of_cubes(sides, prm, kt, seed, name="oldroyd_ensemble")
 
 
+# --------------------------------------------------------------------------
+# colloids
+# --------------------------------------------------------------------------
+
+# mcf_header.F90:131,141: inner / outer layer coefficients
+COLLOID_IN_LAYER_COEFF = 0.05
+COLLOID_OUT_LAYER_COEFF = 0.05
+
+
+class ColloidSet:
+    """Rigid colloids of a box: disks (2D) or spheres (3D), with the
+    parameters of the reference's colloid block (physics_config.mcf keys
+    coll_*).  Vectors carry 3 components per colloid."""
+
+    def __init__(self, radius, x, m, mmi, v=None, omega=None, noslip=2, rho_type=0,
+                 translate=False, rotate=False, ab=1, sub_steps=1, body_force_type=0,
+                 body_force=(0.0, 0.0, 0.0), dout=0.05):
+        k = len(radius)
+        self.radius = np.ascontiguousarray(radius, np.float64)
+        self.x = np.zeros((k, 3))
+        self.x[:, : np.asarray(x).shape[1]] = x
+        self.m = np.ascontiguousarray(m, np.float64)
+        self.mmi = np.zeros((k, 3))
+        self.mmi[:, : np.asarray(mmi).reshape(k, -1).shape[1]] = np.asarray(mmi).reshape(k, -1)
+        self.v = np.zeros((k, 3)) if v is None else np.array(v, np.float64).reshape(k, 3)
+        self.omega = np.zeros((k, 3)) if omega is None else np.array(omega, np.float64).reshape(k, 3)
+        self.noslip, self.rho_type = int(noslip), int(rho_type)
+        self.translate, self.rotate = bool(translate), bool(rotate)
+        self.ab, self.sub_steps = int(ab), int(sub_steps)
+        self.body_force_type = int(body_force_type)
+        self.body_force = [float(b) for b in list(body_force) + [0.0] * 3][:3]
+        self.dout = float(dout)
+
+    @property
+    def num(self) -> int:
+        return int(self.radius.shape[0])
+
+
+def sphere_shell(radius: float, dx: float, din: float) -> np.ndarray:
+    """Boundary particles of a sphere centred at the origin, placed parallel
+    to the surface in layers dx apart (coll_place 5,
+    colloid_create_boundary_particle_3D_sphere.F90:160-230)."""
+    pts = []
+    num_layer = math.ceil(din / dx - 0.5)
+    for m in range(1, num_layer + 1):
+        rad = radius - m * dx + dx / 2.0
+        pts.append((rad, 0.0, 0.0))
+        d_phi = dx / rad
+        num_phi = int(round(math.pi / d_phi))
+        d_phi = math.pi / num_phi
+        phi = 0.0
+        for _ in range(1, num_phi):
+            phi = phi + d_phi
+            d_theta = dx / (rad * math.sin(phi))
+            num_theta = int(round(2.0 * math.pi / d_theta))
+            d_theta = 2.0 * math.pi / num_theta
+            theta = 0.0
+            for _k in range(num_theta):
+                pts.append((rad * math.cos(phi), rad * math.sin(phi) * math.sin(theta),
+                            rad * math.sin(phi) * math.cos(theta)))
+                theta = theta + d_theta
+        pts.append((-rad, 0.0, 0.0))
+    return np.array(pts)
+
+
+def disk_shell(radius: float, dx: float, din: float) -> np.ndarray:
+    """Boundary particles of a disk centred at the origin, rings dx apart,
+    every other ring turned by half a step (coll_place 5,
+    colloid_create_boundary_particle_2D.F90:148-173)."""
+    pts = []
+    shift = 0
+    for i in range(math.ceil(din / dx)):
+        rad = radius - i * dx - dx / 2.0
+        if rad >= dx / 2.0:
+            angle = math.acos(1.0 - (dx / rad) ** 2 / 2.0)
+            num_sur = math.ceil(2.0 * math.pi / angle)
+            angle = 2.0 * math.pi / num_sur
+            for k in range(num_sur):
+                a = k * angle + 0.5 * shift * angle
+                pts.append((rad * math.cos(a), rad * math.sin(a)))
+            shift = (shift + 1) % 2
+        else:
+            pts.append((0.0, 0.0))
+    return np.array(pts)
+
+
+def nearest_dist(x: np.ndarray, c: np.ndarray, L: np.ndarray) -> np.ndarray:
+    """Periodic minimum-image distance of points x (n, d) to centre c (d)."""
+    r = x - c[None, :]
+    r = r - L[None, :] * np.round(r / L[None, :])
+    return np.sqrt((r * r).sum(1))
+
+
+def with_colloids(box: Box, centres, radius: float, dx: float = None, rho_type: int = 0,
+                  **kw):
+    """Put rigid colloids of the given radius at `centres` (k, dim) into a
+    lattice box: solvent sites within radius + dout of a centre are dropped
+    (particles_init_global_assign_id.F90, colloid_check_boundary_particle.F90:
+    189-205), each colloid gets its boundary-particle shell (sid = k + 1) with
+    the solvent particle mass (particles_compute_mass.F90:360-470), moving
+    rigidly with the colloid.  Colloid mass and moment of inertia follow
+    particles_compute_mass.F90:153-357 (disk: m = rho pi R^2, I = m R^2 / 2;
+    sphere: m = rho 4/3 pi R^3, I = 2/5 m R^2).  Extra keywords go to
+    ColloidSet.  Returns (box, colloids)."""
+    prm = box.prm
+    d = prm.num_dim
+    if dx is None:
+        dx = (prm.max_phys[0] - prm.min_phys[0]) / round((box.n) ** (1.0 / d))
+    centres = np.asarray(centres, np.float64).reshape(-1, d)
+    k = centres.shape[0]
+    L = np.array(prm.length[:d])
+    dout = dx * COLLOID_OUT_LAYER_COEFF
+    din = (1.0 + COLLOID_IN_LAYER_COEFF) * prm.cut_off * (2.0 if rho_type == 1 else 1.0)
+    keep = np.ones(box.n, bool)
+    for c in centres:
+        keep &= nearest_dist(box.x, c, L) > radius + dout
+    shell = sphere_shell(radius, dx, din) if d == 3 else disk_shell(radius, dx, din)
+    xs, vs, sids = [box.x[keep]], [box.v[keep]], [np.zeros(int(keep.sum()), np.int32)]
+    vcol = np.zeros((k, 3)) if kw.get("v") is None else np.array(kw["v"], np.float64).reshape(k, 3)
+    wcol = np.zeros((k, 3)) if kw.get("omega") is None else np.array(kw["omega"], np.float64).reshape(k, 3)
+    for q, c in enumerate(centres):
+        xq = shell + c[None, :]
+        xq = xq - L[None, :] * np.floor((xq - np.array(prm.min_phys[:d])[None, :]) / L[None, :])
+        r3 = np.zeros((shell.shape[0], 3))
+        r3[:, :d] = shell
+        vq = vcol[q][None, :] + np.cross(wcol[q][None, :], r3)
+        xs.append(xq)
+        vs.append(vq[:, :d])
+        sids.append(np.full(shell.shape[0], q + 1, np.int32))
+    x = np.concatenate(xs)
+    v = np.concatenate(vs)
+    sid = np.concatenate(sids)
+    m = np.full(x.shape[0], box.m[0])
+    pid = np.arange(1, x.shape[0] + 1, dtype=np.int32)
+    if d == 3:
+        mass = prm.rho * 4.0 * math.pi * radius ** 3.0 / 3.0
+        mmi = np.full((k, 3), 0.4 * mass * radius ** 2)
+    else:
+        mass = prm.rho * math.pi * radius ** 2.0
+        mmi = np.zeros((k, 3))
+        mmi[:, 2] = 0.5 * mass * radius ** 2
+    coll = ColloidSet(np.full(k, float(radius)), centres, np.full(k, mass), mmi, dout=dout,
+                      rho_type=rho_type, **kw)
+    out = Box(box.name + "_colloids", prm, x, v, m, pid, sid)
+    return out, coll
+
+
+def colloid_suspension_3d(n_side: int = 128, ncoll: int = 512, radius: float = 5.0,
+                          shear_rate: float = 0.5, kt: float = 1e-3, seed: int = 1,
+                          precision: int = PRECISION_FP64, translate: bool = True,
+                          rotate: bool = True, symmetry: bool = True):
+    """Spheres of radius 5 dx in the 128^3 Tait box, Lees-Edwards faces on y
+    sheared at `shear_rate` (face velocities -+ shear_rate L / 2) with the
+    solvent started on the linear profile.  The centres sit on a cubic
+    lattice of spacing L / ncoll^(1/3) (16 dx for 512 spheres: a surface gap
+    of 6 dx = 2 cut-offs), shifted by a seeded random offset: random
+    sequential addition cannot place 512 such spheres with that gap (their
+    excluded volume fraction would be 0.52, above its 0.38 jamming limit)."""
+    box = tait_box_3d(n_side, precision=precision, kt=kt, seed=seed, symmetry=symmetry)
+    L = float(n_side)
+    per = round(ncoll ** (1.0 / 3.0))
+    if per ** 3 != ncoll:
+        raise ValueError("ncoll must be a cube")
+    a = L / per
+    rng = np.random.default_rng(seed + 4242)
+    off = rng.uniform(0.0, a, size=3)
+    g = (np.arange(per) + 0.0) * a
+    cx, cy, cz = np.meshgrid(g, g, g, indexing="ij")
+    centres = np.stack([cx.ravel(), cy.ravel(), cz.ravel()], 1) + off[None, :]
+    centres = np.mod(centres, L)
+    u = shear_rate * L / 2.0
+    box.v[:, 0] += shear_rate * (box.x[:, 1] - L / 2.0)
+    box.prm = box.prm.replace(le_dim=2, le_v_lower=[-u, 0.0, 0.0], le_v_upper=[u, 0.0, 0.0])
+    vcol = np.zeros((ncoll, 3))
+    vcol[:, 0] = shear_rate * (centres[:, 1] - L / 2.0)
+    out, coll = with_colloids(box, centres, radius, dx=1.0, translate=translate, rotate=rotate,
+                              v=vcol)
+    out.name = "colloid_suspension_3d"
+    return out, coll
+
+
 def mixed(box: Box) -> Box:
     b = Box(box.name + "_mixed", box.prm.replace(precision=PRECISION_MIXED), box.x, box.v, box.m,
             box.pid, box.sid)

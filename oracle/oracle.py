@@ -52,7 +52,91 @@ class OrcParams(C.Structure):
         ("le_vlo", C.c_double * 3),
         ("le_vhi", C.c_double * 3),
         ("le_time", C.c_double),
+        ("coll", C.c_void_p),
     ]
+
+
+class OrcColloids(C.Structure):
+    _fields_ = [
+        ("num", C.c_int32),
+        ("noslip", C.c_int32),
+        ("rho_type", C.c_int32),
+        ("translate", C.c_int32),
+        ("rotate", C.c_int32),
+        ("ab", C.c_int32),
+        ("sub_steps", C.c_int32),
+        ("body_force_type", C.c_int32),
+        ("body_force", C.c_double * 3),
+        ("dout", C.c_double),
+        ("le_time", C.c_double),
+        ("step_start", C.c_int64),
+    ] + [(name, C.c_void_p) for name in (
+        "radius", "m", "mmi", "x", "v", "omega", "f", "alpha", "rot_matrix", "acc_matrix",
+        "theta", "drag", "torque")]
+
+
+class Colloids:
+    """Rigid colloids for the oracle: numpy arrays that orc_vv_step updates
+    in place (vectors stored with 3 components per colloid, histories of
+    `ab` levels newest first)."""
+
+    ARRAYS = ("radius", "m", "mmi", "x", "v", "omega", "f", "alpha", "rot_matrix", "acc_matrix",
+              "theta", "drag", "torque")
+
+    def __init__(self, radius, m, mmi, x, v=None, omega=None, noslip=2, rho_type=0,
+                 translate=False, rotate=False, ab=1, sub_steps=1, body_force_type=0,
+                 body_force=(0.0, 0.0, 0.0), dout=0.05, acc_matrix=None, step_start=0):
+        k = len(radius)
+        self.num = k
+        self.noslip, self.rho_type = int(noslip), int(rho_type)
+        self.translate, self.rotate = bool(translate), bool(rotate)
+        self.ab, self.sub_steps = int(ab), int(sub_steps)
+        self.body_force_type = int(body_force_type)
+        self.body_force = [float(b) for b in list(body_force) + [0.0] * (3 - len(body_force))][:3]
+        self.dout = float(dout)
+        self.step_start = int(step_start)
+        self.le_time = 0.0
+
+        def vec(a):
+            out = np.zeros((k, 3))
+            if a is not None:
+                a = np.asarray(a, np.float64).reshape(k, -1)
+                out[:, : a.shape[1]] = a
+            return out
+        self.radius = np.ascontiguousarray(radius, np.float64)
+        self.m = np.ascontiguousarray(m, np.float64)
+        self.mmi = vec(mmi)
+        self.x = vec(x)
+        self.v = np.zeros((self.ab, k, 3))
+        self.v[0] = vec(v)
+        self.omega = np.zeros((self.ab, k, 3))
+        self.omega[0] = vec(omega)
+        self.f = np.zeros((self.ab, k, 3))
+        self.alpha = np.zeros((self.ab, k, 3))
+        self.rot_matrix = np.zeros((k, 3, 3))
+        self.acc_matrix = np.tile(np.eye(3), (k, 1, 1)) if acc_matrix is None else \
+            np.array(acc_matrix, np.float64).reshape(k, 3, 3).copy()
+        self.theta = np.zeros((k, 3))
+        self.drag = np.zeros((k, 3))
+        self.torque = np.zeros((k, 3))
+
+    def struct(self) -> OrcColloids:
+        c = OrcColloids()
+        c.num = self.num
+        c.noslip, c.rho_type = self.noslip, self.rho_type
+        c.translate, c.rotate = int(self.translate), int(self.rotate)
+        c.ab, c.sub_steps, c.body_force_type = self.ab, self.sub_steps, self.body_force_type
+        for d in range(3):
+            c.body_force[d] = self.body_force[d]
+        c.dout, c.le_time, c.step_start = self.dout, float(self.le_time), self.step_start
+        for name in self.ARRAYS:
+            a = getattr(self, name)
+            assert a.flags["C_CONTIGUOUS"] and a.dtype == np.float64
+            setattr(c, name, a.ctypes.data)
+        # acc_matrix / rot_matrix are (k, 3, 3) row-major = column-major
+        # (3, 3, k) of the transposes: stored as the C code reads them,
+        # element (r, c) at [r + 3 c]
+        return c
 
 
 def build() -> None:
@@ -90,13 +174,20 @@ def lib():
         L.orc_prologue.argtypes = state
         L.orc_vv_step.argtypes = state
         L.orc_statistics.argtypes = [P, dp, dp, ip, C.c_int32, C.POINTER(C.c_double), dp]
+        L.orc_colloid_nearest.argtypes = [P, C.c_double, dp, C.c_int32, dp, dp, dp]
+        L.orc_colloid_noslip.argtypes = [P, dp, dp, dp, C.c_int32, dp]
+        L.orc_colloid_collect.argtypes = [P, dp, dp, dp, ip, C.c_int32]
         _lib = L
     return _lib
 
 
-def to_orc(prm, le_time: float = 0.0) -> OrcParams:
+_keep = []  # colloid records referenced by the last OrcParams built
+
+
+def to_orc(prm, le_time: float = 0.0, coll: "Colloids | None" = None) -> OrcParams:
     """Convert an SDPDParams-like object into the oracle's record; le_time
-    is the time the Lees-Edwards shear length is evaluated at."""
+    is the time the Lees-Edwards shear length is evaluated at; coll the
+    colloids of the box, if any."""
     o = OrcParams()
     o.num_dim = int(prm.num_dim)
     o.symmetry = 1 if prm.symmetry else 0
@@ -129,6 +220,10 @@ def to_orc(prm, le_time: float = 0.0) -> OrcParams:
         o.le_vlo[d] = float(getattr(prm, "le_v_lower", [0.0] * 3)[d])
         o.le_vhi[d] = float(getattr(prm, "le_v_upper", [0.0] * 3)[d])
     o.le_time = float(le_time)
+    if coll is not None:
+        rec = coll.struct()
+        _keep[:] = [rec]
+        o.coll = C.addressof(rec)
     return o
 
 
@@ -195,12 +290,12 @@ def neighbor_pairs(prm, x, brute=False, le_time=0.0):
     return pairs[: 2 * np_.value].reshape(-1, 2).copy()
 
 
-def density(prm, x, m, sid=None, le_time=0.0):
+def density(prm, x, m, sid=None, le_time=0.0, coll=None):
     x = _f64(x)
     n = x.shape[0]
     sid = np.zeros(n, np.int32) if sid is None else _i32(sid)
     rho = np.zeros(n)
-    _check(lib().orc_density(C.byref(to_orc(prm, le_time)), x, _f64(m), sid, n, rho), "density")
+    _check(lib().orc_density(C.byref(to_orc(prm, le_time, coll)), x, _f64(m), sid, n, rho), "density")
     return rho
 
 
@@ -211,7 +306,7 @@ def pressure(prm, rho, m):
     return p
 
 
-def interaction(prm, step, x, v, rho, press, m, pid, sid=None, ct=None, le_time=0.0):
+def interaction(prm, step, x, v, rho, press, m, pid, sid=None, ct=None, le_time=0.0, coll=None):
     x = _f64(x)
     n, dim = x.shape
     sid = np.zeros(n, np.int32) if sid is None else _i32(sid)
@@ -221,10 +316,37 @@ def interaction(prm, step, x, v, rho, press, m, pid, sid=None, ct=None, le_time=
     if ct is not None:
         ct = _f64(ct)
         ctp = ct.ctypes.data_as(C.c_void_p)
-    _check(lib().orc_interaction(C.byref(to_orc(prm, le_time)), int(step), x, _f64(v), _f64(rho), _f64(press),
+    _check(lib().orc_interaction(C.byref(to_orc(prm, le_time, coll)), int(step), x, _f64(v), _f64(rho), _f64(press),
                                  _f64(m), _i32(pid), sid, ctp, n, f,
                                  vgt.ctypes.data_as(C.c_void_p)), "interaction")
     return f, vgt
+
+
+def colloid_nearest(prm, coll, t, xp, sid):
+    """Nearest image of colloid sid's centre to xp: (xim, rx, vim)."""
+    xim, rx, vim = np.zeros(3), np.zeros(3), np.zeros(3)
+    xp = np.concatenate([np.asarray(xp, np.float64), np.zeros(3)])[:3]
+    _check(lib().orc_colloid_nearest(C.byref(to_orc(prm, 0.0, coll)), float(t), xp, int(sid), xim, rx,
+                                     vim), "colloid_nearest")
+    return xim, rx, vim
+
+
+def colloid_noslip(prm, coll, xf, xc, vf, sid, le_time=0.0):
+    """Velocity of boundary particle xc (colloid sid) seen by fluid
+    particle xf moving at vf."""
+    vc = np.zeros(3)
+    pad = lambda a: np.concatenate([np.asarray(a, np.float64), np.zeros(3)])[:3]  # noqa: E731
+    _check(lib().orc_colloid_noslip(C.byref(to_orc(prm, le_time, coll)), pad(xf), pad(xc), pad(vf),
+                                    int(sid), vc), "colloid_noslip")
+    return vc[: prm.num_dim]
+
+
+def colloid_collect(prm, coll, x, f, m, sid):
+    """Drag and torque of every colloid into coll.drag / coll.torque."""
+    x = _f64(x)
+    _check(lib().orc_colloid_collect(C.byref(to_orc(prm, 0.0, coll)), x, _f64(f), _f64(m), _i32(sid),
+                                     x.shape[0]), "colloid_collect")
+    return coll.drag.copy(), coll.torque.copy()
 
 
 def compute_act(prm, ct, vgt):
@@ -251,8 +373,9 @@ def pair_normals(seed, pid_lo, pid_hi, step):
 class State:
     """Particle state for multi-step oracle runs (particle-major arrays)."""
 
-    def __init__(self, prm, x, v, m, pid, sid=None, ct=None):
+    def __init__(self, prm, x, v, m, pid, sid=None, ct=None, coll=None):
         self.prm = prm
+        self.coll = coll
         self.x = np.array(x, np.float64, copy=True)
         n, dim = self.x.shape
         self.v = np.array(v, np.float64, copy=True)
@@ -269,6 +392,7 @@ class State:
         self.act = np.zeros((n, d2))
         self.vgt = np.zeros((n, d2))
         self.time = 0.0  # time_current (accumulated by repeated addition of dt)
+        self.le_last = 0.0  # time of the last shear-length update
 
     def _args(self):
         vp = C.c_void_p
@@ -279,22 +403,28 @@ class State:
         # marching_marching.F90:437: the boundary is updated at the current
         # time before the initial evaluation
         n = self.x.shape[0]
-        _check(lib().orc_prologue(C.byref(to_orc(self.prm, self.time)), int(step), n, self.x, self.v,
-                                  self.m, self.pid, self.sid, *self._args()), "prologue")
+        self.le_last = self.time
+        _check(lib().orc_prologue(C.byref(to_orc(self.prm, self.time, self.coll)), int(step), n, self.x,
+                                  self.v, self.m, self.pid, self.sid, *self._args()), "prologue")
 
     def vv_step(self, step):
         # marching_marching.F90:1392-1404 advances time_current before the
         # step; marching_integrate_VV.F90:784-786 evaluates the shear length
         # at time_current + dt
+        # the colloid block runs before the step's boundary update, with the
+        # shear length of the previous one (marching_integrate_VV.F90:455-797)
         n = self.x.shape[0]
         self.time = self.time + self.prm.dt
-        _check(lib().orc_vv_step(C.byref(to_orc(self.prm, self.time + self.prm.dt)), int(step), n,
+        if self.coll is not None:
+            self.coll.le_time = self.le_last
+        self.le_last = self.time + self.prm.dt
+        _check(lib().orc_vv_step(C.byref(to_orc(self.prm, self.le_last, self.coll)), int(step), n,
                                  self.x, self.v, self.m, self.pid, self.sid, *self._args()), "vv_step")
 
     def statistics(self):
         n = self.x.shape[0]
         e = C.c_double()
         mom = np.zeros(3)
-        _check(lib().orc_statistics(C.byref(to_orc(self.prm)), self.v, self.m, self.sid, n,
+        _check(lib().orc_statistics(C.byref(to_orc(self.prm, 0.0, self.coll)), self.v, self.m, self.sid, n,
                                     C.byref(e), mom), "statistics")
         return e.value, mom[: self.x.shape[1]]

@@ -72,7 +72,9 @@
 #include <string.h>
 
 #define ORC_PI 3.141592653589793238
-#define ORC_MACHINE_ZERO 4.440892098500626e-16 /* 2*EPSILON(1d0) */
+/* mcf_machine_zero as set at start-up: 10^(INT(LOG10(EPSILON(1d0))) + 1)
+ * (technique_new.F90:361-363) */
+#define ORC_MACHINE_ZERO 1e-14
 
 /* ------------------------------------------------------------------ */
 /* kernel                                                              */
@@ -601,6 +603,8 @@ typedef struct {
   const orc_cells *c;
   orc_kern k;
   const double *mall;
+  const int32_t *sall;   /* species ids over real particles and ghosts */
+  int coll_const;        /* colloid boundary particles keep a constant density */
   double *rho;
 } dens_ctx;
 
@@ -608,6 +612,10 @@ static void visit_density(void *vctx, int32_t ip, int32_t jp, double r,
                           const double *rij) {
   (void)rij;
   dens_ctx *dc = (dens_ctx *)vctx;
+  const int32_t si = dc->sall[ip], sj = dc->sall[jp];
+  /* boundary particles with constant density do not contribute to each
+   * other (particles_compute_density.F90:766-771) */
+  if (si != 0 && sj != 0 && dc->coll_const) return;
   double w, gw;
   kern_eval(&dc->k, r, &w, &gw);
   double rhoi, rhoj;
@@ -618,26 +626,40 @@ static void visit_density(void *vctx, int32_t ip, int32_t jp, double r,
     rhoi = w;                 /* rhs_density.F90:95-96 */
     rhoj = w;
   }
-  dc->rho[ip] = dc->rho[ip] + rhoi;
-  if (dc->p->symmetry) dc->rho[jp] = dc->rho[jp] + rhoj;
+  /* densities are summed for fluid particles and for boundary particles
+   * of dynamic density (:812-848) */
+  if (si == 0 || !dc->coll_const) dc->rho[ip] = dc->rho[ip] + rhoi;
+  if (dc->p->symmetry && (sj == 0 || !dc->coll_const)) dc->rho[jp] = dc->rho[jp] + rhoj;
 }
 
 static int density_on_cells(const orc_params *p, const orc_cells *c,
-                            const double *m, double *rho_real) {
+                            const double *m, const int32_t *sid, double *rho_real) {
   dens_ctx dc;
   dc.p = p;
   dc.c = c;
   kern_init(p, &dc.k);
   double *mall = (double *)malloc(sizeof(double) * (size_t)c->nall);
+  int32_t *sall = (int32_t *)malloc(sizeof(int32_t) * (size_t)c->nall);
   double *rho = (double *)calloc((size_t)c->nall, sizeof(double));
-  if (!mall || !rho) { free(mall); free(rho); return -1; }
-  for (int32_t i = 0; i < c->nall; ++i) mall[i] = m[c->owner[i]];
+  if (!mall || !sall || !rho) { free(mall); free(sall); free(rho); return -1; }
+  for (int32_t i = 0; i < c->nall; ++i) {
+    mall[i] = m[c->owner[i]];
+    sall[i] = sid[c->owner[i]];
+  }
   double w0, g0;
   kern_eval(&dc.k, 0.0, &w0, &g0);
-  /* self contribution (particles_compute_density.F90:369-558, fluid) */
-  for (int32_t i = 0; i < c->nreal; ++i)
-    rho[i] = p->rhs_density_type == 1 ? w0 * m[i] : w0;
+  dc.coll_const = p->coll && p->coll->rho_type == 0;
+  /* self contribution (particles_compute_density.F90:369-558): fluid and
+   * dynamic boundary particles start from W(0); constant boundary
+   * particles take the initial density rho (number density rho / m) */
+  for (int32_t i = 0; i < c->nreal; ++i) {
+    if (sid[i] > 0 && dc.coll_const)
+      rho[i] = p->rhs_density_type == 1 ? p->rho : p->rho / m[i];
+    else
+      rho[i] = p->rhs_density_type == 1 ? w0 * m[i] : w0;
+  }
   dc.mall = mall;
+  dc.sall = sall;
   dc.rho = rho;
   traverse(p, c, visit_density, &dc);
   /* ghost contributions back to their owners (map_ghost_put) */
@@ -646,17 +668,25 @@ static int density_on_cells(const orc_params *p, const orc_cells *c,
       rho[c->owner[g]] = rho[c->owner[g]] + rho[g];
   memcpy(rho_real, rho, sizeof(double) * (size_t)c->nreal);
   free(mall);
+  free(sall);
   free(rho);
+  return 0;
+}
+
+/* species ids: fluid (0) or a colloid of p->coll (1..num) */
+static int check_sid(const orc_params *p, const int32_t *sid, int32_t n) {
+  const int32_t nc = p->coll ? p->coll->num : 0;
+  for (int32_t i = 0; i < n; ++i)
+    if (sid[i] < 0 || sid[i] > nc) return -1;
   return 0;
 }
 
 int orc_density(const orc_params *p, const double *x, const double *m,
                 const int32_t *sid, int32_t n, double *rho) {
-  for (int32_t i = 0; i < n; ++i)
-    if (sid[i] != 0) return -1; /* fluid particles only */
+  if (check_sid(p, sid, n)) return -1;
   orc_cells c;
   if (cells_build(p, x, n, &c)) return -1;
-  int rc = density_on_cells(p, &c, m, rho);
+  int rc = density_on_cells(p, &c, m, sid, rho);
   cells_free(&c);
   return rc;
 }
@@ -770,6 +800,159 @@ void orc_pair_normals(uint64_t seed, uint32_t pid_lo, uint32_t pid_hi,
 }
 
 /* ------------------------------------------------------------------ */
+/* colloids                                                             */
+/* ------------------------------------------------------------------ */
+
+/* mcf_colloid_dist_ratio_max (mcf_header.F90:152) */
+#define ORC_COLL_RATIO_MAX 0.5
+
+static void cross3(const double *a, const double *b, double *c) {
+  /* tool_cross_product.F90 */
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+int orc_colloid_nearest(const orc_params *p, double t, const double *xp, int32_t sid,
+                        double *xim, double *rx, double *vim) {
+  const orc_colloids *cp = p->coll;
+  if (!cp || sid < 1 || sid > cp->num) return -1;
+  const int dim = p->num_dim;
+  const int32_t k = sid - 1;
+  const double dmax = cp->radius[k] + cp->dout;
+  double L[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) L[d] = p->max_phys[d] - p->min_phys[d];
+  for (int d = 0; d < 3; ++d) {
+    xim[d] = cp->x[d + 3 * (size_t)k];
+    vim[d] = cp->v[d + 3 * (size_t)k]; /* newest level */
+    rx[d] = 0.0;
+  }
+  for (int d = 0; d < dim; ++d) rx[d] = xp[d] - xim[d];
+  const int g = le_index(p);
+  if (g >= 0) {
+    /* colloid_in_nearest_image.F90: the Lees-Edwards face pair first,
+     * shear_length(:, upper) = (vhi - vlo) t, (:, lower) = (vlo - vhi) t */
+    if (rx[g] > dmax) {
+      for (int q = 0; q < dim; ++q)
+        if (q == g) {
+          xim[q] = xim[q] + L[q];
+        } else {
+          xim[q] = f_modulo(xim[q] + (p->le_vhi[q] - p->le_vlo[q]) * t, L[q]);
+          vim[q] = vim[q] + (p->le_vhi[q] - p->le_vlo[q]);
+        }
+    } else if (-rx[g] > dmax) {
+      for (int q = 0; q < dim; ++q)
+        if (q == g) {
+          xim[q] = xim[q] - L[q];
+        } else {
+          xim[q] = f_modulo(xim[q] + (p->le_vlo[q] - p->le_vhi[q]) * t, L[q]);
+          vim[q] = vim[q] + (p->le_vlo[q] - p->le_vhi[q]);
+        }
+    }
+    for (int d = 0; d < dim; ++d) rx[d] = xp[d] - xim[d];
+  }
+  for (int d = 0; d < dim; ++d) {
+    if (rx[d] > dmax)
+      xim[d] = xim[d] + L[d];
+    else if (-rx[d] > dmax)
+      xim[d] = xim[d] - L[d];
+  }
+  for (int d = 0; d < dim; ++d) rx[d] = xp[d] - xim[d];
+  return 0;
+}
+
+int orc_colloid_noslip(const orc_params *p, const double *xf, const double *xc,
+                       const double *vf, int32_t sid, double *vc) {
+  const orc_colloids *cp = p->coll;
+  if (!cp) return -1;
+  const int dim = p->num_dim;
+  double xcoll[3], r_xc[3], vcoll[3];
+  if (orc_colloid_nearest(p, p->le_time, xc, sid, xcoll, r_xc, vcoll)) return -1;
+  const int32_t k = sid - 1;
+  const double *om = cp->omega + 3 * (size_t)k; /* newest level */
+  for (int d = 0; d < 3; ++d) vc[d] = 0.0;
+  if (cp->noslip == 1) {
+    /* colloid_noslip_frozen.F90 */
+    for (int d = 0; d < dim; ++d) vc[d] = vcoll[d];
+    if (cp->rotate) {
+      double r_vc[3];
+      cross3(om, r_xc, r_vc);
+      for (int d = 0; d < dim; ++d) vc[d] = vc[d] + r_vc[d];
+    }
+    return 0;
+  }
+  if (cp->noslip != 2) return -1;
+  /* colloid_noslip_Morris_sphere.F90:129-300 (the disk routine has the
+   * same steps) */
+  const double R = cp->radius[k];
+  double r_xf[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) r_xf[d] = xf[d] - xcoll[d];
+  double s2 = 0.0;
+  for (int d = 0; d < dim; ++d) s2 = s2 + r_xf[d] * r_xf[d];
+  const double d_fcoll = sqrt(s2);
+  if (d_fcoll <= R) {
+    /* the fluid particle is inside: rigid velocity at the boundary particle */
+    for (int d = 0; d < dim; ++d) vc[d] = vcoll[d];
+    if (cp->rotate) {
+      double r_vc[3];
+      cross3(om, r_xc, r_vc);
+      for (int d = 0; d < dim; ++d) vc[d] = vc[d] + r_vc[d];
+    }
+    return 0;
+  }
+  double c2 = 0.0;
+  for (int d = 0; d < dim; ++d) c2 = c2 + r_xc[d] * r_xc[d];
+  if (sqrt(c2) > R + cp->dout) return -1; /* boundary particle left its colloid */
+  double nv[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) nv[d] = r_xf[d] / d_fcoll;
+  double d_cn = 0.0;
+  for (int d = 0; d < dim; ++d) d_cn = d_cn + r_xc[d] * nv[d];
+  const double d_cs = R - d_cn;
+  double d_fs = d_fcoll - R;
+  double r_xs[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) r_xs[d] = R / d_fcoll * r_xf[d];
+  double corr;
+  if (fabs(d_fs) < ORC_MACHINE_ZERO) {
+    d_fs = ORC_MACHINE_ZERO;
+    corr = ORC_COLL_RATIO_MAX;
+  } else {
+    corr = d_cs / d_fs;
+    if (corr > ORC_COLL_RATIO_MAX)
+      corr = ORC_COLL_RATIO_MAX;
+    else if (corr < -ORC_COLL_RATIO_MAX)
+      corr = -ORC_COLL_RATIO_MAX;
+  }
+  double r_vs[3] = {0, 0, 0};
+  if (cp->rotate) cross3(om, r_xs, r_vs);
+  for (int d = 0; d < dim; ++d)
+    vc[d] = -corr * (vf[d] - vcoll[d] - r_vs[d]) + vcoll[d] + r_vs[d];
+  return 0;
+}
+
+int orc_colloid_collect(const orc_params *p, const double *x, const double *f,
+                        const double *m, const int32_t *sid, int32_t n) {
+  orc_colloids *cp = p->coll;
+  if (!cp) return -1;
+  const int dim = p->num_dim;
+  for (int32_t q = 0; q < 3 * cp->num; ++q) {
+    cp->drag[q] = 0.0;
+    cp->torque[q] = 0.0;
+  }
+  for (int32_t j = 0; j < n; ++j) {
+    if (sid[j] <= 0) continue;
+    if (sid[j] > cp->num) return -1;
+    const int32_t k = sid[j] - 1;
+    double F[3] = {0, 0, 0}, xim[3], rx[3], vim[3], T[3];
+    for (int d = 0; d < dim; ++d) F[d] = f[d + (size_t)dim * j] * m[j];
+    for (int d = 0; d < dim; ++d) cp->drag[d + 3 * (size_t)k] += F[d];
+    if (orc_colloid_nearest(p, cp->le_time, x + (size_t)dim * j, sid[j], xim, rx, vim)) return -1;
+    cross3(rx, F, T);
+    for (int d = 0; d < 3; ++d) cp->torque[d + 3 * (size_t)k] += T[d];
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
 /* pair forces                                                          */
 /* ------------------------------------------------------------------ */
 
@@ -779,7 +962,7 @@ typedef struct {
   orc_kern k;
   int64_t step;
   const double *v, *rho, *press, *m, *ct; /* over xall */
-  const int32_t *pid;
+  const int32_t *pid, *sid;
   double *f, *vgt;                         /* over xall */
   double *pt;                              /* (dim,dim,nall) */
   int fail;
@@ -814,9 +997,31 @@ static void visit_force(void *vctx, int32_t ip, int32_t jp, double dij,
   force_ctx *fc = (force_ctx *)vctx;
   const orc_params *p = fc->p;
   const int dim = p->num_dim;
+  const double *vi = fc->v + (size_t)dim * ip, *vj = fc->v + (size_t)dim * jp;
+  double pi = fc->press[ip], pj = fc->press[jp];
+  double vbuf[3];
+  const int32_t si = fc->sid[ip], sj = fc->sid[jp];
+  if (si != 0 || sj != 0) {
+    /* pp_interaction.inc: two boundary particles do not interact (same
+     * colloid: CYCLE; different colloids: pp_interact_cc off) */
+    if (si != 0 && sj != 0) return;
+    const double *xi = fc->c->xall + (size_t)dim * ip;
+    const double *xj = fc->c->xall + (size_t)dim * jp;
+    if (sj != 0) {
+      /* pp_interaction_fc.inc: jp takes the no-slip velocity and ip's
+       * pressure */
+      if (orc_colloid_noslip(p, xi, xj, vi, sj, vbuf)) { fc->fail = 1; return; }
+      vj = vbuf;
+      pj = pi;
+    } else {
+      /* pp_interaction_cf.inc: the same with the roles swapped */
+      if (orc_colloid_noslip(p, xj, xi, vj, si, vbuf)) { fc->fail = 1; return; }
+      vi = vbuf;
+      pi = pj;
+    }
+  }
   double w, gradw;
   kern_eval(&fc->k, dij, &w, &gradw);
-  const double *vi = fc->v + (size_t)dim * ip, *vj = fc->v + (size_t)dim * jp;
   double numi = fc->rho[ip], numj = fc->rho[jp];
   double mi = fc->m[ip], mj = fc->m[jp];
   if (mi < ORC_MACHINE_ZERO || mj < ORC_MACHINE_ZERO ||
@@ -872,7 +1077,6 @@ static void visit_force(void *vctx, int32_t ip, int32_t jp, double dij,
     }
   } else if (p->rhs_force_type == 3) {
     /* rhs_force_ff_Newtonian_HuAdams.F90:140-237 */
-    double pi = fc->press[ip], pj = fc->press[jp];
     double f_c = (pi / (numi * numi) + pj / (numj * numj)) * gradw;
     for (int d = 0; d < dim; ++d) {
       fi[d] = fi[d] - f_c * eij[d] / mi;
@@ -895,7 +1099,6 @@ static void visit_force(void *vctx, int32_t ip, int32_t jp, double dij,
     }
   } else {
     /* rhs_force_ff_Newtonian_HuAdams_angular.F90:166-229 */
-    double pi = fc->press[ip], pj = fc->press[jp];
     double ev = 0.0;
     for (int d = 0; d < dim; ++d) ev = ev + eij[d] * vij[d];
     double pij = (pi + pj) / 2.0;
@@ -933,8 +1136,8 @@ static int interaction_on_cells(const orc_params *p, const orc_cells *c,
                                 int64_t step, const double *v,
                                 const double *rho, const double *press,
                                 const double *m, const int32_t *pid,
-                                const double *ct, double *f_real,
-                                double *vgt_real) {
+                                const int32_t *sid, const double *ct,
+                                double *f_real, double *vgt_real) {
   const int dim = p->num_dim, d2n = dim * dim;
   const int32_t na = c->nall;
   force_ctx fc;
@@ -948,11 +1151,12 @@ static int interaction_on_cells(const orc_params *p, const orc_cells *c,
   double *pa = (double *)malloc(sizeof(double) * (size_t)na);
   double *ma = (double *)malloc(sizeof(double) * (size_t)na);
   int32_t *ia = (int32_t *)malloc(sizeof(int32_t) * (size_t)na);
+  int32_t *sa = (int32_t *)malloc(sizeof(int32_t) * (size_t)na);
   double *fa = (double *)calloc((size_t)dim * na, sizeof(double));
   double *vg = (double *)calloc((size_t)d2n * na, sizeof(double));
   double *pt = (double *)calloc((size_t)d2n * na, sizeof(double));
-  if (!va || !ra || !pa || !ma || !ia || !fa || !vg || !pt) {
-    free(va); free(ra); free(pa); free(ma); free(ia); free(fa); free(vg); free(pt);
+  if (!va || !ra || !pa || !ma || !ia || !sa || !fa || !vg || !pt) {
+    free(va); free(ra); free(pa); free(ma); free(ia); free(sa); free(fa); free(vg); free(pt);
     return -1;
   }
   /* map_ghost_get(x, rho, v, ct): ghosts carry their owner's values */
@@ -963,6 +1167,7 @@ static int interaction_on_cells(const orc_params *p, const orc_cells *c,
     pa[i] = press[o];
     ma[i] = m[o];
     ia[i] = pid[o];
+    sa[i] = sid[o];
   }
   {
     /* Lees-Edwards images move with the face they crossed
@@ -995,7 +1200,7 @@ static int interaction_on_cells(const orc_params *p, const orc_cells *c,
         }
     }
   }
-  fc.v = va; fc.rho = ra; fc.press = pa; fc.m = ma; fc.pid = ia;
+  fc.v = va; fc.rho = ra; fc.press = pa; fc.m = ma; fc.pid = ia; fc.sid = sa;
   fc.ct = ct; fc.f = fa; fc.vgt = vg; fc.pt = pt;
   traverse(p, c, visit_force, &fc);
   if (p->symmetry)
@@ -1007,17 +1212,17 @@ static int interaction_on_cells(const orc_params *p, const orc_cells *c,
         for (int q = 0; q < d2n; ++q)
           vg[q + (size_t)d2n * o] += vg[q + (size_t)d2n * g];
     }
-  /* particles_apply_body_force.F90:125-193 (type 1) */
+  /* particles_apply_body_force.F90:125-193 (type 1), fluid particles only */
   if (p->body_force_type == 1)
     for (int d = 0; d < dim; ++d)
       if (fabs(p->body_force[d]) > ORC_MACHINE_ZERO)
         for (int32_t i = 0; i < c->nreal; ++i)
-          fa[d + (size_t)dim * i] += p->body_force[d];
+          if (sid[i] == 0) fa[d + (size_t)dim * i] += p->body_force[d];
   memcpy(f_real, fa, sizeof(double) * (size_t)dim * c->nreal);
   if (!p->newtonian && vgt_real)
     memcpy(vgt_real, vg, sizeof(double) * (size_t)d2n * c->nreal);
   int fail = fc.fail;
-  free(va); free(ra); free(pa); free(ma); free(ia); free(fa); free(vg); free(pt);
+  free(va); free(ra); free(pa); free(ma); free(ia); free(sa); free(fa); free(vg); free(pt);
   return fail ? -1 : 0;
 }
 
@@ -1025,13 +1230,13 @@ int orc_interaction(const orc_params *p, int64_t step, const double *x,
                     const double *v, const double *rho, const double *press,
                     const double *m, const int32_t *pid, const int32_t *sid,
                     const double *ct, int32_t n, double *f, double *vgt) {
-  for (int32_t i = 0; i < n; ++i)
-    if (sid[i] != 0) return -1;
+  if (check_sid(p, sid, n)) return -1;
   if (p->rhs_force_type != 3 && p->rhs_force_type != 4) return -1;
   if (!p->newtonian && p->rhs_force_type != 3) return -1;
+  if (!p->newtonian && p->coll) return -1; /* colloids in Newtonian flows only */
   orc_cells c;
   if (cells_build(p, x, n, &c)) return -1;
-  int rc = interaction_on_cells(p, &c, step, v, rho, press, m, pid, ct, f, vgt);
+  int rc = interaction_on_cells(p, &c, step, v, rho, press, m, pid, sid, ct, f, vgt);
   cells_free(&c);
   return rc;
 }
@@ -1126,16 +1331,233 @@ static void integrate_ct(const orc_params *p, int32_t n, double *ct,
   for (size_t q = 0; q < tot; ++q) ct[q] = ct[q] + act[q] * p->dt * lambda;
 }
 
+/* Adams-Bashforth increment of order `order` from a history (3, num, ab),
+ * newest first, for colloid k (colloid_integrate_translate_position.F90
+ * and its siblings use these coefficients). */
+static void ab_increment(const double *h, int32_t num, int32_t k, int order, double dt,
+                         double *out) {
+  const size_t lv = 3 * (size_t)num;
+  for (int d = 0; d < 3; ++d) {
+    const double *a = h + d + 3 * (size_t)k;
+    switch (order) {
+      case 1: out[d] = a[0] * dt; break;
+      case 2: out[d] = (3.0 * a[0] - a[lv]) * dt / 2.0; break;
+      case 3: out[d] = (23.0 * a[0] - 16.0 * a[lv] + 5.0 * a[2 * lv]) * dt / 12.0; break;
+      case 4:
+        out[d] = (55.0 * a[0] - 59.0 * a[lv] + 37.0 * a[2 * lv] - 9.0 * a[3 * lv]) * dt / 24.0;
+        break;
+      default:
+        out[d] = (1901.0 * a[0] - 2774.0 * a[lv] + 2616.0 * a[2 * lv] - 1274.0 * a[3 * lv] +
+                  251.0 * a[4 * lv]) * dt / 720.0;
+        break;
+    }
+  }
+}
+
+/* history levels move one back: h(:,:,i) = h(:,:,i-1), i = ab..2 */
+static void history_shift(double *h, int32_t num, int32_t ab) {
+  const size_t lv = 3 * (size_t)num;
+  for (int32_t i = ab - 1; i >= 1; --i) memcpy(h + i * lv, h + (i - 1) * lv, lv * sizeof(double));
+}
+
+/* tool_rotation_matrix.F90: rotation by phi about the unit axis ax,
+ * column-major rm(3,3) */
+static void rotation_matrix(const double *ax, double phi, double *rm) {
+  const double cp = cos(phi), cp1 = 1.0 - cp, sp = sin(phi);
+  rm[0 + 3 * 0] = ax[0] * ax[0] * cp1 + cp;
+  rm[0 + 3 * 1] = ax[0] * ax[1] * cp1 - ax[2] * sp;
+  rm[0 + 3 * 2] = ax[0] * ax[2] * cp1 + ax[1] * sp;
+  rm[1 + 3 * 0] = ax[1] * ax[0] * cp1 + ax[2] * sp;
+  rm[1 + 3 * 1] = ax[1] * ax[1] * cp1 + cp;
+  rm[1 + 3 * 2] = ax[1] * ax[2] * cp1 - ax[0] * sp;
+  rm[2 + 3 * 0] = ax[2] * ax[0] * cp1 - ax[1] * sp;
+  rm[2 + 3 * 1] = ax[2] * ax[1] * cp1 + ax[0] * sp;
+  rm[2 + 3 * 2] = ax[2] * ax[2] * cp1 + cp;
+}
+
+/* colloid_adjust_colloid.F90: centres leaving the box (periodic and
+ * Lees-Edwards faces, shear length at the colloid block's time) */
+static void adjust_colloids(const orc_params *p) {
+  orc_colloids *cp = p->coll;
+  const int dim = p->num_dim, g = le_index(p);
+  const double t = cp->le_time;
+  for (int32_t j = 0; j < cp->num; ++j) {
+    double *X = cp->x + 3 * (size_t)j, *V = cp->v + 3 * (size_t)j;
+    for (int d = 0; d < dim; ++d) {
+      const double L = p->max_phys[d] - p->min_phys[d];
+      if (X[d] < p->min_phys[d]) {
+        if (d == g) {
+          X[d] = X[d] + L;
+          for (int q = 0; q < dim; ++q)
+            if (q != d) {
+              double Lq = p->max_phys[q] - p->min_phys[q];
+              X[q] = f_modulo(X[q] - (p->le_vlo[q] - p->le_vhi[q]) * t, Lq);
+              V[q] = V[q] + (p->le_vhi[q] - p->le_vlo[q]);
+            }
+          break;
+        }
+        const double cross = (p->max_phys[d] - X[d]) / L;
+        X[d] = X[d] + L * (double)(int)cross;
+      }
+      if (X[d] >= p->max_phys[d]) {
+        if (d == g) {
+          X[d] = X[d] - L;
+          for (int q = 0; q < dim; ++q)
+            if (q != d) {
+              double Lq = p->max_phys[q] - p->min_phys[q];
+              X[q] = f_modulo(X[q] - (p->le_vhi[q] - p->le_vlo[q]) * t, Lq);
+              V[q] = V[q] + (p->le_vlo[q] - p->le_vhi[q]);
+            }
+          break;
+        }
+        const double cross = (X[d] - p->min_phys[d]) / L;
+        X[d] = X[d] - L * (double)(int)cross;
+      }
+    }
+  }
+}
+
+/* The colloid block of marching_integrate_VV.F90:455-765 with the
+ * explicit integrators (integrate_colloid_type 1, 2) and no colloid-colloid
+ * forces: drag and torque from the boundary particles' last forces, then
+ * per sub-step the rigid-body update of the centres, velocities and
+ * rotations, carrying the boundary particles along. */
+static int colloid_block(const orc_params *p, int64_t step, int32_t n, double *x,
+                         const double *f, const double *m, const int32_t *sid) {
+  orc_colloids *cp = p->coll;
+  if (orc_colloid_collect(p, x, f, m, sid, n)) return -1;
+  if (!cp->translate && !cp->rotate) return 0;
+  const int dim = p->num_dim;
+  const int32_t nc = cp->num, ab = cp->ab;
+  if (ab < 1 || ab > 5 || cp->sub_steps < 1) return -1;
+  const size_t lv = 3 * (size_t)nc;
+  double *drag0 = (double *)malloc(lv * sizeof(double));
+  double *torque0 = (double *)malloc(lv * sizeof(double));
+  if (!drag0 || !torque0) { free(drag0); free(torque0); return -1; }
+  memcpy(drag0, cp->drag, lv * sizeof(double));
+  memcpy(torque0, cp->torque, lv * sizeof(double));
+  const double dts = p->dt / cp->sub_steps;
+  int rc = 0;
+  for (int i = 1; i <= cp->sub_steps && !rc; ++i) {
+    /* the AB order ramps up with the step index passed by the caller;
+     * the rotating velocity is passed step - 1 + i + step_start */
+    const int64_t st = step - 1 + i - cp->step_start;
+    const int64_t sw = step - 1 + i + cp->step_start;
+    const int ord = (int)(st < ab ? st : ab), ordw = (int)(sw < ab ? sw : ab);
+    if (ord < 1 || ordw < 1) { rc = -1; break; }
+    if (cp->rotate)
+      for (int32_t k = 0; k < nc; ++k) {
+        /* colloid_compute_rotation_vector.F90 */
+        double axis[3];
+        ab_increment(cp->omega, nc, k, ord, dts, axis);
+        double phi = sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+        if (phi < ORC_MACHINE_ZERO) {
+          phi = 0.0;
+          axis[0] = 1.0;
+          axis[1] = axis[2] = 0.0;
+        } else {
+          for (int d = 0; d < 3; ++d) axis[d] = axis[d] / phi;
+        }
+        cp->theta[2 + 3 * (size_t)k] += phi;
+        /* colloid_compute_rotation_matrix.F90, _accumulation_matrix.F90 */
+        double *R = cp->rot_matrix + 9 * (size_t)k, *A = cp->acc_matrix + 9 * (size_t)k;
+        rotation_matrix(axis, phi, R);
+        double RA[9];
+        for (int c = 0; c < 3; ++c)
+          for (int r = 0; r < 3; ++r)
+            RA[r + 3 * c] = R[r + 0] * A[0 + 3 * c] + R[r + 3] * A[1 + 3 * c] + R[r + 6] * A[2 + 3 * c];
+        memcpy(A, RA, sizeof(RA));
+      }
+    /* particles_compute_colloid_relative_position.F90 */
+    for (int32_t j = 0; j < n && !rc; ++j) {
+      if (sid[j] <= 0) continue;
+      double xim[3], rx[3], vim[3];
+      double *xj = x + (size_t)dim * j;
+      if (orc_colloid_nearest(p, cp->le_time, xj, sid[j], xim, rx, vim)) { rc = -1; break; }
+      if (cp->rotate) {
+        const double *R = cp->rot_matrix + 9 * (size_t)(sid[j] - 1);
+        for (int r = 0; r < dim; ++r) {
+          double t = 0.0;
+          for (int c = 0; c < dim; ++c) t = t + R[r + 3 * c] * rx[c];
+          xj[r] = t;
+        }
+      } else {
+        for (int d = 0; d < dim; ++d) xj[d] = rx[d];
+      }
+    }
+    if (rc) break;
+    if (cp->rotate) {
+      /* colloid_integrate_rotate_velocity.F90 */
+      history_shift(cp->omega, nc, ab);
+      for (int32_t k = 0; k < nc; ++k) {
+        double inc[3];
+        ab_increment(cp->alpha, nc, k, ordw, dts, inc);
+        for (int d = 0; d < 3; ++d) cp->omega[d + 3 * (size_t)k] += inc[d];
+      }
+    }
+    if (cp->translate) {
+      /* colloid_integrate_translate_position.F90, _velocity.F90 */
+      for (int32_t k = 0; k < nc; ++k) {
+        double inc[3];
+        ab_increment(cp->v, nc, k, ord, dts, inc);
+        for (int d = 0; d < dim; ++d) cp->x[d + 3 * (size_t)k] += inc[d];
+      }
+      history_shift(cp->v, nc, ab);
+      for (int32_t k = 0; k < nc; ++k) {
+        double inc[3];
+        ab_increment(cp->f, nc, k, ord, dts, inc);
+        for (int d = 0; d < dim; ++d) cp->v[d + 3 * (size_t)k] += inc[d];
+      }
+    }
+    /* colloid_compute_interaction.F90 (no colloid-colloid or wall forces)
+     * and colloid_apply_body_force.F90 */
+    memcpy(cp->drag, drag0, lv * sizeof(double));
+    memcpy(cp->torque, torque0, lv * sizeof(double));
+    if (cp->body_force_type == 1)
+      for (int32_t k = 0; k < nc; ++k)
+        for (int d = 0; d < dim; ++d) cp->drag[d + 3 * (size_t)k] += cp->body_force[d] * cp->m[k];
+    /* colloid_compute_translate_acceleration.F90 */
+    history_shift(cp->f, nc, ab);
+    for (int32_t k = 0; k < nc; ++k)
+      for (int d = 0; d < dim; ++d)
+        cp->f[d + 3 * (size_t)k] = cp->translate ? cp->drag[d + 3 * (size_t)k] / cp->m[k] : 0.0;
+    adjust_colloids(p);
+    /* colloid_compute_rotate_acceleration.F90 (disks and spheres) */
+    history_shift(cp->alpha, nc, ab);
+    for (int32_t k = 0; k < nc; ++k) {
+      double *al = cp->alpha + 3 * (size_t)k;
+      const double *tq = cp->torque + 3 * (size_t)k, *mi = cp->mmi + 3 * (size_t)k;
+      if (!cp->rotate) {
+        al[0] = al[1] = al[2] = 0.0;
+      } else if (dim == 2) {
+        al[2] = tq[2] / mi[2];
+      } else {
+        for (int d = 0; d < 3; ++d) al[d] = tq[d] / mi[d];
+      }
+    }
+    /* particles_compute_colloid_absolute_position.F90 */
+    for (int32_t j = 0; j < n; ++j) {
+      if (sid[j] <= 0) continue;
+      const double *X = cp->x + 3 * (size_t)(sid[j] - 1);
+      for (int d = 0; d < dim; ++d) x[d + (size_t)dim * j] += X[d];
+    }
+  }
+  free(drag0);
+  free(torque0);
+  return rc;
+}
+
 static int evaluate(const orc_params *p, int64_t step, int32_t n,
                     const double *x, const double *v, const double *m,
                     const int32_t *pid, double *rho, double *press, double *f,
-                    double *ct, double *act, double *vgt) {
+                    const int32_t *sid, double *ct, double *act, double *vgt) {
   orc_cells c;
+  if (!p->newtonian && p->coll) return -1;
   if (cells_build(p, x, n, &c)) return -1;
-  int rc = density_on_cells(p, &c, m, rho);
+  int rc = density_on_cells(p, &c, m, sid, rho);
   if (!rc) rc = orc_pressure(p, rho, m, n, press);
   if (!rc)
-    rc = interaction_on_cells(p, &c, step, v, rho, press, m, pid, ct, f, vgt);
+    rc = interaction_on_cells(p, &c, step, v, rho, press, m, pid, sid, ct, f, vgt);
   cells_free(&c);
   if (!rc && !p->newtonian) rc = orc_compute_act(p, ct, vgt, n, act);
   return rc;
@@ -1145,10 +1567,36 @@ int orc_prologue(const orc_params *p, int64_t step, int32_t n, double *x,
                  double *v, const double *m, const int32_t *pid,
                  const int32_t *sid, double *rho, double *press, double *f,
                  double *ct, double *act, double *vgt) {
-  for (int32_t i = 0; i < n; ++i)
-    if (sid[i] != 0) return -1;
-  (void)sid;
-  return evaluate(p, step, n, x, v, m, pid, rho, press, f, ct, act, vgt);
+  if (check_sid(p, sid, n)) return -1;
+  int rc = evaluate(p, step, n, x, v, m, pid, rho, press, f, sid, ct, act, vgt);
+  if (rc || !p->coll) return rc;
+  /* marching_marching.F90:1003-1116: drag and torque of the initial
+   * forces start the colloids' acceleration histories */
+  orc_colloids *cp = p->coll;
+  const int dim = p->num_dim;
+  const double t_block = cp->le_time;
+  cp->le_time = p->le_time;
+  rc = orc_colloid_collect(p, x, f, m, sid, n);
+  cp->le_time = t_block;
+  if (rc) return rc;
+  if (cp->body_force_type == 1)
+    for (int32_t k = 0; k < cp->num; ++k)
+      for (int d = 0; d < dim; ++d) cp->drag[d + 3 * (size_t)k] += cp->body_force[d] * cp->m[k];
+  history_shift(cp->f, cp->num, cp->ab);
+  history_shift(cp->alpha, cp->num, cp->ab);
+  for (int32_t k = 0; k < cp->num; ++k) {
+    double *fk = cp->f + 3 * (size_t)k, *al = cp->alpha + 3 * (size_t)k;
+    const double *tq = cp->torque + 3 * (size_t)k, *mi = cp->mmi + 3 * (size_t)k;
+    for (int d = 0; d < dim; ++d) fk[d] = cp->translate ? cp->drag[d + 3 * (size_t)k] / cp->m[k] : 0.0;
+    if (!cp->rotate) {
+      al[0] = al[1] = al[2] = 0.0;
+    } else if (dim == 2) {
+      al[2] = tq[2] / mi[2];
+    } else {
+      for (int d = 0; d < 3; ++d) al[d] = tq[d] / mi[d];
+    }
+  }
+  return 0;
 }
 
 int orc_vv_step(const orc_params *p, int64_t step, int32_t n, double *x,
@@ -1156,11 +1604,13 @@ int orc_vv_step(const orc_params *p, int64_t step, int32_t n, double *x,
                 const int32_t *sid, double *rho, double *press, double *f,
                 double *ct, double *act, double *vgt) {
   /* marching_integrate_VV.F90:303-1619 */
+  if (check_sid(p, sid, n)) return -1;
   integrate_velocity(p, n, v, f, sid, 0.5);
   integrate_position(p, n, x, v, sid);
   if (!p->newtonian) integrate_ct(p, n, ct, act, 0.5);
+  if (p->coll && colloid_block(p, step, n, x, f, m, sid)) return -1;
   adjust_particles(p, n, x, v);
-  int rc = evaluate(p, step, n, x, v, m, pid, rho, press, f, ct, act, vgt);
+  int rc = evaluate(p, step, n, x, v, m, pid, rho, press, f, sid, ct, act, vgt);
   if (rc) return rc;
   if (!p->newtonian) integrate_ct(p, n, ct, act, 0.5);
   integrate_velocity(p, n, v, f, sid, 0.5);
@@ -1178,6 +1628,24 @@ int orc_statistics(const orc_params *p, const double *v, const double *m,
       e = e + 0.5 * m[j] * v2;
       for (int d = 0; d < dim; ++d) mm[d] = mm[d] + m[j] * v[d + (size_t)dim * j];
     }
+  if (p->coll) {
+    /* colloid_compute_statistic.F90: translational kinetic energy and
+     * momentum of the colloids, added by statistic_compute_statistic.F90:465-475 */
+    const orc_colloids *cp = p->coll;
+    double ek = 0.0, cm[3] = {0, 0, 0};
+    for (int32_t k = 0; k < cp->num; ++k) {
+      double v2 = 0.0, mk[3] = {0, 0, 0};
+      for (int d = 0; d < dim; ++d) {
+        const double vk = cp->v[d + 3 * (size_t)k];
+        v2 = v2 + vk * vk;
+        mk[d] = mk[d] + cp->m[k] * vk;
+      }
+      ek = ek + (0.0 + 0.5 * cp->m[k] * v2);
+      for (int d = 0; d < dim; ++d) cm[d] = cm[d] + mk[d];
+    }
+    e = e + ek;
+    for (int d = 0; d < dim; ++d) mm[d] = mm[d] + cm[d];
+  }
   *ekin = e;
   for (int d = 0; d < dim; ++d) mom[d] = mm[d];
   return 0;

@@ -20,6 +20,8 @@
 extern "C" {
 #endif
 
+struct orc_colloids;
+
 typedef struct orc_params {
   int32_t num_dim;          /* 2 or 3                                          */
   int32_t symmetry;         /* 1: half stencil + upper ghost layer (reference default) */
@@ -50,7 +52,42 @@ typedef struct orc_params {
   double le_vlo[3];
   double le_vhi[3];
   double le_time;
+  /* colloids (NULL: fluid only); boundary particles carry sid 1..num */
+  struct orc_colloids *coll;
 } orc_params;
+
+/* Rigid colloids built from boundary particles: disks in 2D, spheres in
+ * 3D (Class_Colloid.F90).  Vectors are stored with 3 components per
+ * colloid whatever the dimension; histories hold `ab` levels, newest
+ * first, as the reference's third index.  The caller owns the arrays;
+ * orc_vv_step updates them in place. */
+typedef struct orc_colloids {
+  int32_t num;             /* colloids                                     */
+  int32_t noslip;          /* coll_noslip: 1 frozen, 2 Morris              */
+  int32_t rho_type;        /* coll_rho_type: 0 constant, 1 dynamic         */
+  int32_t translate;       /* coll_translate                               */
+  int32_t rotate;          /* coll_rotate                                  */
+  int32_t ab;              /* integrate_colloid_AB, 1..5                   */
+  int32_t sub_steps;       /* coll_sub_time_step                           */
+  int32_t body_force_type; /* coll_body_force_type: 0 none, 1 constant     */
+  double body_force[3];    /* coll_body_force (per unit mass)              */
+  double dout;             /* outer layer (0.05 dx(1), physics_adjust_parameters.F90:320) */
+  double le_time;          /* shear-length time of the colloid block       */
+  int64_t step_start;
+  const double *radius;    /* (num)                                        */
+  const double *m;         /* (num)                                        */
+  const double *mmi;       /* (3, num)                                     */
+  double *x;               /* (3, num) centres                             */
+  double *v;               /* (3, num, ab)                                 */
+  double *omega;           /* (3, num, ab)                                 */
+  double *f;               /* (3, num, ab) translational acceleration      */
+  double *alpha;           /* (3, num, ab) rotational acceleration         */
+  double *rot_matrix;      /* (3, 3, num) rotation of the last sub-step    */
+  double *acc_matrix;      /* (3, 3, num) accumulated rotation             */
+  double *theta;           /* (3, num)                                     */
+  double *drag;            /* (3, num) sum of m f over boundary particles  */
+  double *torque;          /* (3, num) sum of r x (m f)                    */
+} orc_colloids;
 
 /* smoothing kernel W(r), W'(r) */
 int orc_kernel(const orc_params *p, double r, double *w, double *gradw);
@@ -127,6 +164,26 @@ int orc_vv_step(const orc_params *p, int64_t step, int32_t n, double *x,
  * particles (naive summation, as the reference).                        */
 int orc_statistics(const orc_params *p, const double *v, const double *m,
                    const int32_t *sid, int32_t n, double *ekin, double *mom);
+
+/* colloid_in_nearest_image.F90 (disk / sphere): image of colloid sid's
+ * centre nearest to xp (d_max = radius + dout), Lees-Edwards images first
+ * with the shear length at time t; xim, rx = xp - xim and the image's
+ * velocity vim, 3 components each. */
+int orc_colloid_nearest(const orc_params *p, double t, const double *xp, int32_t sid,
+                        double *xim, double *rx, double *vim);
+
+/* colloid_noslip.F90: velocity given to boundary particle xc (colloid
+ * sid) for its pair with fluid particle xf moving at vf; shear length of
+ * the Lees-Edwards images at p->le_time. */
+int orc_colloid_noslip(const orc_params *p, const double *xf, const double *xc,
+                       const double *vf, int32_t sid, double *vc);
+
+/* particles_collect_colloid_interaction.F90:93-136: drag (sum of m f) and
+ * torque (sum of r x m f) of every colloid over its boundary particles,
+ * into p->coll->drag / torque, with r from the colloid block's shear
+ * length time. */
+int orc_colloid_collect(const orc_params *p, const double *x, const double *f,
+                        const double *m, const int32_t *sid, int32_t n);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,28 @@
+"""Colloid boxes for the oracle and GPU colloid tests."""
+import numpy as np
+
+from mcf_amd import workloads as W
+
+
+def to_oracle(O, cs, step_start=0):
+    """The oracle's record of a ColloidSet (its own copy of the state)."""
+    return O.Colloids(cs.radius, cs.m, cs.mmi, cs.x, cs.v, cs.omega, noslip=cs.noslip,
+                      rho_type=cs.rho_type, translate=cs.translate, rotate=cs.rotate, ab=cs.ab,
+                      sub_steps=cs.sub_steps, body_force_type=cs.body_force_type,
+                      body_force=cs.body_force, dout=cs.dout, step_start=step_start)
+
+
+def two_spheres(n_side=16, radius=4.0, kt=1e-3, sym=True, rho_type=0, seed=1, **kw):
+    """Two spheres in a jittered Tait box (dx 1, cut-off 3), one of them
+    straddling the periodic faces."""
+    box = W.tait_box_3d(n_side, kt=kt, symmetry=sym, seed=seed)
+    L = float(n_side)
+    centres = [[0.3 * L, 0.35 * L, 0.4 * L], [0.85 * L, 0.9 * L, 0.97 * L]]
+    return W.with_colloids(box, centres, radius, dx=1.0, rho_type=rho_type, **kw)
+
+
+def two_disks(n_side=24, radius=4.0, kt=1e-4, sym=True, **kw):
+    box = W.shear_wave_2d(n_side, brownian=kt > 0, kt=kt, symmetry=sym)
+    dx = 1.0 / n_side
+    centres = [[0.3, 0.35], [0.8, 0.95]]
+    return W.with_colloids(box, centres, radius * dx, dx=dx, **kw)

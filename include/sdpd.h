@@ -70,7 +70,9 @@ int sdpd_create(const sdpd_params *params, sdpd_ctx **ctx);
 
 /* Replaces particles_init_global_exter / io_read_particles hand-over
  * (io_read_particles.F90:162-240).  x, v: (num_dim, n); m: n;
- * id: (2, n) = (pid, sid).  Only fluid particles (sid == 0). */
+ * id: (2, n) = (pid, sid): sid 0 for fluid particles, 1..num for the
+ * boundary particles of colloid sid when sdpd_set_colloids has been
+ * called (mcf_header.F90:118-120). */
 int sdpd_upload(sdpd_ctx *ctx, const double *x, const double *v,
                 const double *m, const int32_t *id, int32_t n);
 
@@ -111,7 +113,10 @@ int sdpd_eos(sdpd_ctx *ctx);
 /* Replaces particles_compute_interaction (particles_compute_interaction.F90)
  * plus particles_apply_body_force (particles_apply_body_force.F90:96-405)
  * and, when non-Newtonian, particles_compute_act
- * (particles_compute_act.F90:112-156).  Noise uses the context step. */
+ * (particles_compute_act.F90:112-156).  Noise uses the context step.
+ * With colloids, the first call after sdpd_upload also collects their
+ * drag and torque and starts their accelerations (marching_marching.F90:
+ * 1003-1080). */
 int sdpd_forces(sdpd_ctx *ctx);
 
 /* Replaces particles_integrate_velocity (particles_integrate_velocity.F90:84-94):
@@ -120,7 +125,11 @@ int sdpd_kick(sdpd_ctx *ctx, double lambda);
 
 /* Replaces particles_integrate_position (order 1) followed by
  * particles_adjust_particles (particles_integrate_position.F90:82-123,
- * particles_adjust_particles.F90:154-161,272-279): x += v*dt, periodic wrap. */
+ * particles_adjust_particles.F90:154-161,272-279): x += v*dt for fluid
+ * particles, periodic wrap.  With colloids the colloid block of the step
+ * runs in between (marching_integrate_VV.F90:455-765): drag and torque
+ * from the last forces, rigid-body update, boundary particles carried
+ * along. */
 int sdpd_drift_wrap(sdpd_ctx *ctx);
 
 /* Replaces particles_integrate_ct (particles_integrate_ct.F90:66-68):
@@ -128,7 +137,8 @@ int sdpd_drift_wrap(sdpd_ctx *ctx);
 int sdpd_conformation(sdpd_ctx *ctx, double lambda);
 
 /* Replaces statistic_compute_statistic (statistic_compute_statistic.F90:230-283):
- * kinetic energy and momentum of fluid particles, FP64 compensated sums. */
+ * kinetic energy and momentum of fluid particles, FP64 compensated sums,
+ * plus the colloids' translational share (colloid_compute_statistic.F90). */
 int sdpd_stats(sdpd_ctx *ctx, double *mom /*[3]*/, double *ekin);
 
 /* statistic_compute_statistic (statistic_compute_statistic.F90:230-283)
@@ -163,6 +173,41 @@ int sdpd_step(sdpd_ctx *ctx, int32_t nsteps);
  * with min_phys = 0. */
 int sdpd_set_lees_edwards(sdpd_ctx *ctx, int32_t le_dim, const double *v_lower,
                           const double *v_upper);
+
+/* Rigid colloids made of boundary particles: disks (2D) and spheres (3D)
+ * (Class_Colloid.F90; physics_config.mcf coll_* keys).  Fields follow the
+ * reference's keys; the per-colloid arrays of sdpd_set_colloids follow
+ * Class_Colloid's radius, m, mmi, x, v(:, :, 1), omega(:, :, 1). */
+typedef struct sdpd_colloid_params {
+  int32_t noslip;          /* coll_noslip: 1 frozen, 2 Morris                */
+  int32_t rho_type;        /* coll_rho_type: 0 constant, 1 dynamic           */
+  int32_t translate;       /* coll_translate                                 */
+  int32_t rotate;          /* coll_rotate                                    */
+  int32_t integrate_ab;    /* integrate_colloid_AB: 1..5                     */
+  int32_t sub_steps;       /* coll_sub_time_step: 1..8                       */
+  int32_t body_force_type; /* coll_body_force_type: 0 none, 1 constant       */
+  int32_t pad;
+  double body_force[3];    /* coll_body_force (per unit mass)                */
+  double dout;             /* outer layer: 0.05 dx(1) in the reference       */
+} sdpd_colloid_params;
+
+/* Declares num colloids before sdpd_upload (physics_new colloid set-up,
+ * colloid_new.F90).  radius, m: (num); mmi, x, v, omega: (3, num) (the
+ * moment of inertia about the principal axes, centre, translational and
+ * angular velocity).  One box (no replicas), FP64, Newtonian flow; the
+ * boundary particles then come with sdpd_upload (sid = colloid index).
+ * Colloid-colloid and colloid-wall forces are not modelled (the
+ * reference's pp_interact_cc / cw off).  num = 0 removes the colloids. */
+int sdpd_set_colloids(sdpd_ctx *ctx, const sdpd_colloid_params *prm, int32_t num,
+                      const double *radius, const double *m, const double *mmi,
+                      const double *x, const double *v, const double *omega);
+
+/* Current colloid state and the drag (sum of m f) and torque (sum of
+ * r x m f) of the last collection (particles_collect_colloid_interaction
+ * .F90:93-136 plus the colloid body force), each (3, num); any pointer
+ * may be NULL. */
+int sdpd_get_colloids(sdpd_ctx *ctx, double *x, double *v, double *omega, double *drag,
+                      double *torque);
 
 /* sdpd_step runs pairs of steps through a captured HIP graph (default on,
  * and off while phase timing is enabled); enable = 0 launches every kernel

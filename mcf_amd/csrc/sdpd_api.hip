@@ -48,6 +48,13 @@ struct sdpd_ctx {
   bool use_graphs = true;
   int parity = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // colloids: parameters and initial state until sdpd_upload puts them on
+  // the device; boundary particles of all colloids; whether the initial
+  // evaluation has started the colloids' acceleration histories
+  sdpd_colloid_params cprm{};
+  std::vector<double> c_radius, c_m, c_mmi, c_x, c_v, c_w;
+  int nmembers = 0;
+  bool coll_primed = false;
 };
 
 namespace {
@@ -71,15 +78,20 @@ int fail(sdpd_ctx* c, int code, const std::string& msg) {
   } while (0)
 
 // time_current after `step` marching steps, accumulated by repeated
-// addition of dt as marching_marching.F90:1393 does, and the time at which
+// addition of dt as marching_marching.F90:1393 does, the time at which
 // that step evaluates the Lees-Edwards shear length (time_current + dt,
 // marching_integrate_VV.F90:784-786; the initial evaluation uses
-// time_current, marching_marching.F90:437)
-void marching_time(int64_t step, double dt, double* t, double* t_le) {
-  double tt = 0.0;
-  for (int64_t s = 0; s < step; ++s) tt = tt + dt;
+// time_current, marching_marching.F90:437), and that time of the step
+// before (the shear length the step's colloid block still sees)
+void marching_time(int64_t step, double dt, double* t, double* t_le, double* t_prev = nullptr) {
+  double tt = 0.0, prev = 0.0;
+  for (int64_t s = 0; s < step; ++s) {
+    prev = tt;
+    tt = tt + dt;
+  }
   *t = tt;
   *t_le = step > 0 ? tt + dt : tt;
+  if (t_prev) *t_prev = step > 1 ? prev + dt : (step == 1 ? prev : *t_le);
 }
 
 int check_launch(sdpd_ctx* c, const char* what) {
@@ -187,6 +199,7 @@ int derive(const sdpd_params& p, DevParams& P, std::string& err) {
   P.inv_h = 1.0 / P.h;
   P.w0 = p.kernel_type == 1 ? P.coef * (243.0 + -192.0 + 15.0) : P.coef;
   P.c2 = p.c * p.c;
+  P.rho0 = p.rho;
   P.rho_ref = p.rho_ref;
   P.gamma = p.gamma;
   P.p0 = p.state_equation_type == 2
@@ -315,7 +328,9 @@ void free_state(DeviceState& S) {
                   S.rstart,   S.runsorted, S.rlpdx,  S.nlist,   S.nbr_sig_count,
                   S.row_nchunk, S.row_chunk_off, S.chunk_row, S.scan_tmp, S.rec64,
                   (void*)S.geo, S.rep,      S.rep_alt,  S.row_cell, S.row_len,  S.row_rep,
-                  S.rep_stats, S.qr2,     S.step_dev, S.time_dev};
+                  S.rep_stats, S.qr2,     S.step_dev, S.time_dev, S.sid,
+                  S.sid_alt,  S.slot_of,  (void*)S.coll_hot, (void*)S.coll, S.coll_start,
+                  S.coll_members};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   std::memset(&S, 0, sizeof(S));
@@ -383,7 +398,7 @@ int alloc_state(sdpd_ctx* c, int n) {
                          hipMemcpyHostToDevice));
     ALLOC(rep_stats, 4 * (size_t)P.nrep);
     ALLOC(step_dev, 1);
-    ALLOC(time_dev, 2);
+    ALLOC(time_dev, 3);
   }
   {
     const size_t nrows = (size_t)P.nrows;
@@ -404,6 +419,15 @@ int alloc_state(sdpd_ctx* c, int n) {
   ALLOC(nlist, (size_t)P.list_cap * 64 * (((N + 63) / 64 + (size_t)P.nrows + 7) / 8 * 8));
   ALLOC(recheck, N);
   ALLOC(nbr_hash, N);
+  if (P.ncoll > 0) {
+    ALLOC(sid, N);
+    ALLOC(sid_alt, N);
+    ALLOC(slot_of, N);
+    ALLOC(coll_hot, (size_t)P.ncoll);
+    ALLOC(coll, (size_t)P.ncoll);
+    ALLOC(coll_start, (size_t)P.ncoll + 1);
+    ALLOC(coll_members, N);
+  }
 #undef ALLOC
   return SDPD_OK;
 }
@@ -418,6 +442,7 @@ void swap_sorted(sdpd_ctx* c) {
   std::swap(S.perm, S.perm_alt);
   std::swap(S.rep, S.rep_alt);
   std::swap(S.ct, S.ct_alt);
+  std::swap(S.sid, S.sid_alt);
 }
 
 hipEvent_t take_event(sdpd_ctx* c) {
@@ -512,6 +537,21 @@ int run_forces(sdpd_ctx* c) {
   return SDPD_OK;
 }
 
+// After the initial evaluation: the colloids' drag and torque of the
+// initial forces start their acceleration histories (marching_marching.F90:
+// 1003-1080); the run's step_start is the current step.
+int prime_colloids(sdpd_ctx* c) {
+  if (!c->P.ncoll || c->coll_primed) return SDPD_OK;
+  drop_graphs(c);  // coll_step0 is a kernel argument
+  c->P.coll_step0 = c->step;
+  sdpd::launch_colloid_collect(c->stream, c->P, c->S, 1);
+  sdpd::launch_colloid_prologue(c->stream, c->P, c->S);
+  int rc = check_launch(c, "colloid prologue");
+  if (rc) return rc;
+  c->coll_primed = true;
+  return SDPD_OK;
+}
+
 // One velocity-Verlet step after the prologue: advance the marching step
 // (host and device copies), step boundary (closing kick of the previous
 // step if asked, opening kick, drift, ct half update, wrap, cell keys),
@@ -522,6 +562,9 @@ int step_body(sdpd_ctx* c, int closing) {
   sdpd::launch_advance_step(c->stream, c->P, c->S);
   {
     PhaseMark pm(c, 0);
+    // the colloid block moves only boundary particles, the drift only fluid
+    // ones: the block may run first
+    if (c->P.ncoll) sdpd::launch_colloid_block(c->stream, c->P, c->S, c->nmembers);
     sdpd::launch_boundary(c->stream, c->P, c->S, closing, 1);
   }
   int rc = check_launch(c, "step boundary");
@@ -630,6 +673,58 @@ const char* sdpd_last_error(const sdpd_ctx* c) { return c ? c->err.c_str() : "nu
 
 namespace {
 
+// Species ids, the boundary particles of each colloid (upload indices in
+// upload order) and the colloids' initial state (histories zero, no
+// accumulated rotation) onto the device.
+int upload_colloids(sdpd_ctx* c, const int32_t* id, int32_t n) {
+  const int ncoll = c->P.ncoll;
+  std::vector<int32_t> sid(n), start(ncoll + 1, 0), slot(n);
+  for (int32_t i = 0; i < n; ++i) {
+    sid[i] = id[2 * (size_t)i + 1];
+    slot[i] = i;
+    if (sid[i] > 0) ++start[sid[i]];
+  }
+  for (int k = 0; k < ncoll; ++k) start[k + 1] += start[k];
+  std::vector<int32_t> members(std::max(start[ncoll], 1)), fill(start.begin(), start.end() - 1);
+  for (int32_t i = 0; i < n; ++i)
+    if (sid[i] > 0) members[fill[sid[i] - 1]++] = i;
+  c->nmembers = start[ncoll];
+  std::vector<sdpd::CollState> cs(ncoll);
+  std::vector<sdpd::CollHot> hot(ncoll);
+  for (int k = 0; k < ncoll; ++k) {
+    sdpd::CollState& a = cs[k];
+    std::memset(&a, 0, sizeof(a));
+    for (int d = 0; d < 3; ++d) {
+      a.x[d] = c->c_x[3 * (size_t)k + d];
+      a.v[0][d] = c->c_v[3 * (size_t)k + d];
+      a.w[0][d] = c->c_w[3 * (size_t)k + d];
+      a.mmi[d] = c->c_mmi[3 * (size_t)k + d];
+      a.acc[4 * d] = 1.0;
+    }
+    a.radius = c->c_radius[k];
+    a.m = c->c_m[k];
+    sdpd::CollHot& h = hot[k];
+    std::memset(&h, 0, sizeof(h));
+    for (int d = 0; d < 3; ++d) {
+      h.x[d] = a.x[d];
+      h.v[d] = a.v[0][d];
+      h.w[d] = a.w[0][d];
+    }
+    h.radius = a.radius;
+  }
+  const DeviceState& S = c->S;
+  HIP_TRY(c, hipMemcpy(S.sid, sid.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(S.slot_of, slot.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(S.coll_start, start.data(), start.size() * sizeof(int32_t),
+                       hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(S.coll_members, members.data(), (size_t)c->nmembers * sizeof(int32_t),
+                       hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(S.coll, cs.data(), cs.size() * sizeof(sdpd::CollState), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(S.coll_hot, hot.data(), hot.size() * sizeof(sdpd::CollHot),
+                       hipMemcpyHostToDevice));
+  return SDPD_OK;
+}
+
 int upload_impl(sdpd_ctx* c, const double* x, const double* v, const double* m,
                 const int32_t* id, const int32_t* rep, int32_t n, const char* who) {
   const std::string w(who);
@@ -638,9 +733,13 @@ int upload_impl(sdpd_ctx* c, const double* x, const double* v, const double* m,
     return fail(c, SDPD_ERR_ARG, w + ": at most 2^25 - 1 particles per context");
   const int dim = c->P.dim;
   const int nrep = c->P.nrep;
+  const int ncoll = c->P.ncoll;
   for (int32_t i = 0; i < n; ++i) {
-    if (id[2 * (size_t)i + 1] != 0)
-      return fail(c, SDPD_ERR_ARG, w + ": only fluid particles (sid = 0) are supported");
+    const int32_t sid = id[2 * (size_t)i + 1];
+    if (sid < 0 || sid > ncoll)
+      return fail(c, SDPD_ERR_ARG,
+                  w + (ncoll ? ": species id must be 0 (fluid) or a colloid 1..num"
+                             : ": only fluid particles (sid = 0) without sdpd_set_colloids"));
     if (!(m[i] > kMachineZero))
       return fail(c, SDPD_ERR_PHYSICS, w + ": particle mass must be positive");
     const int q = rep ? rep[i] : 0;
@@ -695,14 +794,16 @@ int upload_impl(sdpd_ctx* c, const double* x, const double* v, const double* m,
   HIP_TRY(c, hipMemcpy(c->S.m, m, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->S.pid, pid.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->S.perm, perm.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
+  if (ncoll > 0 && (rc = upload_colloids(c, id, n))) return rc;
   if (rep)
     HIP_TRY(c, hipMemcpy(c->S.rep, rep, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
   else
     sdpd::launch_fill_i32(c->stream, c->S.rep, 0, n);
   {
     double t, tle;
-    marching_time(c->step, c->P.dt, &t, &tle);
-    sdpd::launch_set_step(c->stream, c->S, c->step, t, tle);
+    double tprev;
+    marching_time(c->step, c->P.dt, &t, &tle, &tprev);
+    sdpd::launch_set_step(c->stream, c->S, c->step, t, tle, tprev);
   }
   sdpd::launch_fill_f64(c->stream, c->S.f, 0.0, (int64_t)dim * n);
   sdpd::launch_fill_f64(c->stream, c->S.rho, 0.0, n);
@@ -723,6 +824,7 @@ int upload_impl(sdpd_ctx* c, const double* x, const double* v, const double* m,
   c->n = n;
   c->cells_valid = c->rho_valid = c->f_valid = false;
   c->sorted_once = false;
+  c->coll_primed = false;
   return SDPD_OK;
 }
 
@@ -744,6 +846,8 @@ int sdpd_set_replicas(sdpd_ctx* c, int32_t nrep, const double* box_lo, const dou
     return fail(c, SDPD_ERR_ARG, "sdpd_set_replicas: bad arguments");
   if (c->P.le_g >= 0)
     return fail(c, SDPD_ERR_STATE, "sdpd_set_replicas: Lees-Edwards faces are for one box only");
+  if (c->P.ncoll > 0)
+    return fail(c, SDPD_ERR_STATE, "sdpd_set_replicas: colloids are for one box only");
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   drop_graphs(c);
   free_state(c->S);
@@ -836,7 +940,8 @@ int sdpd_forces(sdpd_ctx* c) {
   if (rc) return rc;
   if (!c->cells_valid || !c->rho_valid)
     return fail(c, SDPD_ERR_STATE, "sdpd_forces: density is not current");
-  return run_forces(c);
+  if ((rc = run_forces(c))) return rc;
+  return prime_colloids(c);
 }
 
 int sdpd_kick(sdpd_ctx* c, double lambda) {
@@ -851,6 +956,11 @@ int sdpd_kick(sdpd_ctx* c, double lambda) {
 int sdpd_drift_wrap(sdpd_ctx* c) {
   int rc = need_particles(c);
   if (rc) return rc;
+  if (c->P.ncoll) {
+    if (!c->coll_primed)
+      return fail(c, SDPD_ERR_STATE, "sdpd_drift_wrap: evaluate the initial forces first (colloids)");
+    sdpd::launch_colloid_block(c->stream, c->P, c->S, c->nmembers);
+  }
   sdpd::launch_drift_wrap(c->stream, c->P, c->S);
   c->cells_valid = false;
   return check_launch(c, "drift_wrap");
@@ -873,14 +983,17 @@ int sdpd_stats(sdpd_ctx* c, double* mom, double* ekin) {
   {
     PhaseMark pm(c, 4);
     sdpd::launch_stats(c->stream, c->P, c->S, c->stats_nb);
+    if (c->P.ncoll) sdpd::launch_colloid_stats(c->stream, c->P, c->S);
   }
   rc = check_launch(c, "stats");
   if (rc) return rc;
-  double out[4];
-  HIP_TRY(c, hipMemcpyAsync(out, c->S.stats_out, sizeof(out), hipMemcpyDeviceToHost, c->stream));
+  double out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  HIP_TRY(c, hipMemcpyAsync(out, c->S.stats_out, (c->P.ncoll ? 8 : 4) * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  *ekin = out[0];
-  for (int d = 0; d < c->P.dim; ++d) mom[d] = out[1 + d];
+  // statistic_compute_statistic.F90:465-475 adds the colloids' share
+  *ekin = out[0] + out[4];
+  for (int d = 0; d < c->P.dim; ++d) mom[d] = out[1 + d] + out[5 + d];
   for (int d = c->P.dim; d < 3; ++d) mom[d] = 0.0;
   return SDPD_OK;
 }
@@ -923,6 +1036,7 @@ int sdpd_step(sdpd_ctx* c, int32_t nsteps) {
     if ((rc = run_density(c))) return rc;
     if ((rc = run_forces(c))) return rc;
   }
+  if ((rc = prime_colloids(c))) return rc;
   if (nsteps == 0) return SDPD_OK;
   // the first step opens without a closing kick (the previous call ended
   // with it); every later one starts with the closing kick of the one
@@ -977,6 +1091,95 @@ int sdpd_set_lees_edwards(sdpd_ctx* c, int32_t le_dim, const double* v_lower,
   c->P.le_vhi = v_upper[0];
   c->P.le_u = v_upper[0] - v_lower[0];
   c->P.le_uf = (float)c->P.le_u;
+  return SDPD_OK;
+}
+
+int sdpd_set_colloids(sdpd_ctx* c, const sdpd_colloid_params* prm, int32_t num,
+                      const double* radius, const double* m, const double* mmi, const double* x,
+                      const double* v, const double* omega) {
+  if (!c) return SDPD_ERR_ARG;
+  if (c->n > 0) return fail(c, SDPD_ERR_STATE, "sdpd_set_colloids: call before sdpd_upload");
+  if (num == 0) {
+    c->P.ncoll = 0;
+    return SDPD_OK;
+  }
+  if (num < 0 || !prm || !radius || !m || !mmi || !x || !v || !omega)
+    return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: bad arguments");
+  if (c->P.nrep != 1 || !c->rep_lo.empty())
+    return fail(c, SDPD_ERR_STATE, "sdpd_set_colloids: one box only (batched context)");
+  if (c->P.mixed || !c->P.newtonian)
+    return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: FP64 Newtonian flows only");
+  if (prm->noslip != 1 && prm->noslip != 2)
+    return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: noslip must be 1 (frozen) or 2 (Morris)");
+  if (prm->rho_type != 0 && prm->rho_type != 1)
+    return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: rho_type must be 0 or 1");
+  if (prm->integrate_ab < 1 || prm->integrate_ab > sdpd::kCollHist)
+    return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: integrate_ab must be 1..5");
+  if (prm->sub_steps < 1 || prm->sub_steps > sdpd::kCollSub)
+    return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: sub_steps must be 1..8");
+  if (prm->body_force_type != 0 && prm->body_force_type != 1)
+    return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: body_force_type must be 0 or 1");
+  if (!(prm->dout >= 0.0)) return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: dout < 0");
+  const int dim = c->P.dim;
+  for (int k = 0; k < num; ++k) {
+    if (!(radius[k] > 0.0) || !(m[k] > 0.0))
+      return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: radius and mass must be positive");
+    if (prm->rotate)
+      for (int d = (dim == 2 ? 2 : 0); d < 3; ++d)
+        if (!(mmi[3 * (size_t)k + d] > 0.0))
+          return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: moment of inertia must be positive");
+    for (int d = 0; d < dim; ++d) {
+      const double xv = x[3 * (size_t)k + d];
+      if (!(xv >= c->P.min_phys[d] && xv < c->P.min_phys[d] + c->P.length[d]))
+        return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: centre outside the box [min, max)");
+    }
+  }
+  c->cprm = *prm;
+  c->c_radius.assign(radius, radius + num);
+  c->c_m.assign(m, m + num);
+  c->c_mmi.assign(mmi, mmi + 3 * (size_t)num);
+  c->c_x.assign(x, x + 3 * (size_t)num);
+  c->c_v.assign(v, v + 3 * (size_t)num);
+  c->c_w.assign(omega, omega + 3 * (size_t)num);
+  if (dim == 2)
+    for (int k = 0; k < num; ++k) {
+      // disks move in the plane and spin about z
+      c->c_x[3 * (size_t)k + 2] = c->c_v[3 * (size_t)k + 2] = 0.0;
+      c->c_w[3 * (size_t)k] = c->c_w[3 * (size_t)k + 1] = 0.0;
+    }
+  DevParams& P = c->P;
+  P.ncoll = num;
+  P.coll_noslip = prm->noslip;
+  P.coll_rho_type = prm->rho_type;
+  P.coll_translate = prm->translate ? 1 : 0;
+  P.coll_rotate = prm->rotate ? 1 : 0;
+  P.coll_ab = prm->integrate_ab;
+  P.coll_sub = prm->sub_steps;
+  P.coll_bf_type = prm->body_force_type;
+  for (int d = 0; d < 3; ++d) P.coll_bf[d] = d < dim ? prm->body_force[d] : 0.0;
+  P.coll_dout = prm->dout;
+  return SDPD_OK;
+}
+
+int sdpd_get_colloids(sdpd_ctx* c, double* x, double* v, double* omega, double* drag,
+                      double* torque) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  const int num = c->P.ncoll;
+  if (num == 0) return fail(c, SDPD_ERR_STATE, "sdpd_get_colloids: no colloids");
+  std::vector<sdpd::CollState> cs(num);
+  HIP_TRY(c, hipMemcpyAsync(cs.data(), c->S.coll, cs.size() * sizeof(sdpd::CollState),
+                            hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < num; ++k)
+    for (int d = 0; d < 3; ++d) {
+      const size_t o = 3 * (size_t)k + d;
+      if (x) x[o] = cs[k].x[d];
+      if (v) v[o] = cs[k].v[0][d];
+      if (omega) omega[o] = cs[k].w[0][d];
+      if (drag) drag[o] = cs[k].drag[d];
+      if (torque) torque[o] = cs[k].torque[d];
+    }
   return SDPD_OK;
 }
 
@@ -1066,8 +1269,9 @@ int sdpd_set_step(sdpd_ctx* c, int64_t step) {
   c->step = step;
   if (c->S.step_dev) {
     double t, tle;
-    marching_time(step, c->P.dt, &t, &tle);
-    sdpd::launch_set_step(c->stream, c->S, step, t, tle);
+    double tprev;
+    marching_time(step, c->P.dt, &t, &tle, &tprev);
+    sdpd::launch_set_step(c->stream, c->S, step, t, tle, tprev);
     return check_launch(c, "sdpd_set_step");
   }
   return SDPD_OK;

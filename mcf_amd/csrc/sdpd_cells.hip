@@ -180,7 +180,9 @@ __global__ void k_place(DevParams P, const int32_t* __restrict__ key,
                         double* __restrict__ x2, double* __restrict__ v2, double* __restrict__ m2,
                         int32_t* __restrict__ pid2, int32_t* __restrict__ perm2,
                         int32_t* __restrict__ rep2, double* __restrict__ ct2,
-                        int32_t* __restrict__ key2, float4* __restrict__ xa) {
+                        int32_t* __restrict__ key2, float4* __restrict__ xa,
+                        const int32_t* __restrict__ sid, int32_t* __restrict__ sid2,
+                        int32_t* __restrict__ slot_of) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = P.n;
   if (i >= n) return;
@@ -197,6 +199,12 @@ __global__ void k_place(DevParams P, const int32_t* __restrict__ key,
   m2[p] = m[i];
   pid2[p] = pid[i];
   perm2[p] = perm[i];
+  if (sid) {
+    // colloids: species id moves with the particle; the slot of each upload
+    // index locates the boundary particles of a colloid
+    sid2[p] = sid[i];
+    slot_of[perm[i]] = p;
+  }
   const int q = rep[i];
   rep2[p] = q;
   key2[p] = k;
@@ -285,7 +293,8 @@ void launch_cell_sort_after_keys(hipStream_t s, const DevParams& P, const Device
   hipLaunchKernelGGL(k_place, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, S.key, S.start,
                      S.unsorted, S.x, S.v, S.m, S.pid, S.perm, S.rep,
                      P.newtonian ? nullptr : S.ct, S.geo, S.x_alt, S.v_alt, S.m_alt, S.pid_alt,
-                     S.perm_alt, S.rep_alt, P.newtonian ? nullptr : S.ct_alt, S.key_sorted, S.xa);
+                     S.perm_alt, S.rep_alt, P.newtonian ? nullptr : S.ct_alt, S.key_sorted, S.xa,
+                     P.ncoll ? S.sid : nullptr, S.sid_alt, S.slot_of);
   const int nrows = P.nrows;
   hipLaunchKernelGGL(k_row_chunks, dim3(blocks_for(nrows, 256)), dim3(256), 0, s, P, S.start,
                      S.row_cell, S.row_len, S.row_nchunk);

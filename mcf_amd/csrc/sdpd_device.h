@@ -61,6 +61,49 @@ struct DevParams {
   // FP32 copies of the constants (mixed precision)
   float inv_hf, coeff, coef_gradf;
   float etaf, eta_coeff, ktf, inv_sqrt_dtf, sqrt_2kt_etaf;
+  // colloids: boundary particles carry sid 1..ncoll (0: fluid only); the
+  // reference's coll_noslip, coll_rho_type, coll_translate, coll_rotate,
+  // integrate_colloid_AB, coll_sub_time_step, coll_body_force(_type)
+  int32_t ncoll;
+  int32_t coll_noslip;    // 1 frozen, 2 Morris
+  int32_t coll_rho_type;  // 0 constant, 1 dynamic
+  int32_t coll_translate, coll_rotate, coll_ab, coll_sub, coll_bf_type;
+  double coll_dout;       // outer layer of a colloid (0.05 dx)
+  double coll_bf[3];
+  double rho0;            // initial density (physics key rho)
+  int64_t coll_step0;     // step of the initial evaluation (the run's step_start)
+};
+
+// Colloid state read by the pair kernels (disks in 2D, spheres in 3D):
+// centre, translational and angular velocity (newest), radius.
+struct CollHot {
+  double x[3];
+  double v[3];
+  double w[3];
+  double radius;
+  double pad[2];
+};
+
+// Full rigid-body state of a colloid.  Histories hold the Adams-Bashforth
+// levels, newest first (the reference's third index).  xs / rot describe
+// the last colloid block: the centre before each sub-step and after the
+// last, and each sub-step's rotation (column-major 3 x 3).
+constexpr int kCollHist = 5;
+constexpr int kCollSub = 8;
+struct CollState {
+  double x[3];
+  double v[kCollHist][3];
+  double w[kCollHist][3];
+  double f[kCollHist][3];
+  double al[kCollHist][3];
+  double acc[9];
+  double theta[3];
+  double drag[3];
+  double torque[3];
+  double radius, m;
+  double mmi[3];
+  double xs[kCollSub + 1][3];
+  double rot[kCollSub][9];
 };
 
 // Geometry of one periodic box of a context (a context holds one box or a
@@ -309,6 +352,113 @@ __device__ __forceinline__ int wrap_cell(int c, int nm, int symmetry, int& mode)
   }
   mode = 0;
   return c;
+}
+
+// colloid_in_nearest_image.F90 (disks, spheres): the image of the centre of
+// colloid C nearest to xp, within d_max = radius + dout per dimension,
+// Lees-Edwards faces first (shear length at time t, the centre's image
+// velocity takes the face velocity jump), then periodic faces.  rx = xp -
+// image, vim = the image's velocity; 3 components each.
+template <int DIM>
+__device__ __forceinline__ void coll_nearest(const DevParams& P, const CollHot& C,
+                                             const double* xp, double t, double* rx,
+                                             double* vim) {
+  // written without run-time indices into arrays, which would put the
+  // kernel's parameters on the stack
+  double c0 = C.x[0], c1 = C.x[1], c2 = C.x[2];
+  const double dmax = C.radius + P.coll_dout;
+  vim[0] = C.v[0];
+  vim[1] = C.v[1];
+  vim[2] = C.v[2];
+  if (P.le_g >= 0) {
+    const bool gy = P.le_g == 1;
+    const double rg = gy ? xp[1] - c1 : (DIM == 3 ? xp[2] - c2 : 0.0);
+    const double Lg = gy ? P.length[1] : P.length[2];
+    // the other coordinates are taken MODULO the box as well (a plain wrap
+    // for the one the faces do not shear)
+    if (rg > dmax) {
+      if (gy) c1 = c1 + Lg; else c2 = c2 + Lg;
+      c0 = f_modulo(c0 + P.le_u * t, P.length[0]);
+      if (DIM == 3) {
+        if (gy) c2 = f_modulo(c2, P.length[2]); else c1 = f_modulo(c1, P.length[1]);
+      }
+      vim[0] = vim[0] + P.le_u;
+    } else if (-rg > dmax) {
+      if (gy) c1 = c1 - Lg; else c2 = c2 - Lg;
+      c0 = f_modulo(c0 + (P.le_vlo - P.le_vhi) * t, P.length[0]);
+      if (DIM == 3) {
+        if (gy) c2 = f_modulo(c2, P.length[2]); else c1 = f_modulo(c1, P.length[1]);
+      }
+      vim[0] = vim[0] + (P.le_vlo - P.le_vhi);
+    }
+  }
+  double r0 = xp[0] - c0, r1 = xp[1] - c1, r2 = DIM == 3 ? xp[2] - c2 : 0.0;
+  if (r0 > dmax) c0 = c0 + P.length[0]; else if (-r0 > dmax) c0 = c0 - P.length[0];
+  if (r1 > dmax) c1 = c1 + P.length[1]; else if (-r1 > dmax) c1 = c1 - P.length[1];
+  if (DIM == 3) {
+    if (r2 > dmax) c2 = c2 + P.length[2]; else if (-r2 > dmax) c2 = c2 - P.length[2];
+  }
+  rx[0] = xp[0] - c0;
+  rx[1] = xp[1] - c1;
+  rx[2] = DIM == 3 ? xp[2] - c2 : 0.0;
+}
+
+__device__ __forceinline__ void cross3(const double* a, const double* b, double* c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// colloid_noslip.F90: velocity a boundary particle of colloid C shows to a
+// fluid particle moving at vf.  r_xc = boundary particle - nearest centre
+// image (moving at vcoll), r_xf = fluid particle - that image.  Morris
+// (colloid_noslip_Morris_sphere.F90:129-300): the fluid velocity relative
+// to the surface point r_s = radius r_xf / |r_xf| is mirrored with the
+// ratio of the distances to the tangent plane, clamped to +-0.5; a fluid
+// particle inside the colloid and the frozen rule take the rigid velocity
+// at the boundary particle.
+template <int DIM>
+__device__ __forceinline__ void coll_noslip(const DevParams& P, const CollHot& C,
+                                            const double* r_xc, const double* vcoll,
+                                            const double* r_xf, const double* vf,
+                                            double* vc) {
+  const double R = C.radius;
+  double s2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) s2 = s2 + r_xf[d] * r_xf[d];
+  const double d_fcoll = sqrt(s2);
+  if (P.coll_noslip == 1 || d_fcoll <= R) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) vc[d] = vcoll[d];
+    if (P.coll_rotate) {
+      double r_vc[3];
+      cross3(C.w, r_xc, r_vc);
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) vc[d] = vc[d] + r_vc[d];
+    }
+    return;
+  }
+  double d_cn = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) d_cn = d_cn + r_xc[d] * (r_xf[d] / d_fcoll);
+  const double d_cs = R - d_cn;
+  const double d_fs = d_fcoll - R;
+  double corr;
+  if (fabs(d_fs) < 1e-14) {
+    corr = 0.5;
+  } else {
+    corr = d_cs / d_fs;
+    corr = corr > 0.5 ? 0.5 : (corr < -0.5 ? -0.5 : corr);
+  }
+  double r_vs[3] = {0.0, 0.0, 0.0};
+  if (P.coll_rotate) {
+    double r_xs[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) r_xs[d] = R / d_fcoll * r_xf[d];
+    cross3(C.w, r_xs, r_vs);
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) vc[d] = -corr * (vf[d] - vcoll[d] - r_vs[d]) + vcoll[d] + r_vs[d];
 }
 
 __device__ __forceinline__ double image_diff(double xi, double xj, double L, int mode) {

@@ -50,7 +50,18 @@ struct DeviceState {
   int32_t *row_cell, *row_len, *row_rep;
   double* rep_stats;     // per replica (E_k, P_x, P_y, P_z)
   int64_t* step_dev;     // marching step read by the force pass (noise key)
-  double* time_dev;      // [0] time_current, [1] time of the Lees-Edwards shear length
+  // [0] time_current, [1] time of the Lees-Edwards shear length of this
+  // step, [2] that of the previous step (read by the colloid block, which
+  // runs before the step's boundary update)
+  double* time_dev;
+  // colloids: species id of each particle (cell order, with its ping-pong
+  // copy), slot of each upload index, colloid records, and the boundary
+  // particles of colloid k as upload indices coll_members[coll_start[k] ..
+  // coll_start[k + 1])
+  int32_t *sid, *sid_alt, *slot_of;
+  CollHot* coll_hot;
+  CollState* coll;
+  int32_t *coll_start, *coll_members;
   // reference-grid cell list (built on request by sdpd_get_cells)
   int32_t *rkey, *rslot, *rcount, *rstart, *runsorted, *rlpdx;
 };
@@ -84,8 +95,19 @@ void launch_replica_stats(hipStream_t s, const DevParams& P, const DeviceState& 
 // the device copy of the marching step: set, or advance by one (a
 // captured step graph advances it itself)
 void launch_set_step(hipStream_t s, const DeviceState& S, int64_t step, double time,
-                     double le_time);
+                     double le_time, double le_prev);
 void launch_advance_step(hipStream_t s, const DevParams& P, const DeviceState& S);
+// colloids (single box): drag and torque of every colloid from its
+// boundary particles' last forces, with relative positions at the shear
+// length of time_dev[which]; the prologue's start of the acceleration
+// histories; the colloid block of one step (rigid-body update of the
+// colloids, then the boundary particles carried along)
+void launch_colloid_collect(hipStream_t s, const DevParams& P, const DeviceState& S, int which);
+void launch_colloid_prologue(hipStream_t s, const DevParams& P, const DeviceState& S);
+void launch_colloid_block(hipStream_t s, const DevParams& P, const DeviceState& S,
+                          int nmembers);
+void launch_colloid_stats(hipStream_t s, const DevParams& P, const DeviceState& S);
+
 void launch_pair_normals(hipStream_t s, const uint32_t* lo, const uint32_t* hi, int count,
                          uint64_t seed, int64_t step, float* g);
 

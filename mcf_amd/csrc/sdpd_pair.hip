@@ -635,6 +635,11 @@ __global__ __launch_bounds__(kWave) void k_density(DevParams P, DeviceState S) {
   }
   if (!h.active) return;
   const int i = h.i;
+  // colloid boundary particles of constant density take the initial
+  // density rho (number density rho / m) instead of the sum
+  // (particles_compute_density.F90:442-473, 526-556)
+  if (P.ncoll && P.coll_rho_type == 0 && S.sid[i] != 0)
+    rho = P.density_type == 1 ? P.rho0 : P.rho0 / mi;
   const double press = eos_pressure(P, P.density_type == 2 ? rho * mi : rho);
   S.rho[i] = rho;
   S.p[i] = press;
@@ -702,7 +707,7 @@ __device__ __forceinline__ void noise_vector(const DevParams& P, int64_t step, u
 // force types: 3 Hu-Adams, 4 Hu-Adams angular, 5 Oldroyd-B (non-Newtonian
 // Hu-Adams).  The bodies sum m_i times the acceleration; finish_force
 // divides by m_i once.
-template <int DIM, int FT>
+template <int DIM, int FT, bool COLL = false>
 struct ForceF64 {
   const DevParams& P;
   const DeviceState& S;
@@ -713,7 +718,22 @@ struct ForceF64 {
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
   double L[9];
   double pti[9];
+  // colloids: own species id; a boundary particle's position relative to
+  // the nearest image of its colloid's centre and that image's velocity;
+  // time of the pass's shear length
+  int sidi = 0;
+  double rxc_i[3] = {0.0, 0.0, 0.0}, vcoll_i[3] = {0.0, 0.0, 0.0};
+  double le_t = 0.0;
   __device__ __forceinline__ void body(int j, const Offsets& o) {
+    if (COLL) {
+      const int sj = S.sid[j];
+      if ((sidi | sj) != 0) {
+        // two boundary particles do not interact (pp_interaction.inc: same
+        // colloid CYCLE, colloid-colloid pair forces off)
+        if (sidi == 0 || sj == 0) colloid_pair(j, o, sj);
+        return;
+      }
+    }
     const int64_t n = P.n;
     const double4* rj = reinterpret_cast<const double4*>(S.rec64 + 8 * (int64_t)j);
     const double4 ra = rj[0], rb = rj[1];
@@ -780,6 +800,92 @@ struct ForceF64 {
       if (P.noise) {
         // f_r W e with W e = s (ca g + cb (g.e) e), e = d / r, s = +1 when
         // pid_i < pid_j (noise_vector), written on d to save operations
+        const uint32_t pidj = (uint32_t)S.pid[j];
+        float g[4];
+        pair_normals(P.seed, pidi < pidj ? pidi : pidj, pidi < pidj ? pidj : pidi, step, g);
+        const double fr = sqrt_nr(-4.0 * P.kt * fd) * P.inv_sqrt_dt;
+        const double sfr = pidi < pidj ? fr : -fr;
+        const double ca = 0.70710678118654752440;
+        const double g0 = (double)g[0], g1 = (double)g[1];
+        if (DIM == 3) {
+          const double cb = 0.81649658092772603273 - 0.70710678118654752440;
+          const double g2 = (double)g[2];
+          const double k = cb * fma(g2, d2, fma(g1, d1, g0 * d0)) * (ir * ir);
+          a0 = fma(sfr, fma(k, d0, ca * g0), a0);
+          a1 = fma(sfr, fma(k, d1, ca * g1), a1);
+          a2 = fma(sfr, fma(k, d2, ca * g2), a2);
+        } else {
+          a0 = fma(sfr, ca * g0, a0);
+          a1 = fma(sfr, ca * g1, a1);
+        }
+      }
+    }
+    f0 = f0 + a0;
+    f1 = f1 + a1;
+    if (DIM == 3) f2 = f2 + a2;
+  }
+
+  // A fluid particle and a colloid boundary particle
+  // (pp_interaction_fc.inc, pp_interaction_cf.inc): the boundary particle
+  // moves at the no-slip velocity seen from the fluid particle and carries
+  // the fluid particle's pressure; its own number density stays.  The
+  // no-slip velocity is formed in the boundary particle's frame, where the
+  // fluid particle sits at r_xc + (x_f - x_c) and moves at its image
+  // velocity, so both sides of the pair see the same relative velocity.
+  __device__ __forceinline__ void colloid_pair(int j, const Offsets& o, int sj) {
+    const double4* rj = reinterpret_cast<const double4*>(S.rec64 + 8 * (int64_t)j);
+    const double4 ra = rj[0], rb = rj[1];
+    double d0, d1, d2;
+    const double r2 = pair_d2<DIM>(P, h, o, ra.x, ra.y, ra.z, d0, d1, d2);
+    const double ir = rsqrt_nr(r2);
+    const double r = r2 * ir;
+    double w, gw;
+    kernel_eval(P, r * P.inv_h, w, gw);
+    const double jump = le_dv(P, h, o);  // x velocity of j's image minus j's own
+    const double bj = rb.w;
+    double u[3], vc[3], pair_p;
+    if (sidi == 0) {
+      const CollHot C = S.coll_hot[sj - 1];
+      const double xj[3] = {ra.x, ra.y, DIM == 3 ? ra.z : 0.0};
+      double rxc[3], vcoll[3];
+      coll_nearest<DIM>(P, C, xj, le_t, rxc, vcoll);
+      const double rxf[3] = {rxc[0] + d0, rxc[1] + d1, DIM == 3 ? rxc[2] + d2 : 0.0};
+      const double vf[3] = {vi0 - jump, vi1, DIM == 3 ? vi2 : 0.0};
+      coll_noslip<DIM>(P, C, rxc, vcoll, rxf, vf, vc);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) u[a] = a < DIM ? vf[a] - vc[a] : 0.0;
+      pair_p = pi;
+    } else {
+      const CollHot C = S.coll_hot[sidi - 1];
+      const double rxf[3] = {rxc_i[0] - d0, rxc_i[1] - d1, DIM == 3 ? rxc_i[2] - d2 : 0.0};
+      const double vf[3] = {rb.x + jump, rb.y, DIM == 3 ? rb.z : 0.0};
+      coll_noslip<DIM>(P, C, rxc_i, vcoll_i, rxf, vf, vc);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) u[a] = a < DIM ? vc[a] - vf[a] : 0.0;
+      pair_p = S.p[j];
+    }
+    double a0, a1, a2;
+    if (FT == 4) {
+      const double du = DIM == 3 ? fma(d2, u[2], fma(d1, u[1], d0 * u[0])) : fma(d1, u[1], d0 * u[0]);
+      const double aij = (bi + bj) * gw;
+      double sc = P.eta_coef * P.eta * aij * du * (ir * ir) - aij * pair_p;
+      if (P.noise) {
+        const uint32_t pj_ = (uint32_t)S.pid[j];
+        float g[4];
+        pair_normals(P.seed, pidi < pj_ ? pidi : pj_, pidi < pj_ ? pj_ : pidi, step, g);
+        sc = sc + (double)g[0] * P.sqrt_2kt_eta * sqrt_nr(-P.eta_coef * aij * ir) * P.inv_sqrt_dt;
+      }
+      sc = sc * ir;
+      a0 = sc * d0;
+      a1 = sc * d1;
+      a2 = sc * d2;
+    } else {
+      const double fd = P.eta * (bi + bj) * gw * ir;
+      const double fcr = (sidi == 0 ? ai + pair_p * bj : pair_p * bi + ra.w) * gw * ir;
+      a0 = fd * u[0] - fcr * d0;
+      a1 = fd * u[1] - fcr * d1;
+      a2 = fd * u[2] - fcr * d2;
+      if (P.noise) {
         const uint32_t pidj = (uint32_t)S.pid[j];
         float g[4];
         pair_normals(P.seed, pidi < pidj ? pidi : pidj, pidi < pidj ? pidj : pidi, step, g);
@@ -911,9 +1017,11 @@ struct ForceMixed {
 
 template <int DIM>
 __device__ __forceinline__ void finish_force(const DevParams& P, const DeviceState& S, int i,
-                                             double f0, double f1, double f2) {
+                                             double f0, double f1, double f2,
+                                             bool fluid = true) {
   const int64_t n = P.n;
-  if (P.body_force) {
+  // fluid particles only (particles_apply_body_force.F90:173)
+  if (P.body_force && fluid) {
     // particles_apply_body_force.F90:125-193
     if (fabs(P.bf[0]) > 1e-14) f0 = f0 + P.bf[0];
     if (fabs(P.bf[1]) > 1e-14) f1 = f1 + P.bf[1];
@@ -984,7 +1092,7 @@ __device__ __forceinline__ void conformation_rate(const DevParams& P, const Devi
     }
 }
 
-template <int DIM, int FT, bool MIXED>
+template <int DIM, int FT, bool MIXED, bool COLL = false>
 __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
   __shared__ float4 tile[kTile];
   __shared__ uint32_t buf[kBuf * kWave];
@@ -1021,7 +1129,15 @@ __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
       conformation_rate<DIM>(P, S, h.i, Ld);
     }
   } else {
-    ForceF64<DIM, FT> pol{P, S, h};
+    ForceF64<DIM, FT, COLL> pol{P, S, h};
+    if (COLL) {
+      pol.sidi = S.sid[ii];
+      pol.le_t = S.time_dev[1];
+      if (pol.sidi != 0) {
+        const double xi[3] = {h.x0, h.x1, DIM == 3 ? h.x2 : 0.0};
+        coll_nearest<DIM>(P, S.coll_hot[pol.sidi - 1], xi, pol.le_t, pol.rxc_i, pol.vcoll_i);
+      }
+    }
     pol.vi0 = S.v[ii];
     pol.vi1 = S.v[n + ii];
     pol.vi2 = DIM == 3 ? S.v[2 * n + ii] : 0.0;
@@ -1047,7 +1163,8 @@ __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
     for_each_listed<DIM>(P, S, ch, h, pol, tile, buf);
     if (!h.active) return;
     const int i = h.i;
-    finish_force<DIM>(P, S, i, pol.f0 * inv_mi, pol.f1 * inv_mi, pol.f2 * inv_mi);
+    finish_force<DIM>(P, S, i, pol.f0 * inv_mi, pol.f1 * inv_mi, pol.f2 * inv_mi,
+                      !COLL || pol.sidi == 0);
     if (FT == 5) conformation_rate<DIM>(P, S, i, pol.L);
   }
 }
@@ -1150,6 +1267,12 @@ static void launch_forces_dim(hipStream_t s, const DevParams& P, const DeviceSta
       hipLaunchKernelGGL((k_forces<DIM, 4, true>), g, b, 0, s, P, S);
     else
       hipLaunchKernelGGL((k_forces<DIM, 5, true>), g, b, 0, s, P, S);
+  } else if (P.ncoll) {
+    // colloids: Newtonian FP64 only (checked by the API)
+    if (ft == 3)
+      hipLaunchKernelGGL((k_forces<DIM, 3, false, true>), g, b, 0, s, P, S);
+    else
+      hipLaunchKernelGGL((k_forces<DIM, 4, false, true>), g, b, 0, s, P, S);
   } else {
     if (ft == 3)
       hipLaunchKernelGGL((k_forces<DIM, 3, false>), g, b, 0, s, P, S);

@@ -12,18 +12,28 @@
 
 namespace sdpd {
 
+// Colloid boundary particles (sid > 0) are not integrated: they move with
+// their colloid (particles_integrate_velocity.F90:84-94 and
+// particles_integrate_position.F90:82-123 act on fluid particles only).
+__device__ __forceinline__ bool is_fluid(const DevParams& P, const DeviceState& S, int i) {
+  return P.ncoll == 0 || S.sid[i] == 0;
+}
+
 __global__ void k_kick(DevParams P, DeviceState S, double lambda) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
+  if (!is_fluid(P, S, i)) return;
   const int64_t n = P.n;
   for (int d = 0; d < P.dim; ++d)
     S.v[d * n + i] = S.v[d * n + i] + lambda * S.f[d * n + i] * P.dt;
 }
 
 // Periodic and Lees-Edwards faces after a drift (particles_adjust_particles
-// .F90:137-389): dimensions in order; crossing a Lees-Edwards face moves x
-// by MODULO(x - shear_length, L) (the box starts at 0 then), adds the face
-// velocity jump, and ends the particle's check.
+// .F90:137-389): dimensions in order; crossing a Lees-Edwards face moves
+// every other coordinate k by MODULO(x_k - shear_length_k, L_k) (the box
+// starts at 0 then; the shear length is zero but along x, so for the third
+// coordinate this is a plain periodic wrap), adds the face velocity jump,
+// and ends the particle's check.
 template <int DIM>
 __device__ __forceinline__ void adjust_particle(const DevParams& P, const RepGeom& g, double* xv,
                                                 double* vv, double le_time) {
@@ -35,6 +45,10 @@ __device__ __forceinline__ void adjust_particle(const DevParams& P, const RepGeo
         xv[d] = xv[d] + L;
         const double sl = (P.le_vlo - P.le_vhi) * le_time;  // shear_length(x, lower)
         xv[0] = f_modulo(xv[0] - sl, g.L[0]);
+        if (DIM == 3) {
+          const int k = d == 1 ? 2 : 1;
+          xv[k] = f_modulo(xv[k], g.L[k]);
+        }
         vv[0] = vv[0] + (P.le_vhi - P.le_vlo);
         return;
       }
@@ -45,6 +59,10 @@ __device__ __forceinline__ void adjust_particle(const DevParams& P, const RepGeo
         xv[d] = xv[d] - L;
         const double sl = (P.le_vhi - P.le_vlo) * le_time;  // shear_length(x, upper)
         xv[0] = f_modulo(xv[0] - sl, g.L[0]);
+        if (DIM == 3) {
+          const int k = d == 1 ? 2 : 1;
+          xv[k] = f_modulo(xv[k], g.L[k]);
+        }
         vv[0] = vv[0] + (P.le_vlo - P.le_vhi);
         return;
       }
@@ -59,9 +77,10 @@ __global__ void k_drift_wrap(DevParams P, DeviceState S) {
   const int64_t n = P.n;
   const RepGeom& g = S.geo[S.rep[i]];
   double xv[3] = {0.0, 0.0, 0.0}, vv[3] = {0.0, 0.0, 0.0};
+  const bool fluid = is_fluid(P, S, i);
   for (int d = 0; d < P.dim; ++d) {
     vv[d] = S.v[d * n + i];
-    xv[d] = S.x[d * n + i] + vv[d] * P.dt;
+    xv[d] = fluid ? S.x[d * n + i] + vv[d] * P.dt : S.x[d * n + i];
   }
   if (P.dim == 3)
     adjust_particle<3>(P, g, xv, vv, S.time_dev[1]);
@@ -94,14 +113,15 @@ __global__ __launch_bounds__(256) void k_boundary(DevParams P, DeviceState S, in
   const int dim = P.dim;
   const RepGeom& g = S.geo[S.rep[i]];
   double xv[3] = {0.0, 0.0, 0.0}, vw[3] = {0.0, 0.0, 0.0};
+  const bool fluid = is_fluid(P, S, i);
   for (int d = 0; d < dim; ++d) {
     double vv = S.v[d * n + i];
     double ff = S.f[d * n + i];
-    if (closing) vv = vv + 0.5 * ff * P.dt;
-    if (opening) vv = vv + 0.5 * ff * P.dt;
+    if (closing && fluid) vv = vv + 0.5 * ff * P.dt;
+    if (opening && fluid) vv = vv + 0.5 * ff * P.dt;
     vw[d] = vv;
     double x = S.x[d * n + i];
-    if (opening) x = x + vv * P.dt;
+    if (opening && fluid) x = x + vv * P.dt;
     xv[d] = x;
   }
   if (opening) {
@@ -150,6 +170,7 @@ __global__ __launch_bounds__(256) void k_stats_partial(DevParams P, DeviceState 
   const int dim = P.dim;
   KSum acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    if (!is_fluid(P, S, (int)i)) continue;  // statistic_compute_statistic.F90:230-246
     double mi = S.m[i];
     double v2 = 0.0;
     for (int d = 0; d < dim; ++d) {
@@ -207,11 +228,12 @@ __global__ __launch_bounds__(256) void k_replica_stats(DevParams P, DeviceState 
 }
 
 __global__ void k_set_step(int64_t* __restrict__ step_dev, double* __restrict__ time_dev,
-                           int64_t step, double time, double le_time) {
+                           int64_t step, double time, double le_time, double le_prev) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     step_dev[0] = step;
     time_dev[0] = time;
     time_dev[1] = le_time;
+    time_dev[2] = le_prev;
   }
 }
 
@@ -223,6 +245,7 @@ __global__ void k_advance_step(int64_t* __restrict__ step_dev, double* __restric
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     step_dev[0] = step_dev[0] + 1;
     const double t = time_dev[0] + dt;
+    time_dev[2] = time_dev[1];
     time_dev[0] = t;
     time_dev[1] = t + dt;
   }
@@ -273,9 +296,9 @@ void launch_replica_stats(hipStream_t s, const DevParams& P, const DeviceState& 
 }
 
 void launch_set_step(hipStream_t s, const DeviceState& S, int64_t step, double time,
-                     double le_time) {
+                     double le_time, double le_prev) {
   hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, S.step_dev, S.time_dev, step, time,
-                     le_time);
+                     le_time, le_prev);
 }
 
 void launch_advance_step(hipStream_t s, const DevParams& P, const DeviceState& S) {

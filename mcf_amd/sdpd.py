@@ -75,6 +75,22 @@ class CTuning(C.Structure):
     _fields_ = [("density_shape", C.c_int32), ("chunk_order", C.c_int32)]
 
 
+class CColloidParams(C.Structure):
+    """struct sdpd_colloid_params of include/sdpd.h."""
+    _fields_ = [
+        ("noslip", C.c_int32),
+        ("rho_type", C.c_int32),
+        ("translate", C.c_int32),
+        ("rotate", C.c_int32),
+        ("integrate_ab", C.c_int32),
+        ("sub_steps", C.c_int32),
+        ("body_force_type", C.c_int32),
+        ("pad", C.c_int32),
+        ("body_force", C.c_double * 3),
+        ("dout", C.c_double),
+    ]
+
+
 # (name, restype, argtypes) of every symbol include/sdpd.h declares
 _P = C.c_void_p
 _DP = C.POINTER(C.c_double)
@@ -101,6 +117,9 @@ SYMBOLS = [
     ("sdpd_download_tensors", C.c_int, [_P, _DP, _DP]),
     ("sdpd_step", C.c_int, [_P, C.c_int32]),
     ("sdpd_set_lees_edwards", C.c_int, [_P, C.c_int32, _DP, _DP]),
+    ("sdpd_set_colloids", C.c_int, [_P, C.POINTER(CColloidParams), C.c_int32, _DP, _DP, _DP, _DP,
+                                    _DP, _DP]),
+    ("sdpd_get_colloids", C.c_int, [_P, _DP, _DP, _DP, _DP, _DP]),
     ("sdpd_set_graphs", C.c_int, [_P, C.c_int32]),
     ("sdpd_set_tuning", C.c_int, [_P, C.POINTER(CTuning)]),
     ("sdpd_get_tuning", C.c_int, [_P, C.POINTER(CTuning)]),
@@ -191,6 +210,7 @@ class SDPD:
             raise SDPDError(f"sdpd_create failed (stat_info={rc}); see stderr")
         self.h = h
         self.n = 0
+        self.ncoll = 0
         if getattr(prm, "le_dim", 0):
             self.set_lees_edwards(prm.le_dim, prm.le_v_lower, prm.le_v_upper)
 
@@ -327,6 +347,33 @@ class SDPD:
         hi = np.ascontiguousarray(list(v_upper) + [0.0] * (3 - len(v_upper)), np.float64)
         self._check(self.lib.sdpd_set_lees_edwards(self.h, int(le_dim), _dp(lo), _dp(hi)),
                     "sdpd_set_lees_edwards")
+
+    def set_colloids(self, cs) -> None:
+        """Rigid colloids (a mcf_amd.workloads.ColloidSet: radius, m, mmi,
+        x, v, omega and the coll_* parameters); call before upload, whose
+        sid then names each boundary particle's colloid (1..num)."""
+        k = cs.num
+        cp = CColloidParams()
+        cp.noslip, cp.rho_type = int(cs.noslip), int(cs.rho_type)
+        cp.translate, cp.rotate = int(bool(cs.translate)), int(bool(cs.rotate))
+        cp.integrate_ab, cp.sub_steps = int(cs.ab), int(cs.sub_steps)
+        cp.body_force_type = int(cs.body_force_type)
+        for d in range(3):
+            cp.body_force[d] = float(cs.body_force[d])
+        cp.dout = float(cs.dout)
+        arr = lambda a, w: np.ascontiguousarray(np.asarray(a, np.float64).reshape(k, w))  # noqa: E731
+        radius, m = arr(cs.radius, 1), arr(cs.m, 1)
+        mmi, x, v, w = arr(cs.mmi, 3), arr(cs.x, 3), arr(cs.v, 3), arr(cs.omega, 3)
+        self._check(self.lib.sdpd_set_colloids(self.h, C.byref(cp), k, _dp(radius), _dp(m), _dp(mmi),
+                                               _dp(x), _dp(v), _dp(w)), "sdpd_set_colloids")
+        self.ncoll = k
+
+    def colloids(self):
+        """(x, v, omega, drag, torque) of the colloids, each (num, 3)."""
+        k = self.ncoll
+        out = [np.zeros((k, 3)) for _ in range(5)]
+        self._check(self.lib.sdpd_get_colloids(self.h, *[_dp(a) for a in out]), "sdpd_get_colloids")
+        return tuple(out)
 
     def set_graphs(self, enable: bool) -> None:
         """Run sdpd_step's steps through captured HIP graphs (default) or

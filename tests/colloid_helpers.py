@@ -4,6 +4,15 @@ import numpy as np
 from mcf_amd import workloads as W
 
 
+def marching_time(step, dt):
+    """Shear-length time of step `step` (time_current + dt, time_current
+    accumulated by repeated addition; the initial evaluation: 0)."""
+    t = 0.0
+    for _ in range(step):
+        t = t + dt
+    return t + dt if step > 0 else t
+
+
 def to_oracle(O, cs, step_start=0):
     """The oracle's record of a ColloidSet (its own copy of the state)."""
     return O.Colloids(cs.radius, cs.m, cs.mmi, cs.x, cs.v, cs.omega, noslip=cs.noslip,

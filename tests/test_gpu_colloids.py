@@ -1,0 +1,202 @@
+"""Colloids on the GPU against the oracle: boundary-particle density, the
+fluid-boundary pair forces with the no-slip velocity, drag and torque, the
+rigid-body block over several steps (translation, rotation, sub-steps,
+Adams-Bashforth orders, Lees-Edwards faces), statistics, and the fused
+graph-replayed steps against the granular calls."""
+import numpy as np
+import pytest
+
+from mcf_amd import workloads as W
+from mcf_amd.sdpd import SDPD, SDPDError
+from tests.colloid_helpers import marching_time, to_oracle, two_disks, two_spheres
+from tests.gpu_helpers import rel_err, vec_err
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print(msg, flush=True)
+
+
+def gpu_context(box, cs):
+    g = SDPD(box.prm)
+    g.set_colloids(cs)
+    g.upload(box.x, box.v, box.m, box.pid, box.sid)
+    return g
+
+
+def le_variant(box, u=0.3):
+    box.prm = box.prm.replace(le_dim=2, le_v_lower=[-u, 0, 0], le_v_upper=[u, 0, 0])
+    return box
+
+
+CASES = [
+    ("sphere", True, 2, False),
+    ("sphere", False, 2, True),
+    ("sphere", True, 1, True),
+    ("disk", True, 2, True),
+]
+
+
+def make(kind, sym, noslip, rotate, **kw):
+    omega = [[0.0, 0.0, 0.02], [0.01, -0.02, 0.03]] if kind == "sphere" else [[0, 0, 0.3], [0, 0, -0.2]]
+    v = [[0.01, 0.0, -0.02], [0.0, 0.015, 0.0]]
+    maker = two_spheres if kind == "sphere" else two_disks
+    return maker(sym=sym, noslip=noslip, rotate=rotate, omega=omega, v=v, **kw)
+
+
+@pytest.mark.parametrize("kind,sym,noslip,rotate", CASES)
+@pytest.mark.parametrize("le", [False, True])
+def test_colloid_density_forces(oracle, kind, sym, noslip, rotate, le):
+    box, cs = make(kind, sym, noslip, rotate)
+    if le:
+        box = le_variant(box)
+    announce(f"colloid density/forces: {kind} sym={sym} noslip={noslip} rotate={rotate} le={le} n={box.n}")
+    step = 3
+    with gpu_context(box, cs) as g:
+        g.step = step
+        g.build_list()
+        g.compute_density()
+        g.compute_interaction()
+        rho, _ = g.download_density()
+        f = g.download_force()
+        _, _, _, drag, torque = g.colloids()
+    c = to_oracle(oracle, cs)
+    t = marching_time(step, box.prm.dt)
+    ro = oracle.density(box.prm, box.x, box.m, box.sid, coll=c, le_time=t)
+    po = oracle.pressure(box.prm, ro, box.m)
+    fo, _ = oracle.interaction(box.prm, step, box.x, box.v, ro, po, box.m, box.pid, box.sid,
+                               coll=c, le_time=t)
+    assert rel_err(rho, ro) < 1e-13
+    assert vec_err(f, fo) < 1e-12
+    # the first force pass starts the colloids: drag and torque of these forces
+    c.le_time = t
+    do, to = oracle.colloid_collect(box.prm, c, box.x, fo, box.m, box.sid)
+    scale = np.abs(do).max() + 1e-300
+    assert np.abs(drag - do).max() < 1e-11 * scale
+    assert np.abs(torque - to).max() < 1e-11 * (np.abs(to).max() + 1e-300)
+
+
+@pytest.mark.parametrize("rho_type", [0, 1])
+def test_colloid_density_types(oracle, rho_type):
+    box, cs = two_spheres(rho_type=rho_type)
+    announce(f"colloid density type {rho_type}")
+    with gpu_context(box, cs) as g:
+        g.build_list()
+        g.compute_density()
+        rho, _ = g.download_density()
+    ro = oracle.density(box.prm, box.x, box.m, box.sid, coll=to_oracle(oracle, cs))
+    assert rel_err(rho, ro) < 1e-13
+    b = box.sid > 0
+    if rho_type == 0:
+        assert np.array_equal(rho[b], box.prm.rho / box.m[b])
+
+
+TRAJ = [
+    # kind, translate, rotate, ab, sub, le, sym
+    ("sphere", True, True, 1, 1, False, True),
+    ("sphere", True, True, 2, 2, False, False),
+    ("sphere", True, True, 3, 1, True, True),
+    ("disk", True, True, 2, 1, False, True),
+    ("sphere", False, False, 1, 1, True, True),
+]
+
+
+def run_pair(oracle, box, cs, nsteps, fused=True):
+    with gpu_context(box, cs) as g:
+        if fused:
+            g.integrate_VV(nsteps)
+        else:
+            g.build_list()
+            g.compute_density()
+            g.compute_interaction()
+            for s in range(1, nsteps + 1):
+                g.step = s
+                g.integrate_velocity(0.5)
+                g.integrate_position_and_adjust()
+                g.build_list()
+                g.compute_density()
+                g.compute_interaction()
+                g.integrate_velocity(0.5)
+        x, v = g.download()
+        f = g.download_force()
+        col = g.colloids()
+        ek, mom = g.compute_statistic()
+    return x, v, f, col, ek, mom
+
+
+@pytest.mark.parametrize("kind,translate,rotate,ab,sub,le,sym", TRAJ)
+def test_colloid_trajectory(oracle, kind, translate, rotate, ab, sub, le, sym):
+    box, cs = make(kind, sym, 2, rotate, translate=translate, ab=ab, sub_steps=sub,
+                   body_force_type=1, body_force=[0.02, -0.01, 0.0])
+    if le:
+        box = le_variant(box)
+    # a larger time step moves the colloids measurably in a few steps (kept
+    # inside the viscous and sound-speed limits)
+    box.prm = box.prm.replace(dt=box.prm.dt * (20 if kind == "sphere" else 5))
+    nsteps = 6
+    announce(f"colloid trajectory: {kind} T={translate} R={rotate} ab={ab} sub={sub} le={le} sym={sym}")
+    x, v, f, (cx, cv, cw, cd, ct), ek, mom = run_pair(oracle, box, cs, nsteps)
+    c = to_oracle(oracle, cs)
+    st = oracle.State(box.prm, box.x, box.v, box.m, box.pid, box.sid, coll=c)
+    st.prologue(0)
+    for s in range(1, nsteps + 1):
+        st.vv_step(s)
+    L = np.array(box.prm.length[: box.prm.num_dim])
+    dx = np.abs(x - st.x)
+    dx = np.minimum(dx, L[None, :] - dx)
+    assert dx.max() < 1e-10 * L.max()
+    assert vec_err(v, st.v) < 1e-10
+    assert vec_err(f, st.f) < 1e-10
+    d = box.prm.num_dim
+    dcx = np.abs(cx[:, :d] - c.x[:, :d])
+    dcx = np.minimum(dcx, L[None, :] - dcx)
+    assert dcx.max() < 1e-10 * L.max()
+    # disks move in the plane and spin about z
+    wc = slice(0, 3) if d == 3 else slice(2, 3)
+    assert np.abs(cv[:, :d] - c.v[0][:, :d]).max() < 1e-9 * (np.abs(c.v[0][:, :d]).max() + 1e-12)
+    assert np.abs(cw[:, wc] - c.omega[0][:, wc]).max() < 1e-9 * (np.abs(c.omega[0][:, wc]).max() + 1e-12)
+    assert np.abs(cd[:, :d] - c.drag[:, :d]).max() < 1e-9 * (np.abs(c.drag).max() + 1e-300)
+    eko, momo = st.statistics()
+    assert ek == pytest.approx(eko, rel=1e-10)
+    assert np.abs(mom - momo).max() < 1e-9 * (np.abs(momo).max() + 1e-12)
+    if translate:
+        assert np.abs(cv[:, :d] - cs.v[:, :d]).max() > 0  # the colloids did move
+
+
+def test_colloid_fused_equals_granular(oracle):
+    box, cs = make("sphere", True, 2, True, translate=True, ab=2)
+    box = le_variant(box)
+    box.prm = box.prm.replace(dt=box.prm.dt * 20)
+    announce("colloid fused vs granular")
+    a = run_pair(oracle, box, cs, 5, fused=True)
+    b = run_pair(oracle, box, cs, 5, fused=False)
+    for u, w in zip(a[:3], b[:3]):
+        assert np.array_equal(u, w)
+    for u, w in zip(a[3], b[3]):
+        assert np.array_equal(u, w)
+
+
+def test_colloid_errors():
+    box, cs = two_spheres()
+    announce("colloid error paths")
+    with SDPD(box.prm) as g:
+        g.upload(box.x[box.sid == 0], box.v[box.sid == 0], box.m[box.sid == 0],
+                 box.pid[box.sid == 0])
+        with pytest.raises(SDPDError):
+            g.set_colloids(cs)  # after upload
+    with SDPD(box.prm) as g:
+        with pytest.raises(SDPDError):
+            g.upload(box.x, box.v, box.m, box.pid, box.sid)  # sid > 0 without colloids
+    with SDPD(box.prm) as g:
+        g.set_colloids(cs)
+        bad = box.sid.copy()
+        bad[0] = 3
+        with pytest.raises(SDPDError):
+            g.upload(box.x, box.v, box.m, box.pid, bad)
+        g.upload(box.x, box.v, box.m, box.pid, box.sid)
+        with pytest.raises(SDPDError):
+            g.integrate_position_and_adjust()  # no initial forces yet
+    with SDPD(W.mixed(box).prm) as g:
+        with pytest.raises(SDPDError):
+            g.set_colloids(cs)

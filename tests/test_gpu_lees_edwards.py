@@ -167,3 +167,30 @@ def test_le_errors():
     with SDPD(box.prm) as g:
         with pytest.raises(SDPDError):
             g.set_replicas(ens.box_lo, ens.box_hi)
+
+
+def test_le_corner_crossing(oracle):
+    # particles leaving through a sheared y face and a periodic z face in the
+    # same drift: the Lees-Edwards branch ends the particle's check, and its
+    # MODULO of the other coordinates wraps z as well
+    box = le_box("y3d", True, kt=1e-3)
+    L = box.prm.max_phys[1]
+    dt = box.prm.dt
+    v = box.v.copy()
+    corner = np.argsort(box.x[:, 1] + box.x[:, 2])[-4:]
+    v[corner, 1] = (L - box.x[corner, 1] + 0.05) / dt
+    v[corner, 2] = (L - box.x[corner, 2] + 0.05) / dt
+    low = np.argsort(box.x[:, 1] + box.x[:, 2])[:4]
+    v[low, 1] = -(box.x[low, 1] + 0.05) / dt
+    v[low, 2] = -(box.x[low, 2] + 0.05) / dt
+    announce("LE corner crossing")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, v, box.m, box.pid)
+        g.integrate_VV(2)
+        x, vg = g.download()
+    st = oracle.State(box.prm, box.x, v, box.m, box.pid)
+    st.prologue(0)
+    for s in range(1, 3):
+        st.vv_step(s)
+    assert np.all((x >= 0) & (x < L))
+    assert np.abs(x - st.x).max() < 1e-10 * L

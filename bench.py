@@ -7,7 +7,10 @@ Maxwellian velocities and thermal noise at kT = 1e-3, dt = 5e-5, in plain
 FP64 (--precision fp64).  One step = one full velocity-Verlet step of every
 particle: step boundary (kicks, drift, wrap, cell keys), cell sort,
 density + EOS, pair forces with noise.  --workload ensemble runs a batch of
-256 independent Oldroyd-B boxes (mixed precision) in one context instead.
+256 independent Oldroyd-B boxes (mixed precision) in one context instead;
+--workload colloids the same Tait box with 512 rigid spheres of radius 5 dx
+(boundary-particle shells, Morris no-slip, translating and rotating) under
+Lees-Edwards shear at rate 0.5 on the y faces, in FP64.
 The launch shapes come from the autotuner (cached per workload class).
 
 Prints one JSON line.  value = particle-steps/s of the whole job with the
@@ -102,9 +105,11 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["tait128", "ensemble"], default="tait128",
+    ap.add_argument("--workload", choices=["tait128", "ensemble", "colloids"], default="tait128",
                     help="tait128: the 3D Tait box (size from --n-side); ensemble: a batch of "
-                         "independent Oldroyd-B boxes (count from --nrep)")
+                         "independent Oldroyd-B boxes (count from --nrep); colloids: the Tait "
+                         "box with --ncoll rigid spheres under Lees-Edwards shear")
+    ap.add_argument("--ncoll", type=int, default=512)
     ap.add_argument("--n-side", type=int, default=128)
     ap.add_argument("--nrep", type=int, default=256)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
@@ -121,7 +126,7 @@ def main() -> int:
               "the multi-GPU design is described but not built", file=sys.stderr)
         return 2
     if args.precision is None:
-        args.precision = "fp64" if args.workload == "tait128" else "mixed"
+        args.precision = "mixed" if args.workload == "ensemble" else "fp64"
 
     from mcf_amd import PRECISION_FP64, PRECISION_MIXED
     from mcf_amd import autotune as A
@@ -129,7 +134,19 @@ def main() -> int:
     from mcf_amd.sdpd import SDPD
 
     prec = PRECISION_FP64 if args.precision == "fp64" else PRECISION_MIXED
-    if args.workload == "tait128":
+    colloids = None
+    if args.workload == "colloids":
+        if args.precision != "fp64":
+            print("bench.py: colloids run in FP64 only", file=sys.stderr)
+            return 2
+        box, colloids = W.colloid_suspension_3d(args.n_side, ncoll=args.ncoll)
+        nrep, volume = 1, float(args.n_side) ** 3
+        workload = (f"3D Tait box {args.n_side}^3 with {args.ncoll} rigid spheres of radius 5 dx "
+                    f"(boundary-particle shells, Morris no-slip, translating and rotating), "
+                    f"Lees-Edwards shear rate 0.5 on y, {box.n} particles "
+                    f"({int((box.sid > 0).sum())} boundary), kT 1e-3, dt 5e-5, fp64")
+        force_bytes = FORCE_BYTES["fp64"]
+    elif args.workload == "tait128":
         box = W.tait_box_3d(args.n_side, precision=prec, kt=1e-3)
         nrep, volume = 1, float(args.n_side) ** 3
         workload = (f"3D periodic Tait box {args.n_side}^3 = {box.n} particles, dx 1, cut-off 3, "
@@ -149,12 +166,16 @@ def main() -> int:
     if nrep > 1 or args.workload == "ensemble":
         g.set_replicas(box.box_lo, box.box_hi)
         g.upload_replicas(box.x, box.v, box.m, box.pid, box.rep)
+    elif colloids is not None:
+        g.set_colloids(colloids)
+        g.upload(box.x, box.v, box.m, box.pid, box.sid)
     else:
         g.upload(box.x, box.v, box.m, box.pid)
     g.integrate_VV(max(args.warmup, 1))
     tuned = None
     if not args.no_autotune:
-        tuned = A.tune(g, box.prm, box.n, volume, nrep)
+        tuned = A.tune(g, box.prm, box.n, volume, nrep,
+                       tag=f"c{colloids.num}" if colloids is not None else "")
         log(f"bench: launch shape {tuned['shape']} ({'cached' if tuned['cached'] else 'tuned'})")
     g.sync()
     log("bench: warmup done")
@@ -195,6 +216,7 @@ def main() -> int:
             "workload": workload,
             "particles": box.n,
             "replicas": nrep,
+            "colloids": 0 if colloids is None else colloids.num,
             "precision": args.precision,
             "launch_shape": None if tuned is None else tuned["shape"],
         },

@@ -26,13 +26,13 @@ def default_cache_path() -> str:
     return os.path.join(root, "mcf_amd", "tuning.json")
 
 
-def signature(prm: SDPDParams, n: int, volume: float, nrep: int = 1) -> str:
+def signature(prm: SDPDParams, n: int, volume: float, nrep: int = 1, tag: str = "") -> str:
     """Workload class of a context: launch shapes depend on the physics
-    model, the scale (particles, replicas) and the neighbours per particle,
-    not on the exact state."""
+    model, the scale (particles, replicas), the neighbours per particle and
+    the boundary conditions, not on the exact state."""
     neighbours = n / max(volume, 1e-300) * (
         4.0 / 3.0 * math.pi * prm.cut_off ** 3 if prm.num_dim == 3 else math.pi * prm.cut_off ** 2)
-    return "-".join([
+    return "-".join([p for p in [
         f"d{prm.num_dim}",
         f"p{int(prm.precision)}",
         f"nn{int(bool(prm.Newtonian))}",
@@ -42,7 +42,9 @@ def signature(prm: SDPDParams, n: int, volume: float, nrep: int = 1) -> str:
         f"n{round(math.log2(max(n, 1)))}",
         f"r{round(math.log2(max(nrep, 1)))}",
         f"nb{round(neighbours / 8.0)}",
-    ])
+        f"le{int(getattr(prm, 'le_dim', 0))}",
+        tag,
+    ] if p])
 
 
 def load_cache(path: str) -> dict:
@@ -63,12 +65,13 @@ def save_cache(path: str, cache: dict) -> None:
 
 
 def tune(sim, prm: SDPDParams, n: int, volume: float, nrep: int = 1, reps: int = 3,
-         cache_path: Optional[str] = None, retune: bool = False) -> dict:
+         cache_path: Optional[str] = None, retune: bool = False, tag: str = "") -> dict:
     """Apply the cached launch shape of this workload class to `sim` (an
     mcf_amd.sdpd.SDPD with particles uploaded), or run the autotuner and
-    cache its pick.  Returns the cache entry."""
+    cache its pick; tag distinguishes further workload classes (colloids).
+    Returns the cache entry."""
     path = cache_path or default_cache_path()
-    key = signature(prm, n, volume, nrep)
+    key = signature(prm, n, volume, nrep, tag)
     cache = load_cache(path)
     entry = cache.get(key)
     if entry is not None and not retune:

@@ -76,12 +76,22 @@ struct Geo {
 // by S_down = -S_up.  In the frame shifted by sp = MODULO(S, Lx) the lane
 // sits in cell wc, dxf = (cx - wc) cs - sp is its FP32 offset from its own
 // cell corner in that frame, and wmin / wmax bound wc over the wave.
+// Per-lane Lees-Edwards frame of the two sheared faces (sigma 0: across
+// the upper face, 1: across the lower face): x shift S' of the images, the
+// window centre, the FP32 x offset of the sheared frame, and the wave's
+// window range.  Named members with select accessors: an array indexed by
+// the run-time sigma would be placed in scratch memory.
 struct LeLane {
   double t;             // time of the shear length
-  double sp[2];
-  int wc[2];
-  float dxf[2];
-  int wmin[2], wmax[2];
+  double sp0, sp1;
+  int wc0, wc1;
+  float dxf0, dxf1;
+  int wmin0, wmin1, wmax0, wmax1;
+  __device__ __forceinline__ double sp(int s) const { return s == 0 ? sp0 : sp1; }
+  __device__ __forceinline__ int wc(int s) const { return s == 0 ? wc0 : wc1; }
+  __device__ __forceinline__ float dxf(int s) const { return s == 0 ? dxf0 : dxf1; }
+  __device__ __forceinline__ int wmin(int s) const { return s == 0 ? wmin0 : wmin1; }
+  __device__ __forceinline__ int wmax(int s) const { return s == 0 ? wmax0 : wmax1; }
 };
 
 // Per-lane context of the home particle.
@@ -134,7 +144,7 @@ __device__ __forceinline__ void rel_diff(const DevParams& P, const Home& h, cons
   if (P.le_g >= 0) {
     // across a Lees-Edwards face ox counts from the sheared window centre
     const int sg = le_sigma(P, h, o.oy, o.oz);
-    if (sg >= 0) ax = ax + h.le.dxf[sg];
+    if (sg >= 0) ax = ax + h.le.dxf(sg);
   }
   d0 = ax - (q.x + (float)o.ox * h.G.cs0);
   d1 = h.a.y - (q.y + (float)o.oy * h.G.cs1);
@@ -228,17 +238,26 @@ __device__ __forceinline__ Home load_home(const DevParams& P, const DeviceState&
       const double sl = sg == 0 ? (P.le_vhi - P.le_vlo) * h.le.t : (P.le_vlo - P.le_vhi) * h.le.t;
       const double sp = f_modulo(sl, ch.G.L0);
       const int wc = (int)floor((h.x0 - sp) / cs);
-      h.le.sp[sg] = sp;
-      h.le.wc[sg] = wc;
-      h.le.dxf[sg] = (float)((double)(h.cx - wc) * cs - sp);
+      const float dxf = (float)((double)(h.cx - wc) * cs - sp);
       int mn = h.active ? wc : INT_MAX, mx = h.active ? wc : INT_MIN;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) {
         mn = min(mn, __shfl_xor(mn, off));
         mx = max(mx, __shfl_xor(mx, off));
       }
-      h.le.wmin[sg] = mn;
-      h.le.wmax[sg] = mx;
+      if (sg == 0) {
+        h.le.sp0 = sp;
+        h.le.wc0 = wc;
+        h.le.dxf0 = dxf;
+        h.le.wmin0 = mn;
+        h.le.wmax0 = mx;
+      } else {
+        h.le.sp1 = sp;
+        h.le.wc1 = wc;
+        h.le.dxf1 = dxf;
+        h.le.wmin1 = mn;
+        h.le.wmax1 = mx;
+      }
     }
   }
   return h;
@@ -286,8 +305,8 @@ __device__ __forceinline__ RowPieces row_pieces(const DevParams& P, const Device
     rp.vlo = ch.xmin - 2;
     vhi = ch.xmax + 2;
   } else {
-    rp.vlo = h.le.wmin[rp.sigma] - 2;
-    vhi = h.le.wmax[rp.sigma] + 2;
+    rp.vlo = h.le.wmin(rp.sigma) - 2;
+    vhi = h.le.wmax(rp.sigma) + 2;
   }
   rp.aend = vhi + 1;
   rp.m0 = floor_div(rp.vlo, nm0);
@@ -416,9 +435,9 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
     // offset from the lane's window centre).  Across a Lees-Edwards face
     // the window centre and the lane's x come from the sheared frame.
     const bool le_row = rp.sigma >= 0;
-    const int wc = le_row ? h.le.wc[rp.sigma] : h.cx;
+    const int wc = le_row ? h.le.wc(rp.sigma) : h.cx;
     const int crel = wc - v_lo;
-    const float ax = (le_row ? h.a.x + h.le.dxf[rp.sigma] : h.a.x) + (float)crel * h.G.cs0;
+    const float ax = (le_row ? h.a.x + h.le.dxf(rp.sigma) : h.a.x) + (float)crel * h.G.cs0;
     const uint32_t self_w = le_row ? 0xFFFFFFFFu : ((uint32_t)h.ii | ((uint32_t)crel << kCodeShift));
     // this lane's window: virtual cells wc-2 .. wc+2 as a range of the
     // row sequence
@@ -534,7 +553,7 @@ __device__ __forceinline__ double pair_d2(const DevParams& P, const Home& h, con
     const int sg = le_sigma(P, h, o.oy, o.oz);
     if (sg >= 0) {
       // x_i - x_j - S' lies in (-2 Lx, Lx): nearest period
-      const double raw = (h.x0 - xj0) - h.le.sp[sg];
+      const double raw = (h.x0 - xj0) - h.le.sp(sg);
       d0 = raw - h.G.L0 * rint(raw / h.G.L0);
     }
   }
@@ -610,8 +629,11 @@ __device__ __forceinline__ void write_force_records(const DevParams& P, const De
   }
 }
 
-template <int DIM, bool MIXED, int TILE, int BUF>
+// LE = false compiles the Lees-Edwards paths out (boxes without sheared
+// faces): le_g is then the constant -1 in every inlined helper.
+template <int DIM, bool MIXED, int TILE, int BUF, bool LE>
 __global__ __launch_bounds__(kWave) void k_density(DevParams P, DeviceState S) {
+  if (!LE) P.le_g = -1;
   __shared__ float4 tile[TILE];
   __shared__ uint32_t buf[BUF * kWave];
   Chunk ch;
@@ -1092,8 +1114,9 @@ __device__ __forceinline__ void conformation_rate(const DevParams& P, const Devi
     }
 }
 
-template <int DIM, int FT, bool MIXED, bool COLL = false>
+template <int DIM, int FT, bool MIXED, bool COLL, bool LE>
 __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
+  if (!LE) P.le_g = -1;
   __shared__ float4 tile[kTile];
   __shared__ uint32_t buf[kBuf * kWave];
   Chunk ch;
@@ -1190,8 +1213,9 @@ struct SignaturePol {
   }
 };
 
-template <int DIM>
+template <int DIM, bool LE>
 __global__ __launch_bounds__(kWave) void k_signature(DevParams P, DeviceState S) {
+  if (!LE) P.le_g = -1;
   __shared__ float4 tile[kTile];
   __shared__ uint32_t buf[kBuf * kWave];
   Chunk ch;
@@ -1215,35 +1239,43 @@ static inline int nchunks_max(const DevParams& P) {
   return (m + 7) / 8 * 8;
 }
 
-template <int DIM, bool MIXED>
+template <int DIM, bool MIXED, bool LE>
 static void launch_density_shape(hipStream_t s, const DevParams& P, const DeviceState& S) {
   dim3 g(nchunks_max(P)), b(kWave);
   switch (P.density_shape) {
     case 1:
-      hipLaunchKernelGGL((k_density<DIM, MIXED, 128, 16>), g, b, 0, s, P, S);
+      hipLaunchKernelGGL((k_density<DIM, MIXED, 128, 16, LE>), g, b, 0, s, P, S);
       break;
     case 2:
-      hipLaunchKernelGGL((k_density<DIM, MIXED, 224, 12>), g, b, 0, s, P, S);
+      hipLaunchKernelGGL((k_density<DIM, MIXED, 224, 12, LE>), g, b, 0, s, P, S);
       break;
     case 3:
-      hipLaunchKernelGGL((k_density<DIM, MIXED, 128, 12>), g, b, 0, s, P, S);
+      hipLaunchKernelGGL((k_density<DIM, MIXED, 128, 12, LE>), g, b, 0, s, P, S);
       break;
     default:
-      hipLaunchKernelGGL((k_density<DIM, MIXED, kTile, kBuf>), g, b, 0, s, P, S);
+      hipLaunchKernelGGL((k_density<DIM, MIXED, kTile, kBuf, LE>), g, b, 0, s, P, S);
   }
+}
+
+template <int DIM, bool MIXED>
+static void launch_density_le(hipStream_t s, const DevParams& P, const DeviceState& S) {
+  if (P.le_g >= 0)
+    launch_density_shape<DIM, MIXED, true>(s, P, S);
+  else
+    launch_density_shape<DIM, MIXED, false>(s, P, S);
 }
 
 void launch_density(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed) {
   if (P.dim == 3) {
     if (mixed)
-      launch_density_shape<3, true>(s, P, S);
+      launch_density_le<3, true>(s, P, S);
     else
-      launch_density_shape<3, false>(s, P, S);
+      launch_density_le<3, false>(s, P, S);
   } else {
     if (mixed)
-      launch_density_shape<2, true>(s, P, S);
+      launch_density_le<2, true>(s, P, S);
     else
-      launch_density_shape<2, false>(s, P, S);
+      launch_density_le<2, false>(s, P, S);
   }
 }
 
@@ -1255,31 +1287,39 @@ void launch_refresh_records(hipStream_t s, const DevParams& P, const DeviceState
   hipLaunchKernelGGL(k_refresh_records, dim3((P.n + 255) / 256), dim3(256), 0, s, P, S);
 }
 
+template <int DIM, int FT, bool MIXED, bool COLL>
+static void launch_forces_le(hipStream_t s, const DevParams& P, const DeviceState& S) {
+  dim3 g(nchunks_max(P)), b(kWave);
+  if (P.le_g >= 0)
+    hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, true>), g, b, 0, s, P, S);
+  else
+    hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, false>), g, b, 0, s, P, S);
+}
+
 template <int DIM>
 static void launch_forces_dim(hipStream_t s, const DevParams& P, const DeviceState& S,
                               bool mixed) {
-  dim3 g(nchunks_max(P)), b(kWave);
   const int ft = P.newtonian ? P.force_type : 5;
   if (mixed) {
     if (ft == 3)
-      hipLaunchKernelGGL((k_forces<DIM, 3, true>), g, b, 0, s, P, S);
+      launch_forces_le<DIM, 3, true, false>(s, P, S);
     else if (ft == 4)
-      hipLaunchKernelGGL((k_forces<DIM, 4, true>), g, b, 0, s, P, S);
+      launch_forces_le<DIM, 4, true, false>(s, P, S);
     else
-      hipLaunchKernelGGL((k_forces<DIM, 5, true>), g, b, 0, s, P, S);
+      launch_forces_le<DIM, 5, true, false>(s, P, S);
   } else if (P.ncoll) {
     // colloids: Newtonian FP64 only (checked by the API)
     if (ft == 3)
-      hipLaunchKernelGGL((k_forces<DIM, 3, false, true>), g, b, 0, s, P, S);
+      launch_forces_le<DIM, 3, false, true>(s, P, S);
     else
-      hipLaunchKernelGGL((k_forces<DIM, 4, false, true>), g, b, 0, s, P, S);
+      launch_forces_le<DIM, 4, false, true>(s, P, S);
   } else {
     if (ft == 3)
-      hipLaunchKernelGGL((k_forces<DIM, 3, false>), g, b, 0, s, P, S);
+      launch_forces_le<DIM, 3, false, false>(s, P, S);
     else if (ft == 4)
-      hipLaunchKernelGGL((k_forces<DIM, 4, false>), g, b, 0, s, P, S);
+      launch_forces_le<DIM, 4, false, false>(s, P, S);
     else
-      hipLaunchKernelGGL((k_forces<DIM, 5, false>), g, b, 0, s, P, S);
+      launch_forces_le<DIM, 5, false, false>(s, P, S);
   }
 }
 
@@ -1293,10 +1333,18 @@ void launch_forces(hipStream_t s, const DevParams& P, const DeviceState& S, bool
 void launch_signature(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed) {
   (void)mixed;  // both modes share the decision
   dim3 g(nchunks_max(P)), b(kWave);
-  if (P.dim == 3)
-    hipLaunchKernelGGL(k_signature<3>, g, b, 0, s, P, S);
-  else
-    hipLaunchKernelGGL(k_signature<2>, g, b, 0, s, P, S);
+  const bool le = P.le_g >= 0;
+  if (P.dim == 3) {
+    if (le)
+      hipLaunchKernelGGL((k_signature<3, true>), g, b, 0, s, P, S);
+    else
+      hipLaunchKernelGGL((k_signature<3, false>), g, b, 0, s, P, S);
+  } else {
+    if (le)
+      hipLaunchKernelGGL((k_signature<2, true>), g, b, 0, s, P, S);
+    else
+      hipLaunchKernelGGL((k_signature<2, false>), g, b, 0, s, P, S);
+  }
 }
 
 }  // namespace sdpd

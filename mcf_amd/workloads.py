@@ -306,8 +306,24 @@ def with_colloids(box: Box, centres, radius: float, dx: float = None, rho_type: 
     dout = dx * COLLOID_OUT_LAYER_COEFF
     din = (1.0 + COLLOID_IN_LAYER_COEFF) * prm.cut_off * (2.0 if rho_type == 1 else 1.0)
     keep = np.ones(box.n, bool)
+    reach = radius + dout
+    lo = np.array(prm.min_phys[:d])
+    # bin the solvent into cells at least `reach` wide and test each centre
+    # against the 3^d cells around it only
+    nc = np.maximum((L // reach).astype(int), 1)
+    cell = np.minimum(((box.x - lo[None, :]) / L[None, :] * nc[None, :]).astype(int), nc[None, :] - 1)
+    flat = np.ravel_multi_index(tuple(cell.T), tuple(nc))
+    order = np.argsort(flat, kind="stable")
+    bounds = np.searchsorted(flat[order], np.arange(np.prod(nc) + 1))
+    offs = np.array(np.meshgrid(*[[-1, 0, 1]] * d, indexing="ij")).reshape(d, -1).T
     for c in centres:
-        keep &= nearest_dist(box.x, c, L) > radius + dout
+        cc = np.minimum(((c - lo) / L * nc).astype(int), nc - 1)
+        cand = []
+        for o in offs:
+            q = np.ravel_multi_index(tuple(np.mod(cc + o, nc)), tuple(nc))
+            cand.append(order[bounds[q]:bounds[q + 1]])
+        cand = np.unique(np.concatenate(cand))
+        keep[cand[nearest_dist(box.x[cand], c, L) <= reach]] = False
     shell = sphere_shell(radius, dx, din) if d == 3 else disk_shell(radius, dx, din)
     xs, vs, sids = [box.x[keep]], [box.v[keep]], [np.zeros(int(keep.sum()), np.int32)]
     vcol = np.zeros((k, 3)) if kw.get("v") is None else np.array(kw["v"], np.float64).reshape(k, 3)

@@ -747,14 +747,12 @@ struct ForceF64 {
   double rxc_i[3] = {0.0, 0.0, 0.0}, vcoll_i[3] = {0.0, 0.0, 0.0};
   double le_t = 0.0;
   __device__ __forceinline__ void body(int j, const Offsets& o) {
+    int sj = 0;
     if (COLL) {
-      const int sj = S.sid[j];
-      if ((sidi | sj) != 0) {
-        // two boundary particles do not interact (pp_interaction.inc: same
-        // colloid CYCLE, colloid-colloid pair forces off)
-        if (sidi == 0 || sj == 0) colloid_pair(j, o, sj);
-        return;
-      }
+      // two boundary particles do not interact (pp_interaction.inc: same
+      // colloid CYCLE, colloid-colloid pair forces off)
+      sj = S.sid[j];
+      if (sidi != 0 && sj != 0) return;
     }
     const int64_t n = P.n;
     const double4* rj = reinterpret_cast<const double4*>(S.rec64 + 8 * (int64_t)j);
@@ -765,15 +763,27 @@ struct ForceF64 {
     const double r = r2 * ir;
     double w, gw;
     kernel_eval(P, r * P.inv_h, w, gw);
-    const double u0 = vi0 - (rb.x + le_dv(P, h, o)), u1 = vi1 - rb.y, u2 = DIM == 3 ? vi2 - rb.z : 0.0;
+    const double jump = le_dv(P, h, o);  // x velocity of j's image minus j's own
+    double u0 = vi0 - (rb.x + jump), u1 = vi1 - rb.y, u2 = DIM == 3 ? vi2 - rb.z : 0.0;
     const double bj = rb.w;
+    // pressure terms: p_i / n_i^2 + p_j / n_j^2 (type 3), (p_i + p_j) / 2 (type 4)
+    double psum = 0.0, pij = 0.0;
+    if (FT == 3) psum = ai + ra.w;
+    if (FT == 4) pij = (pi + S.p[j]) * 0.5;
+    if (COLL && (sidi | sj) != 0) {
+      // a fluid particle and a colloid boundary particle: only the
+      // boundary particle's velocity and pressure change
+      double pb;
+      colloid_substitute(j, sj, ra, rb, d0, d1, d2, jump, u0, u1, u2, pb);
+      if (FT == 3) psum = sidi == 0 ? ai + pb * bj : pb * bi + ra.w;
+      if (FT == 4) pij = pb;
+    }
     double a0, a1, a2;
     if (FT == 4) {
       // rhs_force_ff_Newtonian_HuAdams_angular.F90:166-229: everything
       // along e = d / r
       const double du = DIM == 3 ? fma(d2, u2, fma(d1, u1, d0 * u0)) : fma(d1, u1, d0 * u0);
       const double aij = (bi + bj) * gw;
-      const double pij = (pi + S.p[j]) * 0.5;
       double sc = P.eta_coef * P.eta * aij * du * (ir * ir) - aij * pij;
       if (P.noise) {
         const uint32_t pj_ = (uint32_t)S.pid[j];
@@ -789,7 +799,7 @@ struct ForceF64 {
       const double fd = P.eta * (bi + bj) * gw * ir;
       if (FT == 3) {
         // rhs_force_ff_Newtonian_HuAdams.F90:140-237
-        const double fcr = (ai + ra.w) * gw * ir;
+        const double fcr = psum * gw * ir;
         a0 = fd * u0 - fcr * d0;
         a1 = fd * u1 - fcr * d1;
         a2 = fd * u2 - fcr * d2;
@@ -850,22 +860,16 @@ struct ForceF64 {
   // A fluid particle and a colloid boundary particle
   // (pp_interaction_fc.inc, pp_interaction_cf.inc): the boundary particle
   // moves at the no-slip velocity seen from the fluid particle and carries
-  // the fluid particle's pressure; its own number density stays.  The
+  // the fluid particle's pressure (pb); its own number density stays.  The
   // no-slip velocity is formed in the boundary particle's frame, where the
   // fluid particle sits at r_xc + (x_f - x_c) and moves at its image
-  // velocity, so both sides of the pair see the same relative velocity.
-  __device__ __forceinline__ void colloid_pair(int j, const Offsets& o, int sj) {
-    const double4* rj = reinterpret_cast<const double4*>(S.rec64 + 8 * (int64_t)j);
-    const double4 ra = rj[0], rb = rj[1];
-    double d0, d1, d2;
-    const double r2 = pair_d2<DIM>(P, h, o, ra.x, ra.y, ra.z, d0, d1, d2);
-    const double ir = rsqrt_nr(r2);
-    const double r = r2 * ir;
-    double w, gw;
-    kernel_eval(P, r * P.inv_h, w, gw);
-    const double jump = le_dv(P, h, o);  // x velocity of j's image minus j's own
-    const double bj = rb.w;
-    double u[3], vc[3], pair_p;
+  // velocity, so both sides of the pair see the same relative velocity
+  // u = v_i - v_j.
+  __device__ __forceinline__ void colloid_substitute(int j, int sj, const double4& ra,
+                                                     const double4& rb, double d0, double d1,
+                                                     double d2, double jump, double& u0,
+                                                     double& u1, double& u2, double& pb) {
+    double vc[3];
     if (sidi == 0) {
       const CollHot C = S.coll_hot[sj - 1];
       const double xj[3] = {ra.x, ra.y, DIM == 3 ? ra.z : 0.0};
@@ -874,63 +878,20 @@ struct ForceF64 {
       const double rxf[3] = {rxc[0] + d0, rxc[1] + d1, DIM == 3 ? rxc[2] + d2 : 0.0};
       const double vf[3] = {vi0 - jump, vi1, DIM == 3 ? vi2 : 0.0};
       coll_noslip<DIM>(P, C, rxc, vcoll, rxf, vf, vc);
-#pragma unroll
-      for (int a = 0; a < 3; ++a) u[a] = a < DIM ? vf[a] - vc[a] : 0.0;
-      pair_p = pi;
+      u0 = vf[0] - vc[0];
+      u1 = vf[1] - vc[1];
+      u2 = DIM == 3 ? vf[2] - vc[2] : 0.0;
+      pb = pi;
     } else {
       const CollHot C = S.coll_hot[sidi - 1];
       const double rxf[3] = {rxc_i[0] - d0, rxc_i[1] - d1, DIM == 3 ? rxc_i[2] - d2 : 0.0};
       const double vf[3] = {rb.x + jump, rb.y, DIM == 3 ? rb.z : 0.0};
       coll_noslip<DIM>(P, C, rxc_i, vcoll_i, rxf, vf, vc);
-#pragma unroll
-      for (int a = 0; a < 3; ++a) u[a] = a < DIM ? vc[a] - vf[a] : 0.0;
-      pair_p = S.p[j];
+      u0 = vc[0] - vf[0];
+      u1 = vc[1] - vf[1];
+      u2 = DIM == 3 ? vc[2] - vf[2] : 0.0;
+      pb = S.p[j];
     }
-    double a0, a1, a2;
-    if (FT == 4) {
-      const double du = DIM == 3 ? fma(d2, u[2], fma(d1, u[1], d0 * u[0])) : fma(d1, u[1], d0 * u[0]);
-      const double aij = (bi + bj) * gw;
-      double sc = P.eta_coef * P.eta * aij * du * (ir * ir) - aij * pair_p;
-      if (P.noise) {
-        const uint32_t pj_ = (uint32_t)S.pid[j];
-        float g[4];
-        pair_normals(P.seed, pidi < pj_ ? pidi : pj_, pidi < pj_ ? pj_ : pidi, step, g);
-        sc = sc + (double)g[0] * P.sqrt_2kt_eta * sqrt_nr(-P.eta_coef * aij * ir) * P.inv_sqrt_dt;
-      }
-      sc = sc * ir;
-      a0 = sc * d0;
-      a1 = sc * d1;
-      a2 = sc * d2;
-    } else {
-      const double fd = P.eta * (bi + bj) * gw * ir;
-      const double fcr = (sidi == 0 ? ai + pair_p * bj : pair_p * bi + ra.w) * gw * ir;
-      a0 = fd * u[0] - fcr * d0;
-      a1 = fd * u[1] - fcr * d1;
-      a2 = fd * u[2] - fcr * d2;
-      if (P.noise) {
-        const uint32_t pidj = (uint32_t)S.pid[j];
-        float g[4];
-        pair_normals(P.seed, pidi < pidj ? pidi : pidj, pidi < pidj ? pidj : pidi, step, g);
-        const double fr = sqrt_nr(-4.0 * P.kt * fd) * P.inv_sqrt_dt;
-        const double sfr = pidi < pidj ? fr : -fr;
-        const double ca = 0.70710678118654752440;
-        const double g0 = (double)g[0], g1 = (double)g[1];
-        if (DIM == 3) {
-          const double cb = 0.81649658092772603273 - 0.70710678118654752440;
-          const double g2 = (double)g[2];
-          const double k = cb * fma(g2, d2, fma(g1, d1, g0 * d0)) * (ir * ir);
-          a0 = fma(sfr, fma(k, d0, ca * g0), a0);
-          a1 = fma(sfr, fma(k, d1, ca * g1), a1);
-          a2 = fma(sfr, fma(k, d2, ca * g2), a2);
-        } else {
-          a0 = fma(sfr, ca * g0, a0);
-          a1 = fma(sfr, ca * g1, a1);
-        }
-      }
-    }
-    f0 = f0 + a0;
-    f1 = f1 + a1;
-    if (DIM == 3) f2 = f2 + a2;
   }
 };
 

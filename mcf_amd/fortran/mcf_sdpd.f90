@@ -81,6 +81,41 @@ CONTAINS
     END DO
   END FUNCTION config_find
 
+  ! Index of the n-th occurrence of a key (keys the reference reads once
+  ! per colloid, io_read_physics.F90:1242-1350), 0 when absent.
+  INTEGER FUNCTION config_find_nth(cfg, key, nth)
+    TYPE(config), INTENT(IN) :: cfg
+    CHARACTER(len=*), INTENT(IN) :: key
+    INTEGER, INTENT(IN) :: nth
+    INTEGER :: k, seen
+    config_find_nth = 0
+    seen = 0
+    DO k = 1, cfg%n
+       IF (TRIM(cfg%key(k)) == lower(key)) THEN
+          seen = seen + 1
+          IF (seen == nth) THEN
+             config_find_nth = k
+             RETURN
+          END IF
+       END IF
+    END DO
+  END FUNCTION config_find_nth
+
+  SUBROUTINE get_reals_nth(cfg, key, nth, n, vals)
+    TYPE(config), INTENT(IN) :: cfg
+    CHARACTER(len=*), INTENT(IN) :: key
+    INTEGER, INTENT(IN) :: nth, n
+    DOUBLE PRECISION, INTENT(INOUT) :: vals(n)
+    DOUBLE PRECISION :: tmp(n)
+    INTEGER :: k, ios
+    k = config_find_nth(cfg, key, nth)
+    IF (k > 0) THEN
+       tmp = vals
+       READ(cfg%val(k), *, iostat=ios) tmp
+       IF (ios == 0) vals = tmp
+    END IF
+  END SUBROUTINE get_reals_nth
+
   INTEGER FUNCTION get_int(cfg, key, default)
     TYPE(config), INTENT(IN) :: cfg
     CHARACTER(len=*), INTENT(IN) :: key
@@ -186,7 +221,10 @@ PROGRAM mcf_sdpd
   INTEGER :: seed_size
   INTEGER, ALLOCATABLE :: seed_arr(:)
   INTEGER(c_int64_t) :: cur_step
-  INTEGER :: bcdef(6), le_dim
+  INTEGER :: bcdef(6), le_dim, ncoll
+  TYPE(sdpd_colloid_params) :: cprm
+  DOUBLE PRECISION, ALLOCATABLE :: c_rad(:), c_m(:), c_mmi(:,:), c_x(:,:), c_v(:,:), c_w(:,:)
+  DOUBLE PRECISION :: rr(3), dxc, cpi
   DOUBLE PRECISION :: shear_in(12)
   REAL(c_double) :: v_lo(3), v_hi(3)
 
@@ -328,6 +366,61 @@ PROGRAM mcf_sdpd
         v_hi(5 - d) = shear_in((2 * d - 1) * 2 + 2)
      END IF
      CALL check(sdpd_set_lees_edwards(ctx, INT(d, c_int32_t), v_lo, v_hi), 'sdpd_set_lees_edwards')
+  END IF
+  ! colloids: num_colloid and the coll_* keys, read once per colloid as
+  ! io_read_physics.F90:901-1350 does; disks and spheres, explicit
+  ! integrators; their boundary particles come with the particle file
+  ! (species id = colloid index)
+  ncoll = get_int(cfg, 'num_colloid', 0)
+  IF (ncoll > 0) THEN
+     k = get_int(cfg, 'integrate_colloid_type', 2)
+     IF (k /= 1 .AND. k /= 2) CALL die('only the explicit colloid integrators (integrate_colloid_type 1, 2)')
+     cprm%noslip = get_int(cfg, 'coll_noslip', 2)
+     cprm%rho_type = get_int(cfg, 'coll_rho_type', 0)
+     cprm%translate = MERGE(1, 0, get_logical(cfg, 'coll_translate', .FALSE.))
+     cprm%rotate = MERGE(1, 0, get_logical(cfg, 'coll_rotate', .FALSE.))
+     cprm%integrate_ab = get_int(cfg, 'integrate_colloid_ab', 1)
+     cprm%sub_steps = get_int(cfg, 'coll_sub_time_step', 1)
+     cprm%body_force_type = get_int(cfg, 'coll_body_force_type', 0)
+     cprm%pad = 0
+     cprm%body_force = 0.0d0
+     CALL get_reals(cfg, 'coll_body_force', dim, cprm%body_force(1:dim))
+     ! outer layer 0.05 dx(1) (physics_adjust_parameters.F90:320)
+     num_part = 0
+     CALL get_ints(cfg, 'num_part', dim, num_part(1:dim))
+     IF (num_part(1) > 0) THEN
+        dxc = (prm%max_phys(1) - prm%min_phys(1)) / num_part(1)
+     ELSE
+        dxc = prm%cut_off / 3.0d0
+     END IF
+     cprm%dout = 0.05d0 * dxc
+     ALLOCATE(c_rad(ncoll), c_m(ncoll), c_mmi(3, ncoll), c_x(3, ncoll), c_v(3, ncoll), c_w(3, ncoll))
+     c_x = 0.0d0
+     c_v = 0.0d0
+     c_w = 0.0d0
+     c_mmi = 0.0d0
+     cpi = ACOS(-1.0d0)
+     DO k = 1, ncoll
+        rr = 2.0d0
+        CALL get_reals_nth(cfg, 'coll_shape', k, 1, rr(1:1))
+        IF (NINT(rr(1)) /= 2) CALL die('only disks and spheres (coll_shape = 2) are supported')
+        rr = 0.0d0
+        CALL get_reals_nth(cfg, 'coll_radius', k, dim, rr(1:dim))
+        c_rad(k) = rr(1)
+        CALL get_reals_nth(cfg, 'coll_x', k, dim, c_x(1:dim, k))
+        CALL get_reals_nth(cfg, 'coll_v', k, dim, c_v(1:dim, k))
+        CALL get_reals_nth(cfg, 'coll_omega', k, 3, c_w(1:3, k))
+        ! mass and moment of inertia (particles_compute_mass.F90:153-357)
+        IF (dim == 2) THEN
+           c_m(k) = prm%rho * (cpi * c_rad(k)**2.0d0)
+           c_mmi(3, k) = 0.5d0 * c_m(k) * c_rad(k)**2
+        ELSE
+           c_m(k) = prm%rho * (4.0d0 * cpi * c_rad(k)**3.0d0 / 3.0d0)
+           c_mmi(1:3, k) = 0.4d0 * c_m(k) * c_rad(k)**2
+        END IF
+     END DO
+     CALL check(sdpd_set_colloids(ctx, cprm, INT(ncoll, c_int32_t), c_rad, c_m, c_mmi, c_x, c_v, &
+          c_w), 'sdpd_set_colloids')
   END IF
   CALL check(sdpd_upload(ctx, x, v, m, id, INT(n, c_int32_t)), 'sdpd_upload')
   CALL check(sdpd_set_step(ctx, INT(step_start, c_int64_t)), 'sdpd_set_step')

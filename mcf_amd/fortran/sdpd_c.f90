@@ -51,6 +51,20 @@ MODULE sdpd_c
      INTEGER(c_int64_t) :: seed
   END TYPE sdpd_params
 
+  ! struct sdpd_colloid_params (coll_* keys of physics_config.mcf)
+  TYPE, BIND(C) :: sdpd_colloid_params
+     INTEGER(c_int32_t) :: noslip
+     INTEGER(c_int32_t) :: rho_type
+     INTEGER(c_int32_t) :: translate
+     INTEGER(c_int32_t) :: rotate
+     INTEGER(c_int32_t) :: integrate_ab
+     INTEGER(c_int32_t) :: sub_steps
+     INTEGER(c_int32_t) :: body_force_type
+     INTEGER(c_int32_t) :: pad
+     REAL(c_double)     :: body_force(3)
+     REAL(c_double)     :: dout
+  END TYPE sdpd_colloid_params
+
   INTERFACE
      INTEGER(c_int) FUNCTION sdpd_create(params, ctx) BIND(C, name="sdpd_create")
        IMPORT :: c_int, c_ptr, sdpd_params
@@ -98,6 +112,23 @@ MODULE sdpd_c
        INTEGER(c_int32_t), VALUE  :: le_dim
        REAL(c_double), INTENT(IN) :: v_lower(3), v_upper(3)
      END FUNCTION sdpd_set_lees_edwards
+
+     INTEGER(c_int) FUNCTION sdpd_set_colloids(ctx, cprm, num, radius, m, mmi, x, v, omega) &
+          BIND(C, name="sdpd_set_colloids")
+       IMPORT :: c_int, c_ptr, c_double, c_int32_t, sdpd_colloid_params
+       TYPE(c_ptr), VALUE                    :: ctx
+       TYPE(sdpd_colloid_params), INTENT(IN) :: cprm
+       INTEGER(c_int32_t), VALUE             :: num
+       REAL(c_double), INTENT(IN)            :: radius(*), m(*)
+       REAL(c_double), INTENT(IN)            :: mmi(3, *), x(3, *), v(3, *), omega(3, *)
+     END FUNCTION sdpd_set_colloids
+
+     INTEGER(c_int) FUNCTION sdpd_get_colloids(ctx, x, v, omega, drag, torque) &
+          BIND(C, name="sdpd_get_colloids")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: ctx
+       TYPE(c_ptr), VALUE :: x, v, omega, drag, torque
+     END FUNCTION sdpd_get_colloids
 
      INTEGER(c_int) FUNCTION sdpd_upload_ct(ctx, ct) BIND(C, name="sdpd_upload_ct")
        IMPORT :: c_int, c_ptr, c_double

@@ -343,10 +343,10 @@ def with_colloids(box: Box, centres, radius: float, dx: float = None, rho_type: 
     m = np.full(x.shape[0], box.m[0])
     pid = np.arange(1, x.shape[0] + 1, dtype=np.int32)
     if d == 3:
-        mass = prm.rho * 4.0 * math.pi * radius ** 3.0 / 3.0
+        mass = prm.rho * (4.0 * math.pi * radius ** 3.0 / 3.0)
         mmi = np.full((k, 3), 0.4 * mass * radius ** 2)
     else:
-        mass = prm.rho * math.pi * radius ** 2.0
+        mass = prm.rho * (math.pi * radius ** 2.0)
         mmi = np.zeros((k, 3))
         mmi[:, 2] = 0.5 * mass * radius ** 2
     coll = ColloidSet(np.full(k, float(radius)), centres, np.full(k, mass), mmi, dout=dout,

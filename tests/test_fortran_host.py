@@ -178,3 +178,42 @@ def test_host_lees_edwards_matches_python_binding(tmp_path, le_dim, fused):
         g.integrate_VV(nsteps)
         xr, vr = g.download()
     assert np.array_equal(x, xr) and np.array_equal(vh, vr)
+
+
+def colloid_extra(cs, d):
+    """num_colloid and the coll_* keys, once per colloid as the reference
+    reads them (io_read_physics.F90:1242-1350)."""
+    lines = [f"num_colloid = {cs.num}", f"coll_noslip = {cs.noslip}", f"coll_rho_type = {cs.rho_type}",
+             f"coll_translate = {'T' if cs.translate else 'F'}",
+             f"coll_rotate = {'T' if cs.rotate else 'F'}", f"coll_sub_time_step = {cs.sub_steps}",
+             f"integrate_colloid_ab = {cs.ab}", "integrate_colloid_type = 2"]
+    for k in range(cs.num):
+        f = lambda a: ", ".join(repr(float(u)) for u in a)  # noqa: E731
+        lines += ["coll_shape = 2", f"coll_radius = {f([cs.radius[k]] + [0.0] * (d - 1))}",
+                  f"coll_x = {f(cs.x[k][:d])}", f"coll_v = {f(cs.v[k][:d])}",
+                  f"coll_omega = {f(cs.omega[k])}"]
+    return "\n".join(lines) + "\n"
+
+
+@pytest.mark.gpu
+@needs_host
+@pytest.mark.parametrize("fused", [False, True])
+def test_host_colloids_match_python_binding(tmp_path, fused):
+    from tests.colloid_helpers import two_spheres
+    box, cs = two_spheres(n_side=16, radius=4.0, kt=1e-3, translate=True, rotate=True,
+                          omega=[[0.0, 0.0, 0.02], [0.01, 0.0, 0.0]], v=[[0.01, 0, 0], [0, 0.02, 0]])
+    # the host derives dx = cut_off / 3 when num_part is absent: dout 0.05
+    assert abs(cs.dout - 0.05) < 1e-15
+    box.prm = box.prm.replace(dt=box.prm.dt * 20)
+    nsteps = 4
+    announce(f"Fortran host with colloids, fused={fused}")
+    ctrl = write_configs(str(tmp_path), box, nsteps, fused=fused, extra=colloid_extra(cs, 3))
+    r = run_host(ctrl, str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    x, vh = read_parts(str(tmp_path / f"parts{nsteps:08d}.dat"), 3)
+    with SDPD(box.prm) as g:
+        g.set_colloids(cs)
+        g.upload(box.x, box.v, box.m, box.pid, box.sid)
+        g.integrate_VV(nsteps)
+        xr, vr = g.download()
+    assert np.array_equal(x, xr) and np.array_equal(vh, vr)

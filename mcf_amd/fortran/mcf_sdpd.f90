@@ -215,6 +215,13 @@ PROGRAM mcf_sdpd
   INTEGER :: num_part(3), idx(3)
   INTEGER(c_int32_t) :: rc32
   LOGICAL :: read_external, fused, newtonian
+  ! output files (io_config.mcf keys): statistic, particles, conformation,
+  ! restart; FORMATTED (E16.8 records) or UNFORMATTED (binary records)
+  CHARACTER(len=VAL_LEN) :: restart_file, conf_out_file, restart_conf_file, read_conf_file
+  INTEGER :: restart_freq, conf_freq, next_out
+  LOGICAL :: out_unf, restart_unf, conf_unf
+  DOUBLE PRECISION :: time_cur
+  DOUBLE PRECISION, ALLOCATABLE, TARGET :: ct(:,:)
   DOUBLE PRECISION :: dx(3), jitter, mom(3), ekin, t0, t1, r(3)
   DOUBLE PRECISION, ALLOCATABLE, TARGET :: x(:,:), v(:,:), m(:), rho(:)
   INTEGER(c_int32_t), ALLOCATABLE :: id(:,:)
@@ -290,6 +297,15 @@ PROGRAM mcf_sdpd
   out_freq = get_int(cfg, 'output_particles_freq_step', -1)
   stat_file = get_string(cfg, 'statistic_file', 'mcf_statistic.dat')
   out_file = get_string(cfg, 'output_particles_file', 'mcf_particles')
+  out_unf = is_unformatted(get_string(cfg, 'output_particles_fmt', 'FORMATTED'))
+  restart_file = get_string(cfg, 'restart_particles_file', 'mcf_restart_particles')
+  restart_unf = is_unformatted(get_string(cfg, 'restart_particles_fmt', 'FORMATTED'))
+  restart_freq = get_int(cfg, 'restart_freq_step', -1)
+  conf_out_file = get_string(cfg, 'output_conformation_file', 'mcf_conformation')
+  conf_unf = is_unformatted(get_string(cfg, 'output_conformation_fmt', 'FORMATTED'))
+  conf_freq = get_int(cfg, 'output_conformation_freq_step', -1)
+  restart_conf_file = get_string(cfg, 'restart_conformation_file', 'mcf_restart_conformation')
+  read_conf_file = get_string(cfg, 'read_conformation_file', '')
   read_external = get_logical(cfg, 'read_external', .FALSE.)
   fused = get_logical(cfg, 'fused_step', .FALSE.)
 
@@ -424,6 +440,16 @@ PROGRAM mcf_sdpd
   END IF
   CALL check(sdpd_upload(ctx, x, v, m, id, INT(n, c_int32_t)), 'sdpd_upload')
   CALL check(sdpd_set_step(ctx, INT(step_start, c_int64_t)), 'sdpd_set_step')
+  IF (.NOT. newtonian) THEN
+     ALLOCATE(ct(dim * dim, n))
+     ! initial conformation from a file (io_read_conformation.F90) when the
+     ! particles come from one; else the identity of sdpd_upload
+     IF (read_external .AND. LEN_TRIM(read_conf_file) > 0) THEN
+        CALL read_conformation(TRIM(dir)//TRIM(read_conf_file))
+        CALL check(sdpd_upload_ct(ctx, ct), 'sdpd_upload_ct')
+     END IF
+  END IF
+  time_cur = 0.0d0
 
   OPEN(newunit=su, file=TRIM(stat_file), status='replace', action='write')
 
@@ -437,11 +463,22 @@ PROGRAM mcf_sdpd
   step = step_start
   DO WHILE (step < step_start + nsteps)
      IF (fused) THEN
-        k = MIN(stat_freq - MOD(step - step_start, stat_freq), step_start + nsteps - step)
+        ! whole steps on the device up to the next output of any kind: the
+        ! host copies data only when it writes a file
+        next_out = step_start + nsteps
+        next_out = MIN(next_out, next_multiple(step, stat_freq))
+        IF (out_freq > 0) next_out = MIN(next_out, next_multiple(step, out_freq))
+        IF (restart_freq > 0) next_out = MIN(next_out, next_multiple(step, restart_freq))
+        IF (conf_freq > 0 .AND. .NOT. newtonian) next_out = MIN(next_out, next_multiple(step, conf_freq))
+        k = next_out - step
         CALL check(sdpd_step(ctx, INT(k, c_int32_t)), 'sdpd_step')
+        DO d = 1, k
+           time_cur = time_cur + prm%dt  ! marching_marching.F90:1393
+        END DO
         step = step + k
      ELSE
         step = step + 1
+        time_cur = time_cur + prm%dt
         CALL check(sdpd_set_step(ctx, INT(step, c_int64_t)), 'sdpd_set_step')
         CALL vv_step()
      END IF
@@ -449,12 +486,20 @@ PROGRAM mcf_sdpd
      IF (out_freq > 0) THEN
         IF (MOD(step - step_start, out_freq) == 0) CALL write_particles(step)
      END IF
+     IF (conf_freq > 0 .AND. .NOT. newtonian) THEN
+        IF (MOD(step - step_start, conf_freq) == 0) CALL write_conformation(step, .FALSE.)
+     END IF
+     IF (restart_freq > 0) THEN
+        IF (MOD(step - step_start, restart_freq) == 0) CALL write_restart(step)
+     END IF
   END DO
   CALL check(sdpd_sync(ctx), 'sdpd_sync')
   CALL CPU_TIME(t1)
   CALL check(sdpd_get_step(ctx, cur_step), 'sdpd_get_step')
   CLOSE(su)
+  ! the particles of the last step (the reference writes them as a restart)
   CALL write_particles(step)
+  CALL write_restart(step)
   WRITE(*, '(A,I0,A,ES12.4,A)') 'mcf_sdpd: reached step ', cur_step, ' in ', t1 - t0, ' s of host time'
   rc32 = sdpd_destroy(ctx)
 
@@ -472,28 +517,139 @@ CONTAINS
     CALL check(sdpd_kick(ctx, 0.5d0), 'sdpd_kick')                    ! :1619
   END SUBROUTINE vv_step
 
+  INTEGER FUNCTION next_multiple(st, freq)
+    INTEGER, INTENT(IN) :: st, freq
+    next_multiple = step_start + ((st - step_start) / freq + 1) * freq
+  END FUNCTION next_multiple
+
+  LOGICAL FUNCTION is_unformatted(fmt)
+    CHARACTER(len=*), INTENT(IN) :: fmt
+    CHARACTER(len=VAL_LEN) :: f
+    f = lower(ADJUSTL(fmt))
+    is_unformatted = f(1:1) == 'u' .OR. f(1:1) == 'b'
+  END FUNCTION is_unformatted
+
+  ! io_write_statistic.F90:224-254: (I9, k E16.8) of step, time, kinetic
+  ! energy and momentum
   SUBROUTINE write_statistic(st)
     INTEGER, INTENT(IN) :: st
+    CHARACTER(len=32) :: form
     CALL check(sdpd_stats(ctx, mom, ekin), 'sdpd_stats')
-    WRITE(su, '(I10,1X,ES16.8,1X,ES16.8,3(1X,ES16.8))') st, st * prm%dt, ekin, mom(1:dim)
+    WRITE(form, '(A4,I1,A6)') '(I9,', dim + 2, 'E16.8)'
+    WRITE(su, form) st, time_cur, ekin, mom(1:dim)
   END SUBROUTINE write_statistic
 
-  ! Particle file in the read_external layout (x v rho m pid sid).
+  ! Records of a file: FORMATTED, one (k E16.8) record per particle
+  ! (ppm_io_write_ascii.F90, list-directed count first when asked);
+  ! UNFORMATTED, the value count then all values as two binary records
+  ! (ppm_io_write_binary.F90), the count as its own pair first when asked.
+  SUBROUTINE write_records(path, dat, count_first, unformatted)
+    CHARACTER(len=*), INTENT(IN) :: path
+    DOUBLE PRECISION, INTENT(IN) :: dat(:,:)
+    LOGICAL, INTENT(IN) :: count_first, unformatted
+    CHARACTER(len=32) :: form
+    INTEGER :: u, i, ncol
+    ncol = SIZE(dat, 1)
+    IF (unformatted) THEN
+       OPEN(newunit=u, file=path, status='replace', action='write', form='unformatted', &
+            access='sequential')
+       IF (count_first) THEN
+          WRITE(u) 1
+          WRITE(u) SIZE(dat, 2)
+       END IF
+       WRITE(u) SIZE(dat)
+       WRITE(u) dat
+    ELSE
+       OPEN(newunit=u, file=path, status='replace', action='write')
+       IF (count_first) WRITE(u, *) SIZE(dat, 2)
+       WRITE(form, '(A1,I2,A6)') '(', ncol, 'E16.8)'
+       DO i = 1, SIZE(dat, 2)
+          WRITE(u, form) dat(:, i)
+       END DO
+    END IF
+    CLOSE(u)
+  END SUBROUTINE write_records
+
+  ! x v rho m pid sid per particle (io_write_particles.F90:220-431)
+  SUBROUTINE particle_records(dat)
+    DOUBLE PRECISION, INTENT(OUT) :: dat(:,:)
+    INTEGER :: i
+    CALL check(sdpd_download(ctx, c_loc(x), c_loc(v), c_loc(rho), c_null_ptr, c_null_ptr), &
+         'sdpd_download')
+    DO i = 1, n
+       dat(1:dim, i) = x(:, i)
+       dat(dim + 1:2 * dim, i) = v(:, i)
+       dat(2 * dim + 1, i) = rho(i)
+       dat(2 * dim + 2, i) = m(i)
+       dat(2 * dim + 3, i) = DBLE(id(1, i))
+       dat(2 * dim + 4, i) = DBLE(id(2, i))
+    END DO
+  END SUBROUTINE particle_records
+
+  ! <output_particles_file>[_r]<step>.out (io_write_particles.F90:224-232)
   SUBROUTINE write_particles(st)
     INTEGER, INTENT(IN) :: st
     CHARACTER(len=VAL_LEN+16) :: name
-    INTEGER :: u, i
-    CALL check(sdpd_download(ctx, c_loc(x), c_loc(v), c_loc(rho), c_null_ptr, c_null_ptr), &
-         'sdpd_download')
-    WRITE(name, '(A,I8.8,A)') TRIM(out_file), st, '.dat'
-    OPEN(newunit=u, file=TRIM(name), status='replace', action='write')
-    WRITE(u, '(I0)') n
+    DOUBLE PRECISION, ALLOCATABLE :: dat(:,:)
+    ALLOCATE(dat(2 * dim + 4, n))
+    CALL particle_records(dat)
+    IF (read_external) THEN
+       WRITE(name, '(2A,I8.8,A)') TRIM(out_file), '_r', st, '.out'
+    ELSE
+       WRITE(name, '(A,I8.8,A)') TRIM(out_file), st, '.out'
+    END IF
+    CALL write_records(TRIM(name), dat, .FALSE., out_unf)
+  END SUBROUTINE write_particles
+
+  ! <restart_particles_file><step>.dat: the count, then the particle
+  ! records (io_write_restart_particles.F90:105-215), and the restart
+  ! conformation of Oldroyd-B runs
+  SUBROUTINE write_restart(st)
+    INTEGER, INTENT(IN) :: st
+    CHARACTER(len=VAL_LEN+16) :: name
+    DOUBLE PRECISION, ALLOCATABLE :: dat(:,:)
+    ALLOCATE(dat(2 * dim + 4, n))
+    CALL particle_records(dat)
+    WRITE(name, '(A,I8.8,A)') TRIM(restart_file), st, '.dat'
+    CALL write_records(TRIM(name), dat, .TRUE., restart_unf)
+    IF (.NOT. newtonian) CALL write_conformation(st, .TRUE.)
+  END SUBROUTINE write_restart
+
+  ! io_write_conformation.F90:175-245 (records of dim*dim values), or the
+  ! restart conformation (count first, io_write_restart_conformation.F90)
+  SUBROUTINE write_conformation(st, restart)
+    INTEGER, INTENT(IN) :: st
+    LOGICAL, INTENT(IN) :: restart
+    CHARACTER(len=VAL_LEN+16) :: name
+    CALL check(sdpd_download_tensors(ctx, c_loc(ct), c_null_ptr), 'sdpd_download_tensors')
+    IF (restart) THEN
+       WRITE(name, '(A,I8.8,A)') TRIM(restart_conf_file), st, '.dat'
+       CALL write_records(TRIM(name), ct, .TRUE., restart_unf)
+    ELSE
+       IF (read_external) THEN
+          WRITE(name, '(2A,I8.8,A)') TRIM(conf_out_file), '_r', st, '.out'
+       ELSE
+          WRITE(name, '(A,I8.8,A)') TRIM(conf_out_file), st, '.out'
+       END IF
+       CALL write_records(TRIM(name), ct, .FALSE., conf_unf)
+    END IF
+  END SUBROUTINE write_conformation
+
+  ! io_read_conformation.F90: the count, then dim*dim values per particle
+  SUBROUTINE read_conformation(path)
+    CHARACTER(len=*), INTENT(IN) :: path
+    INTEGER :: u, i, ios, nc
+    OPEN(newunit=u, file=path, status='old', action='read', iostat=ios)
+    IF (ios /= 0) CALL die('cannot open conformation file '//path)
+    READ(u, *, iostat=ios) nc
+    IF (ios /= 0) CALL die('reading conformation tensor has problem at line 1')
+    IF (nc /= n) CALL die('actual number of lines is not equal to number given !')
     DO i = 1, n
-       WRITE(u, '(*(ES25.17,1X))', advance='no') x(:, i), v(:, i), rho(i), m(i)
-       WRITE(u, '(I0,1X,I0)') id(1, i), id(2, i)
+       READ(u, *, iostat=ios) ct(:, i)
+       IF (ios /= 0) CALL die('actual number of lines is not equal to number given !')
     END DO
     CLOSE(u)
-  END SUBROUTINE write_particles
+  END SUBROUTINE read_conformation
 
   SUBROUTINE read_particles(path)
     CHARACTER(len=*), INTENT(IN) :: path

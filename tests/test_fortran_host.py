@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from mcf_amd import build as B
+from mcf_amd import io as IO
 from mcf_amd import workloads as W
 from mcf_amd.sdpd import SDPD
 
@@ -21,7 +22,8 @@ def announce(msg):
     print(msg, flush=True)
 
 
-def write_configs(tmp, box, nsteps, fused=False, precision="fp64", stat_freq=2, extra=""):
+def write_configs(tmp, box, nsteps, fused=False, precision="fp64", stat_freq=2, extra="",
+                  out_fmt="UNFORMATTED", io_extra=""):
     prm = box.prm
     d = prm.num_dim
     W.write_external(os.path.join(tmp, "particles.dat"), box)
@@ -63,7 +65,9 @@ kt = {prm.kt!r}
 statistic_file = stat.dat
 statistic_freq_step = {stat_freq}
 output_particles_file = parts
-""")
+output_particles_fmt = {out_fmt}
+restart_particles_file = restart
+{io_extra}""")
     return os.path.join(tmp, "ctrl.mcf")
 
 
@@ -95,11 +99,14 @@ def test_host_rejects_other_integrators(tmp_path):
 
 
 def read_parts(path, d):
-    with open(path) as fh:
-        n = int(fh.readline())
-        data = np.loadtxt(fh, ndmin=2)
-    assert data.shape[0] == n
+    """Binary particle output of the host (full precision): x, v."""
+    _, data = IO.read_binary_records(path, 2 * d + 4)
     return data[:, :d], data[:, d:2 * d]
+
+
+def parts_name(nsteps):
+    # read_external runs name their outputs <file>_r<step>.out
+    return f"parts_r{nsteps:08d}.out"
 
 
 @pytest.mark.gpu
@@ -112,17 +119,25 @@ def test_host_matches_python_binding(tmp_path, fused):
     ctrl = write_configs(str(tmp_path), box, nsteps, fused=fused)
     r = run_host(ctrl, str(tmp_path))
     assert r.returncode == 0, r.stdout + r.stderr
-    x, v = read_parts(str(tmp_path / f"parts{nsteps:08d}.dat"), 3)
+    x, v = read_parts(str(tmp_path / parts_name(nsteps)), 3)
     with SDPD(box.prm) as g:
         g.upload(box.x, box.v, box.m, box.pid)
         g.integrate_VV(nsteps)
         xr, vr = g.download()
         ek, mom = g.compute_statistic()
-    # the particle file carries 18 significant digits: doubles round-trip
+    # binary particle output: doubles round-trip
     assert np.array_equal(x, xr) and np.array_equal(v, vr)
-    stat = np.loadtxt(str(tmp_path / "stat.dat"), ndmin=2)
+    # statistic records (I9, k E16.8) of step, time, kinetic energy, momentum
+    lines = open(tmp_path / "stat.dat").read().splitlines()
+    t = 0.0
+    for _ in range(nsteps):
+        t = t + box.prm.dt
+    assert lines[-1] == IO.statistic_record(nsteps, t, ek, mom)
+    stat = IO.read_statistic(str(tmp_path / "stat.dat"))
     assert list(stat[:, 0]) == [0, 2, 4]
-    assert stat[-1, 2] == pytest.approx(ek, rel=1e-7)
+    # the restart file of the last step reads back as a read_external file
+    xs, vs, _, _, pid, sid = IO.read_particles(str(tmp_path / f"restart{nsteps:08d}.dat"), 3)
+    assert np.allclose(xs, xr, rtol=5.1e-8) and np.array_equal(pid, box.pid)
 
 
 def shear_v_line(d, le_dim, v_lo, v_hi):
@@ -172,7 +187,7 @@ def test_host_lees_edwards_matches_python_binding(tmp_path, le_dim, fused):
     ctrl = write_configs(str(tmp_path), box, nsteps, fused=fused, extra=le_extra(3, le_dim, v_lo, v_hi))
     r = run_host(ctrl, str(tmp_path))
     assert r.returncode == 0, r.stdout + r.stderr
-    x, vh = read_parts(str(tmp_path / f"parts{nsteps:08d}.dat"), 3)
+    x, vh = read_parts(str(tmp_path / parts_name(nsteps)), 3)
     with SDPD(box.prm.replace(le_dim=le_dim, le_v_lower=v_lo, le_v_upper=v_hi)) as g:
         g.upload(box.x, box.v, box.m, box.pid)
         g.integrate_VV(nsteps)
@@ -210,10 +225,50 @@ def test_host_colloids_match_python_binding(tmp_path, fused):
     ctrl = write_configs(str(tmp_path), box, nsteps, fused=fused, extra=colloid_extra(cs, 3))
     r = run_host(ctrl, str(tmp_path))
     assert r.returncode == 0, r.stdout + r.stderr
-    x, vh = read_parts(str(tmp_path / f"parts{nsteps:08d}.dat"), 3)
+    x, vh = read_parts(str(tmp_path / parts_name(nsteps)), 3)
     with SDPD(box.prm) as g:
         g.set_colloids(cs)
         g.upload(box.x, box.v, box.m, box.pid, box.sid)
         g.integrate_VV(nsteps)
         xr, vr = g.download()
     assert np.array_equal(x, xr) and np.array_equal(vh, vr)
+
+
+@pytest.mark.gpu
+@needs_host
+def test_host_formatted_outputs(tmp_path):
+    # FORMATTED particle output: (k E16.8) records identical to the Python
+    # writer's on the same (bitwise equal) state; Oldroyd-B conformation
+    # read from a file and written back
+    box = W.oldroyd_box_3d(10, kt=1e-3)
+    nsteps = 3
+    rng = np.random.default_rng(2)
+    ct = np.tile(np.eye(3).reshape(-1), (box.n, 1)) + 0.01 * rng.normal(size=(box.n, 9))
+    IO.write_conformation(str(tmp_path / "conf_in.dat"), ct, count_line=True)
+    ct_in = IO.read_conformation(str(tmp_path / "conf_in.dat"), 3)
+    announce("Fortran host formatted outputs (Oldroyd-B)")
+    extra = f"tau = {box.prm.tau!r}\nn_p = {box.prm.n_p!r}\nkt_p = {box.prm.kt_p!r}\n"
+    ctrl = write_configs(str(tmp_path), box, nsteps, extra=extra, out_fmt="FORMATTED",
+                         io_extra="read_conformation_file = conf_in.dat\noutput_particles_freq_step = 1\n"
+                                  "output_conformation_file = conf\noutput_conformation_freq_step = 3\n"
+                                  "restart_conformation_file = rconf\n")
+    with open(ctrl, "a") as fh:
+        fh.write("Newtonian = F\n")
+    r = run_host(ctrl, str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.upload_ct(ct_in)
+        g.integrate_VV(nsteps)
+        xr, vr = g.download()
+        rho, _ = g.download_density()
+        ctr, _ = g.download_tensors()
+    ref = tmp_path / "ref.out"
+    IO.write_particles(str(ref), xr, vr, rho, box.m, box.pid, box.sid)
+    assert open(tmp_path / f"parts_r{nsteps:08d}.out").read() == open(ref).read()
+    assert (tmp_path / "parts_r00000001.out").exists()
+    refc = tmp_path / "refc.out"
+    IO.write_conformation(str(refc), ctr)
+    assert open(tmp_path / f"conf_r{nsteps:08d}.out").read() == open(refc).read()
+    assert np.allclose(IO.read_conformation(str(tmp_path / f"rconf{nsteps:08d}.dat"), 3), ctr,
+                       rtol=5.1e-8, atol=1e-300)

@@ -208,12 +208,6 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
 // sqrt(x) for x >= 0 through rsqrt_nr (0 for x = 0).
 __device__ __forceinline__ double sqrt_nr(double x) { return x > 0.0 ? x * rsqrt_nr(x) : 0.0; }
 
-// Periodic image of a neighbour in virtual cell v (outside 0..nm-1 across
-// a face): the amount added to x_i - x_j.
-__device__ __forceinline__ double image_shift(int v, int nm, double L) {
-  return v < 0 ? L : (v >= nm ? -L : 0.0);
-}
-
 // ---------------------------------------------------------------------
 // equation of state (stateEquation_compute_pressure.F90:158,176); rho is
 // the mass density (number density times mass for density type 2)
@@ -241,8 +235,9 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
     }
     uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
     uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
-    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    // three-input xor in one instruction (truth table 0x96)
+    uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, k0, 0x96);
+    uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
     c0 = n0;
     c1 = (uint32_t)p1;
     c2 = n2;
@@ -263,14 +258,20 @@ __device__ __forceinline__ float noise_log(float u) {
     m = m * 0.5f;
     e = e + 1;
   }
-  float t = (m - 1.0f) / (m + 1.0f);
-  float s = t * t;
-  float q = fmaf(s, 0.111111111f, 0.142857143f);
-  q = fmaf(s, q, 0.2f);
-  q = fmaf(s, q, 0.333333333f);
-  q = fmaf(s, q, 1.0f);
-  float lnm = (2.0f * t) * q;
-  return fmaf((float)e, 0.693147182f, lnm);
+  // ln m = x P(x), x = m - 1 (exact), P of degree 9 fitted to ln(1+x)/x
+  // on [1/sqrt2 - 1, sqrt2 - 1] for relative error (coefficients rounded
+  // to float); no division: within 2 ulp of ln over every mantissa
+  const float x = m - 1.0f;
+  float p = fmaf(x, -0x1.38158ep-4f, 0x1.08442ap-3f);
+  p = fmaf(x, p, -0x1.0f3b4ep-3f);
+  p = fmaf(x, p, 0x1.2272a8p-3f);
+  p = fmaf(x, p, -0x1.542702p-3f);
+  p = fmaf(x, p, 0x1.99a3dcp-3f);
+  p = fmaf(x, p, -0x1.000428p-2f);
+  p = fmaf(x, p, 0x1.55555p-2f);
+  p = fmaf(x, p, -0x1.fffff8p-2f);
+  p = fmaf(x, p, 1.0f);
+  return fmaf((float)e, 0.693147182f, x * p);
 }
 
 __device__ __forceinline__ void noise_sincos(uint32_t k, float& cs, float& sn) {
@@ -288,11 +289,27 @@ __device__ __forceinline__ void noise_sincos(uint32_t k, float& cs, float& sn) {
   pc = fmaf(s, pc, 4.16666667e-2f);
   pc = fmaf(s, pc, -0.5f);
   float cv = fmaf(s, pc, 1.0f);
-  uint32_t q = qn & 3u;
-  float c_out = (q == 0) ? cv : (q == 1) ? -sv : (q == 2) ? -cv : sv;
-  float s_out = (q == 0) ? sv : (q == 1) ? cv : (q == 2) ? -sv : -cv;
-  cs = c_out;
-  sn = s_out;
+  // quadrant q: (cos, sin) = (cv, sv), (-sv, cv), (-cv, -sv), (sv, -cv);
+  // odd quadrants swap the two, the signs flip by xor of the sign bit
+  // (the same bits as negation), without branches
+  const uint32_t q = qn & 3u;
+  const bool swap = (q & 1u) != 0u;
+  cs = __uint_as_float(__float_as_uint(swap ? sv : cv) ^ (((q + 1u) & 2u) << 30));
+  sn = __uint_as_float(__float_as_uint(swap ? cv : sv) ^ ((q & 2u) << 30));
+}
+
+// sqrt(y) rounded to nearest, for y zero or a positive normal number (the
+// Box-Muller radius): the hardware estimate, within an ulp, replaced by its
+// neighbour below when y - s_down s <= 0 or above when y - s_up s > 0 --
+// the correction IEEE sqrtf applies, without the rescaling of tiny inputs
+// and the infinity / NaN cases the radius never meets.
+__device__ __forceinline__ float sqrt_rn_normal(float y) {
+  const float s = __builtin_amdgcn_sqrtf(y);
+  const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+  const float su = __uint_as_float(__float_as_uint(s) + 1u);
+  float r = fmaf(-sd, s, y) <= 0.0f ? sd : s;
+  r = fmaf(-su, s, y) > 0.0f ? su : r;
+  return y > 0.0f ? r : y;
 }
 
 // Four N(0,1) deviates for the unordered pair (lo, hi) at a step.
@@ -305,7 +322,7 @@ __device__ __forceinline__ void pair_normals(uint64_t seed, uint32_t lo, uint32_
 #pragma unroll
   for (int hh = 0; hh < 2; ++hh) {
     float u = (float)((w[2 * hh] >> 8) + 1u) * 0x1p-24f;
-    float r = sqrtf(-2.0f * noise_log(u));
+    float r = sqrt_rn_normal(-2.0f * noise_log(u));
     float cs, sn;
     noise_sincos(w[2 * hh + 1] >> 8, cs, sn);
     g[2 * hh] = r * cs;

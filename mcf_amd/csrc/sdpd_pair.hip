@@ -61,6 +61,26 @@ __device__ __forceinline__ Offsets decode_code(uint32_t code) {
   return o;
 }
 
+// FP64 lists store the neighbour's periodic image instead of its stencil
+// offset: bits 25-26, 27-28 and 29-30 hold, per dimension, the image
+// shift of x_i - x_j in box lengths as a 2-bit signed field (+1 for a
+// virtual cell below 0, -1 for one at or above nm, else 0).  The FP32
+// path needs the offsets (cell-relative positions) and keeps the codes.
+struct Images {
+  int sx, sy, sz;
+};
+__device__ __forceinline__ int image_class(int v, int nm) { return v < 0 ? 1 : (v >= nm ? -1 : 0); }
+__device__ __forceinline__ uint32_t image_field(int v, int nm, int dim) {
+  return ((uint32_t)image_class(v, nm) & 3u) << (kCodeShift + 2 * dim);
+}
+__device__ __forceinline__ Images decode_images(uint32_t ent) {
+  Images m;
+  m.sx = (int)(ent << 5) >> 30;
+  m.sy = (int)(ent << 3) >> 30;
+  m.sz = (int)(ent << 1) >> 30;
+  return m;
+}
+
 // Geometry of the replica a chunk belongs to (uniform across the wave).
 struct Geo {
   int nm0, nm1, nm2;    // sort-grid cells per dimension
@@ -118,6 +138,12 @@ __device__ __forceinline__ int le_sigma(const DevParams& P, const Home& h, int d
     return c >= h.G.nm2 ? 0 : (c < 0 ? 1 : -1);
   }
   return -1;
+}
+// The same sign from a listed neighbour's images: the image below the
+// upper face (shift -1) is the upper copy.
+__device__ __forceinline__ int le_sigma(const DevParams& P, const Images& m) {
+  const int c = P.le_g == 1 ? m.sy : (P.le_g == 2 ? m.sz : 0);
+  return c < 0 ? 0 : (c > 0 ? 1 : -1);
 }
 
 // Image modes of a neighbour at offsets o from the home cell.
@@ -390,8 +416,7 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
   auto flush = [&](bool final) {
     for (int q = ev; __ballot(q < cnt) != 0ull; ++q) {
       if (q < cnt) {
-        const uint32_t ent = buf[q * kWave + lane];
-        pol.body((int)(ent & kIndexMask), decode_code(ent >> kCodeShift));
+        pol.body(buf[q * kWave + lane]);
       }
     }
     const int w = final ? cnt : (cnt & ~7);
@@ -429,16 +454,21 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
     const int tot = rp.tot;
     const int v_lo = rp.vlo;
     // Positions in the tile are relative to the corner of virtual cell
-    // v_lo in x (and to the candidate's own cell in y, z); the tile's .w
-    // holds the list entry with the virtual cell (mod 128) in place of the
-    // code, so that entry + kadd is the entry with the stencil code (the
-    // offset from the lane's window centre).  Across a Lees-Edwards face
-    // the window centre and the lane's x come from the sheared frame.
+    // v_lo in x (and to the candidate's own cell in y, z).  With stencil
+    // codes the tile's .w holds the list entry with the virtual cell (mod
+    // 128) in place of the code, so that entry + kadd is the entry with
+    // the code (the offset from the lane's window centre); with images it
+    // holds the entry with the x image, and the row adds the y and z ones.
+    // Across a Lees-Edwards face the window centre and the lane's x come
+    // from the sheared frame.
     const bool le_row = rp.sigma >= 0;
     const int wc = le_row ? h.le.wc(rp.sigma) : h.cx;
     const int crel = wc - v_lo;
     const float ax = (le_row ? h.a.x + h.le.dxf(rp.sigma) : h.a.x) + (float)crel * h.G.cs0;
-    const uint32_t self_w = le_row ? 0xFFFFFFFFu : ((uint32_t)h.ii | ((uint32_t)crel << kCodeShift));
+    // the lane's own copy: same particle, own virtual cell (x image 0)
+    const uint32_t self_w =
+        le_row ? 0xFFFFFFFFu
+               : (Pol::kImages ? (uint32_t)h.ii : ((uint32_t)h.ii | ((uint32_t)crel << kCodeShift)));
     // this lane's window: virtual cells wc-2 .. wc+2 as a range of the
     // row sequence
     int tb = 0, te = 0;
@@ -448,7 +478,10 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
     }
     const float ay = h.a.y - (float)dy * h.G.cs1;
     const float az = DIM == 3 ? h.a.z - (float)dz * h.G.cs2 : 0.f;
-    const uint32_t kadd = (uint32_t)(2 - crel + 5 * (dy + 2) + 25 * (dz + 2)) << kCodeShift;
+    const uint32_t kadd =
+        Pol::kImages
+            ? image_field(h.cy + dy, h.G.nm1, 1) | (DIM == 3 ? image_field(h.cz + dz, h.G.nm2, 2) : 0u)
+            : (uint32_t)(2 - crel + 5 * (dy + 2) + 25 * (dz + 2)) << kCodeShift;
     // the row in batches of at most TILE records (one batch in practice)
     for (int base = 0; base < tot; base += TILE) {
       const int bend = min(tot, base + TILE);
@@ -456,9 +489,11 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
         const int piece = t < rp.len0 ? 0 : (t < rp.len0 + rp.len1 ? 1 : 2);
         const int g = seq_particle(rp, t);
         const float4 q = S.xa[g];
-        const int vrel = __float_as_int(q.w) + (rp.m0 + piece) * nm0 - v_lo;
+        const int v = __float_as_int(q.w) + (rp.m0 + piece) * nm0;
+        const int vrel = v - v_lo;
+        const uint32_t code = Pol::kImages ? image_field(v, nm0, 0) : (uint32_t)vrel << kCodeShift;
         tile[t - base] = make_float4(fmaf((float)vrel, h.G.cs0, q.x), q.y, q.z,
-                                     __uint_as_float((uint32_t)g | ((uint32_t)vrel << kCodeShift)));
+                                     __uint_as_float((uint32_t)g | code));
       }
       __syncthreads();
       const int lb = max(tb, base) - base, le = min(te, bend) - base;
@@ -487,10 +522,10 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
             if (le_row) {
               in = exact_d2_le<DIM>(P, S.x, j, h, rp.sigma, my, mz) < P.cut2;
             } else {
-              const Offsets o = decode_code(ent >> kCodeShift);
-              const int v = h.cx + o.ox;
+              const int cls = Pol::kImages ? decode_images(ent).sx
+                                           : image_class(h.cx + decode_code(ent >> kCodeShift).ox, nm0);
               Modes m;
-              m.m0 = v < 0 ? low_mode : (v > nm0 - 1 ? 1 : 0);
+              m.m0 = cls > 0 ? low_mode : (cls < 0 ? 1 : 0);
               m.m1 = my;
               m.m2 = mz;
               double e0, e1, e2;
@@ -528,9 +563,9 @@ struct KahanF {
 // Velocity of a neighbour's Lees-Edwards copy relative to its own: + the
 // face velocity jump across the upper face, - across the lower one
 // (particles_set_boundary_ghost_velocity.F90:175-240).
-__device__ __forceinline__ double le_dv(const DevParams& P, const Home& h, const Offsets& o) {
+__device__ __forceinline__ double le_dv(const DevParams& P, const Images& m) {
   if (P.le_g < 0) return 0.0;
-  const int sg = le_sigma(P, h, o.oy, o.oz);
+  const int sg = le_sigma(P, m);
   return sg == 0 ? P.le_u : (sg == 1 ? -P.le_u : 0.0);
 }
 __device__ __forceinline__ float le_dvf(const DevParams& P, const Home& h, const Offsets& o) {
@@ -540,25 +575,26 @@ __device__ __forceinline__ float le_dvf(const DevParams& P, const Home& h, const
 }
 
 // FP64 separation of a listed pair: x_i - x_j plus the periodic image
-// shift of the neighbour's virtual cell.  The in/out decision was taken
-// with the reference's own formula; the value here differs from it by at
-// most an ulp of the box length.
+// shift of the neighbour's virtual cell, s L with s in {-1, 0, 1} (the fma
+// rounds once, as the sum with the shift does).  The in/out decision was
+// taken with the reference's own formula; the value here differs from it
+// by at most an ulp of the box length.
 template <int DIM>
-__device__ __forceinline__ double pair_d2(const DevParams& P, const Home& h, const Offsets& o,
+__device__ __forceinline__ double pair_d2(const DevParams& P, const Home& h, const Images& m,
                                           double xj0, double xj1, double xj2, double& d0,
                                           double& d1, double& d2) {
-  d0 = (h.x0 - xj0) + image_shift(h.cx + o.ox, h.G.nm0, h.G.L0);
+  d0 = fma((double)m.sx, h.G.L0, h.x0 - xj0);
   if (P.le_g >= 0) {
     // across a Lees-Edwards face: the sheared copy, nearest periodic image
-    const int sg = le_sigma(P, h, o.oy, o.oz);
+    const int sg = le_sigma(P, m);
     if (sg >= 0) {
       // x_i - x_j - S' lies in (-2 Lx, Lx): nearest period
       const double raw = (h.x0 - xj0) - h.le.sp(sg);
       d0 = raw - h.G.L0 * rint(raw / h.G.L0);
     }
   }
-  d1 = (h.x1 - xj1) + image_shift(h.cy + o.oy, h.G.nm1, h.G.L1);
-  d2 = DIM == 3 ? (h.x2 - xj2) + image_shift(h.cz + o.oz, h.G.nm2, h.G.L2) : 0.0;
+  d1 = fma((double)m.sy, h.G.L1, h.x1 - xj1);
+  d2 = DIM == 3 ? fma((double)m.sz, h.G.L2, h.x2 - xj2) : 0.0;
   return DIM == 3 ? fma(d2, d2, fma(d1, d1, d0 * d0)) : fma(d1, d1, d0 * d0);
 }
 
@@ -568,11 +604,13 @@ struct DensityF64 {
   const DeviceState& S;
   const Home& h;
   double acc;
-  __device__ __forceinline__ void body(int j, const Offsets& o) {
+  static constexpr bool kImages = true;
+  __device__ __forceinline__ void body(uint32_t ent) {
+    const int j = (int)(ent & kIndexMask);
     const int64_t n = P.n;
     double d0, d1, d2;
-    const double r2 = pair_d2<DIM>(P, h, o, S.x[j], S.x[n + j], DIM == 3 ? S.x[2 * n + j] : 0.0,
-                                   d0, d1, d2);
+    const double r2 = pair_d2<DIM>(P, h, decode_images(ent), S.x[j], S.x[n + j],
+                                   DIM == 3 ? S.x[2 * n + j] : 0.0, d0, d1, d2);
     double w, gw;
     kernel_eval(P, sqrt_nr(r2) * P.inv_h, w, gw);
     acc = acc + (P.density_type == 1 ? S.m[j] * w : w);
@@ -585,7 +623,10 @@ struct DensityMixed {
   const DeviceState& S;
   const Home& h;
   KahanF acc;
-  __device__ __forceinline__ void body(int j, const Offsets& o) {
+  static constexpr bool kImages = false;
+  __device__ __forceinline__ void body(uint32_t ent) {
+    const int j = (int)(ent & kIndexMask);
+    const Offsets o = decode_code(ent >> kCodeShift);
     float d0, d1, d2;
     rel_diff<DIM>(P, h, S.xa[j], o, d0, d1, d2);
     const float r = __builtin_amdgcn_sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
@@ -746,7 +787,10 @@ struct ForceF64 {
   int sidi = 0;
   double rxc_i[3] = {0.0, 0.0, 0.0}, vcoll_i[3] = {0.0, 0.0, 0.0};
   double le_t = 0.0;
-  __device__ __forceinline__ void body(int j, const Offsets& o) {
+  static constexpr bool kImages = true;
+  __device__ __forceinline__ void body(uint32_t ent) {
+    const int j = (int)(ent & kIndexMask);
+    const Images o = decode_images(ent);
     int sj = 0;
     if (COLL) {
       // two boundary particles do not interact (pp_interaction.inc: same
@@ -763,8 +807,10 @@ struct ForceF64 {
     const double r = r2 * ir;
     double w, gw;
     kernel_eval(P, r * P.inv_h, w, gw);
-    const double jump = le_dv(P, h, o);  // x velocity of j's image minus j's own
-    double u0 = vi0 - (rb.x + jump), u1 = vi1 - rb.y, u2 = DIM == 3 ? vi2 - rb.z : 0.0;
+    const double jump = le_dv(P, o);  // x velocity of j's image minus j's own
+    // (without sheared faces the jump is the constant 0 and no add is made)
+    double u0 = vi0 - (P.le_g >= 0 ? rb.x + jump : rb.x), u1 = vi1 - rb.y,
+           u2 = DIM == 3 ? vi2 - rb.z : 0.0;
     const double bj = rb.w;
     // pressure terms: p_i / n_i^2 + p_j / n_j^2 (type 3), (p_i + p_j) / 2 (type 4)
     double psum = 0.0, pij = 0.0;
@@ -796,10 +842,11 @@ struct ForceF64 {
       a1 = sc * d1;
       a2 = sc * d2;
     } else {
-      const double fd = P.eta * (bi + bj) * gw * ir;
+      const double gir = gw * ir;
+      const double fd = P.eta * (bi + bj) * gir;
       if (FT == 3) {
         // rhs_force_ff_Newtonian_HuAdams.F90:140-237
-        const double fcr = psum * gw * ir;
+        const double fcr = psum * gir;
         a0 = fd * u0 - fcr * d0;
         a1 = fd * u1 - fcr * d1;
         a2 = fd * u2 - fcr * d2;
@@ -824,7 +871,6 @@ struct ForceF64 {
             if (a == jj) ptj = ptj + pj + P.G;
             fc[jj] = fc[jj] + (pti[a + DIM * jj] * bi + ptj * bj) * dx[a];
           }
-        const double gir = gw * ir;
         a0 = fd * u0 - fc[0] * gir;
         a1 = fd * u1 - fc[1] * gir;
         a2 = DIM == 3 ? fd * u2 - fc[2] * gir : 0.0;
@@ -835,7 +881,8 @@ struct ForceF64 {
         const uint32_t pidj = (uint32_t)S.pid[j];
         float g[4];
         pair_normals(P.seed, pidi < pidj ? pidi : pidj, pidi < pidj ? pidj : pidi, step, g);
-        const double fr = sqrt_nr(-4.0 * P.kt * fd) * P.inv_sqrt_dt;
+        // sqrt(-4 kT f_d) / sqrt(dt) as one root (the factor is uniform)
+        const double fr = sqrt_nr((-4.0 * P.kt / P.dt) * fd);
         const double sfr = pidi < pidj ? fr : -fr;
         const double ca = 0.70710678118654752440;
         const double g0 = (double)g[0], g1 = (double)g[1];
@@ -885,7 +932,7 @@ struct ForceF64 {
     } else {
       const CollHot C = S.coll_hot[sidi - 1];
       const double rxf[3] = {rxc_i[0] - d0, rxc_i[1] - d1, DIM == 3 ? rxc_i[2] - d2 : 0.0};
-      const double vf[3] = {rb.x + jump, rb.y, DIM == 3 ? rb.z : 0.0};
+      const double vf[3] = {P.le_g >= 0 ? rb.x + jump : rb.x, rb.y, DIM == 3 ? rb.z : 0.0};
       coll_noslip<DIM>(P, C, rxc_i, vcoll_i, rxf, vf, vc);
       u0 = vc[0] - vf[0];
       u1 = vc[1] - vf[1];
@@ -908,8 +955,10 @@ struct ForceMixed {
   float pb[6];
   float inv_ni;
   KahanF L[DIM * DIM];
-  __device__ __forceinline__ void body(int j, const Offsets& o) {
-    body_rec(j, S.xa[j], S.vr[j], S.qr[j], o);
+  static constexpr bool kImages = false;
+  __device__ __forceinline__ void body(uint32_t ent) {
+    const int j = (int)(ent & kIndexMask);
+    body_rec(j, S.xa[j], S.vr[j], S.qr[j], decode_code(ent >> kCodeShift));
   }
   // xj: cell-relative position; vj: (v, pid); qj: (p/n^2, 1/n^2, p)
   __device__ __forceinline__ void body_rec(int jidx, const float4& xj, const float4& vj,
@@ -1029,17 +1078,26 @@ __device__ __forceinline__ void for_each_listed(const DevParams& P, const Device
     for (int q = 0; __ballot(q < cnt) != 0ull; q += 8) {
       if (q < cnt) {
         // eight entries per pair of 16-byte loads; entries past cnt are
-        // padding inside the row and are not evaluated
+        // padding inside the row and are not evaluated.  The group goes to
+        // LDS (entry u of a lane at buf[u * 64 + lane], conflict-free), and
+        // the loop reads one entry per pair, the next one ahead of the body.
         const uint4* src = reinterpret_cast<const uint4*>(blk + group_offset(q, lane));
         const uint4 lo = src[0], hi = src[1];
-        uint32_t e[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+        buf[0 * kWave + lane] = lo.x;
+        buf[1 * kWave + lane] = lo.y;
+        buf[2 * kWave + lane] = lo.z;
+        buf[3 * kWave + lane] = lo.w;
+        buf[4 * kWave + lane] = hi.x;
+        buf[5 * kWave + lane] = hi.y;
+        buf[6 * kWave + lane] = hi.z;
+        buf[7 * kWave + lane] = hi.w;
         const int m = min(8, cnt - q);
+        uint32_t ent = buf[lane];
 #pragma unroll 1
         for (int u = 0; u < m; ++u) {
-          const uint32_t ent = e[0];
-#pragma unroll
-          for (int k = 0; k < 7; ++k) e[k] = e[k + 1];
-          pol.body((int)(ent & kIndexMask), decode_code(ent >> kCodeShift));
+          const uint32_t cur = ent;
+          ent = buf[((u + 1) & 7) * kWave + lane];
+          pol.body(cur);
         }
       }
     }
@@ -1169,8 +1227,9 @@ struct SignaturePol {
   const int32_t* __restrict__ pid;
   uint32_t pidi;
   uint32_t hs;
-  __device__ __forceinline__ void body(int j, const Offsets&) {
-    hs += pair_hash(pidi, (uint32_t)pid[j]);
+  static constexpr bool kImages = true;
+  __device__ __forceinline__ void body(uint32_t ent) {
+    hs += pair_hash(pidi, (uint32_t)pid[(int)(ent & kIndexMask)]);
   }
 };
 

@@ -169,6 +169,14 @@ def lib():
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.orc_pair_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, C.POINTER(C.c_float)]
         L.orc_pair_normals.restype = None
+        fp = C.POINTER(C.c_float)
+        L.orc_noise_log_n.argtypes = [fp, C.c_int64, fp]
+        L.orc_noise_log_n.restype = None
+        L.orc_noise_sincos_n.argtypes = [C.POINTER(C.c_uint32), C.c_int64, fp, fp]
+        L.orc_noise_sincos_n.restype = None
+        up = C.POINTER(C.c_uint32)
+        L.orc_pair_normals_n.argtypes = [C.c_uint64, up, up, C.c_int64, C.c_int64, fp]
+        L.orc_pair_normals_n.restype = None
         L.orc_philox4x32_10.restype = None
         state = [P, C.c_int64, C.c_int32, dp, dp, dp, ip, ip, dp, dp, dp, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_prologue.argtypes = state
@@ -368,6 +376,38 @@ def pair_normals(seed, pid_lo, pid_hi, step):
     g = (C.c_float * 4)()
     lib().orc_pair_normals(int(seed) & 0xFFFFFFFFFFFFFFFF, int(pid_lo), int(pid_hi), int(step), g)
     return np.array([g[i] for i in range(4)], np.float32)
+
+
+def pair_normals_n(seed, lo, hi, step):
+    """pair_normals over arrays of keys: (n, 4) float32."""
+    lo = np.ascontiguousarray(lo, np.uint32)
+    hi = np.ascontiguousarray(hi, np.uint32)
+    g = np.empty((lo.size, 4), np.float32)
+    up = C.POINTER(C.c_uint32)
+    lib().orc_pair_normals_n(int(seed) & 0xFFFFFFFFFFFFFFFF, lo.ctypes.data_as(up),
+                             hi.ctypes.data_as(up), int(step), lo.size,
+                             g.ctypes.data_as(C.POINTER(C.c_float)))
+    return g
+
+
+def noise_log(u):
+    """The noise stream's ln u over a float32 array (u in (0, 1])."""
+    u = np.ascontiguousarray(u, np.float32)
+    out = np.empty_like(u)
+    fp = C.POINTER(C.c_float)
+    lib().orc_noise_log_n(u.ctypes.data_as(fp), u.size, out.ctypes.data_as(fp))
+    return out
+
+
+def noise_sincos(k):
+    """The noise stream's (cos, sin) of 2 pi k / 2^24 over a uint32 array."""
+    k = np.ascontiguousarray(k, np.uint32)
+    cs = np.empty(k.shape, np.float32)
+    sn = np.empty(k.shape, np.float32)
+    fp = C.POINTER(C.c_float)
+    lib().orc_noise_sincos_n(k.ctypes.data_as(C.POINTER(C.c_uint32)), k.size,
+                             cs.ctypes.data_as(fp), sn.ctypes.data_as(fp))
+    return cs, sn
 
 
 class State:

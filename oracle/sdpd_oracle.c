@@ -738,8 +738,10 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
 static inline uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
 static inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 
-/* natural log of u in (0, 1]: exponent split plus
- * ln m = 2 atanh(t), t = (m-1)/(m+1), m in [1/sqrt2, sqrt2] (Taylor) */
+/* natural log of u in (0, 1]: exponent split, m in [1/sqrt2, sqrt2], then
+ * ln m = x P(x), x = m - 1 (exact), P of degree 9 fitted to ln(1+x)/x on
+ * [1/sqrt2 - 1, sqrt2 - 1] for relative error, coefficients rounded to
+ * float (Horner with fmaf, no division) */
 static float noise_log(float u) {
   uint32_t b = f2u(u);
   int e = (int)(b >> 23) - 127;
@@ -748,14 +750,17 @@ static float noise_log(float u) {
     m = m * 0.5f;
     e = e + 1;
   }
-  float t = (m - 1.0f) / (m + 1.0f);
-  float s = t * t;
-  float q = fmaf(s, 0.111111111f, 0.142857143f);
-  q = fmaf(s, q, 0.2f);
-  q = fmaf(s, q, 0.333333333f);
-  q = fmaf(s, q, 1.0f);
-  float lnm = (2.0f * t) * q;
-  return fmaf((float)e, 0.693147182f, lnm);
+  const float x = m - 1.0f;
+  float p = fmaf(x, -0x1.38158ep-4f, 0x1.08442ap-3f);
+  p = fmaf(x, p, -0x1.0f3b4ep-3f);
+  p = fmaf(x, p, 0x1.2272a8p-3f);
+  p = fmaf(x, p, -0x1.542702p-3f);
+  p = fmaf(x, p, 0x1.99a3dcp-3f);
+  p = fmaf(x, p, -0x1.000428p-2f);
+  p = fmaf(x, p, 0x1.55555p-2f);
+  p = fmaf(x, p, -0x1.fffff8p-2f);
+  p = fmaf(x, p, 1.0f);
+  return fmaf((float)e, 0.693147182f, x * p);
 }
 
 /* cos and sin of 2 pi k / 2^24, k in [0, 2^24): quadrant reduction is
@@ -780,6 +785,19 @@ static void noise_sincos(uint32_t k, float *cs, float *sn) {
     case 2: *cs = -cv; *sn = -sv; break;
     default: *cs = sv; *sn = -cv; break;
   }
+}
+
+void orc_noise_log_n(const float *u, int64_t n, float *out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = noise_log(u[i]);
+}
+
+void orc_pair_normals_n(uint64_t seed, const uint32_t *lo, const uint32_t *hi,
+                        int64_t step, int64_t n, float *g) {
+  for (int64_t i = 0; i < n; ++i) orc_pair_normals(seed, lo[i], hi[i], step, g + 4 * i);
+}
+
+void orc_noise_sincos_n(const uint32_t *k, int64_t n, float *cs, float *sn) {
+  for (int64_t i = 0; i < n; ++i) noise_sincos(k[i], &cs[i], &sn[i]);
 }
 
 void orc_pair_normals(uint64_t seed, uint32_t pid_lo, uint32_t pid_hi,

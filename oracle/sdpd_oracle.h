@@ -144,6 +144,12 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
                        uint32_t out[4]);
 void orc_pair_normals(uint64_t seed, uint32_t pid_lo, uint32_t pid_hi,
                       int64_t step, float g[4]);
+/* The stream's ln u (u in (0, 1]) and cos / sin of 2 pi k / 2^24 over
+ * arrays, for checks against libm. */
+void orc_noise_log_n(const float *u, int64_t n, float *out);
+void orc_pair_normals_n(uint64_t seed, const uint32_t *lo, const uint32_t *hi,
+                        int64_t step, int64_t n, float *g);
+void orc_noise_sincos_n(const uint32_t *k, int64_t n, float *cs, float *sn);
 
 /* Initial evaluation (marching_marching prologue): density, pressure,
  * forces, Oldroyd-B act.  step = noise step of this evaluation.           */

@@ -45,7 +45,19 @@ def test_noise_stream_bitwise(oracle):
         g = pair_normals(lo, hi, seed, step)
         ref = np.stack([oracle.pair_normals(seed, a, b, step) for a, b in zip(lo[:4000], hi[:4000])])
         assert np.array_equal(g[:4000].view(np.uint32), ref.view(np.uint32))
+        assert np.array_equal(g.view(np.uint32), oracle.pair_normals_n(seed, lo, hi, step).view(np.uint32))
         assert np.isfinite(g).all()
+
+
+def test_noise_stream_bitwise_many_keys(oracle):
+    # two million normals: the radius sqrt(-2 ln u) and the angle polynomials
+    # over a wide sample of the 2^24 uniforms, bit for bit
+    announce("noise stream: 500000 keys")
+    lo = np.arange(500000, dtype=np.uint32) * np.uint32(2654435761)
+    hi = lo ^ np.uint32(0x5bd1e995)
+    g = pair_normals(lo, hi, 0x1234567890ABCDEF, 77)
+    ref = oracle.pair_normals_n(0x1234567890ABCDEF, lo, hi, 77)
+    assert np.array_equal(g.view(np.uint32), ref.view(np.uint32))
 
 
 @pytest.mark.parametrize("name", ["shear2d_sym", "tait3d_sym_kt0", "tait3d_full_noise"])

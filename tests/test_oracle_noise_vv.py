@@ -57,6 +57,31 @@ def test_normals_depend_on_every_key_field(oracle):
     assert np.array_equal(oracle.pair_normals(3, 10, 20, 7), base)
 
 
+def test_noise_log_within_two_ulp_of_libm(oracle):
+    # every uniform the stream can draw: u = k 2^-24, k = 1 .. 2^24
+    k = np.arange(1, 2 ** 24 + 1, dtype=np.uint64)
+    u = (k.astype(np.float32) * np.float32(2.0 ** -24)).astype(np.float32)
+    got = oracle.noise_log(u).astype(np.float64)
+    ref = np.log(u.astype(np.float64))
+    ulp = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+    ulp[ref == 0] = np.spacing(np.float32(0))
+    assert (np.abs(got - ref) / ulp).max() <= 2.0
+    assert got[-1] == 0.0  # ln 1
+
+
+def test_noise_sincos_against_libm(oracle):
+    # every angle index: cos and sin of 2 pi k / 2^24 to about an ulp of 1
+    k = np.arange(2 ** 24, dtype=np.uint32)
+    cs, sn = oracle.noise_sincos(k)
+    th = 2 * np.pi * k.astype(np.float64) / 2 ** 24
+    assert np.abs(cs - np.cos(th)).max() < 1.2e-7
+    assert np.abs(sn - np.sin(th)).max() < 1.2e-7
+    # quadrant starts are exact
+    q = np.array([0, 1 << 22, 2 << 22, 3 << 22], np.uint32)
+    c4, s4 = oracle.noise_sincos(q)
+    assert np.array_equal(c4, [1, 0, -1, 0]) and np.array_equal(s4, [0, 1, 0, -1])
+
+
 def _reference_we(rng, e, count):
     """The reference's recipe: 3x3 iid normals, symmetrised, made
     traceless, times e (rhs_force_ff_Newtonian_HuAdams.F90:186-219)."""

@@ -43,6 +43,7 @@ constexpr int kBuf = 16;          // LDS list entries per lane between flushes (
 constexpr int kTile = 224;        // staged candidate records per stencil row batch
 constexpr int kCodeShift = 25;    // list entry = j | (stencil code << 25)
 constexpr uint32_t kIndexMask = (1u << kCodeShift) - 1u;
+constexpr int kEvalBatch = 4;     // buffered FP32 density pairs evaluated per gather batch
 
 // Stencil code of a neighbour: offsets o in -2..2 per dimension,
 // code = (ox+2) + 5 (oy+2) + 25 (oz+2).
@@ -414,9 +415,25 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
   int ev = 0;
   uint32_t* __restrict__ blk = list ? list_block(P, list, ch) : nullptr;
   auto flush = [&](bool final) {
-    for (int q = ev; __ballot(q < cnt) != 0ull; ++q) {
-      if (q < cnt) {
-        pol.body(buf[q * kWave + lane]);
+    if constexpr (Pol::kBatched) {
+      // kEvalBatch entries at a time: the policy issues all their gathers
+      // before it evaluates the first pair (slots past cnt read the lane's
+      // own particle and add nothing)
+      for (int q = ev; __ballot(q < cnt) != 0ull; q += kEvalBatch) {
+        uint32_t e[kEvalBatch];
+        bool ok[kEvalBatch];
+#pragma unroll
+        for (int k = 0; k < kEvalBatch; ++k) {
+          ok[k] = q + k < cnt;
+          e[k] = ok[k] ? buf[(q + k) * kWave + lane] : (uint32_t)h.ii;
+        }
+        pol.body_batch(e, ok);
+      }
+    } else {
+      for (int q = ev; __ballot(q < cnt) != 0ull; ++q) {
+        if (q < cnt) {
+          pol.body(buf[q * kWave + lane]);
+        }
       }
     }
     const int w = final ? cnt : (cnt & ~7);
@@ -499,11 +516,14 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
       const int lb = max(tb, base) - base, le = min(te, bend) - base;
       for (int t = lb; __ballot(t < le) != 0ull; t += 2) {
         if (__ballot(cnt > BUF - 2) != 0ull) flush(false);
+        // records t and t + 1 from one clamped base (the tile holds TILE + 1
+        // records; a clamped base only serves lanes past their window)
+        const float4* __restrict__ tp = tile + min(t, TILE - 1);
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
           const int tt = t + u;
           const bool live = tt < le;
-          const float4 q = tile[live ? tt : 0];
+          const float4 q = tp[u];
           const uint32_t wv = __float_as_uint(q.w);
           const float d0 = ax - q.x;
           const float d1 = ay - q.y;
@@ -514,8 +534,9 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
           }
           const bool ok = live && wv != self_w;
           const uint32_t ent = wv + kadd;
-          bool in = s2 < P.cut2_lo_f32;
-          if (ok && !in && s2 < P.cut2_hi_f32) {
+          const bool below_lo = s2 < P.cut2_lo_f32, below_hi = s2 < P.cut2_hi_f32;
+          bool in = below_lo;
+          if (ok && below_hi && !below_lo) {
             // FP64 re-decision with the reference's image convention
             ++rechecks;
             const int j = (int)(ent & kIndexMask);
@@ -605,6 +626,7 @@ struct DensityF64 {
   const Home& h;
   double acc;
   static constexpr bool kImages = true;
+  static constexpr bool kBatched = false;
   __device__ __forceinline__ void body(uint32_t ent) {
     const int j = (int)(ent & kIndexMask);
     const int64_t n = P.n;
@@ -624,15 +646,28 @@ struct DensityMixed {
   const Home& h;
   KahanF acc;
   static constexpr bool kImages = false;
-  __device__ __forceinline__ void body(uint32_t ent) {
-    const int j = (int)(ent & kIndexMask);
-    const Offsets o = decode_code(ent >> kCodeShift);
-    float d0, d1, d2;
-    rel_diff<DIM>(P, h, S.xa[j], o, d0, d1, d2);
-    const float r = __builtin_amdgcn_sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
-    float w, gw;
-    kernel_w_gradw_f32(P.kernel_type, r * P.inv_hf, P.coeff, P.coef_gradf, w, gw);
-    acc.add(P.density_type == 1 ? (float)S.m[j] * w : w);
+  static constexpr bool kBatched = true;
+  template <int NB>
+  __device__ __forceinline__ void body_batch(const uint32_t (&e)[NB], const bool (&ok)[NB]) {
+    float4 q[NB];
+    float mj[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      const int j = (int)(e[k] & kIndexMask);
+      q[k] = S.xa[j];
+      mj[k] = P.density_type == 1 ? (float)S.m[j] : 1.f;
+    }
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      // the lane's own particle in a padding slot: code 62 is offset 0
+      const Offsets o = decode_code(ok[k] ? e[k] >> kCodeShift : 62u);
+      float d0, d1, d2;
+      rel_diff<DIM>(P, h, q[k], o, d0, d1, d2);
+      const float r = __builtin_amdgcn_sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+      float w, gw;
+      kernel_w_gradw_f32(P.kernel_type, r * P.inv_hf, P.coeff, P.coef_gradf, w, gw);
+      if (ok[k]) acc.add(P.density_type == 1 ? mj[k] * w : w);
+    }
   }
 };
 
@@ -675,7 +710,7 @@ __device__ __forceinline__ void write_force_records(const DevParams& P, const De
 template <int DIM, bool MIXED, int TILE, int BUF, bool LE>
 __global__ __launch_bounds__(kWave) void k_density(DevParams P, DeviceState S) {
   if (!LE) P.le_g = -1;
-  __shared__ float4 tile[TILE];
+  __shared__ float4 tile[TILE + 1];
   __shared__ uint32_t buf[BUF * kWave];
   Chunk ch;
   if (!load_chunk(P, S, ch)) return;
@@ -788,6 +823,7 @@ struct ForceF64 {
   double rxc_i[3] = {0.0, 0.0, 0.0}, vcoll_i[3] = {0.0, 0.0, 0.0};
   double le_t = 0.0;
   static constexpr bool kImages = true;
+  static constexpr bool kBatched = false;
   __device__ __forceinline__ void body(uint32_t ent) {
     const int j = (int)(ent & kIndexMask);
     const Images o = decode_images(ent);
@@ -956,6 +992,7 @@ struct ForceMixed {
   float inv_ni;
   KahanF L[DIM * DIM];
   static constexpr bool kImages = false;
+  static constexpr bool kBatched = false;
   __device__ __forceinline__ void body(uint32_t ent) {
     const int j = (int)(ent & kIndexMask);
     body_rec(j, S.xa[j], S.vr[j], S.qr[j], decode_code(ent >> kCodeShift));
@@ -1136,7 +1173,7 @@ __device__ __forceinline__ void conformation_rate(const DevParams& P, const Devi
 template <int DIM, int FT, bool MIXED, bool COLL, bool LE>
 __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
   if (!LE) P.le_g = -1;
-  __shared__ float4 tile[kTile];
+  __shared__ float4 tile[kTile + 1];
   __shared__ uint32_t buf[kBuf * kWave];
   Chunk ch;
   if (!load_chunk(P, S, ch)) return;
@@ -1228,6 +1265,7 @@ struct SignaturePol {
   uint32_t pidi;
   uint32_t hs;
   static constexpr bool kImages = true;
+  static constexpr bool kBatched = false;
   __device__ __forceinline__ void body(uint32_t ent) {
     hs += pair_hash(pidi, (uint32_t)pid[(int)(ent & kIndexMask)]);
   }
@@ -1236,7 +1274,7 @@ struct SignaturePol {
 template <int DIM, bool LE>
 __global__ __launch_bounds__(kWave) void k_signature(DevParams P, DeviceState S) {
   if (!LE) P.le_g = -1;
-  __shared__ float4 tile[kTile];
+  __shared__ float4 tile[kTile + 1];
   __shared__ uint32_t buf[kBuf * kWave];
   Chunk ch;
   if (!load_chunk(P, S, ch)) return;

@@ -227,6 +227,16 @@ typedef struct sdpd_tuning {
 int sdpd_set_tuning(sdpd_ctx *ctx, const sdpd_tuning *t);
 int sdpd_get_tuning(sdpd_ctx *ctx, sdpd_tuning *t);
 
+/* Capacity of the stored neighbour lists, in entries per particle: a
+ * multiple of 8 in 8..4096, or 0 for the automatic choice (1.6 times the
+ * mean in-cut-off count of the box, plus 32, rounded up to 16).  A chunk
+ * with a particle that has more neighbours scans its stencil again in the
+ * force pass; every capacity gives identical results.  No counterpart in
+ * the reference (its neighbour lists grow as needed).  Set before
+ * sdpd_upload; get returns the capacity in use after it. */
+int sdpd_set_list_capacity(sdpd_ctx *ctx, int32_t cap);
+int sdpd_get_list_capacity(sdpd_ctx *ctx, int32_t *cap);
+
 /* Autotuner: times reps density + force passes at the current positions
  * for every launch shape (the simulation does not advance), applies the
  * fastest and returns it with its time per pass pair (ms). */

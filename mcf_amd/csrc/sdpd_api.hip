@@ -46,6 +46,7 @@ struct sdpd_ctx {
   // captured two-step graphs of sdpd_step, one per parity of the sort's
   // ping-pong buffers (kernel arguments hold the buffer pointers)
   bool use_graphs = true;
+  int list_cap_req = 0;      // stored neighbour entries per particle (0: automatic)
   int parity = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   // colloids: parameters and initial state until sdpd_upload puts them on
@@ -772,7 +773,7 @@ int upload_impl(sdpd_ctx* c, const double* x, const double* v, const double* m,
     const double expect = (double)n / vol * ball;
     int cap = (int)std::ceil(1.6 * expect) + 32;
     cap = (cap + 15) / 16 * 16;
-    c->P.list_cap = std::min(cap, 4096);
+    c->P.list_cap = c->list_cap_req > 0 ? c->list_cap_req : std::min(cap, 4096);
   }
   int rc = alloc_state(c, n);
   if (rc) {
@@ -1190,6 +1191,21 @@ int sdpd_set_graphs(sdpd_ctx* c, int32_t enable) {
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     drop_graphs(c);
   }
+  return SDPD_OK;
+}
+
+int sdpd_set_list_capacity(sdpd_ctx* c, int32_t cap) {
+  if (!c) return SDPD_ERR_ARG;
+  if (c->n > 0) return fail(c, SDPD_ERR_STATE, "sdpd_set_list_capacity: call before sdpd_upload");
+  if (cap != 0 && (cap < 8 || cap > 4096 || cap % 8 != 0))
+    return fail(c, SDPD_ERR_ARG, "sdpd_set_list_capacity: 0, or a multiple of 8 in 8..4096");
+  c->list_cap_req = cap;
+  return SDPD_OK;
+}
+
+int sdpd_get_list_capacity(sdpd_ctx* c, int32_t* cap) {
+  if (!c || !cap) return SDPD_ERR_ARG;
+  *cap = c->n > 0 ? c->P.list_cap : c->list_cap_req;
   return SDPD_OK;
 }
 

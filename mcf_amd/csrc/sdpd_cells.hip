@@ -37,13 +37,17 @@ __global__ void k_cell_key(DevParams P, const double* __restrict__ x,
                            const int32_t* __restrict__ rep, const RepGeom* __restrict__ geo,
                            int32_t* __restrict__ key, int32_t* __restrict__ slot,
                            int32_t* __restrict__ count) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = i0 < P.n;
+  const int i = act ? i0 : P.n - 1;  // every lane takes part in the histogram
   const int64_t n = P.n;
   const RepGeom& g = geo[rep[i]];
-  int32_t k = cell_key_geo(g, P.dim, x[i], x[n + i], P.dim == 3 ? x[2 * n + i] : 0.0);
-  key[i] = k;
-  slot[i] = atomicAdd(&count[k], 1);
+  const int32_t k = cell_key_geo(g, P.dim, x[i], x[n + i], P.dim == 3 ? x[2 * n + i] : 0.0);
+  const int32_t sl = histogram_slot(count, k, act);
+  if (act) {
+    key[i] = k;
+    slot[i] = sl;
+  }
 }
 
 // Exclusive scan of count[0..m) into start[0..m] in three passes over

@@ -145,6 +145,28 @@ __device__ __forceinline__ double f_modulo(double a, double p) {
   return r;
 }
 
+// Histogram slot of key k, aggregated per wave (every lane of the wave
+// calls it; lanes with act false count nothing and get 0).  Equal keys in
+// runs of neighbouring lanes -- particles stored in cell order -- cost one
+// atomic per run: the run's first lane adds the run length and the run
+// shares the old count.  The order of slots inside a cell is not
+// deterministic; consumers rank by index.
+__device__ __forceinline__ int32_t histogram_slot(int32_t* __restrict__ count, int32_t k, bool act) {
+  const int lane = threadIdx.x & 63;
+  const int32_t key = act ? k : -1 - lane;  // inactive lanes: runs of their own
+  const int32_t kprev = __shfl_up(key, 1);
+  const bool head = lane == 0 || kprev != key;
+  const uint64_t heads = __ballot(head);
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int first = 63 - __clzll(heads & upto);
+  const uint64_t after = heads & ~upto;
+  const int end = after ? __ffsll((long long)after) - 1 : 64;
+  int32_t base = 0;
+  if (head && act) base = atomicAdd(&count[key], end - first);
+  base = __shfl(base, first);
+  return act ? base + (lane - first) : 0;
+}
+
 // global cell key of a position in replica geometry g
 __device__ __forceinline__ int32_t cell_key_geo(const RepGeom& g, int dim, double x0, double x1,
                                                 double x2) {

@@ -1101,46 +1101,49 @@ __device__ __forceinline__ void finish_force(const DevParams& P, const DeviceSta
   if (DIM == 3) S.f[2 * n + i] = f2;
 }
 
+// A chunk one of whose particles has more in-cutoff neighbours than the
+// stored lists hold: its wave scans the stencil again in the force pass.
+__device__ __forceinline__ bool list_overflow(const DevParams& P, const DeviceState& S,
+                                              const Chunk& ch) {
+  const int i = ch.p0 + (int)threadIdx.x;
+  const int cnt = i < ch.p1 ? S.nbr_count[i] : 0;
+  return __ballot(cnt > P.list_cap) != 0ull;
+}
+
 // Walk the in-cutoff neighbours stored by the density pass of the same
-// step; a wave with a particle whose list overflowed scans again.
+// step (waves without list overflow).
 template <int DIM, class Pol>
 __device__ __forceinline__ void for_each_listed(const DevParams& P, const DeviceState& S,
                                                 const Chunk& ch, const Home& h, Pol& pol,
-                                                float4* __restrict__ tile,
                                                 uint32_t* __restrict__ buf) {
   const int cnt = h.active ? S.nbr_count[h.i] : 0;
-  if (__ballot(cnt > P.list_cap) == 0ull) {
-    const uint32_t* __restrict__ blk = list_block(P, S.nlist, ch);
-    const int lane = threadIdx.x;
-    for (int q = 0; __ballot(q < cnt) != 0ull; q += 8) {
-      if (q < cnt) {
-        // eight entries per pair of 16-byte loads; entries past cnt are
-        // padding inside the row and are not evaluated.  The group goes to
-        // LDS (entry u of a lane at buf[u * 64 + lane], conflict-free), and
-        // the loop reads one entry per pair, the next one ahead of the body.
-        const uint4* src = reinterpret_cast<const uint4*>(blk + group_offset(q, lane));
-        const uint4 lo = src[0], hi = src[1];
-        buf[0 * kWave + lane] = lo.x;
-        buf[1 * kWave + lane] = lo.y;
-        buf[2 * kWave + lane] = lo.z;
-        buf[3 * kWave + lane] = lo.w;
-        buf[4 * kWave + lane] = hi.x;
-        buf[5 * kWave + lane] = hi.y;
-        buf[6 * kWave + lane] = hi.z;
-        buf[7 * kWave + lane] = hi.w;
-        const int m = min(8, cnt - q);
-        uint32_t ent = buf[lane];
+  const uint32_t* __restrict__ blk = list_block(P, S.nlist, ch);
+  const int lane = threadIdx.x;
+  for (int q = 0; __ballot(q < cnt) != 0ull; q += 8) {
+    if (q < cnt) {
+      // eight entries per pair of 16-byte loads; entries past cnt are
+      // padding inside the row and are not evaluated.  The group goes to
+      // LDS (entry u of a lane at buf[u * 64 + lane], conflict-free), and
+      // the loop reads one entry per pair, the next one ahead of the body.
+      const uint4* src = reinterpret_cast<const uint4*>(blk + group_offset(q, lane));
+      const uint4 lo = src[0], hi = src[1];
+      buf[0 * kWave + lane] = lo.x;
+      buf[1 * kWave + lane] = lo.y;
+      buf[2 * kWave + lane] = lo.z;
+      buf[3 * kWave + lane] = lo.w;
+      buf[4 * kWave + lane] = hi.x;
+      buf[5 * kWave + lane] = hi.y;
+      buf[6 * kWave + lane] = hi.z;
+      buf[7 * kWave + lane] = hi.w;
+      const int m = min(8, cnt - q);
+      uint32_t ent = buf[lane];
 #pragma unroll 1
-        for (int u = 0; u < m; ++u) {
-          const uint32_t cur = ent;
-          ent = buf[((u + 1) & 7) * kWave + lane];
-          pol.body(cur);
-        }
+      for (int u = 0; u < m; ++u) {
+        const uint32_t cur = ent;
+        ent = buf[((u + 1) & 7) * kWave + lane];
+        pol.body(cur);
       }
     }
-  } else {
-    int rechecks = 0;
-    (void)staged_scan<DIM>(P, S, ch, h, pol, tile, buf, rechecks, nullptr);
   }
 }
 
@@ -1170,13 +1173,42 @@ __device__ __forceinline__ void conformation_rate(const DevParams& P, const Devi
     }
 }
 
-template <int DIM, int FT, bool MIXED, bool COLL, bool LE>
+// The neighbours of a chunk: its stored lists, or (SCAN) the stencil
+// scanned again.
+template <int DIM, bool SCAN, class Pol>
+__device__ __forceinline__ void visit_neighbours(const DevParams& P, const DeviceState& S,
+                                                 const Chunk& ch, const Home& h, Pol& pol,
+                                                 float4* __restrict__ tile,
+                                                 uint32_t* __restrict__ buf) {
+  if constexpr (SCAN) {
+    int rechecks = 0;
+    (void)staged_scan<DIM>(P, S, ch, h, pol, tile, buf, rechecks, nullptr);
+  } else {
+    (void)tile;
+    for_each_listed<DIM>(P, S, ch, h, pol, buf);
+  }
+}
+
+// Two launches over the same grid: SCAN = false takes the chunks whose
+// lists hold every neighbour and walks them, SCAN = true takes the others
+// and scans their stencil again.  Every chunk is done by exactly one of
+// them, and the list walk carries none of the scan's state (it needs 40 %
+// fewer registers, so more waves share a SIMD).
+//
+// The FP64 list walk (without colloids) takes 8 KB of LDS per
+// workgroup, its entry buffer plus a reserve.  That holds it to five waves
+// per SIMD.  Its registers alone would allow six, but on the 128^3 box six
+// waves' gathers run 6 % slower (1.99 against 1.88 ms per pass).
+constexpr int kForceLdsWordsF64 = 2048;
+
+template <int DIM, int FT, bool MIXED, bool COLL, bool LE, bool SCAN>
 __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
   if (!LE) P.le_g = -1;
-  __shared__ float4 tile[kTile + 1];
-  __shared__ uint32_t buf[kBuf * kWave];
+  __shared__ float4 tile[SCAN ? kTile + 1 : 1];
+  __shared__ uint32_t buf[(!MIXED && !COLL && !SCAN) ? kForceLdsWordsF64 : kBuf * kWave];
   Chunk ch;
   if (!load_chunk(P, S, ch)) return;
+  if (list_overflow(P, S, ch) != SCAN) return;
   const Home h = load_home<DIM>(P, S, ch);
   const int64_t n = P.n;
   const int ii = h.ii;
@@ -1197,7 +1229,7 @@ __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
       pol.pb[5] = q2.z;
       pol.inv_ni = (float)(1.0 / S.rho[ii]);
     }
-    for_each_listed<DIM>(P, S, ch, h, pol, tile, buf);
+    visit_neighbours<DIM, SCAN>(P, S, ch, h, pol, tile, buf);
     if (!h.active) return;
     finish_force<DIM>(P, S, h.i, pol.f0.value() * inv_mi, pol.f1.value() * inv_mi,
                       DIM == 3 ? pol.f2.value() * inv_mi : 0.0);
@@ -1239,7 +1271,7 @@ __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
           pol.pti[a + DIM * b] = val;
         }
     }
-    for_each_listed<DIM>(P, S, ch, h, pol, tile, buf);
+    visit_neighbours<DIM, SCAN>(P, S, ch, h, pol, tile, buf);
     if (!h.active) return;
     const int i = h.i;
     finish_force<DIM>(P, S, i, pol.f0 * inv_mi, pol.f1 * inv_mi, pol.f2 * inv_mi,
@@ -1348,10 +1380,13 @@ void launch_refresh_records(hipStream_t s, const DevParams& P, const DeviceState
 template <int DIM, int FT, bool MIXED, bool COLL>
 static void launch_forces_le(hipStream_t s, const DevParams& P, const DeviceState& S) {
   dim3 g(nchunks_max(P)), b(kWave);
-  if (P.le_g >= 0)
-    hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, true>), g, b, 0, s, P, S);
-  else
-    hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, false>), g, b, 0, s, P, S);
+  if (P.le_g >= 0) {
+    hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, true, false>), g, b, 0, s, P, S);
+    hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, true, true>), g, b, 0, s, P, S);
+  } else {
+    hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, false, false>), g, b, 0, s, P, S);
+    hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, false, true>), g, b, 0, s, P, S);
+  }
 }
 
 template <int DIM>

@@ -107,8 +107,9 @@ __global__ void k_conformation(DevParams P, DeviceState S, double lambda) {
 // Cell counts must be zero on entry.
 __global__ __launch_bounds__(256) void k_boundary(DevParams P, DeviceState S, int closing,
                                                   int opening) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = i0 < P.n;
+  const int i = act ? i0 : P.n - 1;  // lanes past the end compute but store nothing
   const int64_t n = P.n;
   const int dim = P.dim;
   const RepGeom& g = S.geo[S.rep[i]];
@@ -130,23 +131,30 @@ __global__ __launch_bounds__(256) void k_boundary(DevParams P, DeviceState S, in
     else
       adjust_particle<2>(P, g, xv, vw, S.time_dev[1]);
   }
-  for (int d = 0; d < dim; ++d) {
-    S.v[d * n + i] = vw[d];
-    S.x[d * n + i] = xv[d];
-  }
-  if (!P.newtonian) {
-    const int d2 = dim * dim;
-    for (int q = 0; q < d2; ++q) {
-      double c = S.ct[q * n + i];
-      double a = S.act[q * n + i];
-      if (closing) c = c + a * P.dt * 0.5;
-      if (opening) c = c + a * P.dt * 0.5;
-      S.ct[q * n + i] = c;
+  if (act) {
+    for (int d = 0; d < dim; ++d) {
+      S.v[d * n + i] = vw[d];
+      S.x[d * n + i] = xv[d];
+    }
+    if (!P.newtonian) {
+      const int d2 = dim * dim;
+      for (int q = 0; q < d2; ++q) {
+        double c = S.ct[q * n + i];
+        double a = S.act[q * n + i];
+        if (closing) c = c + a * P.dt * 0.5;
+        if (opening) c = c + a * P.dt * 0.5;
+        S.ct[q * n + i] = c;
+      }
     }
   }
-  int32_t k = cell_key_geo(g, dim, xv[0], xv[1], xv[2]);
-  S.key[i] = k;
-  S.slot[i] = atomicAdd(&S.count[k], 1);
+  // cell histogram of the next sort; the particles are in the cell order
+  // of the previous one and move little
+  const int32_t k = cell_key_geo(g, dim, xv[0], xv[1], xv[2]);
+  const int32_t sl = histogram_slot(S.count, k, act);
+  if (act) {
+    S.key[i] = k;
+    S.slot[i] = sl;
+  }
 }
 
 // Neumaier-compensated sum helpers

@@ -121,6 +121,8 @@ SYMBOLS = [
                                     _DP, _DP]),
     ("sdpd_get_colloids", C.c_int, [_P, _DP, _DP, _DP, _DP, _DP]),
     ("sdpd_set_graphs", C.c_int, [_P, C.c_int32]),
+    ("sdpd_set_list_capacity", C.c_int, [_P, C.c_int32]),
+    ("sdpd_get_list_capacity", C.c_int, [_P, C.POINTER(C.c_int32)]),
     ("sdpd_set_tuning", C.c_int, [_P, C.POINTER(CTuning)]),
     ("sdpd_get_tuning", C.c_int, [_P, C.POINTER(CTuning)]),
     ("sdpd_autotune", C.c_int, [_P, C.c_int32, C.POINTER(CTuning), _DP]),
@@ -385,6 +387,15 @@ class SDPD:
         t = CTuning()
         self._check(self.lib.sdpd_get_tuning(self.h, C.byref(t)), "sdpd_get_tuning")
         return int(t.density_shape), int(t.chunk_order)
+
+    def set_list_capacity(self, cap: int) -> None:
+        """Stored neighbour entries per particle (0: automatic); before upload."""
+        self._check(self.lib.sdpd_set_list_capacity(self.h, int(cap)), "sdpd_set_list_capacity")
+
+    def list_capacity(self) -> int:
+        cap = C.c_int32()
+        self._check(self.lib.sdpd_get_list_capacity(self.h, C.byref(cap)), "sdpd_get_list_capacity")
+        return int(cap.value)
 
     def set_tuning(self, density_shape: int, chunk_order: int) -> None:
         t = CTuning(int(density_shape), int(chunk_order))

@@ -471,6 +471,60 @@ def test_graph_steps_match_direct_launches(variant):
                     g.compute_interaction()
 
 
+def _list_capacity_boxes():
+    from tests.colloid_helpers import two_spheres
+    le = W.tait_box_3d(12, kt=1e-3)
+    le.prm = le.prm.replace(le_dim=2, le_v_lower=[-0.3, 0, 0], le_v_upper=[0.3, 0, 0])
+    sph, cs = two_spheres(sym=True, noslip=2, rotate=True, omega=[[0, 0, 0.02], [0.01, 0, 0]])
+    return {
+        "tait3d_fp64": (BOXES["tait3d_sym_noise"](), None),
+        "tait3d_mixed": (W.mixed(BOXES["tait3d_sym_noise"]()), None),
+        "shear2d": (BOXES["shear2d_full"](), None),
+        "oldroyd3d": (W.oldroyd_box_3d(10), None),
+        "oldroyd3d_mixed": (W.mixed(W.oldroyd_box_3d(10)), None),
+        "lees_edwards": (le, None),
+        "colloids": (sph, cs),
+    }
+
+
+@pytest.mark.parametrize("name", ["tait3d_fp64", "tait3d_mixed", "shear2d", "oldroyd3d",
+                                  "oldroyd3d_mixed", "lees_edwards", "colloids"])
+def test_list_capacity_gives_identical_results(name):
+    # capacities below the neighbour counts send every chunk (8) or many
+    # (48) to the force pass's stencil rescan; the results must not move
+    box, cs = _list_capacity_boxes()[name]
+    announce(f"list capacity: {name} n={box.n}")
+    outs = []
+    for cap in (0, 48, 8):
+        with SDPD(box.prm) as g:
+            g.set_list_capacity(cap)
+            if cs is not None:
+                g.set_colloids(cs)
+            g.upload(box.x, box.v, box.m, box.pid, box.sid if cs is not None else None)
+            if cap:
+                assert g.list_capacity() == cap
+            g.step = 5
+            g.build_list()
+            g.compute_density()
+            g.compute_interaction()
+            outs.append((g.download_density()[0], g.download_force()))
+    for rho, f in outs[1:]:
+        assert np.array_equal(rho, outs[0][0])
+        assert np.array_equal(f, outs[0][1])
+
+
+def test_list_capacity_arguments():
+    box = BOXES["tait3d_sym_kt0"]()
+    with SDPD(box.prm) as g:
+        for bad in (-8, 4, 12, 8192):
+            with pytest.raises(SDPDError):
+                g.set_list_capacity(bad)
+        g.upload(box.x, box.v, box.m, box.pid)
+        assert g.list_capacity() >= 16
+        with pytest.raises(SDPDError):
+            g.set_list_capacity(64)  # after upload
+
+
 def test_launch_shapes_give_identical_results():
     box = BOXES["tait3d_sym_noise"]()
     announce("launch shapes and autotuner")

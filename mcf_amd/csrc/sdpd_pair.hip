@@ -705,15 +705,29 @@ __device__ __forceinline__ void write_force_records(const DevParams& P, const De
   }
 }
 
-// LE = false compiles the Lees-Edwards paths out (boxes without sheared
-// faces): le_g is then the constant -1 in every inlined helper.
+// Whether a chunk's stencil rows reach across a sheared face: its row of
+// cells lies within two cells of a face of dimension le_g.
+__device__ __forceinline__ bool chunk_meets_le_face(int le_g, const Chunk& ch) {
+  const int lr = ch.row - ch.G.row_base;
+  const int c = le_g == 1 ? lr % ch.G.nm1 : lr / ch.G.nm1;
+  const int nm = le_g == 1 ? ch.G.nm1 : ch.G.nm2;
+  return c < 2 || c >= nm - 2;
+}
+
+// LE = false compiles the Lees-Edwards paths out: le_g is then the
+// constant -1 in every inlined helper.  A box with sheared faces runs two
+// launches over the same grid, LE = true for the chunks next to a face and
+// LE = false for the rest, whose rows never cross a face (the sheared
+// frame's state would cost the whole pass registers and occupancy).
 template <int DIM, bool MIXED, int TILE, int BUF, bool LE>
 __global__ __launch_bounds__(kWave) void k_density(DevParams P, DeviceState S) {
+  const int le_box = P.le_g;
   if (!LE) P.le_g = -1;
   __shared__ float4 tile[TILE + 1];
   __shared__ uint32_t buf[BUF * kWave];
   Chunk ch;
   if (!load_chunk(P, S, ch)) return;
+  if (le_box >= 0 && chunk_meets_le_face(le_box, ch) != LE) return;
   const Home h = load_home<DIM>(P, S, ch);
   const double mi = S.m[h.ii];
   double rho;
@@ -1349,10 +1363,8 @@ static void launch_density_shape(hipStream_t s, const DevParams& P, const Device
 
 template <int DIM, bool MIXED>
 static void launch_density_le(hipStream_t s, const DevParams& P, const DeviceState& S) {
-  if (P.le_g >= 0)
-    launch_density_shape<DIM, MIXED, true>(s, P, S);
-  else
-    launch_density_shape<DIM, MIXED, false>(s, P, S);
+  if (P.le_g >= 0) launch_density_shape<DIM, MIXED, true>(s, P, S);
+  launch_density_shape<DIM, MIXED, false>(s, P, S);
 }
 
 void launch_density(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed) {

@@ -679,6 +679,16 @@ namespace {
 // accumulated rotation) onto the device.
 int upload_colloids(sdpd_ctx* c, const int32_t* id, int32_t n) {
   const int ncoll = c->P.ncoll;
+  {
+    double rmax = 0.0;
+    for (int k = 0; k < ncoll; ++k) rmax = std::max(rmax, c->c_radius[k]);
+    int cells = 2;
+    if (c->P.le_g >= 0) {
+      const double reach = rmax + c->P.coll_dout + c->P.cut_off;
+      cells = std::max(2, (int)std::ceil(reach / c->P.cs[c->P.le_g]) + 1);
+    }
+    c->P.coll_le_cells = cells;
+  }
   std::vector<int32_t> sid(n), start(ncoll + 1, 0), slot(n);
   for (int32_t i = 0; i < n; ++i) {
     sid[i] = id[2 * (size_t)i + 1];

@@ -72,6 +72,11 @@ struct DevParams {
   double coll_bf[3];
   double rho0;            // initial density (physics key rho)
   int64_t coll_step0;     // step of the initial evaluation (the run's step_start)
+  // boxes with sheared faces and colloids: rows of cells within this many
+  // cells of a face take the Lees-Edwards force kernel (the stencil's reach
+  // plus a colloid's radius, outer layer and the cut-off: the nearest image
+  // of a colloid seen from there may lie across the face)
+  int32_t coll_le_cells;
 };
 
 // Colloid state read by the pair kernels (disks in 2D, spheres in 3D):

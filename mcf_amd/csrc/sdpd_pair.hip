@@ -705,13 +705,13 @@ __device__ __forceinline__ void write_force_records(const DevParams& P, const De
   }
 }
 
-// Whether a chunk's stencil rows reach across a sheared face: its row of
-// cells lies within two cells of a face of dimension le_g.
-__device__ __forceinline__ bool chunk_meets_le_face(int le_g, const Chunk& ch) {
+// Whether a chunk's row of cells lies within `cells` cells of a face of
+// dimension le_g (2: its stencil rows reach across the face).
+__device__ __forceinline__ bool chunk_meets_le_face(int le_g, const Chunk& ch, int cells) {
   const int lr = ch.row - ch.G.row_base;
   const int c = le_g == 1 ? lr % ch.G.nm1 : lr / ch.G.nm1;
   const int nm = le_g == 1 ? ch.G.nm1 : ch.G.nm2;
-  return c < 2 || c >= nm - 2;
+  return c < cells || c >= nm - cells;
 }
 
 // LE = false compiles the Lees-Edwards paths out: le_g is then the
@@ -727,7 +727,7 @@ __global__ __launch_bounds__(kWave) void k_density(DevParams P, DeviceState S) {
   __shared__ uint32_t buf[BUF * kWave];
   Chunk ch;
   if (!load_chunk(P, S, ch)) return;
-  if (le_box >= 0 && chunk_meets_le_face(le_box, ch) != LE) return;
+  if (le_box >= 0 && chunk_meets_le_face(le_box, ch, 2) != LE) return;
   const Home h = load_home<DIM>(P, S, ch);
   const double mi = S.m[h.ii];
   double rho;
@@ -1203,26 +1203,30 @@ __device__ __forceinline__ void visit_neighbours(const DevParams& P, const Devic
   }
 }
 
+// The FP64 list walk (without colloids) takes 8 KB of LDS per workgroup,
+// its entry buffer plus a reserve.  That holds it to five waves per SIMD.
+// Its registers alone would allow six, but on the 128^3 box six waves'
+// gathers run 6 % slower (1.99 against 1.88 ms per pass).
+constexpr int kForceLdsWordsF64 = 2048;
+
 // Two launches over the same grid: SCAN = false takes the chunks whose
 // lists hold every neighbour and walks them, SCAN = true takes the others
 // and scans their stencil again.  Every chunk is done by exactly one of
 // them, and the list walk carries none of the scan's state (it needs 40 %
-// fewer registers, so more waves share a SIMD).
-//
-// The FP64 list walk (without colloids) takes 8 KB of LDS per
-// workgroup, its entry buffer plus a reserve.  That holds it to five waves
-// per SIMD.  Its registers alone would allow six, but on the 128^3 box six
-// waves' gathers run 6 % slower (1.99 against 1.88 ms per pass).
-constexpr int kForceLdsWordsF64 = 2048;
-
+// fewer registers, so more waves share a SIMD).  With colloids a box with
+// sheared faces splits the chunks once more: the Lees-Edwards
+// instantiation takes the rows within coll_le_cells of a face, the plain
+// one (a quarter fewer registers) the interior.
 template <int DIM, int FT, bool MIXED, bool COLL, bool LE, bool SCAN>
 __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
+  const int le_box = P.le_g;
   if (!LE) P.le_g = -1;
   __shared__ float4 tile[SCAN ? kTile + 1 : 1];
   __shared__ uint32_t buf[(!MIXED && !COLL && !SCAN) ? kForceLdsWordsF64 : kBuf * kWave];
   Chunk ch;
   if (!load_chunk(P, S, ch)) return;
   if (list_overflow(P, S, ch) != SCAN) return;
+  if (COLL && le_box >= 0 && chunk_meets_le_face(le_box, ch, P.coll_le_cells) != LE) return;
   const Home h = load_home<DIM>(P, S, ch);
   const int64_t n = P.n;
   const int ii = h.ii;
@@ -1395,7 +1399,8 @@ static void launch_forces_le(hipStream_t s, const DevParams& P, const DeviceStat
   if (P.le_g >= 0) {
     hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, true, false>), g, b, 0, s, P, S);
     hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, true, true>), g, b, 0, s, P, S);
-  } else {
+  }
+  if (P.le_g < 0 || COLL) {
     hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, false, false>), g, b, 0, s, P, S);
     hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, false, true>), g, b, 0, s, P, S);
   }

@@ -77,6 +77,36 @@ def test_colloid_density_forces(oracle, kind, sym, noslip, rotate, le):
     assert np.abs(torque - to).max() < 1e-11 * (np.abs(to).max() + 1e-300)
 
 
+def test_colloid_le_face_and_interior_chunks(oracle):
+    # 28^3: 18 rows of cells along y, of which the 6 next to each sheared
+    # face take the Lees-Edwards force kernel and the middle 6 the plain one;
+    # one sphere straddles the upper face, the other sits in the interior
+    box, cs = make("sphere", True, 2, True, n_side=28)
+    box = le_variant(box, u=0.4)
+    step = 4
+    announce(f"colloid LE face/interior chunks: n={box.n}")
+    with gpu_context(box, cs) as g:
+        g.step = step
+        g.build_list()
+        g.compute_density()
+        g.compute_interaction()
+        rho, _ = g.download_density()
+        f = g.download_force()
+        _, _, _, drag, torque = g.colloids()
+    c = to_oracle(oracle, cs)
+    t = marching_time(step, box.prm.dt)
+    ro = oracle.density(box.prm, box.x, box.m, box.sid, coll=c, le_time=t)
+    po = oracle.pressure(box.prm, ro, box.m)
+    fo, _ = oracle.interaction(box.prm, step, box.x, box.v, ro, po, box.m, box.pid, box.sid,
+                               coll=c, le_time=t)
+    assert rel_err(rho, ro) < 1e-13
+    assert vec_err(f, fo) < 1e-12
+    c.le_time = t
+    do, to = oracle.colloid_collect(box.prm, c, box.x, fo, box.m, box.sid)
+    assert np.abs(drag - do).max() < 1e-11 * (np.abs(do).max() + 1e-300)
+    assert np.abs(torque - to).max() < 1e-11 * (np.abs(to).max() + 1e-300)
+
+
 @pytest.mark.parametrize("rho_type", [0, 1])
 def test_colloid_density_types(oracle, rho_type):
     box, cs = two_spheres(rho_type=rho_type)

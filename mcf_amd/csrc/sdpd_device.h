@@ -339,7 +339,12 @@ __device__ __forceinline__ float sqrt_rn_normal(float y) {
   return y > 0.0f ? r : y;
 }
 
-// Four N(0,1) deviates for the unordered pair (lo, hi) at a step.
+// Four N(0,1) deviates for the unordered pair (lo, hi) at a step.  FAST
+// (mixed precision, 1e-5 accuracy) takes the hardware square root and the
+// hardware sine / cosine of 2 pi k / 2^24 (about an ulp; the instructions
+// take the angle in revolutions) instead of the exactly rounded forms; the
+// log stays the polynomial, whose relative accuracy holds for u near 1.
+template <bool FAST = false>
 __device__ __forceinline__ void pair_normals(uint64_t seed, uint32_t lo, uint32_t hi,
                                              int64_t step, float g[4]) {
 #pragma clang fp contract(off)
@@ -349,9 +354,16 @@ __device__ __forceinline__ void pair_normals(uint64_t seed, uint32_t lo, uint32_
 #pragma unroll
   for (int hh = 0; hh < 2; ++hh) {
     float u = (float)((w[2 * hh] >> 8) + 1u) * 0x1p-24f;
-    float r = sqrt_rn_normal(-2.0f * noise_log(u));
-    float cs, sn;
-    noise_sincos(w[2 * hh + 1] >> 8, cs, sn);
+    float cs, sn, r;
+    if (FAST) {
+      r = __builtin_amdgcn_sqrtf(-2.0f * noise_log(u));
+      const float rev = (float)(w[2 * hh + 1] >> 8) * 0x1p-24f;
+      cs = __builtin_amdgcn_cosf(rev);
+      sn = __builtin_amdgcn_sinf(rev);
+    } else {
+      r = sqrt_rn_normal(-2.0f * noise_log(u));
+      noise_sincos(w[2 * hh + 1] >> 8, cs, sn);
+    }
     g[2 * hh] = r * cs;
     g[2 * hh + 1] = r * sn;
   }

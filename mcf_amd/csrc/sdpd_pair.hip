@@ -82,6 +82,13 @@ __device__ __forceinline__ Images decode_images(uint32_t ent) {
   return m;
 }
 
+// The cell-relative position of an xa record without its cell index: a
+// 12-byte gather instead of 16.
+__device__ __forceinline__ float4 load_xyz(const float4* __restrict__ rec) {
+  const float* f = reinterpret_cast<const float*>(rec);
+  return make_float4(f[0], f[1], f[2], 0.f);
+}
+
 // Geometry of the replica a chunk belongs to (uniform across the wave).
 struct Geo {
   int nm0, nm1, nm2;    // sort-grid cells per dimension
@@ -654,7 +661,7 @@ struct DensityMixed {
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
       const int j = (int)(e[k] & kIndexMask);
-      q[k] = S.xa[j];
+      q[k] = load_xyz(S.xa + j);
       mj[k] = P.density_type == 1 ? (float)S.m[j] : 1.f;
     }
 #pragma unroll
@@ -1009,7 +1016,7 @@ struct ForceMixed {
   static constexpr bool kBatched = false;
   __device__ __forceinline__ void body(uint32_t ent) {
     const int j = (int)(ent & kIndexMask);
-    body_rec(j, S.xa[j], S.vr[j], S.qr[j], decode_code(ent >> kCodeShift));
+    body_rec(j, load_xyz(S.xa + j), S.vr[j], S.qr[j], decode_code(ent >> kCodeShift));
   }
   // xj: cell-relative position; vj: (v, pid); qj: (p/n^2, 1/n^2, p)
   __device__ __forceinline__ void body_rec(int jidx, const float4& xj, const float4& vj,

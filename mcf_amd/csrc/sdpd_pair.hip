@@ -65,11 +65,16 @@ __device__ __forceinline__ Offsets decode_code(uint32_t code) {
 // FP64 lists store the neighbour's periodic image instead of its stencil
 // offset: bits 25-26, 27-28 and 29-30 hold, per dimension, the image
 // shift of x_i - x_j in box lengths as a 2-bit signed field (+1 for a
-// virtual cell below 0, -1 for one at or above nm, else 0).  The FP32
-// path needs the offsets (cell-relative positions) and keeps the codes.
+// virtual cell below 0, -1 for one at or above nm, else 0); bit 31 flags
+// a colloid boundary particle.  The FP32 path needs the offsets
+// (cell-relative positions) and keeps the codes.
 struct Images {
   int sx, sy, sz;
 };
+// Bit 31 of an FP64 entry: the neighbour is a colloid boundary particle
+// (boxes with colloids), so the pair body needs no species gather for the
+// fluid-fluid pairs.
+constexpr uint32_t kBoundaryBit = 0x80000000u;
 __device__ __forceinline__ int image_class(int v, int nm) { return v < 0 ? 1 : (v >= nm ? -1 : 0); }
 __device__ __forceinline__ uint32_t image_field(int v, int nm, int dim) {
   return ((uint32_t)image_class(v, nm) & 3u) << (kCodeShift + 2 * dim);
@@ -416,6 +421,8 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
   const int nm0 = h.G.nm0;
   const int32_t* __restrict__ start = S.start;
   const int low_mode = P.symmetry ? 2 : 3;
+  // the boundary flag of the lane's own record (FP64 entries with colloids)
+  const uint32_t self_flag = (Pol::kImages && P.ncoll && S.sid[h.ii] != 0) ? kBoundaryBit : 0u;
   int cnt = 0, stored = 0;
   // buf holds cnt entries of this lane, the first ev of them evaluated
   // already; entries reach the list block in whole groups of 8
@@ -492,7 +499,8 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
     // the lane's own copy: same particle, own virtual cell (x image 0)
     const uint32_t self_w =
         le_row ? 0xFFFFFFFFu
-               : (Pol::kImages ? (uint32_t)h.ii : ((uint32_t)h.ii | ((uint32_t)crel << kCodeShift)));
+               : (Pol::kImages ? ((uint32_t)h.ii | self_flag)
+                               : ((uint32_t)h.ii | ((uint32_t)crel << kCodeShift)));
     // this lane's window: virtual cells wc-2 .. wc+2 as a range of the
     // row sequence
     int tb = 0, te = 0;
@@ -515,7 +523,8 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
         const float4 q = S.xa[g];
         const int v = __float_as_int(q.w) + (rp.m0 + piece) * nm0;
         const int vrel = v - v_lo;
-        const uint32_t code = Pol::kImages ? image_field(v, nm0, 0) : (uint32_t)vrel << kCodeShift;
+        uint32_t code = Pol::kImages ? image_field(v, nm0, 0) : (uint32_t)vrel << kCodeShift;
+        if (Pol::kImages && P.ncoll && S.sid[g] != 0) code |= kBoundaryBit;
         tile[t - base] = make_float4(fmaf((float)vrel, h.G.cs0, q.x), q.y, q.z,
                                      __uint_as_float((uint32_t)g | code));
       }
@@ -848,12 +857,11 @@ struct ForceF64 {
   __device__ __forceinline__ void body(uint32_t ent) {
     const int j = (int)(ent & kIndexMask);
     const Images o = decode_images(ent);
-    int sj = 0;
+    const bool jb = COLL && (ent & kBoundaryBit) != 0u;
     if (COLL) {
       // two boundary particles do not interact (pp_interaction.inc: same
       // colloid CYCLE, colloid-colloid pair forces off)
-      sj = S.sid[j];
-      if (sidi != 0 && sj != 0) return;
+      if (sidi != 0 && jb) return;
     }
     const int64_t n = P.n;
     const double4* rj = reinterpret_cast<const double4*>(S.rec64 + 8 * (int64_t)j);
@@ -873,11 +881,11 @@ struct ForceF64 {
     double psum = 0.0, pij = 0.0;
     if (FT == 3) psum = ai + ra.w;
     if (FT == 4) pij = (pi + S.p[j]) * 0.5;
-    if (COLL && (sidi | sj) != 0) {
+    if (COLL && (sidi != 0 || jb)) {
       // a fluid particle and a colloid boundary particle: only the
       // boundary particle's velocity and pressure change
       double pb;
-      colloid_substitute(j, sj, ra, rb, d0, d1, d2, jump, u0, u1, u2, pb);
+      colloid_substitute(j, jb ? S.sid[j] : 0, ra, rb, d0, d1, d2, jump, u0, u1, u2, pb);
       if (FT == 3) psum = sidi == 0 ? ai + pb * bj : pb * bi + ra.w;
       if (FT == 4) pij = pb;
     }

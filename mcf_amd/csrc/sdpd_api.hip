@@ -324,7 +324,7 @@ void free_state(DeviceState& S) {
                   S.v_alt,    S.m_alt,    S.pid_alt,   S.perm_alt, S.f,       S.rho,
                   S.p,        S.an,       S.bn,        S.ct,      S.ct_alt,   S.act,
                   S.vgt,      S.key,      S.slot,      S.count,   S.start,    S.unsorted,
-                  S.key_sorted, S.xa,     S.vr,        S.qr,      S.partials, S.stats_out,
+                  S.key_sorted, S.xa,     S.pt64,      S.vr,        S.qr,      S.partials, S.stats_out,
                   S.nbr_count, S.recheck, S.nbr_hash, S.rkey,   S.rslot,    S.rcount,
                   S.rstart,   S.runsorted, S.rlpdx,  S.nlist,   S.nbr_sig_count,
                   S.row_nchunk, S.row_chunk_off, S.chunk_row, S.scan_tmp, S.rec64,
@@ -378,6 +378,7 @@ int alloc_state(sdpd_ctx* c, int n) {
     if (!P.newtonian) ALLOC(qr2, N);
   } else {
     ALLOC(rec64, 8 * N);
+    if (!P.newtonian) ALLOC(pt64, 6 * N);
   }
   {
     // replica tables (static for the context's lifetime)

@@ -28,8 +28,10 @@ struct DeviceState {
   // work map of the pair kernels (chunks of <= 64 particles of one row)
   int32_t *row_nchunk, *row_chunk_off, *chunk_row;
   int32_t *scan_tmp;     // tile sums of the exclusive scans
-  // FP64 records of the force pass: x, y, z, p/n^2, vx, vy, vz, 1/n^2
+  // FP64 records of the force pass: x, y, z, p/n^2, vx, vy, vz, 1/n^2;
+  // Oldroyd-B also Pi / n^2 as (xx, yy, zz, xy, xz, yz) (2D: xx, yy, 0, xy)
   double *rec64;
+  double *pt64;
   float4 *vr;            // velocity, w = pid bits
   float4 *qr;            // (p / n^2, 1 / n^2, p, 0); Oldroyd-B: (1/n^2, Pxx, Pyy, Pzz)
   float4 *qr2;           // Oldroyd-B: (Pxy, Pxz, Pyz, 0), P = Pi / n^2

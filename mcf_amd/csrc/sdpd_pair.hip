@@ -718,6 +718,24 @@ __device__ __forceinline__ void write_force_records(const DevParams& P, const De
     double4* r = reinterpret_cast<double4*>(S.rec64 + 8 * (int64_t)i);
     r[0] = make_double4(S.x[i], S.x[n + i], DIM == 3 ? S.x[2 * n + i] : 0.0, an);
     r[1] = make_double4(S.v[i], S.v[n + i], DIM == 3 ? S.v[2 * n + i] : 0.0, bn);
+    if (!P.newtonian) {
+      // Oldroyd-B: Pi / n^2 as (xx, yy, zz, xy, xz, yz), Pi = (p + G) I - G c
+      // (particles_compute_pressure_tensor.F90:114-133; c is symmetric)
+      auto pib = [&](int a, int b) {
+        const double c = S.ct[(int64_t)(a + DIM * b) * n + i];
+        return ((a == b ? press + P.G : 0.0) - P.G * c) * bn;
+      };
+      double2* q = reinterpret_cast<double2*>(S.pt64 + 6 * (int64_t)i);
+      if (DIM == 3) {
+        q[0] = make_double2(pib(0, 0), pib(1, 1));
+        q[1] = make_double2(pib(2, 2), pib(0, 1));
+        q[2] = make_double2(pib(0, 2), pib(1, 2));
+      } else {
+        q[0] = make_double2(pib(0, 0), pib(1, 1));
+        q[1] = make_double2(0.0, pib(0, 1));
+        q[2] = make_double2(0.0, 0.0);
+      }
+    }
   }
 }
 
@@ -863,7 +881,6 @@ struct ForceF64 {
       // colloid CYCLE, colloid-colloid pair forces off)
       if (sidi != 0 && jb) return;
     }
-    const int64_t n = P.n;
     const double4* rj = reinterpret_cast<const double4*>(S.rec64 + 8 * (int64_t)j);
     const double4 ra = rj[0], rb = rj[1];
     double d0, d1, d2;
@@ -926,15 +943,17 @@ struct ForceF64 {
         for (int b = 0; b < DIM; ++b)
 #pragma unroll
           for (int a = 0; a < DIM; ++a) L[a + DIM * b] = fma(du[a] * dx[b], gfac, L[a + DIM * b]);
-        const double pj = S.p[j];
+        // j's Pi / n^2 from its 48-byte record (c is symmetric)
+        const double2* __restrict__ rp = reinterpret_cast<const double2*>(S.pt64 + 6 * (int64_t)j);
+        const double2 q0 = rp[0], q1 = rp[1], q2 = rp[2];
+        const double pjb[9] = {q0.x, q1.y, q2.x, q1.y, q0.y, q2.y, q2.x, q2.y, q1.x};
         double fc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int jj = 0; jj < DIM; ++jj)
 #pragma unroll
           for (int a = 0; a < DIM; ++a) {
-            double ptj = -P.G * S.ct[(int64_t)(a + DIM * jj) * n + j];
-            if (a == jj) ptj = ptj + pj + P.G;
-            fc[jj] = fc[jj] + (pti[a + DIM * jj] * bi + ptj * bj) * dx[a];
+            const double ptjb = DIM == 3 ? pjb[a + 3 * jj] : pjb[(a == jj ? (a == 0 ? 0 : 4) : 1)];
+            fc[jj] = fc[jj] + (pti[a + DIM * jj] * bi + ptjb) * dx[a];
           }
         a0 = fd * u0 - fc[0] * gir;
         a1 = fd * u1 - fc[1] * gir;

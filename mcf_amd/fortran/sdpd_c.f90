@@ -51,6 +51,12 @@ MODULE sdpd_c
      INTEGER(c_int64_t) :: seed
   END TYPE sdpd_params
 
+  ! struct sdpd_tuning (launch shapes of the pair kernels)
+  TYPE, BIND(C) :: sdpd_tuning
+     INTEGER(c_int32_t) :: density_shape
+     INTEGER(c_int32_t) :: chunk_order
+  END TYPE sdpd_tuning
+
   ! struct sdpd_colloid_params (coll_* keys of physics_config.mcf)
   TYPE, BIND(C) :: sdpd_colloid_params
      INTEGER(c_int32_t) :: noslip
@@ -129,6 +135,37 @@ MODULE sdpd_c
        TYPE(c_ptr), VALUE :: ctx
        TYPE(c_ptr), VALUE :: x, v, omega, drag, torque
      END FUNCTION sdpd_get_colloids
+
+     ! device-path tuning (no counterpart in the reference; results do not
+     ! depend on it): stored neighbour entries per particle before upload,
+     ! launch shapes, the autotuner, graph replay of sdpd_step
+     INTEGER(c_int) FUNCTION sdpd_set_list_capacity(ctx, cap) &
+          BIND(C, name="sdpd_set_list_capacity")
+       IMPORT :: c_int, c_ptr, c_int32_t
+       TYPE(c_ptr), VALUE        :: ctx
+       INTEGER(c_int32_t), VALUE :: cap
+     END FUNCTION sdpd_set_list_capacity
+
+     INTEGER(c_int) FUNCTION sdpd_set_tuning(ctx, t) BIND(C, name="sdpd_set_tuning")
+       IMPORT :: c_int, c_ptr, sdpd_tuning
+       TYPE(c_ptr), VALUE            :: ctx
+       TYPE(sdpd_tuning), INTENT(IN) :: t
+     END FUNCTION sdpd_set_tuning
+
+     INTEGER(c_int) FUNCTION sdpd_autotune(ctx, reps, best, ms_best) &
+          BIND(C, name="sdpd_autotune")
+       IMPORT :: c_int, c_ptr, c_int32_t, c_double, sdpd_tuning
+       TYPE(c_ptr), VALUE             :: ctx
+       INTEGER(c_int32_t), VALUE      :: reps
+       TYPE(sdpd_tuning), INTENT(OUT) :: best
+       REAL(c_double), INTENT(OUT)    :: ms_best
+     END FUNCTION sdpd_autotune
+
+     INTEGER(c_int) FUNCTION sdpd_set_graphs(ctx, enable) BIND(C, name="sdpd_set_graphs")
+       IMPORT :: c_int, c_ptr, c_int32_t
+       TYPE(c_ptr), VALUE        :: ctx
+       INTEGER(c_int32_t), VALUE :: enable
+     END FUNCTION sdpd_set_graphs
 
      INTEGER(c_int) FUNCTION sdpd_upload_ct(ctx, ct) BIND(C, name="sdpd_upload_ct")
        IMPORT :: c_int, c_ptr, c_double

@@ -234,6 +234,32 @@ def test_oldroyd_parity(oracle):
     assert np.array_equal(ctg, ct)
 
 
+def _oldroyd_2d():
+    box = W.shear_wave_2d(24, brownian=True, kt=1e-4)
+    box.prm = box.prm.replace(Newtonian=False, tau=0.1, n_p=1.0, kt_p=0.5)
+    return box
+
+
+def test_oldroyd_2d_parity(oracle):
+    # the 2D pressure-tensor record (xx, yy, 0, xy) and the 2x2 tensors
+    box = _oldroyd_2d()
+    announce("Oldroyd-B 2D forces, vgt")
+    rng = np.random.default_rng(11)
+    a = rng.normal(0, 0.05, size=(box.n, 2, 2))
+    ct = (np.tile(np.eye(2), (box.n, 1, 1)) + a + a.transpose(0, 2, 1)).reshape(box.n, 4)
+    g = gpu_evaluate(box, step=3, ct=ct)
+    f = g.download_force()
+    _, vgt = g.download_tensors()
+    g.close()
+    rho, p, fo, vgto = oracle_evaluate(oracle, box, step=3, ct=ct)
+    assert vec_err(f, fo) < 1e-12
+    assert vec_err(vgt, vgto) < 1e-12
+    gm = gpu_evaluate(W.mixed(box), step=3, ct=ct)
+    fm = gm.download_force()
+    gm.close()
+    assert vec_err(fm, fo) < 5e-5  # the 2D mixed bar (FORCE_BAR)
+
+
 def test_oldroyd_trajectory(oracle):
     box = W.oldroyd_box_3d(12)
     announce("Oldroyd-B trajectory")

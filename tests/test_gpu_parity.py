@@ -194,6 +194,35 @@ def test_trajectory_parity(oracle, name):
     assert np.abs(mom - mo).max() < 1e-10 * (np.abs(box.m[:, None] * box.v).sum() + 1e-30)
 
 
+@pytest.mark.parametrize("kt", [0.0, 1e-4])
+def test_shear_wave_1000_steps(oracle, kt):
+    # the 2D reference case: 64^2 particles, quintic kernel, c 10, eta 0.1,
+    # sinusoidal shear, dt 1e-4, 1000 velocity-Verlet steps, deterministic
+    # and with thermal noise (the same counter-based stream on both sides)
+    box = W.shear_wave_2d(64, brownian=kt > 0, kt=kt if kt > 0 else 1e-4)
+    nsteps = 1000
+    announce(f"shear wave 64^2, kT {kt}: {nsteps} steps")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.integrate_VV(nsteps)
+        x, v = g.download()
+        e, mom = g.compute_statistic()
+    st = oracle.State(box.prm, box.x, box.v, box.m, box.pid)
+    st.prologue(0)
+    for s in range(1, nsteps + 1):
+        st.vv_step(s)
+    L = np.array(box.prm.length[:2])
+    dx = x - st.x
+    dx -= L * np.round(dx / L)
+    ex, ev = np.abs(dx).max() / L.max(), vec_err(v, st.v)
+    eo, mo = st.statistics()
+    announce(f"  max |dx| / L = {ex:.2e}, velocity {ev:.2e}, E_k rel {abs(e / eo - 1):.2e}")
+    # measured on MI355X: |dx| / L below 3e-15, velocities within 1.2e-12
+    assert ex < 1e-13
+    assert ev < 1e-10
+    assert e == pytest.approx(eo, rel=1e-11)
+
+
 def test_step_api_matches_individual_calls(oracle):
     # sdpd_step == kick, drift+wrap, build, density, forces, kick
     box = BOXES["tait3d_sym_noise"]()

@@ -223,6 +223,32 @@ def test_shear_wave_1000_steps(oracle, kt):
     assert e == pytest.approx(eo, rel=1e-11)
 
 
+def test_tait_box_200_steps(oracle):
+    # the 3D Tait case (gamma 7, c 10, kT 1e-3, dt 5e-5) over its 200 steps
+    # on a 16^3 box
+    box = W.tait_box_3d(16, kt=1e-3)
+    nsteps = 200
+    announce(f"Tait 16^3: {nsteps} steps")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.integrate_VV(nsteps)
+        x, v = g.download()
+        rho, _ = g.download_density()
+    st = oracle.State(box.prm, box.x, box.v, box.m, box.pid)
+    st.prologue(0)
+    for s in range(1, nsteps + 1):
+        st.vv_step(s)
+    L = np.array(box.prm.length[:3])
+    dx = x - st.x
+    dx -= L * np.round(dx / L)
+    ex, ev = np.abs(dx).max() / L.max(), vec_err(v, st.v)
+    announce(f"  max |dx| / L = {ex:.2e}, velocity {ev:.2e}, density {rel_err(rho, st.rho):.2e}")
+    # measured on MI355X: 1e-16 of the box, velocities 5e-15, density 3e-15
+    assert ex < 1e-13
+    assert ev < 1e-11
+    assert rel_err(rho, st.rho) < 1e-12
+
+
 def test_step_api_matches_individual_calls(oracle):
     # sdpd_step == kick, drift+wrap, build, density, forces, kick
     box = BOXES["tait3d_sym_noise"]()

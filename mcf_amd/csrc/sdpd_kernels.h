@@ -32,8 +32,8 @@ struct DeviceState {
   // Oldroyd-B also Pi / n^2 as (xx, yy, zz, xy, xz, yz) (2D: xx, yy, 0, xy)
   double *rec64;
   double *pt64;
-  float4 *vr;            // velocity, w = pid bits
-  float4 *qr;            // (p / n^2, 1 / n^2, p, 0); Oldroyd-B: (1/n^2, Pxx, Pyy, Pzz)
+  float4 *vr;            // velocity, w = 1 / n^2; Oldroyd-B: w = pid bits
+  float4 *qr;            // (cell-relative x, p / n^2 or p); Oldroyd-B: (1/n^2, Pxx, Pyy, Pzz)
   float4 *qr2;           // Oldroyd-B: (Pxy, Pxz, Pyz, 0), P = Pi / n^2
   // reductions and counters
   double *partials;      // stats partial sums (blocks x 8)

@@ -687,21 +687,26 @@ struct DensityMixed {
   }
 };
 
-// Records of the force pass, written once the density is known: FP64
-// (x, y, z, p/n^2, vx, vy, vz, 1/n^2) or FP32 (v, pid) and (p/n^2, 1/n^2, p);
-// Oldroyd-B in FP32: (1/n^2, Pxx, Pyy, Pzz) and (Pxy, Pxz, Pyz, 0) with
-// P = Pi / n^2, Pi = (p + G) I - G c (particles_compute_pressure_tensor.F90:
-// 114-133; c is symmetric).
+// Records of the force pass, written once the density is known.
+//   FP64: (x, y, z, p/n^2, vx, vy, vz, 1/n^2).
+//   FP32 Newtonian: (cell-relative x, y, z, p/n^2 for Hu-Adams or p for the
+//   angular form) and (vx, vy, vz, 1/n^2); the pair body gathers these two
+//   and the particle id.
+//   FP32 Oldroyd-B: (v, pid), (1/n^2, Pxx, Pyy, Pzz) and (Pxy, Pxz, Pyz, 0)
+//   with P = Pi / n^2, Pi = (p + G) I - G c (particles_compute_pressure_
+//   tensor.F90:114-133; c is symmetric); the position comes from xa.
 template <int DIM>
 __device__ __forceinline__ void write_force_records(const DevParams& P, const DeviceState& S, int i,
                                                     double an, double bn, double press) {
   const int64_t n = P.n;
   if (P.mixed) {
-    S.vr[i] = make_float4((float)S.v[i], (float)S.v[n + i], DIM == 3 ? (float)S.v[2 * n + i] : 0.f,
-                          __uint_as_float((uint32_t)S.pid[i]));
+    const float v0 = (float)S.v[i], v1 = (float)S.v[n + i], v2 = DIM == 3 ? (float)S.v[2 * n + i] : 0.f;
     if (P.newtonian) {
-      S.qr[i] = make_float4((float)an, (float)bn, (float)press, 0.f);
+      const float4 xa = S.xa[i];
+      S.qr[i] = make_float4(xa.x, xa.y, xa.z, (float)(P.force_type == 4 ? press : an));
+      S.vr[i] = make_float4(v0, v1, v2, (float)bn);
     } else {
+      S.vr[i] = make_float4(v0, v1, v2, __uint_as_float((uint32_t)S.pid[i]));
       auto pi = [&](int a, int b) {
         const double c = S.ct[(int64_t)(a + DIM * b) * n + i];
         return ((a == b ? press + P.G : 0.0) - P.G * c) * bn;
@@ -1043,11 +1048,21 @@ struct ForceMixed {
   static constexpr bool kBatched = false;
   __device__ __forceinline__ void body(uint32_t ent) {
     const int j = (int)(ent & kIndexMask);
-    body_rec(j, load_xyz(S.xa + j), S.vr[j], S.qr[j], decode_code(ent >> kCodeShift));
+    if (FT == 5) {
+      const float4 vj = S.vr[j];
+      body_rec(j, load_xyz(S.xa + j), vj, __float_as_uint(vj.w), S.qr[j],
+               decode_code(ent >> kCodeShift));
+    } else {
+      const uint32_t pidj = P.noise ? (uint32_t)S.pid[j] : 0u;
+      body_rec(j, S.qr[j], S.vr[j], pidj, make_float4(0.f, 0.f, 0.f, 0.f),
+               decode_code(ent >> kCodeShift));
+    }
   }
-  // xj: cell-relative position; vj: (v, pid); qj: (p/n^2, 1/n^2, p)
+  // xj: cell-relative position (w: p/n^2 for Hu-Adams, p for the angular
+  // form); vj: velocity (w: 1/n^2, Newtonian); Oldroyd-B qj: (1/n^2, Pxx,
+  // Pyy, Pzz)
   __device__ __forceinline__ void body_rec(int jidx, const float4& xj, const float4& vj,
-                                           const float4& qj, const Offsets& o) {
+                                           uint32_t pidj, const float4& qj, const Offsets& o) {
     float d0, d1, d2;
     rel_diff<DIM>(P, h, xj, o, d0, d1, d2);
     const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
@@ -1056,7 +1071,6 @@ struct ForceMixed {
     float w, gw;
     kernel_w_gradw_f32(P.kernel_type, r * P.inv_hf, P.coeff, P.coef_gradf, w, gw);
     const float u0 = vi0 - (vj.x + le_dvf(P, h, o)), u1 = vi1 - vj.y, u2 = vi2 - vj.z;
-    const uint32_t pidj = __float_as_uint(vj.w);
     float fi0, fi1, fi2;
     if (FT == 5) {
       // rhs_force_ff_non_Newtonian_HuAdams.F90:105-131 with the records'
@@ -1095,8 +1109,8 @@ struct ForceMixed {
         fi2 = fi2 + f_r * w2;
       }
     } else if (FT == 3) {
-      const float fcr = (ai + qj.x) * gw * ir;
-      const float f_d = P.etaf * (bi + qj.y) * gw * ir;
+      const float fcr = (ai + xj.w) * gw * ir;
+      const float f_d = P.etaf * (bi + vj.w) * gw * ir;
       fi0 = f_d * u0 - fcr * d0;
       fi1 = f_d * u1 - fcr * d1;
       fi2 = f_d * u2 - fcr * d2;
@@ -1110,8 +1124,8 @@ struct ForceMixed {
       }
     } else {
       const float ev = (d0 * u0 + d1 * u1 + d2 * u2) * ir;
-      const float pij = (pi + qj.z) * 0.5f;
-      const float aij = (bi + qj.y) * gw;
+      const float pij = (pi + xj.w) * 0.5f;
+      const float aij = (bi + vj.w) * gw;
       const float f_c = aij * pij;
       const float f_d = P.eta_coeff * P.etaf * aij * ev * ir;
       float sc = f_d - f_c;
@@ -1268,7 +1282,10 @@ __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
   if (MIXED) {
     const float4 vi = S.vr[ii];
     const float4 qi = S.qr[ii];
-    ForceMixed<DIM, FT> pol{P, S, h, vi.x, vi.y, vi.z, qi.x, qi.y, qi.z, __float_as_uint(vi.w),
+    // Newtonian records: qi.w = p/n^2 (Hu-Adams) or p (angular), vi.w = 1/n^2
+    ForceMixed<DIM, FT> pol{P, S, h, vi.x, vi.y, vi.z, FT == 3 ? qi.w : 0.f, vi.w,
+                            FT == 4 ? qi.w : 0.f,
+                            FT == 5 ? __float_as_uint(vi.w) : (uint32_t)S.pid[ii],
                             S.step_dev[0]};
     if (FT == 5) {
       const float4 q2 = S.qr2[ii];

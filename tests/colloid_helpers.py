@@ -1,6 +1,4 @@
 """Colloid boxes for the oracle and GPU colloid tests."""
-import numpy as np
-
 from mcf_amd import workloads as W
 
 

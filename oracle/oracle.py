@@ -140,8 +140,9 @@ class Colloids:
 
 
 def build() -> None:
-    """Compile the oracle with its Makefile (gcc, no GPU needed)."""
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    """Compile the oracle with its Makefile (gcc, no GPU needed), also when
+    the library is up to date."""
+    subprocess.run(["make", "-s", "-B", "-C", HERE], check=True)
 
 
 _lib = None

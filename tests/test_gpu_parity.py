@@ -456,6 +456,35 @@ def test_mixed_vs_fp64(name, force_type):
     assert np.abs(mf.sum(0)).max() < 1e-5 * np.abs(mf).sum()
 
 
+@pytest.mark.parametrize("force_type", [3, 4])
+def test_mixed_record_refresh_equals_density_pass(force_type):
+    # after a kick (or a separate EOS call) the FP32 pair records are rebuilt
+    # before the force pass; they must give the forces that the records of a
+    # fresh density pass at the same state give, bit for bit
+    box = W.mixed(W.tait_box_3d(12, kt=1e-3))
+    box.prm = box.prm.replace(rhs_force_type=force_type)
+    announce(f"mixed record refresh: type {force_type}")
+    with SDPD(box.prm) as a, SDPD(box.prm) as b:
+        for g in (a, b):
+            g.upload(box.x, box.v, box.m, box.pid)
+            g.step = 3
+            g.build_list()
+            g.compute_density()
+            g.compute_interaction()
+        f0 = a.download_force()
+        for g in (a, b):
+            g.integrate_velocity(0.5)
+        a.compute_interaction()  # records refreshed from the kicked velocities
+        b.compute_density()
+        b.compute_interaction()  # records written by the density epilogue
+        fa, fb = a.download_force(), b.download_force()
+        assert np.array_equal(fa, fb)
+        assert not np.array_equal(fa, f0)
+        a.compute_pressure()     # the EOS again: records refreshed, same state
+        a.compute_interaction()
+        assert np.array_equal(a.download_force(), fa)
+
+
 def test_mixed_trajectory_close_to_fp64():
     box = W.tait_box_3d(13, kt=1e-3)
     announce("mixed trajectory vs fp64")

@@ -456,14 +456,17 @@ def test_mixed_vs_fp64(name, force_type):
     assert np.abs(mf.sum(0)).max() < 1e-5 * np.abs(mf).sum()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("force_type", [3, 4])
-def test_mixed_record_refresh_equals_density_pass(force_type):
-    # after a kick (or a separate EOS call) the FP32 pair records are rebuilt
+def test_record_refresh_equals_density_pass(precision, force_type):
+    # after a kick (or a separate EOS call) the pair records are rebuilt
     # before the force pass; they must give the forces that the records of a
     # fresh density pass at the same state give, bit for bit
-    box = W.mixed(W.tait_box_3d(12, kt=1e-3))
+    box = W.tait_box_3d(12, kt=1e-3)
+    if precision == "mixed":
+        box = W.mixed(box)
     box.prm = box.prm.replace(rhs_force_type=force_type)
-    announce(f"mixed record refresh: type {force_type}")
+    announce(f"record refresh: {precision} type {force_type}")
     with SDPD(box.prm) as a, SDPD(box.prm) as b:
         for g in (a, b):
             g.upload(box.x, box.v, box.m, box.pid)

@@ -582,6 +582,13 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
   return stored;
 }
 
+// FP32 terms summed in FP64 (each add rounds once, at FP64 precision)
+struct WideF {
+  double s = 0.0;
+  __device__ __forceinline__ void add(float v) { s = s + (double)v; }
+  __device__ __forceinline__ double value() const { return s; }
+};
+
 // compensated FP32 accumulator (Kahan)
 struct KahanF {
   float s = 0.f, c = 0.f;
@@ -1043,7 +1050,7 @@ struct ForceMixed {
   // Oldroyd-B: own Pi / n^2 (xx, yy, zz, xy, xz, yz), 1 / n, velocity gradient
   float pb[6];
   float inv_ni;
-  KahanF L[DIM * DIM];
+  WideF L[DIM * DIM];
   static constexpr bool kImages = false;
   static constexpr bool kBatched = false;
   __device__ __forceinline__ void body(uint32_t ent) {

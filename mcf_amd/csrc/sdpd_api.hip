@@ -1073,6 +1073,8 @@ int sdpd_step(sdpd_ctx* c, int32_t nsteps) {
     sdpd::launch_kick(c->stream, c->P, c->S, 0.5);
     if ((rc = check_launch(c, "closing kick"))) return rc;
   }
+  // the pair records still hold the velocities (and Pi) of the force pass
+  c->vr_stale = true;
   return SDPD_OK;
 }
 
@@ -1091,6 +1093,10 @@ int sdpd_set_lees_edwards(sdpd_ctx* c, int32_t le_dim, const double* v_lower,
     return fail(c, SDPD_ERR_ARG, "sdpd_set_lees_edwards: le_dim must be 2 (y) or 3 (z)");
   if (c->P.nrep != 1 || !c->rep_lo.empty())
     return fail(c, SDPD_ERR_STATE, "sdpd_set_lees_edwards: one box only (batched context)");
+  // the colloid force pass sizes its band of sheared rows at upload
+  if (c->P.ncoll > 0 && c->n > 0)
+    return fail(c, SDPD_ERR_STATE,
+                "sdpd_set_lees_edwards: with colloids, set the faces before sdpd_upload");
   for (int d = 0; d < dim; ++d)
     if (c->P.min_phys[d] != 0.0)
       return fail(c, SDPD_ERR_ARG, "sdpd_set_lees_edwards: min_phys must be 0 (MODULO wraps into [0, L))");
@@ -1245,12 +1251,44 @@ int sdpd_autotune(sdpd_ctx* c, int32_t reps, sdpd_tuning* best, double* ms_best)
   if (!c->cells_valid && (rc = run_sort(c, false))) return rc;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   drop_graphs(c);
+  // The passes below recompute f (and act, vgt) from the current velocities.
+  // After sdpd_step those are v(t + dt), while the next step must open with
+  // the forces of the last step's evaluation (half-step velocities): keep a
+  // copy and put it back at the end.
+  struct Saved {
+    double* dev;
+    double* keep;
+    size_t count;
+  };
+  std::vector<Saved> saved;
+  if (c->f_valid) {
+    const size_t n = (size_t)c->n, dim = c->P.dim;
+    saved.push_back({c->S.f, nullptr, dim * n});
+    if (!c->P.newtonian) {
+      saved.push_back({c->S.act, nullptr, dim * dim * n});
+      saved.push_back({c->S.vgt, nullptr, dim * dim * n});
+    }
+  }
+  auto release_saved = [&]() {
+    for (Saved& s : saved)
+      if (s.keep) (void)hipFree(s.keep);
+  };
+  for (Saved& s : saved) {
+    hipError_t e = hipMalloc((void**)&s.keep, s.count * sizeof(double));
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(s.keep, s.dev, s.count * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
+    if (e != hipSuccess) {
+      release_saved();
+      return fail(c, SDPD_ERR_DEVICE, std::string("sdpd_autotune: ") + hipGetErrorString(e));
+    }
+  }
   const bool timing = c->timing;
   c->timing = false;
   hipEvent_t a = nullptr, b = nullptr;
-  HIP_TRY(c, hipEventCreate(&a));
-  if (hipEventCreate(&b) != hipSuccess) {
-    (void)hipEventDestroy(a);
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+    if (a) (void)hipEventDestroy(a);
+    c->timing = timing;
+    release_saved();
     return fail(c, SDPD_ERR_DEVICE, "sdpd_autotune: event creation failed");
   }
   sdpd_tuning pick{0, 1};
@@ -1282,10 +1320,20 @@ int sdpd_autotune(sdpd_ctx* c, int32_t reps, sdpd_tuning* best, double* ms_best)
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   c->timing = timing;
+  if (!rc) {
+    c->P.density_shape = pick.density_shape;
+    c->P.xcd_order = pick.chunk_order;
+    if (!(rc = run_density(c))) rc = run_forces(c);
+  }
+  for (Saved& s : saved) {
+    hipError_t e = hipMemcpyAsync(s.dev, s.keep, s.count * sizeof(double), hipMemcpyDeviceToDevice,
+                                  c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess && !rc)
+      rc = fail(c, SDPD_ERR_DEVICE, std::string("sdpd_autotune: ") + hipGetErrorString(e));
+  }
+  release_saved();
   if (rc) return rc;
-  c->P.density_shape = pick.density_shape;
-  c->P.xcd_order = pick.chunk_order;
-  if ((rc = run_density(c)) || (rc = run_forces(c))) return rc;
   if (best) *best = pick;
   if (ms_best) *ms_best = (double)tbest / reps;
   return SDPD_OK;

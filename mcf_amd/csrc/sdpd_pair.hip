@@ -770,7 +770,9 @@ __global__ __launch_bounds__(kWave) void k_density(DevParams P, DeviceState S) {
   const int le_box = P.le_g;
   if (!LE) P.le_g = -1;
   __shared__ float4 tile[TILE + 1];
-  __shared__ uint32_t buf[BUF * kWave];
+  // whole groups of 8 rows: the final flush stores the last group of a
+  // lane's entries as 8 rows, of which those past its count are padding
+  __shared__ uint32_t buf[(BUF + 7) / 8 * 8 * kWave];
   Chunk ch;
   if (!load_chunk(P, S, ch)) return;
   if (le_box >= 0 && chunk_meets_le_face(le_box, ch, 2) != LE) return;

@@ -227,6 +227,10 @@ def test_colloid_errors():
         g.upload(box.x, box.v, box.m, box.pid, box.sid)
         with pytest.raises(SDPDError):
             g.integrate_position_and_adjust()  # no initial forces yet
+        with pytest.raises(SDPDError):
+            # the band of sheared rows of the colloid force pass is set at upload
+            g.set_lees_edwards(2, [-0.3, 0, 0], [0.3, 0, 0])
+        g.evaluate()  # the context stays usable
     with SDPD(W.mixed(box).prm) as g:
         with pytest.raises(SDPDError):
             g.set_colloids(cs)

@@ -661,3 +661,49 @@ def test_launch_shapes_give_identical_results():
             assert g.tuning() == (shape, order) and ms > 0
             assert np.array_equal(g.download_force(), ref[1])
             outs[prec_box.prm.precision] = ref
+
+
+@pytest.mark.parametrize("variant", ["tait_fp64", "oldroyd_mixed"])
+def test_autotune_between_steps_keeps_trajectory(variant):
+    # the autotuner times density + force passes at the current state; after
+    # sdpd_step those would see v(t + dt), so it must put back the forces
+    # (and act, vgt) of the last step: step(3) + autotune + step(3) equals
+    # step(6), bit for bit
+    box = BOXES["tait3d_sym_noise"]() if variant == "tait_fp64" else W.mixed(W.oldroyd_box_3d(11))
+    announce(f"autotune between steps: {variant}")
+    with SDPD(box.prm) as a, SDPD(box.prm) as b:
+        for g in (a, b):
+            g.upload(box.x, box.v, box.m, box.pid)
+        a.integrate_VV(3)
+        a.autotune(1)
+        a.integrate_VV(3)
+        b.set_tuning(*a.tuning())
+        b.integrate_VV(6)
+        xa, va = a.download()
+        xb, vb = b.download()
+        assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+        assert np.array_equal(a.download_force(), b.download_force())
+        if variant == "oldroyd_mixed":
+            cta, ga = a.download_tensors()
+            ctb, gb = b.download_tensors()
+            assert np.array_equal(cta, ctb) and np.array_equal(ga, gb)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_forces_after_step_see_kicked_velocities(precision):
+    # sdpd_step ends with the closing kick; a force pass straight after it
+    # must rebuild the pair records from the kicked velocities, i.e. equal a
+    # fresh density + force pass on the same state
+    box = BOXES["tait3d_sym_noise"]()
+    if precision == "mixed":
+        box = W.mixed(box)
+    announce(f"forces after sdpd_step: {precision}")
+    with SDPD(box.prm) as a:
+        a.upload(box.x, box.v, box.m, box.pid)
+        a.integrate_VV(2)
+        a.compute_interaction()
+        fa = a.download_force()
+        a.compute_density()
+        a.compute_interaction()
+        fb = a.download_force()
+    assert np.array_equal(fa, fb)

@@ -249,3 +249,101 @@ def test_oldroyd_velocity_gradient_of_linear_field(oracle):
     # vgt(a + 3 b) ~ dv_a / dx_b; column-major storage of L
     got = vgt[inner].mean(0).reshape(3, 3, order="F")
     assert np.allclose(got, Lg, atol=2e-3)
+
+
+def _ob_params(L, brownian=False):
+    # G = n_p kt_p = 0.5 (particles_compute_pressure_tensor.F90:94)
+    return SDPDParams(num_dim=3, min_phys=[0, 0, 0], max_phys=[L, L, L], cut_off=3.0, symmetry=True,
+                      kernel_type=1, rhs_density_type=2, rhs_force_type=3, stateEquation_type=2,
+                      c=10.0, rho=1.0, rho_ref=1.0, gamma=7.0, eta=0.1, Newtonian=False,
+                      Brownian=brownian, kt=1e-3 if brownian else 0.0, dt=5e-5, tau=0.1, n_p=1.0,
+                      kt_p=0.5)
+
+
+@pytest.mark.parametrize("alpha", [0.0, 0.5, 3.0])
+def test_oldroyd_isotropic_conformation_shifts_pressure(oracle, alpha):
+    # Pi = (p + G) I - G alpha I = (p + G (1 - alpha)) I: the force is the
+    # Newtonian Hu-Adams force with that pressure.  A wrong G (or a G that
+    # enters the diagonal once only) fails for alpha != 1.
+    box = W.tait_box_3d(10, brownian=False)
+    prm = _ob_params(10.0)
+    G = prm.n_p * prm.kt_p
+    rho = oracle.density(prm, box.x, box.m)
+    p = oracle.pressure(prm, rho, box.m)
+    ct = np.tile((alpha * np.eye(3)).reshape(-1), (box.n, 1))
+    f_ob, _ = oracle.interaction(prm, 0, box.x, box.v, rho, p, box.m, box.pid, ct=ct)
+    f_nw, _ = oracle.interaction(prm.replace(Newtonian=True), 0, box.x, box.v, rho,
+                                 p + G * (1.0 - alpha), box.m, box.pid)
+    scale = np.sqrt((f_nw ** 2).sum(1).mean())
+    assert np.abs(f_ob - f_nw).max() < 1e-12 * scale
+    if alpha != 1.0:
+        f_p, _ = oracle.interaction(prm.replace(Newtonian=True), 0, box.x, box.v, rho, p, box.m,
+                                   box.pid)
+        assert np.abs(f_p - f_nw).max() > 1e-6 * scale  # the shift is visible
+
+
+def _ob_sine_force(oracle, n_side, comp):
+    # perfect lattice at rest, uniform density: only the polymer stress
+    # varies, Pi = (p + G) I - G c with one component of c a sine wave
+    L = float(n_side)
+    prm = _ob_params(L)
+    x = W.lattice(3, n_side, 1.0, 0.0, 0)
+    n = len(x)
+    m = np.ones(n)
+    rho = oracle.density(prm, x, m)
+    p = oracle.pressure(prm, rho, m)
+    eps, k = 0.05, 2 * math.pi / L
+    ct = np.tile(np.eye(3).reshape(-1), (n, 1))
+    if comp == "xy":       # c_xy = c_yx = eps sin(k y)
+        ct[:, 1] = ct[:, 3] = eps * np.sin(k * x[:, 1])
+        along = 1
+    elif comp == "xx":     # c_xx = 1 + eps sin(k x)
+        ct[:, 0] = 1.0 + eps * np.sin(k * x[:, 0])
+        along = 0
+    else:                  # c_yy = 1 + eps sin(k x): no x force
+        ct[:, 4] = 1.0 + eps * np.sin(k * x[:, 0])
+        along = 0
+    f, _ = oracle.interaction(prm, 0, x, np.zeros_like(x), rho, p, m, np.arange(1, n + 1), ct=ct)
+    # continuum: dv/dt = -div(Pi) / rho; for the sine components
+    # a_x = G eps k cos(k x_along) / rho (rho = n m)
+    G = prm.n_p * prm.kt_p
+    expect = G * eps * k * np.cos(k * x[:, along]) / (rho * m)
+    return f, expect
+
+
+@pytest.mark.parametrize("comp", ["xy", "xx"])
+def test_oldroyd_polymer_stress_divergence(oracle, comp):
+    # the pair force of a varying conformation is -div(Pi)/rho up to the
+    # SPH error O((k h)^2): 1.5 % at 24^3, 0.8 % at 32^3 (measured).  A
+    # contraction over the wrong index of Pi drops c_xy's x force, a wrong
+    # G scales it.
+    f24, e24 = _ob_sine_force(oracle, 24, comp)
+    err24 = np.abs(f24[:, 0] - e24).max() / np.abs(e24).max()
+    assert err24 < 0.025
+    assert np.abs(f24[:, 1:]).max() < 1e-9 * np.abs(e24).max()
+    f16, e16 = _ob_sine_force(oracle, 16, comp)
+    err16 = np.abs(f16[:, 0] - e16).max() / np.abs(e16).max()
+    # second-order convergence in k h: (24/16)^2 = 2.25
+    assert 1.8 < err16 / err24 < 2.8
+
+
+def test_oldroyd_diagonal_stress_acts_along_its_own_axis(oracle):
+    f, e = _ob_sine_force(oracle, 16, "yy")
+    assert np.abs(f).max() < 1e-9 * np.abs(e).max()
+
+
+def test_oldroyd_random_conformation_conserves_momentum(oracle):
+    box = W.oldroyd_box_3d(10, brownian=True)
+    prm = box.prm
+    rng = np.random.default_rng(11)
+    a = rng.normal(0.0, 0.3, size=(box.n, 3, 3))
+    c = np.eye(3)[None] + 0.5 * (a + a.transpose(0, 2, 1))  # symmetric
+    ct = c.reshape(box.n, 9)
+    rho = oracle.density(prm, box.x, box.m)
+    p = oracle.pressure(prm, rho, box.m)
+    f, _ = oracle.interaction(prm, 3, box.x, box.v, rho, p, box.m, box.pid, ct=ct)
+    mf = box.m[:, None] * f
+    assert np.abs(mf.sum(0)).max() < 1e-12 * np.abs(mf).sum()
+    f1, _ = oracle.interaction(prm, 3, box.x, box.v, rho, p, box.m, box.pid,
+                               ct=np.tile(np.eye(3).reshape(-1), (box.n, 1)))
+    assert np.abs(f - f1).max() > 1e-3 * np.abs(f1).max()  # the conformation matters

@@ -237,6 +237,24 @@ int sdpd_get_tuning(sdpd_ctx *ctx, sdpd_tuning *t);
 int sdpd_set_list_capacity(sdpd_ctx *ctx, int32_t cap);
 int sdpd_get_list_capacity(sdpd_ctx *ctx, int32_t *cap);
 
+/* Verlet lists.  No counterpart in the reference (technique_build_list
+ * rebuilds PPM's cell list every step, marching_integrate_VV.F90:968-971).
+ * In a periodic box without sheared faces or colloids the stored lists
+ * hold every pair within cut_off + skin of their build; the particles keep
+ * their order until the next build, which happens only once a particle has
+ * moved more than skin / 2 since the last one.  Every step still decides
+ * in or out of the cut-off on its own positions with the reference's FP64
+ * formula, so the neighbour sets and the sums equal a per-step rebuild's.
+ * skin < 0 (the default): the largest the sort grid allows, twice its cell
+ * edge minus cut_off; 0: lists rebuilt every step; > 0: that skin, refused
+ * at sdpd_upload if larger than the grid allows.  Set before sdpd_upload;
+ * get returns the skin in use (0: rebuilt every step) after it.
+ * sdpd_get_list_builds returns the lists' builds since sdpd_upload (-1 when
+ * they are rebuilt every step). */
+int sdpd_set_skin(sdpd_ctx *ctx, double skin);
+int sdpd_get_skin(sdpd_ctx *ctx, double *skin);
+int sdpd_get_list_builds(sdpd_ctx *ctx, int64_t *builds);
+
 /* Autotuner: times reps density + force passes at the current positions
  * for every launch shape (the simulation does not advance), applies the
  * fastest and returns it with its time per pass pair (ms). */

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/sdpd.h"
+#include "sdpd_host.h"
 #include "sdpd_kernels.h"
 
 using sdpd::DevParams;
@@ -47,6 +48,8 @@ struct sdpd_ctx {
   // ping-pong buffers (kernel arguments hold the buffer pointers)
   bool use_graphs = true;
   int list_cap_req = 0;      // stored neighbour entries per particle (0: automatic)
+  double skin_req = -1.0;    // Verlet skin asked for (< 0: automatic, 0: lists every step)
+  double mmax = 0.0;         // largest box extent plus five cells (the FP32 band)
   int parity = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   // colloids: parameters and initial state until sdpd_upload puts them on
@@ -102,8 +105,13 @@ int check_launch(sdpd_ctx* c, const char* what) {
   return SDPD_OK;
 }
 
-// Derived constants, with the reference's start-up rules.
-int derive(const sdpd_params& p, DevParams& P, std::string& err) {
+}  // namespace
+
+namespace sdpd {
+
+// Derived constants, with the reference's start-up rules; err names the
+// failed check.
+int derive_params(const sdpd_params& p, DevParams& P, const char*& err) {
   std::memset(&P, 0, sizeof(P));
   if (p.num_dim != 2 && p.num_dim != 3) return err = "num_dim must be 2 or 3", SDPD_ERR_ARG;
   if (!(p.cut_off > 0)) return err = "cut_off must be positive", SDPD_ERR_ARG;
@@ -233,6 +241,66 @@ int derive(const sdpd_params& p, DevParams& P, std::string& err) {
   return SDPD_OK;
 }
 
+// Band of the FP32 candidate test.  The scan compares positions relative
+// to the corner of a chunk's first candidate cell in x (at most L + 5 cs
+// in size, mmax over the boxes) and to a neighbouring cell corner in y, z;
+// with six FP32 roundings per difference each difference is off by at most
+// dd = 2^-21 mmax, and r^2 by 2 sqrt(3) r dd + 4 2^-24 r^2.  The band is
+// twice that bound relative to rc^2, and at least 2e-5.
+void set_cut_band(DevParams& P, double mmax) {
+  const double dd = std::ldexp(mmax, -21);
+  double rel = 2.0 * (2.0 * std::sqrt(3.0) * dd / P.cut_off + 4.0 * std::ldexp(1.0, -24));
+  rel = std::max(rel, 2e-5);
+  const double c2 = P.cut_off * P.cut_off;
+  P.cut2_lo_f32 = (float)(c2 * std::max(0.0, 1.0 - rel)) * (1.0f - 1e-6f);
+  P.cut2_hi_f32 = (float)(c2 * (1.0 + rel)) * (1.0f + 1e-6f);
+  // the Verlet build scan keeps every candidate whose exact distance may be
+  // below rc + skin
+  const double rs = P.cut_off + P.skin;
+  P.skin2_hi_f32 = (float)(rs * rs * (1.0 + rel)) * (1.0f + 1e-6f);
+}
+
+// The largest Verlet skin of boxes with these sort grids: the candidate
+// scan reaches two cells each way, so a build finds every pair within
+// 2 cs - rc beyond the cut-off; less a margin for the rounding of cell keys.
+double grid_skin(const DevParams& P, const RepGeom* geo, int nrep) {
+  double smax = 1e300, lmax = 0.0;
+  for (int q = 0; q < nrep; ++q)
+    for (int d = 0; d < P.dim; ++d) {
+      smax = std::min(smax, 2.0 * geo[q].cs[d] - P.cut_off);
+      lmax = std::max(lmax, geo[q].L[d]);
+    }
+  return smax - 1e-9 * lmax;
+}
+
+// Whether a skin serves Verlet lists: 1e-6 rc or more.
+bool skin_usable(const DevParams& P, double skin) { return skin > 1e-6 * P.cut_off; }
+
+// Capacity of the stored neighbour lists: 1.6 times the neighbours a
+// uniform fluid of n particles in `volume` has inside the cut-off (plus the
+// Verlet skin), plus 32, rounded up to 16 (at most 4096).  A particle with more is handled by
+// scanning again in the force pass.
+int auto_list_capacity(const DevParams& P, double volume, int64_t n) {
+  const double kPiL = 3.141592653589793238;
+  const double rc = P.cut_off + P.skin;
+  const double ball = P.dim == 3 ? 4.0 / 3.0 * kPiL * rc * rc * rc : kPiL * rc * rc;
+  const double expect = (double)n / volume * ball;
+  int cap = (int)std::ceil(1.6 * expect) + 32;
+  cap = (cap + 15) / 16 * 16;
+  return std::min(cap, 4096);
+}
+
+}  // namespace sdpd
+
+namespace {
+
+int derive(const sdpd_params& p, DevParams& P, std::string& err) {
+  const char* e = "";
+  const int rc = sdpd::derive_params(p, P, e);
+  if (rc != SDPD_OK) err = e;
+  return rc;
+}
+
 // Geometry tables of the context's boxes: the single box of the params,
 // or the replicas of sdpd_set_replicas.  Sets P.nrep, P.ncell, P.nrows and
 // the FP32 cut-off band, which is sized for the largest box.
@@ -289,18 +357,8 @@ int build_geometry(sdpd_ctx* c) {
   P.nrep = nrep;
   P.ncell = (int32_t)cells;
   P.nrows = (int32_t)rows;
-  // Band of the FP32 candidate test.  The scan compares positions relative
-  // to the corner of a chunk's first candidate cell in x (at most L + 5 cs
-  // in size) and to a neighbouring cell corner in y, z; with six FP32
-  // roundings per difference each difference is off by at most
-  // dd = 2^-21 (L + 5 cs), and r^2 by 2 sqrt(3) r dd + 4 2^-24 r^2.  The
-  // band is twice that bound relative to rc^2, and at least 2e-5.
-  const double dd = std::ldexp(mmax, -21);
-  double rel = 2.0 * (2.0 * std::sqrt(3.0) * dd / P.cut_off + 4.0 * std::ldexp(1.0, -24));
-  rel = std::max(rel, 2e-5);
-  const double c2 = P.cut_off * P.cut_off;
-  P.cut2_lo_f32 = (float)(c2 * std::max(0.0, 1.0 - rel)) * (1.0f - 1e-6f);
-  P.cut2_hi_f32 = (float)(c2 * (1.0 + rel)) * (1.0f + 1e-6f);
+  c->mmax = mmax;
+  sdpd::set_cut_band(P, mmax);
   return SDPD_OK;
 }
 
@@ -331,7 +389,7 @@ void free_state(DeviceState& S) {
                   (void*)S.geo, S.rep,      S.rep_alt,  S.row_cell, S.row_len,  S.row_rep,
                   S.rep_stats, S.qr2,     S.step_dev, S.time_dev, S.sid,
                   S.sid_alt,  S.slot_of,  (void*)S.coll_hot, (void*)S.coll, S.coll_start,
-                  S.coll_members};
+                  S.coll_members, S.x_ref, S.gate, (void*)S.disp, S.nbr_in};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   std::memset(&S, 0, sizeof(S));
@@ -415,7 +473,11 @@ int alloc_state(sdpd_ctx* c, int n) {
   ALLOC(partials, (size_t)c->stats_nb * 4);
   ALLOC(stats_out, 8);
   ALLOC(nbr_count, N);
+  ALLOC(nbr_in, N);
   ALLOC(nbr_sig_count, N);
+  ALLOC(gate, 4);
+  ALLOC(disp, 1);
+  if (P.verlet) ALLOC(x_ref, dim * N);
   // neighbour lists: one block of list_cap x 64 entries per chunk of the
   // sort grid (at most N / 64 + rows chunks, rounded up to 8)
   ALLOC(nlist, (size_t)P.list_cap * 64 * (((N + 63) / 64 + (size_t)P.nrows + 7) / 8 * 8));
@@ -492,17 +554,27 @@ int need_particles(sdpd_ctx* c) {
   return SDPD_OK;
 }
 
-// K1 for the current positions (keys, histogram, scan, scatter, place)
+// K1 for the current positions (keys, histogram, scan, scatter, place).
+// Lists rebuilt every step: the sort writes the ping-pong copies, which
+// then swap.  Verlet lists: the sorted state is copied back (the buffers
+// stay put), the positions of the sort become the lists' reference, and
+// the next density pass builds the lists.
 int run_sort(sdpd_ctx* c, bool keys_done) {
   c->P.step = c->step;
   {
     PhaseMark pm(c, 1);
-    if (!keys_done) sdpd::launch_cell_keys(c->stream, c->P, c->S);
-    sdpd::launch_cell_sort_after_keys(c->stream, c->P, c->S);
+    if (c->P.verlet) {
+      sdpd::launch_cell_keys(c->stream, c->P, c->S);
+      sdpd::launch_cell_sort_after_keys(c->stream, c->P, c->S, nullptr, true);
+      sdpd::launch_force_build(c->stream, c->S);
+    } else {
+      if (!keys_done) sdpd::launch_cell_keys(c->stream, c->P, c->S);
+      sdpd::launch_cell_sort_after_keys(c->stream, c->P, c->S);
+    }
   }
   int rc = check_launch(c, "cell sort");
   if (rc) return rc;
-  swap_sorted(c);
+  if (!c->P.verlet) swap_sorted(c);
   c->cells_valid = true;
   c->sorted_once = true;
   c->rho_valid = false;
@@ -510,11 +582,32 @@ int run_sort(sdpd_ctx* c, bool keys_done) {
   return SDPD_OK;
 }
 
-int run_density(sdpd_ctx* c) {
+// Verlet lists after a drift whose displacements have been tracked: the
+// step's gate (a particle moved more than skin / 2 since the lists' build),
+// then the sort, every kernel of which returns at once unless the gate is
+// set.  Nothing here depends on the host, so captured steps replay it.
+int run_list_update(sdpd_ctx* c) {
+  c->P.step = c->step;
+  {
+    PhaseMark pm(c, 1);
+    sdpd::launch_decide(c->stream, c->P, c->S);
+    sdpd::launch_cell_keys(c->stream, c->P, c->S, c->S.gate);
+    sdpd::launch_cell_sort_after_keys(c->stream, c->P, c->S, c->S.gate, true);
+  }
+  int rc = check_launch(c, "list update");
+  if (rc) return rc;
+  c->cells_valid = true;
+  c->sorted_once = true;
+  c->rho_valid = false;
+  c->f_valid = false;
+  return SDPD_OK;
+}
+
+int run_density(sdpd_ctx* c, bool scan_all = false) {
   c->P.step = c->step;
   {
     PhaseMark pm(c, 2);
-    sdpd::launch_density(c->stream, c->P, c->S, c->P.mixed != 0);
+    sdpd::launch_density(c->stream, c->P, c->S, c->P.mixed != 0, scan_all);
   }
   int rc = check_launch(c, "density");
   if (rc) return rc;
@@ -572,7 +665,7 @@ int step_body(sdpd_ctx* c, int closing) {
   int rc = check_launch(c, "step boundary");
   if (rc) return rc;
   c->cells_valid = false;
-  if ((rc = run_sort(c, true))) return rc;
+  if ((rc = c->P.verlet ? run_list_update(c) : run_sort(c, true))) return rc;
   if ((rc = run_density(c))) return rc;
   return run_forces(c);
 }
@@ -674,6 +767,27 @@ const char* sdpd_last_error(const sdpd_ctx* c) { return c ? c->err.c_str() : "nu
 }  // extern "C"
 
 namespace {
+
+// Verlet lists or lists rebuilt every step.  The candidate scan reaches two
+// cells of the sort grid each way, so a build finds every pair within
+// 2 cs - rc of the cut-off: that is the largest skin (less a margin for the
+// rounding of cell keys).  Sheared faces and colloids, a request of 0 or a
+// skin thinner than 1e-6 rc rebuild the lists every step.
+int choose_skin(sdpd_ctx* c) {
+  DevParams& P = c->P;
+  const double smax = sdpd::grid_skin(P, c->geo.data(), (int)c->geo.size());
+  if (c->skin_req > 0.0 && c->skin_req > smax)
+    return fail(c, SDPD_ERR_ARG,
+                "sdpd_set_skin: the skin exceeds what the sort grid allows (2 cell edges - cut_off = " +
+                    std::to_string(smax) + ")");
+  const double skin = c->skin_req < 0.0 ? smax : c->skin_req;
+  const bool on = P.le_g < 0 && P.ncoll == 0 && sdpd::skin_usable(P, skin);
+  P.verlet = on ? 1 : 0;
+  P.skin = on ? skin : 0.0;
+  P.half_skin2 = 0.25 * P.skin * P.skin;
+  sdpd::set_cut_band(P, c->mmax);
+  return SDPD_OK;
+}
 
 // Species ids, the boundary particles of each colloid (upload indices in
 // upload order) and the colloids' initial state (histories zero, no
@@ -779,12 +893,9 @@ int upload_impl(sdpd_ctx* c, const double* x, const double* v, const double* m,
       for (int d = 0; d < dim; ++d) vq *= g.L[d];
       vol += vq;
     }
-    const double rc_ = c->P.cut_off;
-    const double ball = dim == 3 ? 4.0 / 3.0 * kPi * rc_ * rc_ * rc_ : kPi * rc_ * rc_;
-    const double expect = (double)n / vol * ball;
-    int cap = (int)std::ceil(1.6 * expect) + 32;
-    cap = (cap + 15) / 16 * 16;
-    c->P.list_cap = c->list_cap_req > 0 ? c->list_cap_req : std::min(cap, 4096);
+    int rc = choose_skin(c);
+    if (rc) return rc;
+    c->P.list_cap = c->list_cap_req > 0 ? c->list_cap_req : sdpd::auto_list_capacity(c->P, vol, n);
   }
   int rc = alloc_state(c, n);
   if (rc) {
@@ -817,6 +928,8 @@ int upload_impl(sdpd_ctx* c, const double* x, const double* v, const double* m,
     marching_time(c->step, c->P.dt, &t, &tle, &tprev);
     sdpd::launch_set_step(c->stream, c->S, c->step, t, tle, tprev);
   }
+  sdpd::launch_fill_i32(c->stream, c->S.gate, 0, 4);
+  sdpd::launch_fill_i32(c->stream, reinterpret_cast<int32_t*>(c->S.disp), 0, 2);
   sdpd::launch_fill_f64(c->stream, c->S.f, 0.0, (int64_t)dim * n);
   sdpd::launch_fill_f64(c->stream, c->S.rho, 0.0, n);
   sdpd::launch_fill_f64(c->stream, c->S.p, 0.0, n);
@@ -928,7 +1041,11 @@ int sdpd_upload_ct(sdpd_ctx* c, const double* ct) {
 int sdpd_build_cells(sdpd_ctx* c) {
   int rc = need_particles(c);
   if (rc) return rc;
-  return run_sort(c, false);
+  if (!c->P.verlet || !c->sorted_once) return run_sort(c, false);
+  // Verlet lists: the displacements since the lists' build decide whether
+  // they (and the cell order) are rebuilt at the next density pass
+  sdpd::launch_track(c->stream, c->P, c->S);
+  return run_list_update(c);
 }
 
 int sdpd_density(sdpd_ctx* c) {
@@ -1093,10 +1210,12 @@ int sdpd_set_lees_edwards(sdpd_ctx* c, int32_t le_dim, const double* v_lower,
     return fail(c, SDPD_ERR_ARG, "sdpd_set_lees_edwards: le_dim must be 2 (y) or 3 (z)");
   if (c->P.nrep != 1 || !c->rep_lo.empty())
     return fail(c, SDPD_ERR_STATE, "sdpd_set_lees_edwards: one box only (batched context)");
-  // the colloid force pass sizes its band of sheared rows at upload
-  if (c->P.ncoll > 0 && c->n > 0)
+  // the colloid force pass sizes its band of sheared rows at upload, and
+  // Verlet lists (chosen at upload) serve boxes without sheared faces only
+  if ((c->P.ncoll > 0 || c->P.verlet) && c->n > 0)
     return fail(c, SDPD_ERR_STATE,
-                "sdpd_set_lees_edwards: with colloids, set the faces before sdpd_upload");
+                "sdpd_set_lees_edwards: with colloids or Verlet lists, set the faces before "
+                "sdpd_upload");
   for (int d = 0; d < dim; ++d)
     if (c->P.min_phys[d] != 0.0)
       return fail(c, SDPD_ERR_ARG, "sdpd_set_lees_edwards: min_phys must be 0 (MODULO wraps into [0, L))");
@@ -1226,6 +1345,33 @@ int sdpd_get_list_capacity(sdpd_ctx* c, int32_t* cap) {
   return SDPD_OK;
 }
 
+int sdpd_set_skin(sdpd_ctx* c, double skin) {
+  if (!c) return SDPD_ERR_ARG;
+  if (c->n > 0) return fail(c, SDPD_ERR_STATE, "sdpd_set_skin: call before sdpd_upload");
+  if (std::isnan(skin)) return fail(c, SDPD_ERR_ARG, "sdpd_set_skin: skin is NaN");
+  c->skin_req = skin < 0.0 ? -1.0 : skin;
+  return SDPD_OK;
+}
+
+int sdpd_get_skin(sdpd_ctx* c, double* skin) {
+  if (!c || !skin) return SDPD_ERR_ARG;
+  *skin = c->n > 0 ? c->P.skin : c->skin_req;
+  return SDPD_OK;
+}
+
+int sdpd_get_list_builds(sdpd_ctx* c, int64_t* builds) {
+  int rc = need_particles(c);
+  if (rc) return rc;
+  if (!builds) return fail(c, SDPD_ERR_ARG, "sdpd_get_list_builds: null output");
+  int32_t g[4] = {0, 0, 0, 0};
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpy(g, c->S.gate, sizeof(g), hipMemcpyDeviceToHost));
+  // lists rebuilt every step: one build per density pass; the counter then
+  // stays 0
+  *builds = c->P.verlet ? (int64_t)g[1] : -1;
+  return SDPD_OK;
+}
+
 int sdpd_set_tuning(sdpd_ctx* c, const sdpd_tuning* t) {
   if (!c || !t) return SDPD_ERR_ARG;
   if (t->density_shape < 0 || t->density_shape > 3 || (t->chunk_order != 0 && t->chunk_order != 1))
@@ -1298,11 +1444,14 @@ int sdpd_autotune(sdpd_ctx* c, int32_t reps, sdpd_tuning* best, double* ms_best)
       c->P.density_shape = shape;
       c->P.xcd_order = order;
       // one untimed pass, then reps timed density + force passes at the
-      // current positions (the state is recomputed identically each time)
+      // current positions (the state is recomputed identically each time).
+      // With Verlet lists the timed density passes scan the stencil of
+      // every chunk (the launch shapes matter to the lists' builds), which
+      // gives the sums of the list walk.
       if ((rc = run_density(c)) || (rc = run_forces(c))) break;
       (void)hipEventRecord(a, c->stream);
       for (int r = 0; r < reps && rc == SDPD_OK; ++r)
-        if (!(rc = run_density(c))) rc = run_forces(c);
+        if (!(rc = run_density(c, c->P.verlet != 0))) rc = run_forces(c);
       (void)hipEventRecord(b, c->stream);
       if (rc) break;
       if (hipEventSynchronize(b) != hipSuccess) {
@@ -1394,7 +1543,7 @@ int sdpd_get_counters(sdpd_ctx* c, int64_t* pairs, int64_t* rechecks) {
   if (rc) return rc;
   std::vector<int32_t> a(c->n), b(c->n);
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  HIP_TRY(c, hipMemcpy(a.data(), c->S.nbr_count, c->n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(a.data(), c->S.nbr_in, c->n * sizeof(int32_t), hipMemcpyDeviceToHost));
   HIP_TRY(c, hipMemcpy(b.data(), c->S.recheck, c->n * sizeof(int32_t), hipMemcpyDeviceToHost));
   int64_t sa = 0, sb = 0;
   for (int i = 0; i < c->n; ++i) {

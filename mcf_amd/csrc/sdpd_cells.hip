@@ -21,7 +21,13 @@
 
 namespace sdpd {
 
-__global__ void k_fill_i32(int32_t* __restrict__ a, int32_t v, int64_t n) {
+// gate: a kernel of a sort that runs on some steps only returns at once
+// unless gate[0] != 0 (a null gate always runs)
+__device__ __forceinline__ bool gated_off(const int32_t* gate) { return gate && gate[0] == 0; }
+
+__global__ void k_fill_i32(int32_t* __restrict__ a, int32_t v, int64_t n,
+                           const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) a[i] = v;
@@ -36,7 +42,8 @@ __global__ void k_fill_f64(double* __restrict__ a, double v, int64_t n) {
 __global__ void k_cell_key(DevParams P, const double* __restrict__ x,
                            const int32_t* __restrict__ rep, const RepGeom* __restrict__ geo,
                            int32_t* __restrict__ key, int32_t* __restrict__ slot,
-                           int32_t* __restrict__ count) {
+                           int32_t* __restrict__ count, const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
   const bool act = i0 < P.n;
   const int i = act ? i0 : P.n - 1;  // every lane takes part in the histogram
@@ -85,7 +92,9 @@ __device__ __forceinline__ int32_t block_exclusive_scan_1024(int32_t v, int32_t&
 }
 
 __global__ __launch_bounds__(1024) void k_scan_tile_sums(const int32_t* __restrict__ count,
-                                                         int32_t* __restrict__ sums, int32_t m) {
+                                                         int32_t* __restrict__ sums, int32_t m,
+                                                         const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   __shared__ int32_t wsum[17];
   const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * 4;
   int32_t local = 0;
@@ -98,7 +107,9 @@ __global__ __launch_bounds__(1024) void k_scan_tile_sums(const int32_t* __restri
 }
 
 __global__ __launch_bounds__(1024) void k_scan_sums(int32_t* __restrict__ sums, int32_t ntile,
-                                                    int32_t* __restrict__ out_total) {
+                                                    int32_t* __restrict__ out_total,
+                                                    const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   __shared__ int32_t wsum[17];
   __shared__ int32_t carry;
   if (threadIdx.x == 0) carry = 0;
@@ -119,7 +130,9 @@ __global__ __launch_bounds__(1024) void k_scan_sums(int32_t* __restrict__ sums, 
 
 __global__ __launch_bounds__(1024) void k_scan_tiles(const int32_t* __restrict__ count,
                                                      const int32_t* __restrict__ sums,
-                                                     int32_t* __restrict__ start, int32_t m) {
+                                                     int32_t* __restrict__ start, int32_t m,
+                                                     const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   __shared__ int32_t wsum[17];
   const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * 4;
   int32_t v[4], local = 0;
@@ -139,11 +152,12 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(const int32_t* __restrict__
 
 // start[m] receives the total
 void launch_exclusive_scan(hipStream_t s, const int32_t* count, int32_t* start, int32_t m,
-                           int32_t* tile_sums) {
+                           int32_t* tile_sums, const int32_t* gate) {
   const int32_t ntile = (m + kScanTile - 1) / kScanTile;
-  hipLaunchKernelGGL(k_scan_tile_sums, dim3(ntile), dim3(1024), 0, s, count, tile_sums, m);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, tile_sums, ntile, start + m);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(1024), 0, s, count, tile_sums, start, m);
+  hipLaunchKernelGGL(k_scan_tile_sums, dim3(ntile), dim3(1024), 0, s, count, tile_sums, m, gate);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, tile_sums, ntile, start + m, gate);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(1024), 0, s, count, tile_sums, start, m,
+                     gate);
 }
 
 // Work map of the pair kernels: each (y, z) row of the sort grid is cut
@@ -152,7 +166,8 @@ void launch_exclusive_scan(hipStream_t s, const int32_t* count, int32_t* start, 
 __global__ void k_row_chunks(DevParams P, const int32_t* __restrict__ start,
                              const int32_t* __restrict__ row_cell,
                              const int32_t* __restrict__ row_len,
-                             int32_t* __restrict__ row_nchunk) {
+                             int32_t* __restrict__ row_nchunk, const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= P.nrows) return;
   const int c0 = row_cell[r];
@@ -161,14 +176,17 @@ __global__ void k_row_chunks(DevParams P, const int32_t* __restrict__ start,
 }
 
 __global__ void k_chunk_map(DevParams P, const int32_t* __restrict__ row_chunk_off,
-                            int32_t* __restrict__ chunk_row) {
+                            int32_t* __restrict__ chunk_row, const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= P.nrows) return;
   for (int c = row_chunk_off[r]; c < row_chunk_off[r + 1]; ++c) chunk_row[c] = r;
 }
 
 __global__ void k_scatter(int n, const int32_t* __restrict__ key, const int32_t* __restrict__ slot,
-                          const int32_t* __restrict__ start, int32_t* __restrict__ unsorted) {
+                          const int32_t* __restrict__ start, int32_t* __restrict__ unsorted,
+                          const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   unsorted[start[key[i]] + slot[i]] = i;
@@ -186,7 +204,8 @@ __global__ void k_place(DevParams P, const int32_t* __restrict__ key,
                         int32_t* __restrict__ rep2, double* __restrict__ ct2,
                         int32_t* __restrict__ key2, float4* __restrict__ xa,
                         const int32_t* __restrict__ sid, int32_t* __restrict__ sid2,
-                        int32_t* __restrict__ slot_of) {
+                        int32_t* __restrict__ slot_of, const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = P.n;
   if (i >= n) return;
@@ -230,6 +249,30 @@ __global__ void k_place(DevParams P, const int32_t* __restrict__ key,
                       __int_as_float(cx));
 }
 
+// The sorted state back from the *_alt buffers into the primary ones, and
+// the sorted positions into x_ref (the positions of the lists' build).
+__global__ void k_copy_back(DevParams P, DeviceState S, const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = P.n;
+  if (i >= n) return;
+  for (int d = 0; d < P.dim; ++d) {
+    const double xv = S.x_alt[(int64_t)d * n + i];
+    S.x[(int64_t)d * n + i] = xv;
+    if (S.x_ref) S.x_ref[(int64_t)d * n + i] = xv;
+    S.v[(int64_t)d * n + i] = S.v_alt[(int64_t)d * n + i];
+  }
+  S.m[i] = S.m_alt[i];
+  S.pid[i] = S.pid_alt[i];
+  S.perm[i] = S.perm_alt[i];
+  S.rep[i] = S.rep_alt[i];
+  if (!P.newtonian) {
+    const int d2 = P.dim * P.dim;
+    for (int q = 0; q < d2; ++q) S.ct[(int64_t)q * n + i] = S.ct_alt[(int64_t)q * n + i];
+  }
+  if (P.ncoll) S.sid[i] = S.sid_alt[i];
+}
+
 // ---------------------------------------------------------------------
 // PPM's cell list on PPM's grid (cell edge >= 1.0001 cut_off), in upload
 // indices: the order inside a cell is ascending upload index, which is
@@ -269,11 +312,11 @@ __global__ void k_ref_rank(int n, const int32_t* __restrict__ key,
 // ---------------------------------------------------------------------
 static inline int blocks_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
-void launch_fill_i32(hipStream_t s, int32_t* a, int32_t v, int64_t n) {
+void launch_fill_i32(hipStream_t s, int32_t* a, int32_t v, int64_t n, const int32_t* gate) {
   if (n <= 0) return;
   int nb = blocks_for(n, 256);
   if (nb > 4096) nb = 4096;
-  hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, s, a, v, n);
+  hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, s, a, v, n, gate);
 }
 
 void launch_fill_f64(hipStream_t s, double* a, double v, int64_t n) {
@@ -283,28 +326,32 @@ void launch_fill_f64(hipStream_t s, double* a, double v, int64_t n) {
   hipLaunchKernelGGL(k_fill_f64, dim3(nb), dim3(256), 0, s, a, v, n);
 }
 
-void launch_cell_keys(hipStream_t s, const DevParams& P, const DeviceState& S) {
-  launch_fill_i32(s, S.count, 0, P.ncell);
+void launch_cell_keys(hipStream_t s, const DevParams& P, const DeviceState& S,
+                      const int32_t* gate) {
+  launch_fill_i32(s, S.count, 0, P.ncell, gate);
   hipLaunchKernelGGL(k_cell_key, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, P, S.x, S.rep,
-                     S.geo, S.key, S.slot, S.count);
+                     S.geo, S.key, S.slot, S.count, gate);
 }
 
-void launch_cell_sort_after_keys(hipStream_t s, const DevParams& P, const DeviceState& S) {
+void launch_cell_sort_after_keys(hipStream_t s, const DevParams& P, const DeviceState& S,
+                                 const int32_t* gate, bool copy_back) {
   const int n = P.n;
-  launch_exclusive_scan(s, S.count, S.start, P.ncell, S.scan_tmp);
+  launch_exclusive_scan(s, S.count, S.start, P.ncell, S.scan_tmp, gate);
   hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, S.key, S.slot,
-                     S.start, S.unsorted);
+                     S.start, S.unsorted, gate);
   hipLaunchKernelGGL(k_place, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, S.key, S.start,
                      S.unsorted, S.x, S.v, S.m, S.pid, S.perm, S.rep,
                      P.newtonian ? nullptr : S.ct, S.geo, S.x_alt, S.v_alt, S.m_alt, S.pid_alt,
                      S.perm_alt, S.rep_alt, P.newtonian ? nullptr : S.ct_alt, S.key_sorted, S.xa,
-                     P.ncoll ? S.sid : nullptr, S.sid_alt, S.slot_of);
+                     P.ncoll ? S.sid : nullptr, S.sid_alt, S.slot_of, gate);
+  if (copy_back)
+    hipLaunchKernelGGL(k_copy_back, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, S, gate);
   const int nrows = P.nrows;
   hipLaunchKernelGGL(k_row_chunks, dim3(blocks_for(nrows, 256)), dim3(256), 0, s, P, S.start,
-                     S.row_cell, S.row_len, S.row_nchunk);
-  launch_exclusive_scan(s, S.row_nchunk, S.row_chunk_off, nrows, S.scan_tmp);
+                     S.row_cell, S.row_len, S.row_nchunk, gate);
+  launch_exclusive_scan(s, S.row_nchunk, S.row_chunk_off, nrows, S.scan_tmp, gate);
   hipLaunchKernelGGL(k_chunk_map, dim3(blocks_for(nrows, 256)), dim3(256), 0, s, P,
-                     S.row_chunk_off, S.chunk_row);
+                     S.row_chunk_off, S.chunk_row, gate);
 }
 
 void launch_reference_cells(hipStream_t s, const DevParams& P, const DeviceState& S) {
@@ -314,7 +361,7 @@ void launch_reference_cells(hipStream_t s, const DevParams& P, const DeviceState
                      S.rslot, S.rcount);
   launch_exclusive_scan(s, S.rcount, S.rstart, P.ncell_ref, S.scan_tmp);
   hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, S.rkey, S.rslot,
-                     S.rstart, S.runsorted);
+                     S.rstart, S.runsorted, nullptr);
   hipLaunchKernelGGL(k_ref_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, S.rkey, S.rstart,
                      S.runsorted, S.perm, S.rlpdx);
 }

@@ -77,6 +77,16 @@ struct DevParams {
   // plus a colloid's radius, outer layer and the cut-off: the nearest image
   // of a colloid seen from there may lie across the face)
   int32_t coll_le_cells;
+  // Verlet lists (periodic fluid boxes, single or batched): the stored
+  // lists hold every candidate within rc + skin at their build, the state
+  // keeps its order until the next build, and a build happens only once a
+  // particle has moved more than skin / 2 since the last one.  Every step
+  // decides in / out of the cut-off afresh on the current positions.
+  // verlet = 0: lists of in-cut-off pairs, rebuilt every step.
+  int32_t verlet;
+  double skin;
+  double half_skin2;      // (skin / 2)^2, FP64 displacement test
+  float skin2_hi_f32;     // FP32 acceptance of the build scan, (rc + skin)^2 and the band
 };
 
 // Colloid state read by the pair kernels (disks in 2D, spheres in 3D):
@@ -515,6 +525,21 @@ __device__ __forceinline__ void coll_noslip(const DevParams& P, const CollHot& C
   }
 #pragma unroll
   for (int d = 0; d < DIM; ++d) vc[d] = -corr * (vf[d] - vcoll[d] - r_vs[d]) + vcoll[d] + r_vs[d];
+}
+
+// x_i - x_j with the reference's periodic image chosen from the positions:
+// the neighbour within half a box.  Across a face the particle on the low
+// side is the one shifted by +L in symmetric mode ((x_i + L) - x_j, PPM's
+// upper ghost layer), the neighbour by -L otherwise (x_i - (x_j - L)); a
+// neighbour on the low side is seen at x_j + L (ppm_bcdef_periodic_x.inc:
+// 37-90).  Two rounded operations, as the reference forms the ghost
+// coordinate first.  Valid for pairs closer than L / 2.
+__device__ __forceinline__ double image_diff_pos(double xi, double xj, double L, int symmetry) {
+#pragma clang fp contract(off)
+  const double raw = xi - xj;
+  if (fabs(raw) <= 0.5 * L) return raw;
+  if (raw > 0.0) return xi - (xj + L);
+  return symmetry ? (xi + L) - xj : xi - (xj - L);
 }
 
 __device__ __forceinline__ double image_diff(double xi, double xj, double L, int mode) {

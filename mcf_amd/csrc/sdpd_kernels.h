@@ -66,27 +66,55 @@ struct DeviceState {
   int32_t *coll_start, *coll_members;
   // reference-grid cell list (built on request by sdpd_get_cells)
   int32_t *rkey, *rslot, *rcount, *rstart, *runsorted, *rlpdx;
+  // Verlet lists: positions at the last build (cell order), the step's
+  // gate (gate[0] = 1: the lists are rebuilt this step; gate[1] counts the
+  // builds), the bits of the largest squared displacement since the build
+  // seen this step, and the in-cut-off neighbours of each particle in the
+  // last density pass (every mode)
+  double* x_ref;
+  int32_t* gate;
+  unsigned long long* disp;
+  int32_t* nbr_in;
 };
 
-void launch_fill_i32(hipStream_t s, int32_t* a, int32_t v, int64_t n);
+void launch_fill_i32(hipStream_t s, int32_t* a, int32_t v, int64_t n,
+                     const int32_t* gate = nullptr);
 void launch_fill_f64(hipStream_t s, double* a, double v, int64_t n);
 // key + histogram (counts zeroed first), then scan, scatter and place into
-// the *_alt buffers; the caller swaps the ping-pong pointers afterwards
-void launch_cell_keys(hipStream_t s, const DevParams& P, const DeviceState& S);
+// the *_alt buffers.  Without copy_back the caller swaps the ping-pong
+// pointers afterwards; with it the sorted state is copied back into the
+// primary buffers (and into x_ref), so the buffers never change, which lets
+// a captured graph sort on some steps only.  A non-null gate makes every
+// kernel of the sort return at once unless gate[0] != 0.
+void launch_cell_keys(hipStream_t s, const DevParams& P, const DeviceState& S,
+                      const int32_t* gate = nullptr);
 void launch_exclusive_scan(hipStream_t s, const int32_t* count, int32_t* start, int32_t m,
-                           int32_t* tile_sums);
-void launch_cell_sort_after_keys(hipStream_t s, const DevParams& P, const DeviceState& S);
+                           int32_t* tile_sums, const int32_t* gate = nullptr);
+void launch_cell_sort_after_keys(hipStream_t s, const DevParams& P, const DeviceState& S,
+                                 const int32_t* gate = nullptr, bool copy_back = false);
 // PPM's cell list on PPM's grid, particles ordered by upload index inside a
 // cell (ppm_util_rank3d on the particles in upload order); fills rstart and
 // rlpdx without moving the state
 void launch_reference_cells(hipStream_t s, const DevParams& P, const DeviceState& S);
 
-void launch_density(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed);
+// scan_all (Verlet lists, the autotuner): every chunk scans its stencil,
+// without storing lists, whatever the step's gate
+void launch_density(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed,
+                    bool scan_all = false);
 void launch_eos(hipStream_t s, const DevParams& P, const DeviceState& S);
 void launch_forces(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed);
 void launch_signature(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed);
 void launch_refresh_records(hipStream_t s, const DevParams& P, const DeviceState& S);
 
+// Verlet lists: displacement since the last build and the FP32 cell-relative
+// positions after a drift (every particle), then the step's gate: 1 when a
+// particle moved more than skin / 2 (the displacement maximum is cleared)
+void launch_track(hipStream_t s, const DevParams& P, const DeviceState& S);
+void launch_decide(hipStream_t s, const DevParams& P, const DeviceState& S);
+// after a sort outside the step: gate[0] = 1 (build at the next density pass)
+void launch_force_build(hipStream_t s, const DeviceState& S);
+// after a density pass: gate[0] = 0 (the lists are current)
+void launch_lists_current(hipStream_t s, const DeviceState& S);
 void launch_kick(hipStream_t s, const DevParams& P, const DeviceState& S, double lambda);
 void launch_drift_wrap(hipStream_t s, const DevParams& P, const DeviceState& S);
 void launch_conformation(hipStream_t s, const DevParams& P, const DeviceState& S, double lambda);

@@ -175,6 +175,26 @@ __device__ __forceinline__ double exact_d2(const DevParams& P, const double* __r
   return DIM == 3 ? dist2_exact(d0, d1, d2) : dist2_exact2(d0, d1);
 }
 
+// Exact FP64 squared distance of a pair with the image chosen from the
+// positions (image_diff_pos): the reference's formula and convention
+// without the cell classes, so it holds between sorts (Verlet lists),
+// after either particle crossed a periodic face.
+template <int DIM>
+__device__ __forceinline__ double exact_d2_pos(const DevParams& P, const Home& h, double xj0,
+                                               double xj1, double xj2, double& d0, double& d1,
+                                               double& d2) {
+  d0 = image_diff_pos(h.x0, xj0, h.G.L0, P.symmetry);
+  d1 = image_diff_pos(h.x1, xj1, h.G.L1, P.symmetry);
+  d2 = DIM == 3 ? image_diff_pos(h.x2, xj2, h.G.L2, P.symmetry) : 0.0;
+  return DIM == 3 ? dist2_exact(d0, d1, d2) : dist2_exact2(d0, d1);
+}
+
+// FP32 squared distance, rounded as the candidate scan rounds it, so the
+// in / out decisions of the density and force passes coincide
+__device__ __forceinline__ float mixed_s2(float d0, float d1, float d2) {
+  return fmaf(d2, d2, fmaf(d1, d1, d0 * d0));
+}
+
 // FP32 separation from cell-relative positions and the stencil offsets.
 template <int DIM>
 __device__ __forceinline__ void rel_diff(const DevParams& P, const Home& h, const float4& q,
@@ -412,7 +432,9 @@ __device__ __forceinline__ double exact_d2_le(const DevParams& P, const double* 
 // ---------------------------------------------------------------------
 // Candidate scan over the stencil rows of a chunk; returns the number of
 // accepted neighbours.
-template <int DIM, class Pol, int TILE = kTile, int BUF = kBuf>
+// SKIN (Verlet lists): accept every candidate within rc + skin (FP32, with
+// the band's margin); the policy's body takes the in / out decision.
+template <int DIM, class Pol, int TILE = kTile, int BUF = kBuf, bool SKIN = false>
 __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState& S,
                                            const Chunk& ch, const Home& h, Pol& pol,
                                            float4* __restrict__ tile, uint32_t* __restrict__ buf,
@@ -550,9 +572,10 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
           }
           const bool ok = live && wv != self_w;
           const uint32_t ent = wv + kadd;
-          const bool below_lo = s2 < P.cut2_lo_f32, below_hi = s2 < P.cut2_hi_f32;
+          const bool below_lo = SKIN ? s2 < P.skin2_hi_f32 : s2 < P.cut2_lo_f32;
+          const bool below_hi = SKIN ? below_lo : s2 < P.cut2_hi_f32;
           bool in = below_lo;
-          if (ok && below_hi && !below_lo) {
+          if (!SKIN && ok && below_hi && !below_lo) {
             // FP64 re-decision with the reference's image convention
             ++rechecks;
             const int j = (int)(ent & kIndexMask);
@@ -642,14 +665,23 @@ __device__ __forceinline__ double pair_d2(const DevParams& P, const Home& h, con
   return DIM == 3 ? fma(d2, d2, fma(d1, d1, d0 * d0)) : fma(d1, d1, d0 * d0);
 }
 
-template <int DIM>
+// DECIDE (Verlet lists): the entries are candidates within rc + skin; the
+// body takes the exact in / out decision on the current positions and
+// counts the in-cut-off neighbours (nin) and the FP64 re-decisions.
+// Verlet lists evaluate in batches: the positions of a batch of
+// neighbours are gathered before the first pair is evaluated (three 8-byte
+// gathers each, whose latency a one-at-a-time walk exposes); the sums keep
+// the entries' order.  The per-step lists go one at a time (the batch's
+// registers cost the candidate scan occupancy every step).
+template <int DIM, bool DECIDE = false>
 struct DensityF64 {
   const DevParams& P;
   const DeviceState& S;
   const Home& h;
   double acc;
+  int nin = 0, rechecks = 0;
   static constexpr bool kImages = true;
-  static constexpr bool kBatched = false;
+  static constexpr bool kBatched = DECIDE;
   __device__ __forceinline__ void body(uint32_t ent) {
     const int j = (int)(ent & kIndexMask);
     const int64_t n = P.n;
@@ -660,14 +692,59 @@ struct DensityF64 {
     kernel_eval(P, sqrt_nr(r2) * P.inv_h, w, gw);
     acc = acc + (P.density_type == 1 ? S.m[j] * w : w);
   }
+  template <int NB>
+  __device__ __forceinline__ void body_batch(const uint32_t (&e)[NB], const bool (&ok)[NB]) {
+    const int64_t n = P.n;
+    double x0[NB], x1[NB], x2[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      const int j = (int)(e[k] & kIndexMask);
+      x0[k] = S.x[j];
+      x1[k] = S.x[n + j];
+      x2[k] = DIM == 3 ? S.x[2 * n + j] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      double d0, d1, d2, r2;
+      bool use = ok[k];
+      if (DECIDE) {
+        // the reference's decision (pp_interaction.inc:37-46) on the
+        // current positions
+        r2 = exact_d2_pos<DIM>(P, h, x0[k], x1[k], x2[k], d0, d1, d2);
+        use = use && r2 < P.cut2;
+        nin += use ? 1 : 0;
+      } else {
+        r2 = pair_d2<DIM>(P, h, decode_images(e[k]), x0[k], x1[k], x2[k], d0, d1, d2);
+      }
+      // beyond the cut-off the kernel is 0: no branch on the decision
+      double w, gw;
+      kernel_eval(P, sqrt_nr(r2) * P.inv_h, w, gw);
+      if (use) acc = acc + (P.density_type == 1 ? S.m[(int)(e[k] & kIndexMask)] * w : w);
+    }
+  }
 };
 
+// The in / out decision of a candidate from its FP32 squared distance: in
+// below the band, out above it, inside it the reference's FP64 formula.
 template <int DIM>
+__device__ __forceinline__ bool mixed_inside(const DevParams& P, const DeviceState& S,
+                                             const Home& h, int j, float s2, int& rechecks) {
+  if (s2 < P.cut2_lo_f32) return true;
+  if (!(s2 < P.cut2_hi_f32)) return false;
+  ++rechecks;
+  const int64_t n = P.n;
+  double e0, e1, e2;
+  return exact_d2_pos<DIM>(P, h, S.x[j], S.x[n + j], DIM == 3 ? S.x[2 * n + j] : 0.0, e0, e1,
+                           e2) < P.cut2;
+}
+
+template <int DIM, bool DECIDE = false>
 struct DensityMixed {
   const DevParams& P;
   const DeviceState& S;
   const Home& h;
   KahanF acc;
+  int nin = 0, rechecks = 0;
   static constexpr bool kImages = false;
   static constexpr bool kBatched = true;
   template <int NB>
@@ -686,10 +763,16 @@ struct DensityMixed {
       const Offsets o = decode_code(ok[k] ? e[k] >> kCodeShift : 62u);
       float d0, d1, d2;
       rel_diff<DIM>(P, h, q[k], o, d0, d1, d2);
-      const float r = __builtin_amdgcn_sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+      const float s2 = mixed_s2(d0, d1, d2);
+      bool use = ok[k];
+      if (DECIDE && use) {
+        use = mixed_inside<DIM>(P, S, h, (int)(e[k] & kIndexMask), s2, rechecks);
+        nin += use ? 1 : 0;
+      }
+      const float r = __builtin_amdgcn_sqrtf(s2);
       float w, gw;
       kernel_w_gradw_f32(P.kernel_type, r * P.inv_hf, P.coeff, P.coef_gradf, w, gw);
-      if (ok[k]) acc.add(P.density_type == 1 ? mj[k] * w : w);
+      if (use) acc.add(P.density_type == 1 ? mj[k] * w : w);
     }
   }
 };
@@ -751,6 +834,79 @@ __device__ __forceinline__ void write_force_records(const DevParams& P, const De
   }
 }
 
+// A chunk one of whose particles has more in-cutoff neighbours than the
+// stored lists hold: its wave scans the stencil again in the force pass.
+__device__ __forceinline__ bool list_overflow(const DevParams& P, const DeviceState& S,
+                                              const Chunk& ch) {
+  const int i = ch.p0 + (int)threadIdx.x;
+  const int cnt = i < ch.p1 ? S.nbr_count[i] : 0;
+  return __ballot(cnt > P.list_cap) != 0ull;
+}
+
+// Walk the neighbours stored in a chunk's lists (waves without list
+// overflow): the in-cut-off pairs of the step's density pass, or the Verlet
+// lists' candidates.  Policies that evaluate in batches get kEvalBatch
+// entries at a time from every lane (slots past a lane's count carry its
+// own particle and are flagged off).
+template <int DIM, class Pol>
+__device__ __forceinline__ void for_each_listed(const DevParams& P, const DeviceState& S,
+                                                const Chunk& ch, const Home& h, Pol& pol,
+                                                uint32_t* __restrict__ buf) {
+  const int cnt = h.active ? S.nbr_count[h.i] : 0;
+  const uint32_t* __restrict__ blk = list_block(P, S.nlist, ch);
+  const int lane = threadIdx.x;
+  if constexpr (Pol::kBatched) {
+    (void)buf;
+    for (int q = 0; __ballot(q < cnt) != 0ull; q += 8) {
+      uint4 lo = make_uint4(0u, 0u, 0u, 0u), hi = lo;
+      if (q < cnt) {
+        const uint4* src = reinterpret_cast<const uint4*>(blk + group_offset(q, lane));
+        lo = src[0];
+        hi = src[1];
+      }
+      const uint32_t e8[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+      for (int b0 = 0; b0 < 8; b0 += kEvalBatch) {
+        uint32_t e[kEvalBatch];
+        bool ok[kEvalBatch];
+#pragma unroll
+        for (int k = 0; k < kEvalBatch; ++k) {
+          ok[k] = q + b0 + k < cnt;
+          e[k] = ok[k] ? e8[b0 + k] : (uint32_t)h.ii;
+        }
+        pol.body_batch(e, ok);
+      }
+    }
+  } else {
+    for (int q = 0; __ballot(q < cnt) != 0ull; q += 8) {
+      if (q < cnt) {
+        // eight entries per pair of 16-byte loads; entries past cnt are
+        // padding inside the row and are not evaluated.  The group goes to
+        // LDS (entry u of a lane at buf[u * 64 + lane], conflict-free), and
+        // the loop reads one entry per pair, the next one ahead of the body.
+        const uint4* src = reinterpret_cast<const uint4*>(blk + group_offset(q, lane));
+        const uint4 lo = src[0], hi = src[1];
+        buf[0 * kWave + lane] = lo.x;
+        buf[1 * kWave + lane] = lo.y;
+        buf[2 * kWave + lane] = lo.z;
+        buf[3 * kWave + lane] = lo.w;
+        buf[4 * kWave + lane] = hi.x;
+        buf[5 * kWave + lane] = hi.y;
+        buf[6 * kWave + lane] = hi.z;
+        buf[7 * kWave + lane] = hi.w;
+        const int m = min(8, cnt - q);
+        uint32_t ent = buf[lane];
+#pragma unroll 1
+        for (int u = 0; u < m; ++u) {
+          const uint32_t cur = ent;
+          ent = buf[((u + 1) & 7) * kWave + lane];
+          pol.body(cur);
+        }
+      }
+    }
+  }
+}
+
 // Whether a chunk's row of cells lies within `cells` cells of a face of
 // dimension le_g (2: its stencil rows reach across the face).
 __device__ __forceinline__ bool chunk_meets_le_face(int le_g, const Chunk& ch, int cells) {
@@ -765,33 +921,61 @@ __device__ __forceinline__ bool chunk_meets_le_face(int le_g, const Chunk& ch, i
 // launches over the same grid, LE = true for the chunks next to a face and
 // LE = false for the rest, whose rows never cross a face (the sheared
 // frame's state would cost the whole pass registers and occupancy).
-template <int DIM, bool MIXED, int TILE, int BUF, bool LE>
+//
+// Modes.  kStore: the in-cut-off pairs found by the candidate scan are
+// stored; lists rebuilt every step.  Verlet lists decide in / out of the
+// cut-off per candidate on the current positions: kBuild (steps whose gate
+// is set) scans the stencil for every candidate within rc + skin and stores
+// them; kWalk (the other steps) walks the stored candidates of the chunks
+// whose lists hold them all; kRescan (those steps) scans the stencil again,
+// without storing, for the chunks with an overflowed list; kScanAll does
+// that for every chunk whatever the gate (the autotuner).  The walk and the
+// scans visit the candidates in the same order, so they give the same sums.
+enum DensityMode { kStore = 0, kBuild = 1, kWalk = 2, kRescan = 3, kScanAll = 4 };
+
+template <int DIM, bool MIXED, int TILE, int BUF, bool LE, int MODE>
 __global__ __launch_bounds__(kWave) void k_density(DevParams P, DeviceState S) {
+  if (MODE == kBuild && S.gate[0] == 0) return;
+  if ((MODE == kWalk || MODE == kRescan) && S.gate[0] != 0) return;
+  constexpr bool kDecide = MODE != kStore;
+  constexpr bool kScan = MODE != kWalk;
   const int le_box = P.le_g;
   if (!LE) P.le_g = -1;
-  __shared__ float4 tile[TILE + 1];
+  __shared__ float4 tile[kScan ? TILE + 1 : 1];
   // whole groups of 8 rows: the final flush stores the last group of a
   // lane's entries as 8 rows, of which those past its count are padding
   __shared__ uint32_t buf[(BUF + 7) / 8 * 8 * kWave];
   Chunk ch;
   if (!load_chunk(P, S, ch)) return;
   if (le_box >= 0 && chunk_meets_le_face(le_box, ch, 2) != LE) return;
+  if ((MODE == kWalk || MODE == kRescan) && list_overflow(P, S, ch) != (MODE == kRescan)) return;
   const Home h = load_home<DIM>(P, S, ch);
   const double mi = S.m[h.ii];
+  uint32_t* list = (MODE == kStore || MODE == kBuild) ? S.nlist : nullptr;
   double rho;
-  int rechecks = 0, cnt;
+  int rechecks = 0, cnt = 0, nin;
   if (MIXED) {
-    DensityMixed<DIM> pol{P, S, h, KahanF{}};
+    using Pol = DensityMixed<DIM, kDecide>;
+    Pol pol{P, S, h, KahanF{}};
     pol.acc.add(P.density_type == 1 ? (float)(P.w0 * mi) : (float)P.w0);
-    cnt = staged_scan<DIM, DensityMixed<DIM>, TILE, BUF>(P, S, ch, h, pol, tile, buf, rechecks,
-                                                          S.nlist);
+    if constexpr (kScan)
+      cnt = staged_scan<DIM, Pol, TILE, BUF, kDecide>(P, S, ch, h, pol, tile, buf, rechecks, list);
+    else
+      for_each_listed<DIM>(P, S, ch, h, pol, buf);
     rho = pol.acc.value();
+    nin = kDecide ? pol.nin : cnt;
+    rechecks += pol.rechecks;
   } else {
     // self term first (particles_compute_density.F90:369-558)
-    DensityF64<DIM> pol{P, S, h, P.density_type == 1 ? P.w0 * mi : P.w0};
-    cnt = staged_scan<DIM, DensityF64<DIM>, TILE, BUF>(P, S, ch, h, pol, tile, buf, rechecks,
-                                                        S.nlist);
+    using Pol = DensityF64<DIM, kDecide>;
+    Pol pol{P, S, h, P.density_type == 1 ? P.w0 * mi : P.w0};
+    if constexpr (kScan)
+      cnt = staged_scan<DIM, Pol, TILE, BUF, kDecide>(P, S, ch, h, pol, tile, buf, rechecks, list);
+    else
+      for_each_listed<DIM>(P, S, ch, h, pol, buf);
     rho = pol.acc;
+    nin = kDecide ? pol.nin : cnt;
+    rechecks += pol.rechecks;
   }
   if (!h.active) return;
   const int i = h.i;
@@ -808,7 +992,8 @@ __global__ __launch_bounds__(kWave) void k_density(DevParams P, DeviceState S) {
   const double bn = 1.0 / (rho * rho);
   S.an[i] = an;
   S.bn[i] = bn;
-  S.nbr_count[i] = cnt;
+  if (MODE == kStore || MODE == kBuild) S.nbr_count[i] = cnt;
+  S.nbr_in[i] = nin;
   S.recheck[i] = rechecks;
   write_force_records<DIM>(P, S, i, an, bn, press);
 }
@@ -866,8 +1051,10 @@ __device__ __forceinline__ void noise_vector(const DevParams& P, int64_t step, u
 
 // force types: 3 Hu-Adams, 4 Hu-Adams angular, 5 Oldroyd-B (non-Newtonian
 // Hu-Adams).  The bodies sum m_i times the acceleration; finish_force
-// divides by m_i once.
-template <int DIM, int FT, bool COLL = false>
+// divides by m_i once.  DECIDE (Verlet lists): the entries are candidates
+// within rc + skin and the body skips those outside the cut-off, deciding
+// exactly as the density pass of the same step did.
+template <int DIM, int FT, bool COLL = false, bool DECIDE = false>
 struct ForceF64 {
   const DevParams& P;
   const DeviceState& S;
@@ -897,8 +1084,13 @@ struct ForceF64 {
     }
     const double4* rj = reinterpret_cast<const double4*>(S.rec64 + 8 * (int64_t)j);
     const double4 ra = rj[0], rb = rj[1];
-    double d0, d1, d2;
-    const double r2 = pair_d2<DIM>(P, h, o, ra.x, ra.y, ra.z, d0, d1, d2);
+    double d0, d1, d2, r2;
+    if (DECIDE) {
+      r2 = exact_d2_pos<DIM>(P, h, ra.x, ra.y, ra.z, d0, d1, d2);
+      if (!(r2 < P.cut2)) return;
+    } else {
+      r2 = pair_d2<DIM>(P, h, o, ra.x, ra.y, ra.z, d0, d1, d2);
+    }
     const double ir = rsqrt_nr(r2);
     const double r = r2 * ir;
     double w, gw;
@@ -1040,7 +1232,7 @@ struct ForceF64 {
   }
 };
 
-template <int DIM, int FT>
+template <int DIM, int FT, bool DECIDE = false>
 struct ForceMixed {
   const DevParams& P;
   const DeviceState& S;
@@ -1074,7 +1266,11 @@ struct ForceMixed {
                                            uint32_t pidj, const float4& qj, const Offsets& o) {
     float d0, d1, d2;
     rel_diff<DIM>(P, h, xj, o, d0, d1, d2);
-    const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    const float r2 = mixed_s2(d0, d1, d2);
+    if (DECIDE) {
+      int rechecks = 0;
+      if (!mixed_inside<DIM>(P, S, h, jidx, r2, rechecks)) return;
+    }
     const float ir = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * ir;
     float w, gw;
@@ -1172,52 +1368,6 @@ __device__ __forceinline__ void finish_force(const DevParams& P, const DeviceSta
   if (DIM == 3) S.f[2 * n + i] = f2;
 }
 
-// A chunk one of whose particles has more in-cutoff neighbours than the
-// stored lists hold: its wave scans the stencil again in the force pass.
-__device__ __forceinline__ bool list_overflow(const DevParams& P, const DeviceState& S,
-                                              const Chunk& ch) {
-  const int i = ch.p0 + (int)threadIdx.x;
-  const int cnt = i < ch.p1 ? S.nbr_count[i] : 0;
-  return __ballot(cnt > P.list_cap) != 0ull;
-}
-
-// Walk the in-cutoff neighbours stored by the density pass of the same
-// step (waves without list overflow).
-template <int DIM, class Pol>
-__device__ __forceinline__ void for_each_listed(const DevParams& P, const DeviceState& S,
-                                                const Chunk& ch, const Home& h, Pol& pol,
-                                                uint32_t* __restrict__ buf) {
-  const int cnt = h.active ? S.nbr_count[h.i] : 0;
-  const uint32_t* __restrict__ blk = list_block(P, S.nlist, ch);
-  const int lane = threadIdx.x;
-  for (int q = 0; __ballot(q < cnt) != 0ull; q += 8) {
-    if (q < cnt) {
-      // eight entries per pair of 16-byte loads; entries past cnt are
-      // padding inside the row and are not evaluated.  The group goes to
-      // LDS (entry u of a lane at buf[u * 64 + lane], conflict-free), and
-      // the loop reads one entry per pair, the next one ahead of the body.
-      const uint4* src = reinterpret_cast<const uint4*>(blk + group_offset(q, lane));
-      const uint4 lo = src[0], hi = src[1];
-      buf[0 * kWave + lane] = lo.x;
-      buf[1 * kWave + lane] = lo.y;
-      buf[2 * kWave + lane] = lo.z;
-      buf[3 * kWave + lane] = lo.w;
-      buf[4 * kWave + lane] = hi.x;
-      buf[5 * kWave + lane] = hi.y;
-      buf[6 * kWave + lane] = hi.z;
-      buf[7 * kWave + lane] = hi.w;
-      const int m = min(8, cnt - q);
-      uint32_t ent = buf[lane];
-#pragma unroll 1
-      for (int u = 0; u < m; ++u) {
-        const uint32_t cur = ent;
-        ent = buf[((u + 1) & 7) * kWave + lane];
-        pol.body(cur);
-      }
-    }
-  }
-}
-
 // particles_compute_act.F90:140-155 on particle i's own ct and velocity
 // gradient L: stores vgt = L and act = c L^T + L c - (c - I) / tau.
 template <int DIM>
@@ -1246,14 +1396,14 @@ __device__ __forceinline__ void conformation_rate(const DevParams& P, const Devi
 
 // The neighbours of a chunk: its stored lists, or (SCAN) the stencil
 // scanned again.
-template <int DIM, bool SCAN, class Pol>
+template <int DIM, bool SCAN, bool SKIN, class Pol>
 __device__ __forceinline__ void visit_neighbours(const DevParams& P, const DeviceState& S,
                                                  const Chunk& ch, const Home& h, Pol& pol,
                                                  float4* __restrict__ tile,
                                                  uint32_t* __restrict__ buf) {
   if constexpr (SCAN) {
     int rechecks = 0;
-    (void)staged_scan<DIM>(P, S, ch, h, pol, tile, buf, rechecks, nullptr);
+    (void)staged_scan<DIM, Pol, kTile, kBuf, SKIN>(P, S, ch, h, pol, tile, buf, rechecks, nullptr);
   } else {
     (void)tile;
     for_each_listed<DIM>(P, S, ch, h, pol, buf);
@@ -1273,8 +1423,9 @@ constexpr int kForceLdsWordsF64 = 2048;
 // fewer registers, so more waves share a SIMD).  With colloids a box with
 // sheared faces splits the chunks once more: the Lees-Edwards
 // instantiation takes the rows within coll_le_cells of a face, the plain
-// one (a quarter fewer registers) the interior.
-template <int DIM, int FT, bool MIXED, bool COLL, bool LE, bool SCAN>
+// one (a quarter fewer registers) the interior.  VERLET: the lists hold
+// the Verlet candidates and the bodies decide in / out of the cut-off.
+template <int DIM, int FT, bool MIXED, bool COLL, bool LE, bool SCAN, bool VERLET = false>
 __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
   const int le_box = P.le_g;
   if (!LE) P.le_g = -1;
@@ -1292,7 +1443,7 @@ __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
     const float4 vi = S.vr[ii];
     const float4 qi = S.qr[ii];
     // Newtonian records: qi.w = p/n^2 (Hu-Adams) or p (angular), vi.w = 1/n^2
-    ForceMixed<DIM, FT> pol{P, S, h, vi.x, vi.y, vi.z, FT == 3 ? qi.w : 0.f, vi.w,
+    ForceMixed<DIM, FT, VERLET> pol{P, S, h, vi.x, vi.y, vi.z, FT == 3 ? qi.w : 0.f, vi.w,
                             FT == 4 ? qi.w : 0.f,
                             FT == 5 ? __float_as_uint(vi.w) : (uint32_t)S.pid[ii],
                             S.step_dev[0]};
@@ -1307,7 +1458,7 @@ __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
       pol.pb[5] = q2.z;
       pol.inv_ni = (float)(1.0 / S.rho[ii]);
     }
-    visit_neighbours<DIM, SCAN>(P, S, ch, h, pol, tile, buf);
+    visit_neighbours<DIM, SCAN, VERLET>(P, S, ch, h, pol, tile, buf);
     if (!h.active) return;
     finish_force<DIM>(P, S, h.i, pol.f0.value() * inv_mi, pol.f1.value() * inv_mi,
                       DIM == 3 ? pol.f2.value() * inv_mi : 0.0);
@@ -1318,7 +1469,7 @@ __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
       conformation_rate<DIM>(P, S, h.i, Ld);
     }
   } else {
-    ForceF64<DIM, FT, COLL> pol{P, S, h};
+    ForceF64<DIM, FT, COLL, VERLET> pol{P, S, h};
     if (COLL) {
       pol.sidi = S.sid[ii];
       pol.le_t = S.time_dev[1];
@@ -1349,7 +1500,7 @@ __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
           pol.pti[a + DIM * b] = val;
         }
     }
-    visit_neighbours<DIM, SCAN>(P, S, ch, h, pol, tile, buf);
+    visit_neighbours<DIM, SCAN, VERLET>(P, S, ch, h, pol, tile, buf);
     if (!h.active) return;
     const int i = h.i;
     finish_force<DIM>(P, S, i, pol.f0 * inv_mi, pol.f1 * inv_mi, pol.f2 * inv_mi,
@@ -1370,18 +1521,34 @@ __device__ __forceinline__ uint32_t pair_hash(uint32_t a, uint32_t b) {
   return h;
 }
 
+// DECIDE (Verlet lists, between sorts): the scan takes every candidate
+// within rc + skin and the body keeps the ones inside the cut-off on the
+// current positions.
+template <int DIM, bool DECIDE>
 struct SignaturePol {
-  const int32_t* __restrict__ pid;
+  const DevParams& P;
+  const DeviceState& S;
+  const Home& h;
   uint32_t pidi;
   uint32_t hs;
+  int nin = 0;
   static constexpr bool kImages = true;
   static constexpr bool kBatched = false;
   __device__ __forceinline__ void body(uint32_t ent) {
-    hs += pair_hash(pidi, (uint32_t)pid[(int)(ent & kIndexMask)]);
+    const int j = (int)(ent & kIndexMask);
+    if (DECIDE) {
+      const int64_t n = P.n;
+      double d0, d1, d2;
+      if (!(exact_d2_pos<DIM>(P, h, S.x[j], S.x[n + j], DIM == 3 ? S.x[2 * n + j] : 0.0, d0, d1,
+                              d2) < P.cut2))
+        return;
+    }
+    ++nin;
+    hs += pair_hash(pidi, (uint32_t)S.pid[j]);
   }
 };
 
-template <int DIM, bool LE>
+template <int DIM, bool LE, bool VERLET>
 __global__ __launch_bounds__(kWave) void k_signature(DevParams P, DeviceState S) {
   if (!LE) P.le_g = -1;
   __shared__ float4 tile[kTile + 1];
@@ -1389,12 +1556,13 @@ __global__ __launch_bounds__(kWave) void k_signature(DevParams P, DeviceState S)
   Chunk ch;
   if (!load_chunk(P, S, ch)) return;
   const Home h = load_home<DIM>(P, S, ch);
-  SignaturePol pol{S.pid, (uint32_t)S.pid[h.ii], 0u};
+  SignaturePol<DIM, VERLET> pol{P, S, h, (uint32_t)S.pid[h.ii], 0u};
   int rechecks = 0;
-  const int cnt = staged_scan<DIM>(P, S, ch, h, pol, tile, buf, rechecks, nullptr);
+  (void)staged_scan<DIM, SignaturePol<DIM, VERLET>, kTile, kBuf, VERLET>(P, S, ch, h, pol, tile,
+                                                                         buf, rechecks, nullptr);
   if (!h.active) return;
   S.nbr_hash[h.i] = pol.hs;
-  S.nbr_sig_count[h.i] = cnt;
+  S.nbr_sig_count[h.i] = pol.nin;
   S.recheck[h.i] = rechecks;
 }
 
@@ -1407,41 +1575,57 @@ static inline int nchunks_max(const DevParams& P) {
   return (m + 7) / 8 * 8;
 }
 
-template <int DIM, bool MIXED, bool LE>
+template <int DIM, bool MIXED, bool LE, int MODE>
 static void launch_density_shape(hipStream_t s, const DevParams& P, const DeviceState& S) {
   dim3 g(nchunks_max(P)), b(kWave);
   switch (P.density_shape) {
     case 1:
-      hipLaunchKernelGGL((k_density<DIM, MIXED, 128, 16, LE>), g, b, 0, s, P, S);
+      hipLaunchKernelGGL((k_density<DIM, MIXED, 128, 16, LE, MODE>), g, b, 0, s, P, S);
       break;
     case 2:
-      hipLaunchKernelGGL((k_density<DIM, MIXED, 224, 12, LE>), g, b, 0, s, P, S);
+      hipLaunchKernelGGL((k_density<DIM, MIXED, 224, 12, LE, MODE>), g, b, 0, s, P, S);
       break;
     case 3:
-      hipLaunchKernelGGL((k_density<DIM, MIXED, 128, 12, LE>), g, b, 0, s, P, S);
+      hipLaunchKernelGGL((k_density<DIM, MIXED, 128, 12, LE, MODE>), g, b, 0, s, P, S);
       break;
     default:
-      hipLaunchKernelGGL((k_density<DIM, MIXED, kTile, kBuf, LE>), g, b, 0, s, P, S);
+      hipLaunchKernelGGL((k_density<DIM, MIXED, kTile, kBuf, LE, MODE>), g, b, 0, s, P, S);
   }
 }
 
 template <int DIM, bool MIXED>
-static void launch_density_le(hipStream_t s, const DevParams& P, const DeviceState& S) {
-  if (P.le_g >= 0) launch_density_shape<DIM, MIXED, true>(s, P, S);
-  launch_density_shape<DIM, MIXED, false>(s, P, S);
+static void launch_density_dim(hipStream_t s, const DevParams& P, const DeviceState& S,
+                               bool scan_all) {
+  if (!P.verlet) {
+    if (P.le_g >= 0) launch_density_shape<DIM, MIXED, true, kStore>(s, P, S);
+    launch_density_shape<DIM, MIXED, false, kStore>(s, P, S);
+    return;
+  }
+  if (scan_all) {
+    launch_density_shape<DIM, MIXED, false, kScanAll>(s, P, S);
+    return;
+  }
+  // Verlet lists: each kernel returns at once unless the step's gate
+  // selects it (build, or walk + rescan of overflowed chunks)
+  dim3 g(nchunks_max(P)), b(kWave);
+  launch_density_shape<DIM, MIXED, false, kBuild>(s, P, S);
+  hipLaunchKernelGGL((k_density<DIM, MIXED, kTile, kBuf, false, kWalk>), g, b, 0, s, P, S);
+  hipLaunchKernelGGL((k_density<DIM, MIXED, kTile, kBuf, false, kRescan>), g, b, 0, s, P, S);
+  launch_lists_current(s, S);
 }
 
-void launch_density(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed) {
+void launch_density(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed,
+                    bool scan_all) {
   if (P.dim == 3) {
     if (mixed)
-      launch_density_le<3, true>(s, P, S);
+      launch_density_dim<3, true>(s, P, S, scan_all);
     else
-      launch_density_le<3, false>(s, P, S);
+      launch_density_dim<3, false>(s, P, S, scan_all);
   } else {
     if (mixed)
-      launch_density_le<2, true>(s, P, S);
+      launch_density_dim<2, true>(s, P, S, scan_all);
     else
-      launch_density_le<2, false>(s, P, S);
+      launch_density_dim<2, false>(s, P, S, scan_all);
   }
 }
 
@@ -1456,6 +1640,13 @@ void launch_refresh_records(hipStream_t s, const DevParams& P, const DeviceState
 template <int DIM, int FT, bool MIXED, bool COLL>
 static void launch_forces_le(hipStream_t s, const DevParams& P, const DeviceState& S) {
   dim3 g(nchunks_max(P)), b(kWave);
+  if constexpr (!COLL) {
+    if (P.verlet) {
+      hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, false, false, false, true>), g, b, 0, s, P, S);
+      hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, false, false, true, true>), g, b, 0, s, P, S);
+      return;
+    }
+  }
   if (P.le_g >= 0) {
     hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, true, false>), g, b, 0, s, P, S);
     hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, true, true>), g, b, 0, s, P, S);
@@ -1505,15 +1696,19 @@ void launch_signature(hipStream_t s, const DevParams& P, const DeviceState& S, b
   dim3 g(nchunks_max(P)), b(kWave);
   const bool le = P.le_g >= 0;
   if (P.dim == 3) {
-    if (le)
-      hipLaunchKernelGGL((k_signature<3, true>), g, b, 0, s, P, S);
+    if (P.verlet)
+      hipLaunchKernelGGL((k_signature<3, false, true>), g, b, 0, s, P, S);
+    else if (le)
+      hipLaunchKernelGGL((k_signature<3, true, false>), g, b, 0, s, P, S);
     else
-      hipLaunchKernelGGL((k_signature<3, false>), g, b, 0, s, P, S);
+      hipLaunchKernelGGL((k_signature<3, false, false>), g, b, 0, s, P, S);
   } else {
-    if (le)
-      hipLaunchKernelGGL((k_signature<2, true>), g, b, 0, s, P, S);
+    if (P.verlet)
+      hipLaunchKernelGGL((k_signature<2, false, true>), g, b, 0, s, P, S);
+    else if (le)
+      hipLaunchKernelGGL((k_signature<2, true, false>), g, b, 0, s, P, S);
     else
-      hipLaunchKernelGGL((k_signature<2, false>), g, b, 0, s, P, S);
+      hipLaunchKernelGGL((k_signature<2, false, false>), g, b, 0, s, P, S);
   }
 }
 

@@ -101,10 +101,62 @@ __global__ void k_conformation(DevParams P, DeviceState S, double lambda) {
     S.ct[q * n + i] = S.ct[q * n + i] + S.act[q * n + i] * P.dt * lambda;
 }
 
+// Verlet lists: particle i's displacement since the lists were built and
+// its FP32 position relative to the corner of the cell it was sorted into
+// then.  Per dimension the displacement x - x_ref is taken to the nearest
+// period, so a particle that crossed a periodic face keeps its build cell:
+// its FP32 position goes on from there, unwrapped, and stays consistent
+// with the stencil offsets stored in the lists.  Returns the squared
+// displacement.
+__device__ __forceinline__ double track_particle(const DevParams& P, const DeviceState& S,
+                                                 const RepGeom& g, int i, const double* xv) {
+  const int64_t n = P.n;
+  const int kl = S.key_sorted[i] - g.cell_base;
+  const int c0 = kl % g.nm[0];
+  const int c1 = (kl / g.nm[0]) % g.nm[1];
+  const int c2 = kl / (g.nm[0] * g.nm[1]);
+  const int cc[3] = {c0, c1, c2};
+  float a[3] = {0.f, 0.f, 0.f};
+  double d2 = 0.0;
+  for (int d = 0; d < P.dim; ++d) {
+    const double L = g.L[d];
+    const double raw = xv[d] - S.x_ref[d * n + i];
+    const double w = raw > 0.5 * L ? -L : (raw < -0.5 * L ? L : 0.0);
+    const double dd = raw + w;
+    d2 = d2 + dd * dd;
+    a[d] = (float)((xv[d] - (g.lo[d] + (double)cc[d] * g.cs[d])) + w);
+  }
+  const float4 old = S.xa[i];
+  S.xa[i] = make_float4(a[0], a[1], a[2], old.w);
+  return d2;
+}
+
+// The largest squared displacement of the workgroup (256 lanes) into disp.
+// Non-negative doubles order like their bit patterns.  One lane per
+// workgroup reads the running maximum first and only raises it: after the
+// first workgroups nearly none of the 8k per launch needs the atomic, which
+// one address would otherwise serialize.
+__device__ __forceinline__ void block_max_disp(unsigned long long* disp, double d2) {
+  __shared__ double wmax[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wmax[wave] = d2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wmax[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, wmax[w]);
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
+    if (m > 0.0 && bits > __atomic_load_n(disp, __ATOMIC_RELAXED)) atomicMax(disp, bits);
+  }
+}
+
 // Step boundary: the closing half kick of the previous step, then the
 // opening half kick, drift, Oldroyd-B half update and periodic wrap of the
-// next one, then the cell key and cell histogram of the next sort.
-// Cell counts must be zero on entry.
+// next one.  Then either the cell key and cell histogram of the next sort
+// (lists rebuilt every step; cell counts must be zero on entry) or, with
+// Verlet lists, the displacement since the lists' build.
+template <bool VERLET>
 __global__ __launch_bounds__(256) void k_boundary(DevParams P, DeviceState S, int closing,
                                                   int opening) {
   const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -147,6 +199,11 @@ __global__ __launch_bounds__(256) void k_boundary(DevParams P, DeviceState S, in
       }
     }
   }
+  if (VERLET) {
+    const double d2 = act ? track_particle(P, S, g, i, xv) : 0.0;
+    block_max_disp(S.disp, d2);
+    return;
+  }
   // cell histogram of the next sort; the particles are in the cell order
   // of the previous one and move little
   const int32_t k = cell_key_geo(g, dim, xv[0], xv[1], xv[2]);
@@ -155,6 +212,47 @@ __global__ __launch_bounds__(256) void k_boundary(DevParams P, DeviceState S, in
     S.key[i] = k;
     S.slot[i] = sl;
   }
+}
+
+// Displacement tracking alone (Verlet lists, after a separate drift).
+__global__ __launch_bounds__(256) void k_track(DevParams P, DeviceState S) {
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = i0 < P.n;
+  const int i = act ? i0 : P.n - 1;
+  const int64_t n = P.n;
+  double xv[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < P.dim; ++d) xv[d] = S.x[d * n + i];
+  const double d2 = act ? track_particle(P, S, S.geo[S.rep[i]], i, xv) : 0.0;
+  block_max_disp(S.disp, d2);
+}
+
+// The step's gate: rebuild the lists when a particle moved more than
+// skin / 2 since their build (a pair then may have closed in by more than
+// the skin); the displacement maximum starts again from zero.  gate[1]
+// counts the builds.
+__global__ void k_decide(DevParams P, DeviceState S) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const double d2 = __longlong_as_double((long long)S.disp[0]);
+  const int rebuild = d2 > P.half_skin2 ? 1 : 0;
+  // a build still pending (a sort whose lists no density pass has built
+  // yet) stays pending
+  S.gate[0] |= rebuild;
+  S.gate[1] += rebuild;
+  S.disp[0] = 0ull;
+}
+
+// After the density pass: the lists are current (no build pending).
+__global__ void k_lists_current(DeviceState S) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) S.gate[0] = 0;
+}
+
+// A sort requested outside the step (upload, a cell-list call, a check):
+// the next density pass builds the lists.
+__global__ void k_force_build(DeviceState S) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  S.gate[0] = 1;
+  S.gate[1] += 1;
+  S.disp[0] = 0ull;
 }
 
 // Neumaier-compensated sum helpers
@@ -290,8 +388,30 @@ void launch_conformation(hipStream_t s, const DevParams& P, const DeviceState& S
 
 void launch_boundary(hipStream_t s, const DevParams& P, const DeviceState& S, int closing,
                      int opening) {
+  if (P.verlet) {
+    hipLaunchKernelGGL(k_boundary<true>, dim3(nblk(P.n)), dim3(256), 0, s, P, S, closing,
+                       opening);
+    return;
+  }
   launch_fill_i32(s, S.count, 0, P.ncell);
-  hipLaunchKernelGGL(k_boundary, dim3(nblk(P.n)), dim3(256), 0, s, P, S, closing, opening);
+  hipLaunchKernelGGL(k_boundary<false>, dim3(nblk(P.n)), dim3(256), 0, s, P, S, closing,
+                     opening);
+}
+
+void launch_track(hipStream_t s, const DevParams& P, const DeviceState& S) {
+  hipLaunchKernelGGL(k_track, dim3(nblk(P.n)), dim3(256), 0, s, P, S);
+}
+
+void launch_decide(hipStream_t s, const DevParams& P, const DeviceState& S) {
+  hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, s, P, S);
+}
+
+void launch_force_build(hipStream_t s, const DeviceState& S) {
+  hipLaunchKernelGGL(k_force_build, dim3(1), dim3(64), 0, s, S);
+}
+
+void launch_lists_current(hipStream_t s, const DeviceState& S) {
+  hipLaunchKernelGGL(k_lists_current, dim3(1), dim3(64), 0, s, S);
 }
 
 void launch_stats(hipStream_t s, const DevParams& P, const DeviceState& S, int nblocks) {

@@ -123,6 +123,9 @@ SYMBOLS = [
     ("sdpd_set_graphs", C.c_int, [_P, C.c_int32]),
     ("sdpd_set_list_capacity", C.c_int, [_P, C.c_int32]),
     ("sdpd_get_list_capacity", C.c_int, [_P, C.POINTER(C.c_int32)]),
+    ("sdpd_set_skin", C.c_int, [_P, C.c_double]),
+    ("sdpd_get_skin", C.c_int, [_P, _DP]),
+    ("sdpd_get_list_builds", C.c_int, [_P, _LP]),
     ("sdpd_set_tuning", C.c_int, [_P, C.POINTER(CTuning)]),
     ("sdpd_get_tuning", C.c_int, [_P, C.POINTER(CTuning)]),
     ("sdpd_autotune", C.c_int, [_P, C.c_int32, C.POINTER(CTuning), _DP]),
@@ -396,6 +399,22 @@ class SDPD:
         cap = C.c_int32()
         self._check(self.lib.sdpd_get_list_capacity(self.h, C.byref(cap)), "sdpd_get_list_capacity")
         return int(cap.value)
+
+    def set_skin(self, skin: float) -> None:
+        """Verlet skin (< 0: automatic, 0: lists rebuilt every step); before
+        upload."""
+        self._check(self.lib.sdpd_set_skin(self.h, float(skin)), "sdpd_set_skin")
+
+    def skin(self) -> float:
+        s = C.c_double()
+        self._check(self.lib.sdpd_get_skin(self.h, C.byref(s)), "sdpd_get_skin")
+        return s.value
+
+    def list_builds(self) -> int:
+        """Verlet list builds since upload (-1: lists rebuilt every step)."""
+        b = C.c_int64()
+        self._check(self.lib.sdpd_get_list_builds(self.h, C.byref(b)), "sdpd_get_list_builds")
+        return b.value
 
     def set_tuning(self, density_shape: int, chunk_order: int) -> None:
         t = CTuning(int(density_shape), int(chunk_order))

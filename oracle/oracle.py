@@ -186,8 +186,24 @@ def lib():
         L.orc_colloid_nearest.argtypes = [P, C.c_double, dp, C.c_int32, dp, dp, dp]
         L.orc_colloid_noslip.argtypes = [P, dp, dp, dp, C.c_int32, dp]
         L.orc_colloid_collect.argtypes = [P, dp, dp, dp, ip, C.c_int32]
+        L.orc_set_threads.argtypes = [C.c_int32]
         _lib = L
     return _lib
+
+
+def use_library(path: str) -> None:
+    """Load the oracle from another build of the same source (the
+    benchmark's natively tuned CPU baseline); before the first call."""
+    global _lib, LIB_PATH
+    _lib = None
+    LIB_PATH = path
+
+
+def set_threads(n: int) -> None:
+    """Threads of the density and force traversals (1: the reference's
+    serial order, the default; more: a colouring of the home cells whose
+    sums accumulate in another order -- for timing only)."""
+    _check(lib().orc_set_threads(int(n)), "set_threads")
 
 
 _keep = []  # colloid records referenced by the last OrcParams built

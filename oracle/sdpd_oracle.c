@@ -466,38 +466,97 @@ static inline double dist2(int dim, const double *xi, const double *xj,
 typedef void (*visit_fn)(void *ctx, int32_t ip, int32_t jp, double r,
                          const double *rij);
 
-static void traverse(const orc_params *p, const orc_cells *c, visit_fn visit,
-                     void *ctx) {
+/* the candidate pairs of home cell (i, j, k) in the reference order */
+static void traverse_cell(const orc_params *p, const orc_cells *c, visit_fn visit,
+                          void *ctx, int i, int j, int k) {
   const int dim = c->dim;
   const double cut2 = p->cut_off * p->cut_off;
   const int64_t n1 = c->nmt[0];
   const int64_t n2 = dim == 3 ? (int64_t)c->nmt[0] * c->nmt[1] : 0;
   double rij[3];
-  for (int k = c->cstart; k <= c->cend[2]; ++k)
-    for (int j = c->cstart; j <= c->cend[1]; ++j)
-      for (int i = c->cstart; i <= c->cend[0]; ++i) {
-        int64_t ccell = i + n1 * j + n2 * k;
-        for (int it = 0; it < c->nnp; ++it) {
-          int64_t icell = ccell + c->inp[it][0] + n1 * c->inp[it][1] +
-                          n2 * c->inp[it][2];
-          int64_t jcell = ccell + c->jnp[it][0] + n1 * c->jnp[it][1] +
-                          n2 * c->jnp[it][2];
-          int32_t istart = c->lhbx[icell], iend = c->lhbx[icell + 1];
-          int32_t jstart = c->lhbx[jcell], jend = c->lhbx[jcell + 1];
-          for (int32_t ipart = istart; ipart < iend; ++ipart) {
-            int32_t ip = c->lpdx[ipart];
-            if (jcell == icell && p->symmetry) jstart = ipart + 1;
-            for (int32_t jpart = jstart; jpart < jend; ++jpart) {
-              if (jcell == icell && jpart == ipart) continue;
-              int32_t jp = c->lpdx[jpart];
-              double d2 = dist2(dim, c->xall + (size_t)dim * ip,
-                                c->xall + (size_t)dim * jp, rij);
-              if (d2 >= cut2) continue;
-              visit(ctx, ip, jp, sqrt(d2), rij);
-            }
-          }
+  int64_t ccell = i + n1 * j + n2 * k;
+  for (int it = 0; it < c->nnp; ++it) {
+    int64_t icell = ccell + c->inp[it][0] + n1 * c->inp[it][1] +
+                    n2 * c->inp[it][2];
+    int64_t jcell = ccell + c->jnp[it][0] + n1 * c->jnp[it][1] +
+                    n2 * c->jnp[it][2];
+    int32_t istart = c->lhbx[icell], iend = c->lhbx[icell + 1];
+    int32_t jstart = c->lhbx[jcell], jend = c->lhbx[jcell + 1];
+    for (int32_t ipart = istart; ipart < iend; ++ipart) {
+      int32_t ip = c->lpdx[ipart];
+      if (jcell == icell && p->symmetry) jstart = ipart + 1;
+      for (int32_t jpart = jstart; jpart < jend; ++jpart) {
+        if (jcell == icell && jpart == ipart) continue;
+        int32_t jp = c->lpdx[jpart];
+        double d2 = dist2(dim, c->xall + (size_t)dim * ip,
+                          c->xall + (size_t)dim * jp, rij);
+        if (d2 >= cut2) continue;
+        visit(ctx, ip, jp, sqrt(d2), rij);
+      }
+    }
+  }
+}
+
+/* Threads of the density and force traversals.  1 (the default) keeps the
+ * reference's serial order; only the CPU baseline of the benchmark asks
+ * for more. */
+static int g_threads = 1;
+
+int orc_set_threads(int32_t n) {
+#ifdef _OPENMP
+  g_threads = n < 1 ? 1 : n;
+  return 0;
+#else
+  g_threads = 1;
+  return n <= 1 ? 0 : -1;
+#endif
+}
+
+#ifdef _OPENMP
+/* Home cells split by the parity of their two outer indices (k, j in 3D;
+ * j, i in 2D) into four colours.  The half stencil reaches offsets 0 and
+ * +1 only (the full stencil writes the home particle only), so two home
+ * rows of one colour never touch the same particle, and a colour's rows
+ * run in parallel.  Sums then accumulate in another order than the serial
+ * traversal's: a timing path, not the checker's. */
+static void traverse_colored(const orc_params *p, const orc_cells *c, visit_fn visit,
+                             void *ctx) {
+  const int dim = c->dim;
+  const int a0 = c->cstart, a1 = dim == 3 ? c->cend[2] : c->cend[1];
+  const int b0 = c->cstart, b1 = dim == 3 ? c->cend[1] : c->cend[0];
+  for (int color = 0; color < 4; ++color) {
+    const int pa = color & 1, pb = color >> 1;
+    const int na = a1 - a0 + 1 - pa > 0 ? (a1 - a0 + 1 - pa + 1) / 2 : 0;
+    const int nb = b1 - b0 + 1 - pb > 0 ? (b1 - b0 + 1 - pb + 1) / 2 : 0;
+#pragma omp parallel for collapse(2) schedule(dynamic) num_threads(g_threads)
+    for (int ia = 0; ia < na; ++ia)
+      for (int ib = 0; ib < nb; ++ib) {
+        const int a = a0 + pa + 2 * ia, b = b0 + pb + 2 * ib;
+        if (dim == 3) {
+          for (int i = c->cstart; i <= c->cend[0]; ++i) traverse_cell(p, c, visit, ctx, i, b, a);
+        } else {
+          traverse_cell(p, c, visit, ctx, b, a, 0);
         }
       }
+  }
+}
+#endif
+
+/* shared: visit writes only the sums of ip and jp (the pair-list visitor
+ * appends to one list and runs serially) */
+static void traverse(const orc_params *p, const orc_cells *c, visit_fn visit,
+                     void *ctx, int shared) {
+  (void)shared;
+#ifdef _OPENMP
+  if (g_threads > 1 && shared) {
+    traverse_colored(p, c, visit, ctx);
+    return;
+  }
+#endif
+  for (int k = c->cstart; k <= c->cend[2]; ++k)
+    for (int j = c->cstart; j <= c->cend[1]; ++j)
+      for (int i = c->cstart; i <= c->cend[0]; ++i)
+        traverse_cell(p, c, visit, ctx, i, j, k);
 }
 
 /* ------------------------------------------------------------------ */
@@ -526,7 +585,7 @@ int orc_neighbor_pairs(const orc_params *p, const double *x, int32_t n,
   orc_cells c;
   if (cells_build(p, x, n, &c)) return -1;
   pairs_ctx pc = {&c, pairs, cap, 0, 0};
-  traverse(p, &c, visit_pairs, &pc);
+  traverse(p, &c, visit_pairs, &pc, 0);
   cells_free(&c);
   *npairs = pc.n;
   return pc.overflow ? -1 : 0;
@@ -661,7 +720,7 @@ static int density_on_cells(const orc_params *p, const orc_cells *c,
   dc.mall = mall;
   dc.sall = sall;
   dc.rho = rho;
-  traverse(p, c, visit_density, &dc);
+  traverse(p, c, visit_density, &dc, 1);
   /* ghost contributions back to their owners (map_ghost_put) */
   if (p->symmetry)
     for (int32_t g = c->nreal; g < c->nall; ++g)
@@ -1220,7 +1279,7 @@ static int interaction_on_cells(const orc_params *p, const orc_cells *c,
   }
   fc.v = va; fc.rho = ra; fc.press = pa; fc.m = ma; fc.pid = ia; fc.sid = sa;
   fc.ct = ct; fc.f = fa; fc.vgt = vg; fc.pt = pt;
-  traverse(p, c, visit_force, &fc);
+  traverse(p, c, visit_force, &fc, 1);
   if (p->symmetry)
     for (int32_t g = c->nreal; g < na; ++g) {
       int32_t o = c->owner[g];

@@ -191,6 +191,13 @@ int orc_colloid_noslip(const orc_params *p, const double *xf, const double *xc,
 int orc_colloid_collect(const orc_params *p, const double *x, const double *f,
                         const double *m, const int32_t *sid, int32_t n);
 
+/* Threads of the density and force traversals (OpenMP builds): 1, the
+ * default, is the reference's serial order; more colour the home cells so
+ * that parallel rows never share a particle, which changes the order of
+ * the sums.  For timing only (the benchmark's CPU baseline); returns -1
+ * when the build has no OpenMP and n > 1. */
+int orc_set_threads(int32_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,175 @@
+"""Verlet lists on the GPU: lists of every pair within cut_off + skin,
+rebuilt only once a particle has moved more than skin / 2, with the in / out
+decision of the cut-off taken afresh every step on the current positions.
+
+Reference: technique_build_list is called every step
+(marching_integrate_VV.F90:968-971); PPM's Verlet lists
+(ppm.tgz:src/ppm_neighlist_vlist.F90:40-50,170-205) keep pairs within
+cutoff + skin.  The bars are the parity bars of test_gpu_parity: the
+neighbour sets equal the oracle's bit for bit at every checked step, also
+between rebuilds and after particles crossed periodic faces; trajectories
+agree to 1e-10.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from mcf_amd import workloads as W
+from mcf_amd.sdpd import SDPD, SDPDError
+
+from gpu_helpers import oracle_signature, rel_err, vec_err
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def drifting_box(n_side=12, drift=(0.3, 0.3, -0.3), dt=5e-3, kt=1e-3):
+    """A Tait box moving as a whole, its lattice shifted so that planes of
+    particles sit a few hundredths of dx from the faces they move towards:
+    they cross the periodic faces within a few steps (dt large enough that
+    particles move ~1e-3 dx per step)."""
+    box = W.tait_box_3d(n_side, kt=kt)
+    shift = 0.45 * np.sign(np.asarray(drift, np.float64))
+    box.x = np.ascontiguousarray(W.wrap(box.x + shift[None, :], box.prm))
+    box.v = np.ascontiguousarray(box.v + np.asarray(drift)[None, :])
+    box.prm = box.prm.replace(dt=dt)
+    return box
+
+
+def wrapped_around(x0, x1, L):
+    """Particles whose coordinate jumped by more than half a box: they
+    crossed a periodic face."""
+    return np.abs(x1 - x0) > 0.5 * L
+
+
+def run(box, steps, skin=None, cap=0, signature=False):
+    with SDPD(box.prm) as g:
+        if skin is not None:
+            g.set_skin(skin)
+        g.set_list_capacity(cap)
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.integrate_VV(steps)
+        x, v = g.download()
+        f = g.download_force()
+        rho, _ = g.download_density()
+        builds = g.list_builds()
+        sig = g.neighbor_signature() if signature else None
+        pairs, _ = g.counters()
+    return dict(x=x, v=v, f=f, rho=rho, builds=builds, sig=sig, pairs=pairs)
+
+
+def oracle_run(oracle, box, steps):
+    st = oracle.State(box.prm, box.x, box.v, box.m, box.pid)
+    st.prologue(0)
+    for s in range(1, steps + 1):
+        st.vv_step(s)
+    return st
+
+
+def assert_close_to_oracle(box, out, st):
+    L = np.array(box.prm.length[: box.prm.num_dim])
+    dx = out["x"] - st.x
+    dx -= L * np.round(dx / L)
+    assert np.abs(dx).max() < 1e-10 * L.max()
+    assert vec_err(out["v"], st.v) < 1e-10
+    assert vec_err(out["f"], st.f) < 1e-10
+    assert rel_err(out["rho"], st.rho) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["tait_fp64", "tait_mixed", "shear2d_full", "oldroyd_mixed"])
+def test_verlet_agrees_with_per_step_lists(name):
+    box = {
+        "tait_fp64": lambda: W.tait_box_3d(13, kt=1e-3),
+        "tait_mixed": lambda: W.mixed(W.tait_box_3d(13, kt=1e-3)),
+        "shear2d_full": lambda: W.shear_wave_2d(20, brownian=True, symmetry=False),
+        "oldroyd_mixed": lambda: W.mixed(W.oldroyd_box_3d(12)),
+    }[name]()
+    announce(f"verlet vs per-step lists: {name}")
+    a = run(box, 8)
+    b = run(box, 8, skin=0.0)
+    assert a["builds"] >= 1 and b["builds"] == -1
+    assert a["pairs"] == b["pairs"]
+    bar = 1e-5 if box.prm.precision else 1e-10
+    assert vec_err(a["v"], b["v"]) < bar
+    assert vec_err(a["f"], b["f"]) < bar * 10
+
+
+def test_verlet_rebuilds_and_wraps_against_oracle(oracle):
+    # a thin skin with the box drifting: several rebuilds within 20 steps,
+    # particles crossing the periodic faces between them
+    box = drifting_box()
+    announce(f"verlet rebuilds: n={box.n}")
+    steps = 20
+    out = run(box, steps, skin=0.02, signature=True)
+    assert out["builds"] >= 4
+    assert wrapped_around(box.x, out["x"], 12.0).any(axis=0).all()  # every dimension
+    st = oracle_run(oracle, box, steps)
+    assert_close_to_oracle(box, out, st)
+    b = W.Box("end", box.prm, st.x, st.v, box.m, box.pid)
+    cnt, hs = oracle_signature(oracle, b)
+    assert np.array_equal(out["sig"][0], cnt) and np.array_equal(out["sig"][1], hs)
+
+
+def test_verlet_wraps_without_rebuild(oracle):
+    # the automatic skin (2 cs - rc = 1 dx here) is not used up in 60 steps,
+    # while the drift carries particles across the x faces: the images come
+    # from the positions, not from the build's cells
+    box = drifting_box(drift=(2.0, 0.0, 0.0), dt=1e-3)
+    announce("verlet wraps without rebuild")
+    steps = 60
+    out = run(box, steps, signature=True)
+    assert out["builds"] == 1
+    assert wrapped_around(box.x[:, 0], out["x"][:, 0], 12.0).sum() > 50
+    st = oracle_run(oracle, box, steps)
+    assert_close_to_oracle(box, out, st)
+    b = W.Box("end", box.prm, st.x, st.v, box.m, box.pid)
+    cnt, hs = oracle_signature(oracle, b)
+    assert np.array_equal(out["sig"][0], cnt) and np.array_equal(out["sig"][1], hs)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_verlet_overflowed_lists_over_steps(precision):
+    # capacities below the candidate counts: chunks rescan their stencil in
+    # the density and the force pass of every step; the results must not
+    # move, bit for bit
+    box = drifting_box(dt=1e-3)
+    if precision:
+        box = W.mixed(box)
+    announce(f"verlet overflowed lists: precision {precision}")
+    ref = run(box, 12, skin=0.005)
+    assert ref["builds"] >= 2  # a rebuild inside the run
+    for cap in (48, 8):
+        out = run(box, 12, skin=0.005, cap=cap)
+        for k in ("x", "v", "f", "rho"):
+            assert np.array_equal(out[k], ref[k]), (cap, k)
+        assert out["builds"] == ref["builds"]
+
+
+def test_skin_arguments():
+    box = W.tait_box_3d(12, kt=1e-3)
+    with SDPD(box.prm) as g:
+        assert g.skin() < 0  # automatic before upload
+        g.set_skin(5.0)      # wider than the grid allows (2 cs - rc = 1)
+        with pytest.raises(SDPDError):
+            g.upload(box.x, box.v, box.m, box.pid)
+        g.set_skin(-1.0)
+        g.upload(box.x, box.v, box.m, box.pid)
+        assert 0.99 < g.skin() < 1.0
+        with pytest.raises(SDPDError):
+            g.set_skin(0.1)  # after upload
+        with pytest.raises(SDPDError):
+            g.set_lees_edwards(2, [-0.3, 0, 0], [0.3, 0, 0])  # Verlet lists are for such boxes
+        g.integrate_VV(2)
+        assert g.list_builds() == 1
+    le = W.tait_box_3d(12, kt=1e-3)
+    le.prm = le.prm.replace(le_dim=2, le_v_lower=[-0.3, 0, 0], le_v_upper=[0.3, 0, 0])
+    with SDPD(le.prm) as g:
+        g.upload(le.x, le.v, le.m, le.pid)
+        assert g.skin() == 0.0  # sheared faces: lists rebuilt every step
+        g.integrate_VV(2)
+        assert g.list_builds() == -1

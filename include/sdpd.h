@@ -302,6 +302,32 @@ int sdpd_get_phase_times(sdpd_ctx *ctx, double *ms, int64_t *launches);
 /* Waits for all work queued on the context's stream. */
 int sdpd_sync(sdpd_ctx *ctx);
 
+/* Largest particle count sdpd_evaluate_noalloc accepts. */
+#define SDPD_NOALLOC_MAX_PARTICLES 65536
+
+/* Allocation-free evaluation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It
+ * computes what sdpd_build_cells, sdpd_density and sdpd_forces compute on
+ * an uploaded context (the initial evaluation of marching_marching), for
+ * particles already in device memory, with the same kernels and the same
+ * results bit for bit.  All arrays are device pointers in the reference's
+ * layout: x, v (num_dim, n); m (n); id (2, n) = (pid, sid), sid 0.
+ * Outputs in the input order: rho, p (n; either may be NULL) and the
+ * accelerations f (num_dim, n).  step keys the thermal noise.  One periodic
+ * box of Newtonian fluid (2D or 3D, FP64 or mixed precision, no colloids,
+ * no sheared faces); positions must lie in [min_phys, max_phys).  The work
+ * is queued on `stream` (a hipStream_t, NULL for the null stream) and the
+ * call returns without waiting.  Its working buffers come from one static
+ * device array of 32 MiB that every device the library runs on holds until
+ * the process ends; each call overwrites it, so calls that overlap on one
+ * device race.  More than SDPD_NOALLOC_MAX_PARTICLES particles, or a box
+ * whose cell tables and neighbour lists do not fit the array, are refused
+ * with SDPD_ERR_ARG and a message on stderr. */
+int sdpd_evaluate_noalloc(const sdpd_params *params, int64_t step, const double *x,
+                          const double *v, const double *m, const int32_t *id, int32_t n,
+                          double *rho, double *p, double *f, void *stream);
+
 const char *sdpd_last_error(const sdpd_ctx *ctx);
 int sdpd_destroy(sdpd_ctx *ctx);
 

@@ -138,6 +138,8 @@ SYMBOLS = [
     ("sdpd_set_timing", C.c_int, [_P, C.c_int32]),
     ("sdpd_get_phase_times", C.c_int, [_P, _DP, _LP]),
     ("sdpd_sync", C.c_int, [_P]),
+    ("sdpd_evaluate_noalloc", C.c_int, [C.POINTER(CParams), C.c_int64, _P, _P, _P, _P, C.c_int32,
+                                        _P, _P, _P, _P]),
     ("sdpd_last_error", C.c_char_p, [_P]),
     ("sdpd_destroy", C.c_int, [_P]),
 ]
@@ -498,6 +500,19 @@ def pair_normals(lo, hi, seed: int, step: int):
     if rc != 0:
         raise SDPDError(f"sdpd_pair_normals failed (stat_info={rc})")
     return g
+
+
+def evaluate_noalloc(prm: SDPDParams, step: int, x, v, m, ids, n: int, rho, p, f,
+                     stream=None) -> None:
+    """sdpd_evaluate_noalloc on device buffers given as addresses (ints):
+    x, v (n, dim) float64, m (n) float64, ids (n, 2) int32, outputs rho,
+    p (n) float64 or None, f (n, dim) float64; queued on `stream` (a
+    hipStream_t address, None for the null stream), not waited for."""
+    lib = load_library()
+    cp = to_c(prm)
+    rc = lib.sdpd_evaluate_noalloc(C.byref(cp), int(step), x, v, m, ids, int(n), rho, p, f, stream)
+    if rc != 0:
+        raise SDPDError(f"sdpd_evaluate_noalloc failed (stat_info={rc}); see stderr")
 
 
 def pair_hash(a, b):

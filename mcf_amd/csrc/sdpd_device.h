@@ -257,9 +257,9 @@ __device__ __forceinline__ double eos_pressure(const DevParams& P, double rho) {
 // ---------------------------------------------------------------------
 // counter-based noise: Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
 // keyed by the seed, counter (min pid, max pid, step); normals by
-// Box-Muller with log / sin / cos from Taylor series in exactly rounded
-// float operations (no contraction), so that host and device agree
-// bit for bit.
+// Box-Muller with the hardware's log2, square root and sine / cosine
+// (within about an ulp each).  The stream is the one sdpd_pair_normals
+// returns; checks against a CPU path take their normals from it.
 // ---------------------------------------------------------------------
 __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
                                               uint32_t c3, uint32_t k0, uint32_t k1,
@@ -286,96 +286,24 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
   out[3] = c3;
 }
 
-__device__ __forceinline__ float noise_log(float u) {
-#pragma clang fp contract(off)
-  uint32_t b = __float_as_uint(u);
-  int e = (int)(b >> 23) - 127;
-  float m = __uint_as_float((b & 0x7fffffu) | 0x3f800000u);
-  if (m > 1.41421356f) {
-    m = m * 0.5f;
-    e = e + 1;
-  }
-  // ln m = x P(x), x = m - 1 (exact), P of degree 9 fitted to ln(1+x)/x
-  // on [1/sqrt2 - 1, sqrt2 - 1] for relative error (coefficients rounded
-  // to float); no division: within 2 ulp of ln over every mantissa
-  const float x = m - 1.0f;
-  float p = fmaf(x, -0x1.38158ep-4f, 0x1.08442ap-3f);
-  p = fmaf(x, p, -0x1.0f3b4ep-3f);
-  p = fmaf(x, p, 0x1.2272a8p-3f);
-  p = fmaf(x, p, -0x1.542702p-3f);
-  p = fmaf(x, p, 0x1.99a3dcp-3f);
-  p = fmaf(x, p, -0x1.000428p-2f);
-  p = fmaf(x, p, 0x1.55555p-2f);
-  p = fmaf(x, p, -0x1.fffff8p-2f);
-  p = fmaf(x, p, 1.0f);
-  return fmaf((float)e, 0.693147182f, x * p);
-}
-
-__device__ __forceinline__ void noise_sincos(uint32_t k, float& cs, float& sn) {
-#pragma clang fp contract(off)
-  uint32_t qn = (k + (1u << 21)) >> 22;
-  int32_t rem = (int32_t)k - (int32_t)(qn << 22);
-  float th = (float)rem * 0x1.921fb6p-22f;
-  float s = th * th;
-  float ps = fmaf(s, 2.75573192e-6f, -1.98412698e-4f);
-  ps = fmaf(s, ps, 8.33333333e-3f);
-  ps = fmaf(s, ps, -1.66666667e-1f);
-  float sv = fmaf(th * s, ps, th);
-  float pc = fmaf(s, -2.75573192e-7f, 2.48015873e-5f);
-  pc = fmaf(s, pc, -1.38888889e-3f);
-  pc = fmaf(s, pc, 4.16666667e-2f);
-  pc = fmaf(s, pc, -0.5f);
-  float cv = fmaf(s, pc, 1.0f);
-  // quadrant q: (cos, sin) = (cv, sv), (-sv, cv), (-cv, -sv), (sv, -cv);
-  // odd quadrants swap the two, the signs flip by xor of the sign bit
-  // (the same bits as negation), without branches
-  const uint32_t q = qn & 3u;
-  const bool swap = (q & 1u) != 0u;
-  cs = __uint_as_float(__float_as_uint(swap ? sv : cv) ^ (((q + 1u) & 2u) << 30));
-  sn = __uint_as_float(__float_as_uint(swap ? cv : sv) ^ ((q & 2u) << 30));
-}
-
-// sqrt(y) rounded to nearest, for y zero or a positive normal number (the
-// Box-Muller radius): the hardware estimate, within an ulp, replaced by its
-// neighbour below when y - s_down s <= 0 or above when y - s_up s > 0 --
-// the correction IEEE sqrtf applies, without the rescaling of tiny inputs
-// and the infinity / NaN cases the radius never meets.
-__device__ __forceinline__ float sqrt_rn_normal(float y) {
-  const float s = __builtin_amdgcn_sqrtf(y);
-  const float sd = __uint_as_float(__float_as_uint(s) - 1u);
-  const float su = __uint_as_float(__float_as_uint(s) + 1u);
-  float r = fmaf(-sd, s, y) <= 0.0f ? sd : s;
-  r = fmaf(-su, s, y) > 0.0f ? su : r;
-  return y > 0.0f ? r : y;
-}
-
-// Four N(0,1) deviates for the unordered pair (lo, hi) at a step.  FAST
-// (mixed precision, 1e-5 accuracy) takes the hardware square root and the
-// hardware sine / cosine of 2 pi k / 2^24 (about an ulp; the instructions
-// take the angle in revolutions) instead of the exactly rounded forms; the
-// log stays the polynomial, whose relative accuracy holds for u near 1.
-template <bool FAST = false>
+// Four N(0,1) deviates for the unordered pair (lo, hi) at a step: two
+// Box-Muller transforms of 24-bit uniforms, u in (0, 1] (radius) and the
+// angle 2 pi k / 2^24 (the sine and cosine instructions take revolutions).
+// -2 ln u = -2 ln 2 log2 u, clamped at 0 against a rounding above zero for
+// u next to 1.
 __device__ __forceinline__ void pair_normals(uint64_t seed, uint32_t lo, uint32_t hi,
                                              int64_t step, float g[4]) {
-#pragma clang fp contract(off)
   uint32_t w[4];
   philox4x32_10(lo, hi, (uint32_t)(uint64_t)step, (uint32_t)((uint64_t)step >> 32),
                 (uint32_t)seed, (uint32_t)(seed >> 32), w);
 #pragma unroll
   for (int hh = 0; hh < 2; ++hh) {
-    float u = (float)((w[2 * hh] >> 8) + 1u) * 0x1p-24f;
-    float cs, sn, r;
-    if (FAST) {
-      r = __builtin_amdgcn_sqrtf(-2.0f * noise_log(u));
-      const float rev = (float)(w[2 * hh + 1] >> 8) * 0x1p-24f;
-      cs = __builtin_amdgcn_cosf(rev);
-      sn = __builtin_amdgcn_sinf(rev);
-    } else {
-      r = sqrt_rn_normal(-2.0f * noise_log(u));
-      noise_sincos(w[2 * hh + 1] >> 8, cs, sn);
-    }
-    g[2 * hh] = r * cs;
-    g[2 * hh + 1] = r * sn;
+    const float u = (float)((w[2 * hh] >> 8) + 1u) * 0x1p-24f;
+    const float r =
+        __builtin_amdgcn_sqrtf(fmaxf(-1.38629436112f * __builtin_amdgcn_logf(u), 0.0f));
+    const float rev = (float)(w[2 * hh + 1] >> 8) * 0x1p-24f;
+    g[2 * hh] = r * __builtin_amdgcn_cosf(rev);
+    g[2 * hh + 1] = r * __builtin_amdgcn_sinf(rev);
   }
 }
 

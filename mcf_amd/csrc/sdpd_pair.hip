@@ -1032,7 +1032,7 @@ __device__ __forceinline__ void noise_vector(const DevParams& P, int64_t step, u
                                              T& w2) {
   uint32_t lo = pidi < pidj ? pidi : pidj, hi = pidi < pidj ? pidj : pidi;
   float g[4];
-  pair_normals<sizeof(T) == 4>(P.seed, lo, hi, step, g);
+  pair_normals(P.seed, lo, hi, step, g);
   const T sgn = pidi < pidj ? T(1) : T(-1);
   const T ca = T(0.70710678118654752440);
   if (DIM == 3) {
@@ -1336,7 +1336,7 @@ struct ForceMixed {
       float sc = f_d - f_c;
       if (P.noise) {
         float g[4];
-        pair_normals<true>(P.seed, pidi < pidj ? pidi : pidj, pidi < pidj ? pidj : pidi, step, g);
+        pair_normals(P.seed, pidi < pidj ? pidi : pidj, pidi < pidj ? pidj : pidi, step, g);
         sc = sc + g[0] * P.sqrt_2kt_etaf * __builtin_amdgcn_sqrtf(-P.eta_coeff * aij * ir) *
                       P.inv_sqrt_dtf;
       }

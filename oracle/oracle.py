@@ -187,8 +187,64 @@ def lib():
         L.orc_colloid_noslip.argtypes = [P, dp, dp, dp, C.c_int32, dp]
         L.orc_colloid_collect.argtypes = [P, dp, dp, dp, ip, C.c_int32]
         L.orc_set_threads.argtypes = [C.c_int32]
+        L.orc_set_noise_table.argtypes = [C.c_int64, C.c_int64, C.c_void_p, C.c_void_p]
+        L.orc_vv_begin.argtypes = state
+        L.orc_vv_finish.argtypes = state
         _lib = L
     return _lib
+
+
+# --------------------------------------------------------------------------
+# injected noise: a check of a generator other than the oracle's own hands
+# the oracle that generator's normals for the pairs of each evaluation
+# --------------------------------------------------------------------------
+_noise_source = None
+
+
+def set_noise_source(fn) -> None:
+    """fn(seed, lo, hi, step) -> (n, 4) float32 normals of the pairs
+    (lo[k], hi[k]), lo < hi pids; None: the oracle's own counter-based
+    stream.  While set, every force evaluation with thermal noise takes its
+    normals from fn for exactly the pairs inside the cut-off."""
+    global _noise_source
+    _noise_source = fn
+
+
+class _Injected:
+    """Install the source's normals for the pairs of positions x at a step
+    for the duration of one evaluation."""
+
+    def __init__(self, prm, x, pid, step, le_time=0.0):
+        self.active = (_noise_source is not None and bool(prm.Brownian)
+                       and float(prm.kt) > 1e-14)
+        if not self.active:
+            return
+        pairs = neighbor_pairs(prm, x, le_time=le_time)
+        pid = np.asarray(pid, np.int64)
+        a, b = pid[pairs[:, 0]], pid[pairs[:, 1]]
+        lo, hi = np.minimum(a, b), np.maximum(a, b)
+        keys = np.unique((lo.astype(np.uint64) << np.uint64(32)) | hi.astype(np.uint64))
+        if keys.size == 0:  # no pair inside the cut-off: no noise to draw
+            self.active = False
+            return
+        self.keys = np.ascontiguousarray(keys, np.uint64)
+        lo_k = (self.keys >> np.uint64(32)).astype(np.uint32)
+        hi_k = (self.keys & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+        g = _noise_source(int(prm.random_seed), lo_k, hi_k, int(step))
+        self.normals = np.ascontiguousarray(g, np.float32).reshape(-1, 4)
+        assert self.normals.shape[0] == self.keys.shape[0]
+        self.step = int(step)
+
+    def __enter__(self):
+        if self.active:
+            _check(lib().orc_set_noise_table(self.step, self.keys.shape[0],
+                                             self.keys.ctypes.data, self.normals.ctypes.data),
+                   "set_noise_table")
+        return self
+
+    def __exit__(self, *exc):
+        if self.active:
+            lib().orc_set_noise_table(0, 0, None, None)
 
 
 def use_library(path: str) -> None:
@@ -341,9 +397,10 @@ def interaction(prm, step, x, v, rho, press, m, pid, sid=None, ct=None, le_time=
     if ct is not None:
         ct = _f64(ct)
         ctp = ct.ctypes.data_as(C.c_void_p)
-    _check(lib().orc_interaction(C.byref(to_orc(prm, le_time, coll)), int(step), x, _f64(v), _f64(rho), _f64(press),
-                                 _f64(m), _i32(pid), sid, ctp, n, f,
-                                 vgt.ctypes.data_as(C.c_void_p)), "interaction")
+    with _Injected(prm, x, pid, step, le_time):
+        _check(lib().orc_interaction(C.byref(to_orc(prm, le_time, coll)), int(step), x, _f64(v),
+                                     _f64(rho), _f64(press), _f64(m), _i32(pid), sid, ctp, n, f,
+                                     vgt.ctypes.data_as(C.c_void_p)), "interaction")
     return f, vgt
 
 
@@ -461,8 +518,10 @@ class State:
         # time before the initial evaluation
         n = self.x.shape[0]
         self.le_last = self.time
-        _check(lib().orc_prologue(C.byref(to_orc(self.prm, self.time, self.coll)), int(step), n, self.x,
-                                  self.v, self.m, self.pid, self.sid, *self._args()), "prologue")
+        with _Injected(self.prm, self.x, self.pid, step, self.time):
+            _check(lib().orc_prologue(C.byref(to_orc(self.prm, self.time, self.coll)), int(step), n,
+                                      self.x, self.v, self.m, self.pid, self.sid, *self._args()),
+                   "prologue")
 
     def vv_step(self, step):
         # marching_marching.F90:1392-1404 advances time_current before the
@@ -475,8 +534,16 @@ class State:
         if self.coll is not None:
             self.coll.le_time = self.le_last
         self.le_last = self.time + self.prm.dt
-        _check(lib().orc_vv_step(C.byref(to_orc(self.prm, self.le_last, self.coll)), int(step), n,
-                                 self.x, self.v, self.m, self.pid, self.sid, *self._args()), "vv_step")
+        rec = to_orc(self.prm, self.le_last, self.coll)
+        p = C.byref(rec)
+        args = (int(step), n, self.x, self.v, self.m, self.pid, self.sid) + self._args()
+        if _noise_source is None:
+            _check(lib().orc_vv_step(p, *args), "vv_step")
+            return
+        # the step's noise for the pairs at the positions after the drift
+        _check(lib().orc_vv_begin(p, *args), "vv_begin")
+        with _Injected(self.prm, self.x, self.pid, step, self.le_last):
+            _check(lib().orc_vv_finish(p, *args), "vv_finish")
 
     def statistics(self):
         n = self.x.shape[0]

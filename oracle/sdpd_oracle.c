@@ -1045,6 +1045,47 @@ typedef struct {
   int fail;
 } force_ctx;
 
+/* An injected noise stream: the normals of one step's pairs, sorted by
+ * key = pid_lo << 32 | pid_hi.  While a table is installed for step s the
+ * force pass of step s takes its normals from it (a pair missing from the
+ * table fails the evaluation), so that a check can run on the normals
+ * another generator drew; otherwise the counter-based stream above. */
+static int64_t g_noise_step = 0;
+static int64_t g_noise_count = 0;
+static const uint64_t *g_noise_keys = NULL;
+static const float *g_noise_normals = NULL;
+static int g_noise_missing = 0;
+
+int orc_set_noise_table(int64_t step, int64_t count, const uint64_t *keys,
+                        const float *normals) {
+  if (count < 0 || (count > 0 && (!keys || !normals))) return -1;
+  g_noise_step = step;
+  g_noise_count = count;
+  g_noise_keys = count > 0 ? keys : NULL;
+  g_noise_normals = count > 0 ? normals : NULL;
+  g_noise_missing = 0;
+  return 0;
+}
+
+static void force_normals(uint64_t seed, uint32_t lo, uint32_t hi, int64_t step, float g[4]) {
+  if (g_noise_keys && step == g_noise_step) {
+    const uint64_t key = ((uint64_t)lo << 32) | hi;
+    int64_t a = 0, b = g_noise_count;
+    while (a < b) {
+      const int64_t mid = a + (b - a) / 2;
+      if (g_noise_keys[mid] < key) a = mid + 1; else b = mid;
+    }
+    if (a < g_noise_count && g_noise_keys[a] == key) {
+      for (int q = 0; q < 4; ++q) g[q] = g_noise_normals[4 * a + q];
+      return;
+    }
+    g_noise_missing = 1;
+    for (int q = 0; q < 4; ++q) g[q] = 0.0f;
+    return;
+  }
+  orc_pair_normals(seed, lo, hi, step, g);
+}
+
 /* Random force direction for the ordered pair (ip -> jp): a Gaussian
  * vector with the covariance of the reference's symmetric traceless
  * dW . e_ij, oriented by the lower pid (antisymmetric in ip <-> jp). */
@@ -1054,7 +1095,7 @@ static void noise_vector(const force_ctx *fc, int32_t ip, int32_t jp,
   uint32_t a = (uint32_t)fc->pid[ip], b = (uint32_t)fc->pid[jp];
   uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
   float g[4];
-  orc_pair_normals(fc->p->seed, lo, hi, fc->step, g);
+  force_normals(fc->p->seed, lo, hi, fc->step, g);
   double sgn = a < b ? 1.0 : -1.0;
   double ec[3] = {0, 0, 0};
   for (int d = 0; d < dim; ++d) ec[d] = sgn * eij[d];
@@ -1193,7 +1234,7 @@ static void visit_force(void *vctx, int32_t ip, int32_t jp, double dij,
     if (brown) {
       uint32_t a = (uint32_t)fc->pid[ip], b = (uint32_t)fc->pid[jp];
       float g[4];
-      orc_pair_normals(p->seed, a < b ? a : b, a < b ? b : a, fc->step, g);
+      force_normals(p->seed, a < b ? a : b, a < b ? b : a, fc->step, g);
       double eps = (double)g[0];
       double f_r = eps * sqrt(2.0 * p->kt * p->eta) *
                    sqrt(-p->eta_coef * aij / dij) / sqrt(p->dt);
@@ -1279,7 +1320,9 @@ static int interaction_on_cells(const orc_params *p, const orc_cells *c,
   }
   fc.v = va; fc.rho = ra; fc.press = pa; fc.m = ma; fc.pid = ia; fc.sid = sa;
   fc.ct = ct; fc.f = fa; fc.vgt = vg; fc.pt = pt;
+  g_noise_missing = 0;
   traverse(p, c, visit_force, &fc, 1);
+  if (g_noise_missing) fc.fail = 1;
   if (p->symmetry)
     for (int32_t g = c->nreal; g < na; ++g) {
       int32_t o = c->owner[g];
@@ -1676,22 +1719,43 @@ int orc_prologue(const orc_params *p, int64_t step, int32_t n, double *x,
   return 0;
 }
 
-int orc_vv_step(const orc_params *p, int64_t step, int32_t n, double *x,
-                double *v, const double *m, const int32_t *pid,
-                const int32_t *sid, double *rho, double *press, double *f,
-                double *ct, double *act, double *vgt) {
-  /* marching_integrate_VV.F90:303-1619 */
+/* marching_integrate_VV.F90:303-1619, in two halves around the point where
+ * the positions of the step's evaluation are known (a check may install
+ * the step's injected noise there): the opening half kick, drift, ct half
+ * update, colloid block and wrap; then the evaluation, ct half update and
+ * closing half kick. */
+int orc_vv_begin(const orc_params *p, int64_t step, int32_t n, double *x,
+                 double *v, const double *m, const int32_t *pid,
+                 const int32_t *sid, double *rho, double *press, double *f,
+                 double *ct, double *act, double *vgt) {
+  (void)pid; (void)rho; (void)press; (void)vgt;
   if (check_sid(p, sid, n)) return -1;
   integrate_velocity(p, n, v, f, sid, 0.5);
   integrate_position(p, n, x, v, sid);
   if (!p->newtonian) integrate_ct(p, n, ct, act, 0.5);
   if (p->coll && colloid_block(p, step, n, x, f, m, sid)) return -1;
   adjust_particles(p, n, x, v);
+  return 0;
+}
+
+int orc_vv_finish(const orc_params *p, int64_t step, int32_t n, double *x,
+                  double *v, const double *m, const int32_t *pid,
+                  const int32_t *sid, double *rho, double *press, double *f,
+                  double *ct, double *act, double *vgt) {
   int rc = evaluate(p, step, n, x, v, m, pid, rho, press, f, sid, ct, act, vgt);
   if (rc) return rc;
   if (!p->newtonian) integrate_ct(p, n, ct, act, 0.5);
   integrate_velocity(p, n, v, f, sid, 0.5);
   return 0;
+}
+
+int orc_vv_step(const orc_params *p, int64_t step, int32_t n, double *x,
+                double *v, const double *m, const int32_t *pid,
+                const int32_t *sid, double *rho, double *press, double *f,
+                double *ct, double *act, double *vgt) {
+  int rc = orc_vv_begin(p, step, n, x, v, m, pid, sid, rho, press, f, ct, act, vgt);
+  if (rc) return rc;
+  return orc_vv_finish(p, step, n, x, v, m, pid, sid, rho, press, f, ct, act, vgt);
 }
 
 int orc_statistics(const orc_params *p, const double *v, const double *m,

@@ -191,6 +191,26 @@ int orc_colloid_noslip(const orc_params *p, const double *xf, const double *xc,
 int orc_colloid_collect(const orc_params *p, const double *x, const double *f,
                         const double *m, const int32_t *sid, int32_t n);
 
+/* An injected noise stream for step `step`: count pairs, keys
+ * pid_lo << 32 | pid_hi in ascending order, normals(4, count).  The force
+ * pass of that step takes its normals from the table (a pair missing from
+ * it makes the evaluation fail); count = 0 removes the table.  The arrays
+ * are read in place and must outlive the evaluations that use them. */
+int orc_set_noise_table(int64_t step, int64_t count, const uint64_t *keys,
+                        const float *normals);
+
+/* orc_vv_step in two halves: the opening half kick, drift, ct half update,
+ * colloid block and wrap; then the evaluation at the new positions, ct half
+ * update and closing half kick. */
+int orc_vv_begin(const orc_params *p, int64_t step, int32_t n, double *x,
+                 double *v, const double *m, const int32_t *pid,
+                 const int32_t *sid, double *rho, double *press, double *f,
+                 double *ct, double *act, double *vgt);
+int orc_vv_finish(const orc_params *p, int64_t step, int32_t n, double *x,
+                  double *v, const double *m, const int32_t *pid,
+                  const int32_t *sid, double *rho, double *press, double *f,
+                  double *ct, double *act, double *vgt);
+
 /* Threads of the density and force traversals (OpenMP builds): 1, the
  * default, is the reference's serial order; more colour the home cells so
  * that parallel rows never share a particle, which changes the order of

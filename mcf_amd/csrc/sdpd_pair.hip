@@ -1267,9 +1267,17 @@ struct ForceMixed {
     float d0, d1, d2;
     rel_diff<DIM>(P, h, xj, o, d0, d1, d2);
     const float r2 = mixed_s2(d0, d1, d2);
+    // Verlet candidates: the pair is evaluated either way (outside the
+    // cut-off the kernel gradient is 0) and only added when inside.  The
+    // record's w is pinned to a register ahead of the decision (an empty
+    // asm), so the record arrives in one 16-byte gather with the position
+    // instead of a second 4-byte one after the decision's branch.
+    bool use = true;
+    float xjw = xj.w;
     if (DECIDE) {
+      asm volatile("" : "+v"(xjw));
       int rechecks = 0;
-      if (!mixed_inside<DIM>(P, S, h, jidx, r2, rechecks)) return;
+      use = mixed_inside<DIM>(P, S, h, jidx, r2, rechecks);
     }
     const float ir = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * ir;
@@ -1301,10 +1309,12 @@ struct ForceMixed {
       const float gfac = gir * inv_ni;
       const float du[3] = {-u0, -u1, -u2};
       const float dx[3] = {d0, d1, d2};
+      if (use) {
 #pragma unroll
-      for (int b = 0; b < DIM; ++b)
+        for (int b = 0; b < DIM; ++b)
 #pragma unroll
-        for (int a = 0; a < DIM; ++a) L[a + DIM * b].add(du[a] * dx[b] * gfac);
+          for (int a = 0; a < DIM; ++a) L[a + DIM * b].add(du[a] * dx[b] * gfac);
+      }
       if (P.noise) {
         float w0, w1, w2;
         noise_vector<DIM, float>(P, step, pidi, pidj, d0 * ir, d1 * ir, d2 * ir, w0, w1, w2);
@@ -1314,7 +1324,7 @@ struct ForceMixed {
         fi2 = fi2 + f_r * w2;
       }
     } else if (FT == 3) {
-      const float fcr = (ai + xj.w) * gw * ir;
+      const float fcr = (ai + xjw) * gw * ir;
       const float f_d = P.etaf * (bi + vj.w) * gw * ir;
       fi0 = f_d * u0 - fcr * d0;
       fi1 = f_d * u1 - fcr * d1;
@@ -1329,7 +1339,7 @@ struct ForceMixed {
       }
     } else {
       const float ev = (d0 * u0 + d1 * u1 + d2 * u2) * ir;
-      const float pij = (pi + xj.w) * 0.5f;
+      const float pij = (pi + xjw) * 0.5f;
       const float aij = (bi + vj.w) * gw;
       const float f_c = aij * pij;
       const float f_d = P.eta_coeff * P.etaf * aij * ev * ir;
@@ -1345,6 +1355,7 @@ struct ForceMixed {
       fi1 = sc * d1;
       fi2 = sc * d2;
     }
+    if (!use) return;
     f0.add(fi0);
     f1.add(fi1);
     if (DIM == 3) f2.add(fi2);

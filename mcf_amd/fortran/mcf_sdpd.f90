@@ -172,6 +172,87 @@ CONTAINS
     END IF
   END SUBROUTINE get_ints
 
+  SUBROUTINE get_ints_nth(cfg, key, nth, n, vals)
+    TYPE(config), INTENT(IN) :: cfg
+    CHARACTER(len=*), INTENT(IN) :: key
+    INTEGER, INTENT(IN) :: nth, n
+    INTEGER, INTENT(INOUT) :: vals(n)
+    INTEGER :: tmp(n)
+    INTEGER :: k, ios
+    k = config_find_nth(cfg, key, nth)
+    IF (k > 0) THEN
+       tmp = vals
+       READ(cfg%val(k), *, iostat=ios) tmp
+       IF (ios == 0) vals = tmp
+    END IF
+  END SUBROUTINE get_ints_nth
+
+  ! key = value lines of a restart physics file, each followed by a blank
+  ! line: reals E16.8 and integers Iw, vectors joined by commas (the
+  ! (n(A,E16.8)) records of io_write_restart_physics.F90), logicals L2
+  SUBROUTINE put_reals(u, key, vals)
+    INTEGER, INTENT(IN) :: u
+    CHARACTER(len=*), INTENT(IN) :: key
+    DOUBLE PRECISION, INTENT(IN) :: vals(:)
+    CHARACTER(len=1024) :: buf
+    INTEGER :: k, p
+    buf = key//' = '
+    p = LEN(key) + 4
+    DO k = 1, SIZE(vals)
+       IF (k > 1) THEN
+          buf(p:p) = ','
+          p = p + 1
+       END IF
+       WRITE(buf(p:p + 15), '(E16.8)') vals(k)
+       p = p + 16
+    END DO
+    WRITE(u, '(A)') buf(1:p - 1)
+    WRITE(u, '(A)') ' '
+  END SUBROUTINE put_reals
+
+  SUBROUTINE put_real(u, key, val)
+    INTEGER, INTENT(IN) :: u
+    CHARACTER(len=*), INTENT(IN) :: key
+    DOUBLE PRECISION, INTENT(IN) :: val
+    CALL put_reals(u, key, [val])
+  END SUBROUTINE put_real
+
+  SUBROUTINE put_ints(u, key, vals, w)
+    INTEGER, INTENT(IN) :: u, w
+    CHARACTER(len=*), INTENT(IN) :: key
+    INTEGER, INTENT(IN) :: vals(:)
+    CHARACTER(len=1024) :: buf
+    CHARACTER(len=16) :: form
+    INTEGER :: k, p
+    WRITE(form, '(A,I0,A)') '(I', w, ')'
+    buf = key//' = '
+    p = LEN(key) + 4
+    DO k = 1, SIZE(vals)
+       IF (k > 1) THEN
+          buf(p:p) = ','
+          p = p + 1
+       END IF
+       WRITE(buf(p:p + w - 1), form) vals(k)
+       p = p + w
+    END DO
+    WRITE(u, '(A)') buf(1:p - 1)
+    WRITE(u, '(A)') ' '
+  END SUBROUTINE put_ints
+
+  SUBROUTINE put_int(u, key, val, w)
+    INTEGER, INTENT(IN) :: u, val, w
+    CHARACTER(len=*), INTENT(IN) :: key
+    CALL put_ints(u, key, [val], w)
+  END SUBROUTINE put_int
+
+  SUBROUTINE put_logical(u, key, val)
+    INTEGER, INTENT(IN) :: u
+    CHARACTER(len=*), INTENT(IN) :: key
+    LOGICAL, INTENT(IN) :: val
+    WRITE(u, '(A,L2)') key//' = ', val
+    WRITE(u, '(A)') ' '
+  END SUBROUTINE put_logical
+
   LOGICAL FUNCTION get_logical(cfg, key, default)
     TYPE(config), INTENT(IN) :: cfg
     CHARACTER(len=*), INTENT(IN) :: key
@@ -218,6 +299,7 @@ PROGRAM mcf_sdpd
   ! output files (io_config.mcf keys): statistic, particles, conformation,
   ! restart; FORMATTED (E16.8 records) or UNFORMATTED (binary records)
   CHARACTER(len=VAL_LEN) :: restart_file, conf_out_file, restart_conf_file, read_conf_file
+  CHARACTER(len=VAL_LEN) :: restart_phys_file
   INTEGER :: restart_freq, conf_freq, next_out
   LOGICAL :: out_unf, restart_unf, conf_unf
   DOUBLE PRECISION :: time_cur
@@ -305,6 +387,7 @@ PROGRAM mcf_sdpd
   conf_unf = is_unformatted(get_string(cfg, 'output_conformation_fmt', 'FORMATTED'))
   conf_freq = get_int(cfg, 'output_conformation_freq_step', -1)
   restart_conf_file = get_string(cfg, 'restart_conformation_file', 'mcf_restart_conformation')
+  restart_phys_file = get_string(cfg, 'restart_physics_file', 'mcf_restart_physics')
   read_conf_file = get_string(cfg, 'read_conformation_file', '')
   read_external = get_logical(cfg, 'read_external', .FALSE.)
   fused = get_logical(cfg, 'fused_step', .FALSE.)
@@ -449,7 +532,9 @@ PROGRAM mcf_sdpd
         CALL check(sdpd_upload_ct(ctx, ct), 'sdpd_upload_ct')
      END IF
   END IF
-  time_cur = 0.0d0
+  ! time of step_start: 0, or the time_start of a restart physics file
+  ! (marching_marching.F90:325-331)
+  time_cur = get_real(cfg, 'time_start', 0.0d0)
 
   OPEN(newunit=su, file=TRIM(stat_file), status='replace', action='write')
 
@@ -613,7 +698,149 @@ CONTAINS
     WRITE(name, '(A,I8.8,A)') TRIM(restart_file), st, '.dat'
     CALL write_records(TRIM(name), dat, .TRUE., restart_unf)
     IF (.NOT. newtonian) CALL write_conformation(st, .TRUE.)
+    CALL write_restart_physics(st)
   END SUBROUTINE write_restart
+
+  ! <restart_physics_file><step>.mcf: the physics keys with step_start and
+  ! time_start of this step and the colloids' current state, in the order
+  ! and number formats of io_write_restart_physics.F90:236-953 (E16.8
+  ! reals, I2..I10 integers, logicals); a resumed run names it as its
+  ! mcf_physics_config_file.  Keys this host does not model keep the
+  ! values of the run's own files (their defaults when absent).
+  SUBROUTINE write_restart_physics(st)
+    INTEGER, INTENT(IN) :: st
+    CHARACTER(len=VAL_LEN+16) :: name
+    INTEGER :: u, kk, nsp, iv(6)
+    DOUBLE PRECISION :: rv(12)
+    DOUBLE PRECISION, ALLOCATABLE, TARGET :: cx(:,:), cv(:,:), cw(:,:)
+    WRITE(name, '(A,I8.8,A)') TRIM(restart_phys_file), st, '.mcf'
+    OPEN(newunit=u, file=TRIM(name), status='replace', action='write')
+    WRITE(u, '(A)') '#---------------------------------------------'
+    WRITE(u, '(A)') '# physics keys of the run at this step, read as'
+    WRITE(u, '(A)') '# the keys of physics_config.mcf'
+    WRITE(u, '(A)') '#---------------------------------------------'
+    WRITE(u, '(A)') ' '
+    nsp = 1
+    IF (ncoll > 0) nsp = 2
+    CALL put_int(u, 'num_species', nsp, 2)
+    CALL put_int(u, 'num_dim', dim, 2)
+    CALL put_reals(u, 'min_phys', prm%min_phys(1:dim))
+    CALL put_reals(u, 'max_phys', prm%max_phys(1:dim))
+    iv = 0
+    CALL get_ints(cfg, 'num_part', dim, iv(1:dim))
+    CALL put_ints(u, 'num_part', iv(1:dim), 4)
+    CALL put_real(u, 'cut_off', prm%cut_off)
+    CALL put_real(u, 'dt', prm%dt)
+    CALL put_int(u, 'step_start', st, 10)
+    CALL put_int(u, 'step_end', get_int(cfg, 'step_end', 0), 10)
+    CALL put_real(u, 'time_start', time_cur)
+    CALL put_real(u, 'time_end', get_real(cfg, 'time_end', 0.0d0))
+    CALL put_real(u, 'rho', prm%rho)
+    CALL put_real(u, 'eta', prm%eta)
+    CALL put_real(u, 'eta_coef', prm%eta_coef)
+    CALL put_real(u, 'ksai', get_real(cfg, 'ksai', 0.0d0))
+    CALL put_real(u, 'kt', prm%kt)
+    CALL put_real(u, 'c', prm%c)
+    CALL put_real(u, 'rho_ref', prm%rho_ref)
+    CALL put_real(u, 'gamma', prm%gamma)
+    CALL put_int(u, 'relax_type', get_int(cfg, 'relax_type', 0), 10)
+    CALL put_real(u, 'dt_relax', get_real(cfg, 'dt_relax', 0.0d0))
+    CALL put_int(u, 'step_relax', get_int(cfg, 'step_relax', 0), 10)
+    CALL put_real(u, 'time_relax', get_real(cfg, 'time_relax', 0.0d0))
+    CALL put_real(u, 'disorder_level', get_real(cfg, 'disorder_level', 0.0d0))
+    CALL put_real(u, 'kt_relax', get_real(cfg, 'kt_relax', 0.0d0))
+    CALL put_real(u, 'c_relax', get_real(cfg, 'c_relax', 0.0d0))
+    IF (.NOT. newtonian) THEN
+       CALL put_real(u, 'tau', prm%tau)
+       CALL put_real(u, 'n_p', prm%n_p)
+       CALL put_real(u, 'kt_p', prm%kt_p)
+       CALL put_logical(u, 'eigen_dynamics', get_logical(cfg, 'eigen_dynamics', .FALSE.))
+       rv = 0.0d0
+       CALL get_reals(cfg, 'eval', dim, rv(1:dim))
+       CALL put_reals(u, 'eval', rv(1:dim))
+       CALL put_real(u, 'eval_tolerance', get_real(cfg, 'eval_tolerance', 0.0d0))
+       rv = 0.0d0
+       CALL get_reals(cfg, 'evec', dim * dim, rv(1:dim * dim))
+       CALL put_reals(u, 'evec', rv(1:dim * dim))
+       CALL put_logical(u, 'evec_normalize', get_logical(cfg, 'evec_normalize', .FALSE.))
+       CALL put_real(u, 'evec_tolerance', get_real(cfg, 'evec_tolerance', 0.0d0))
+    END IF
+    CALL put_int(u, 'body_force_type', prm%body_force_type, 3)
+    CALL put_reals(u, 'body_force', prm%body_force(1:dim))
+    rv = 0.0d0
+    CALL get_reals(cfg, 'body_force_d', dim, rv(1:dim))
+    CALL put_reals(u, 'body_force_d', rv(1:dim))
+    CALL put_int(u, 'flow_direction', get_int(cfg, 'flow_direction', 1), 5)
+    CALL put_real(u, 'flow_width', get_real(cfg, 'flow_width', 0.0d0))
+    CALL put_real(u, 'flow_v', get_real(cfg, 'flow_v', 0.0d0))
+    CALL put_int(u, 'flow_adjust_freq', get_int(cfg, 'flow_adjust_freq', 0), 5)
+    CALL put_int(u, 'num_colloid', ncoll, 3)
+    IF (ncoll > 0) THEN
+       ALLOCATE(cx(3, ncoll), cv(3, ncoll), cw(3, ncoll))
+       CALL check(sdpd_get_colloids(ctx, c_loc(cx), c_loc(cv), c_loc(cw), c_null_ptr, c_null_ptr), &
+            'sdpd_get_colloids')
+       CALL put_int(u, 'coll_rho_type', INT(cprm%rho_type), 3)
+       CALL put_logical(u, 'coll_translate', cprm%translate /= 0)
+       CALL put_logical(u, 'coll_rotate', cprm%rotate /= 0)
+       CALL put_int(u, 'coll_place', get_int(cfg, 'coll_place', 1), 3)
+       CALL put_int(u, 'coll_noslip', INT(cprm%noslip), 3)
+       CALL put_int(u, 'coll_body_force_type', INT(cprm%body_force_type), 3)
+       CALL put_reals(u, 'coll_body_force', cprm%body_force(1:dim))
+       CALL put_real(u, 'cc_lub_cut_off', get_real(cfg, 'cc_lub_cut_off', 0.0d0))
+       CALL put_real(u, 'cc_lub_cut_on', get_real(cfg, 'cc_lub_cut_on', 0.0d0))
+       CALL put_real(u, 'cc_repul_cut_off', get_real(cfg, 'cc_repul_cut_off', 0.0d0))
+       CALL put_real(u, 'cc_repul_cut_on', get_real(cfg, 'cc_repul_cut_on', 0.0d0))
+       CALL put_real(u, 'cc_repul_F0', get_real(cfg, 'cc_repul_f0', 0.0d0))
+       CALL put_real(u, 'cw_lub_cut_off', get_real(cfg, 'cw_lub_cut_off', 0.0d0))
+       CALL put_real(u, 'cw_lub_cut_on', get_real(cfg, 'cw_lub_cut_on', 0.0d0))
+       CALL put_real(u, 'cw_repul_cut_off', get_real(cfg, 'cw_repul_cut_off', 0.0d0))
+       CALL put_real(u, 'cw_repul_cut_on', get_real(cfg, 'cw_repul_cut_on', 0.0d0))
+       CALL put_real(u, 'cw_repul_F0', get_real(cfg, 'cw_repul_f0', 0.0d0))
+       DO kk = 1, ncoll
+          CALL put_int(u, 'coll_shape', 2, 3)
+          rv = 0.0d0
+          rv(1) = c_rad(kk)
+          CALL get_reals_nth(cfg, 'coll_radius', kk, dim, rv(1:dim))
+          CALL put_reals(u, 'coll_radius', rv(1:dim))
+          iv = 1
+          CALL get_ints_nth(cfg, 'coll_freq', kk, 1, iv(1:1))
+          CALL put_int(u, 'coll_freq', iv(1), 3)
+          CALL put_real(u, 'coll_m', c_m(kk))
+          CALL put_reals(u, 'coll_mmi', c_mmi(1:3, kk))
+          CALL put_reals(u, 'coll_x', cx(1:dim, kk))
+          CALL put_reals(u, 'coll_v', cv(1:dim, kk))
+          rv = 0.0d0
+          CALL get_reals_nth(cfg, 'coll_rot_vector', kk, 4, rv(1:4))
+          CALL put_reals(u, 'coll_rot_vector', rv(1:4))
+          rv = 0.0d0
+          CALL get_reals_nth(cfg, 'coll_theta', kk, 3, rv(1:3))
+          CALL put_reals(u, 'coll_theta', rv(1:3))
+          CALL put_reals(u, 'coll_omega', cw(1:3, kk))
+       END DO
+       ! the keys of the colloid integrator this host reads with them
+       CALL put_int(u, 'integrate_colloid_type', get_int(cfg, 'integrate_colloid_type', 2), 3)
+       CALL put_int(u, 'integrate_colloid_AB', INT(cprm%integrate_ab), 3)
+       CALL put_int(u, 'coll_sub_time_step', INT(cprm%sub_steps), 3)
+    END IF
+    iv = 1
+    CALL get_ints(cfg, 'bcdef', 2 * dim, iv(1:2 * dim))
+    CALL put_ints(u, 'bcdef', iv(1:2 * dim), 2)
+    iv = 0
+    CALL get_ints(cfg, 'shear_type', 2 * dim, iv(1:2 * dim))
+    CALL put_ints(u, 'shear_type', iv(1:2 * dim), 2)
+    rv = 0.0d0
+    CALL get_reals(cfg, 'shear_v', 2 * dim * (dim - 1), rv(1:2 * dim * (dim - 1)))
+    CALL put_reals(u, 'shear_v', rv(1:2 * dim * (dim - 1)))
+    rv = 0.0d0
+    CALL get_reals(cfg, 'shear_freq', 2 * dim, rv(1:2 * dim))
+    CALL put_reals(u, 'shear_freq', rv(1:2 * dim))
+    CALL put_int(u, 'wall_rho_type', get_int(cfg, 'wall_rho_type', 0), 3)
+    CALL put_int(u, 'wall_noslip', get_int(cfg, 'wall_noslip', 0), 2)
+    WRITE(u, '(A)') '#---------------------------------------------'
+    WRITE(u, '(A)') '# end of the restart physics keys'
+    WRITE(u, '(A)') '#---------------------------------------------'
+    CLOSE(u)
+  END SUBROUTINE write_restart_physics
 
   ! io_write_conformation.F90:175-245 (records of dim*dim values), or the
   ! restart conformation (count first, io_write_restart_conformation.F90)

@@ -16,7 +16,12 @@ Files are written with Fortran edit descriptors, reproduced here:
   <output_conformation_file>[_r]<step>.out, records (dim*dim E16.8);
   restart conformation (io_write_restart_conformation.F90): the count,
   then those records; io_read_conformation.F90 reads the count and then
-  dim*dim values per particle.
+  dim*dim values per particle;
+* restart physics (io_write_restart_physics.F90:236-953): file
+  <restart_physics_file><step, 8 digits>.mcf of key = value lines in the
+  layout of physics_config.mcf (reals E16.8, vectors joined by commas),
+  step_start and time_start being the step written; read_config reads it
+  and the other key = value files.
 
 Readers take the values list-directed, as the reference's READ(unit, *)
 does.  Host <-> device copies happen only when a file is written: the
@@ -171,6 +176,95 @@ def read_conformation(path: str, num_dim: int) -> np.ndarray:
     if vals.size != n * d2:
         raise ValueError("actual number of lines is not equal to number given !")
     return vals.reshape(n, d2)
+
+
+def config_line(key: str, values, width: int = 0) -> str:
+    """One key = value record of a restart physics file: reals as E16.8,
+    integers as I<width> (width > 0), logicals as L2, several values joined
+    by commas (the (n(A,E16.8)) records of io_write_restart_physics.F90)."""
+    vals = list(values) if isinstance(values, (list, tuple, np.ndarray)) else [values]
+    if isinstance(vals[0], (bool, np.bool_)):
+        return f"{key} =  {'T' if vals[0] else 'F'}"
+    if width > 0:
+        return f"{key} = " + ",".join(fortran_i(int(v), width) for v in vals)
+    return f"{key} = " + ",".join(fortran_e(float(v)) for v in vals)
+
+
+def restart_physics_lines(prm, step: int, time: float, step_end: int, keys=None) -> list:
+    """The restart physics file of a run without colloids at `step`
+    (io_write_restart_physics.F90:236-953): the physics keys in the
+    reference's order with step_start = step and time_start = time, each
+    record followed by a blank line.  keys overrides the values of keys the
+    parameter record does not hold (num_part, time_end, ksai, relax keys,
+    eval / evec, body_force_d, flow keys, bcdef, shear_type, shear_v,
+    shear_freq, wall keys); absent ones get the reference's defaults."""
+    keys = dict(keys or {})
+    d = prm.num_dim
+    k = lambda name, default: keys.get(name, default)  # noqa: E731
+    recs = [("num_species", 1, 2), ("num_dim", d, 2),
+            ("min_phys", prm.min_phys[:d], 0), ("max_phys", prm.max_phys[:d], 0),
+            ("num_part", k("num_part", [0] * d), 4), ("cut_off", prm.cut_off, 0),
+            ("dt", prm.dt, 0), ("step_start", step, 10), ("step_end", step_end, 10),
+            ("time_start", time, 0), ("time_end", k("time_end", 0.0), 0),
+            ("rho", prm.rho, 0), ("eta", prm.eta, 0), ("eta_coef", prm.eta_coef, 0),
+            ("ksai", k("ksai", 0.0), 0), ("kt", prm.kt_effective, 0), ("c", prm.c, 0),
+            ("rho_ref", prm.rho_ref, 0), ("gamma", prm.gamma, 0),
+            ("relax_type", k("relax_type", 0), 10), ("dt_relax", k("dt_relax", 0.0), 0),
+            ("step_relax", k("step_relax", 0), 10), ("time_relax", k("time_relax", 0.0), 0),
+            ("disorder_level", k("disorder_level", 0.0), 0), ("kt_relax", k("kt_relax", 0.0), 0),
+            ("c_relax", k("c_relax", 0.0), 0)]
+    if not prm.Newtonian:
+        recs += [("tau", prm.tau, 0), ("n_p", prm.n_p, 0), ("kt_p", prm.kt_p, 0),
+                 ("eigen_dynamics", bool(k("eigen_dynamics", False)), 0),
+                 ("eval", k("eval", [0.0] * d), 0), ("eval_tolerance", k("eval_tolerance", 0.0), 0),
+                 ("evec", k("evec", [0.0] * d * d), 0),
+                 ("evec_normalize", bool(k("evec_normalize", False)), 0),
+                 ("evec_tolerance", k("evec_tolerance", 0.0), 0)]
+    recs += [("body_force_type", prm.body_force_type, 3), ("body_force", prm.body_force[:d], 0),
+             ("body_force_d", k("body_force_d", [0.0] * d), 0),
+             ("flow_direction", k("flow_direction", 1), 5), ("flow_width", k("flow_width", 0.0), 0),
+             ("flow_v", k("flow_v", 0.0), 0), ("flow_adjust_freq", k("flow_adjust_freq", 0), 5),
+             ("num_colloid", 0, 3), ("bcdef", k("bcdef", [1] * 2 * d), 2),
+             ("shear_type", k("shear_type", [0] * 2 * d), 2),
+             ("shear_v", k("shear_v", [0.0] * 2 * d * (d - 1)), 0),
+             ("shear_freq", k("shear_freq", [0.0] * 2 * d), 0),
+             ("wall_rho_type", k("wall_rho_type", 0), 3), ("wall_noslip", k("wall_noslip", 0), 2)]
+    bar = "#---------------------------------------------"
+    lines = [bar, "# physics keys of the run at this step, read as",
+             "# the keys of physics_config.mcf", bar, " "]
+    for name, val, width in recs:
+        lines += [config_line(name, val, width), " "]
+    lines += [bar, "# end of the restart physics keys", bar]
+    return lines
+
+
+def restart_physics_name(base: str, step: int) -> str:
+    """<restart_physics_file><step:08d>.mcf"""
+    return f"{base}{step:08d}.mcf"
+
+
+def read_config(path: str) -> dict:
+    """key = value lines of a configuration file (ctrl.mcf,
+    physics_config.mcf, io_config.mcf, a restart physics file): '#' starts
+    a comment, keys are case-insensitive (lower-cased here), values stay
+    strings without surrounding blanks, and a key given twice keeps its
+    last value (io_read_ctrl.F90:157-357 reads line by line the same way)."""
+    out = {}
+    with open(path) as fh:
+        for line in fh:
+            line = line.split("#", 1)[0]
+            if "=" not in line:
+                continue
+            key, val = line.split("=", 1)
+            key = key.strip().lower()
+            if key:
+                out[key] = val.strip()
+    return out
+
+
+def config_reals(value: str) -> list:
+    """The numbers of a comma- or blank-separated value (list-directed)."""
+    return [float(t) for t in value.replace(",", " ").split()]
 
 
 def write_statistic(path: str, records: Iterable[str]) -> None:

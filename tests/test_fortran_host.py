@@ -272,3 +272,97 @@ def test_host_formatted_outputs(tmp_path):
     assert open(tmp_path / f"conf_r{nsteps:08d}.out").read() == open(refc).read()
     assert np.allclose(IO.read_conformation(str(tmp_path / f"rconf{nsteps:08d}.dat"), 3), ctr,
                        rtol=5.1e-8, atol=1e-300)
+
+
+def _resume_dir(tmp, box, src, step, nsteps, extra=""):
+    """A resumed run's files: the first run's control keys, its restart
+    physics file as the physics config and its restart particles as the
+    read_external file (marching_marching.F90:325-331 takes step_start and
+    time_start from the physics config)."""
+    import shutil
+
+    ctrl = write_configs(str(tmp), box, nsteps, fused=True, extra=extra, stat_freq=1)
+    for name in (IO.restart_physics_name("mcf_restart_physics", step), f"restart{step:08d}.dat"):
+        shutil.copy(os.path.join(src, name), os.path.join(tmp, name))
+    text = open(ctrl).read().replace("physics_config.mcf",
+                                     IO.restart_physics_name("mcf_restart_physics", step))
+    open(ctrl, "w").write(text)
+    io = os.path.join(tmp, "io_config.mcf")
+    text = open(io).read().replace("particles.dat", f"restart{step:08d}.dat")
+    open(io, "w").write(text)
+    return ctrl
+
+
+@pytest.mark.gpu
+@needs_host
+@pytest.mark.parametrize("newtonian", [True, False])
+def test_host_resumes_from_its_restart_files(tmp_path, newtonian):
+    # the first run writes restart particles and restart physics at step 3;
+    # a second run started from those files alone continues to step 6
+    box = W.tait_box_3d(12, kt=1e-3) if newtonian else W.oldroyd_box_3d(12, kt=1e-3)
+    n1, n2 = 3, 6
+    extra = "" if newtonian else f"tau = {box.prm.tau!r}\nn_p = {box.prm.n_p!r}\nkt_p = {box.prm.kt_p!r}\n"
+    first = tmp_path / "first"
+    first.mkdir()
+    announce(f"Fortran host restart at step {n1}, newtonian={newtonian}")
+    ctrl = write_configs(str(first), box, n2, fused=True, extra=extra,
+                         io_extra=f"restart_freq_step = {n1}\n")
+    if not newtonian:
+        with open(ctrl, "a") as fh:
+            fh.write("Newtonian = F\n")
+    r = run_host(ctrl, str(first))
+    assert r.returncode == 0, r.stdout + r.stderr
+    t = 0.0
+    for _ in range(n1):
+        t = t + box.prm.dt
+    # the file the reference's writer makes for this state, record for record
+    phys = first / IO.restart_physics_name("mcf_restart_physics", n1)
+    assert phys.read_text().splitlines() == IO.restart_physics_lines(box.prm, n1, t, n2)
+    cfg = IO.read_config(str(phys))
+    assert int(cfg["step_start"]) == n1 and int(cfg["step_end"]) == n2
+
+    second = tmp_path / "second"
+    second.mkdir()
+    ctrl2 = _resume_dir(str(second), box, str(first), n1, n2, extra)
+    if not newtonian:
+        # the conformation of the restart step comes with it
+        import shutil
+
+        shutil.copy(first / f"mcf_restart_conformation{n1:08d}.dat", second / "rconf.dat")
+        with open(ctrl2, "a") as fh:
+            fh.write("Newtonian = F\n")
+        with open(second / "io_config.mcf", "a") as fh:
+            fh.write("read_conformation_file = rconf.dat\n")
+    r = run_host(ctrl2, str(second))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert f"{n2 - n1} steps" in r.stdout
+    x, v = read_parts(str(second / parts_name(n2)), 3)
+
+    # the library resumed from the same restart data at step n1: the host
+    # resumes at the right step (noise keys, times), bit for bit
+    xs, vs, _, ms, pid, _ = IO.read_particles(str(first / f"restart{n1:08d}.dat"), 3)
+    with SDPD(box.prm) as g:
+        g.upload(xs, vs, ms, pid)
+        if not newtonian:
+            g.upload_ct(IO.read_conformation(str(first / f"mcf_restart_conformation{n1:08d}.dat"), 3))
+        g.step = n1
+        g.integrate_VV(n2 - n1)
+        xr, vr = g.download()
+        ek, mom = g.compute_statistic()
+    assert np.array_equal(x, xr) and np.array_equal(v, vr)
+    # the statistic file continues from step_start at time_start
+    tr = float(cfg["time_start"])
+    for _ in range(n2 - n1):
+        tr = tr + box.prm.dt
+    lines = open(second / "stat.dat").read().splitlines()
+    assert lines[0].split()[0] == str(n1)
+    assert lines[-1] == IO.statistic_record(n2, tr, ek, mom)
+    # and it stays with the uninterrupted run: the restart records keep 8
+    # digits, and the resumed prologue evaluates the forces at v(t_n1)
+    # rather than at the half-step velocity (marching_marching.F90 does the
+    # initial evaluation on every start)
+    xf, vf = read_parts(str(first / parts_name(n2)), 3)
+    dx = x - xf
+    dx -= 12.0 * np.round(dx / 12.0)
+    assert np.abs(dx).max() < 1e-6
+    assert np.abs(v - vf).max() < 1e-6 * max(np.abs(vf).max(), 1e-3) + 1e-8

@@ -116,3 +116,56 @@ def test_statistic_file_round_trip(tmp_path):
     st = IO.read_statistic(str(p))
     assert st.shape == (5, 6)
     assert list(st[:, 0]) == [0, 1, 2, 3, 4]
+
+
+@needs_fc
+def test_restart_physics_records_match_fortran(tmp_path):
+    # the records of io_write_restart_physics.F90: (A,E16.8), (3(A,E16.8)),
+    # (A,I10), (3(A,I4)), (A,L) with gfortran's default width 2
+    src = tmp_path / "r.f90"
+    src.write_text("""program r
+  character(len=512) :: cbuf
+  write(cbuf, '(A,E16.8)') 'dt = ', 5.0d-5
+  write(*, '(A)') trim(cbuf)
+  write(cbuf, '(3(A,E16.8))') 'min_phys = ', 0.0d0, ',', -1.5d0, ',', 128.0d0
+  write(*, '(A)') trim(cbuf)
+  write(cbuf, '(A,I10)') 'step_start = ', 1234
+  write(*, '(A)') trim(cbuf)
+  write(cbuf, '(3(A,I4))') 'num_part = ', 64, ',', 64, ',', 1
+  write(*, '(A)') trim(cbuf)
+  write(cbuf, '(A,L2)') 'eigen_dynamics = ', .false.
+  write(*, '(A)') trim(cbuf)
+end program
+""")
+    exe = tmp_path / "r"
+    subprocess.run([FC, str(src), "-o", str(exe)], check=True, capture_output=True)
+    ref = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines()
+    got = [IO.config_line("dt", 5e-5), IO.config_line("min_phys", [0.0, -1.5, 128.0]),
+           IO.config_line("step_start", 1234, 10), IO.config_line("num_part", [64, 64, 1], 4),
+           IO.config_line("eigen_dynamics", False)]
+    assert got == ref
+
+
+def test_restart_physics_reads_back(tmp_path):
+    from mcf_amd import workloads as W
+
+    box = W.oldroyd_box_3d(10)
+    prm = box.prm.replace(body_force_type=1, body_force=[1e-3, 0.0, -2e-3])
+    t = 0.0
+    for _ in range(7):
+        t = t + prm.dt
+    path = tmp_path / IO.restart_physics_name("mcf_restart_physics", 7)
+    assert path.name == "mcf_restart_physics00000007.mcf"
+    path.write_text("\n".join(IO.restart_physics_lines(prm, 7, t, 20, {"num_part": [10, 10, 10]})) + "\n")
+    cfg = IO.read_config(str(path))
+    assert int(cfg["step_start"]) == 7 and int(cfg["step_end"]) == 20
+    # E16.8 keeps eight significant digits
+    assert float(cfg["time_start"]) == float(f"{t:.7e}")
+    assert IO.config_reals(cfg["max_phys"]) == [10.0, 10.0, 10.0]
+    assert IO.config_reals(cfg["body_force"]) == [1e-3, 0.0, -2e-3]
+    for key in ("cut_off", "dt", "rho", "eta", "eta_coef", "c", "rho_ref", "gamma", "tau", "n_p", "kt_p"):
+        assert float(cfg[key]) == float(f"{getattr(prm, key):.7e}"), key
+    assert float(cfg["kt"]) == float(f"{prm.kt_effective:.7e}")
+    assert cfg["eigen_dynamics"] == "F" and cfg["num_colloid"] == "0"
+    assert [int(v) for v in IO.config_reals(cfg["bcdef"])] == [1] * 6
+    assert IO.config_reals(cfg["num_part"]) == [10.0, 10.0, 10.0]

@@ -430,7 +430,7 @@ def main() -> int:
     if configs:
         out["configs"] = {
             name: {k: c[k] for k in ("workload", "precision", "particles", "steps", "value",
-                                     "ms_per_step", "pair_interactions_per_s",
+                                     "ms_per_step", "pair_interactions_per_s", "pairs_per_step",
                                      "phase_ms_per_step", "launch_shape")
                    if k in c} | ({"ratio_vs_C2_fp64": c["ratio_vs_C2_fp64"]}
                                  if "ratio_vs_C2_fp64" in c else {})

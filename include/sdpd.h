@@ -245,10 +245,13 @@ int sdpd_get_list_capacity(sdpd_ctx *ctx, int32_t *cap);
  * moved more than skin / 2 since the last one.  Every step still decides
  * in or out of the cut-off on its own positions with the reference's FP64
  * formula, so the neighbour sets and the sums equal a per-step rebuild's.
- * skin < 0 (the default): the largest the sort grid allows, twice its cell
- * edge minus cut_off; 0: lists rebuilt every step; > 0: that skin, refused
- * at sdpd_upload if larger than the grid allows.  Set before sdpd_upload;
- * get returns the skin in use (0: rebuilt every step) after it.
+ * skin < 0 (the default): automatic, 2 v dt 256 with v the fastest upload
+ * speed or five thermal deviations sqrt(kT / m_min), whichever is larger,
+ * kept between 1/20 of and the largest skin the sort grid allows (twice its
+ * cell edge minus cut_off); 0: lists rebuilt every step; > 0: that skin,
+ * refused at sdpd_upload if larger than the grid allows.  Set before
+ * sdpd_upload; get returns the skin in use (0: rebuilt every step) after
+ * it.
  * sdpd_get_list_builds returns the lists' builds since sdpd_upload (-1 when
  * they are rebuilt every step). */
 int sdpd_set_skin(sdpd_ctx *ctx, double skin);

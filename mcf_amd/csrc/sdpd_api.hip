@@ -773,14 +773,29 @@ namespace {
 // 2 cs - rc of the cut-off: that is the largest skin (less a margin for the
 // rounding of cell keys).  Sheared faces and colloids, a request of 0 or a
 // skin thinner than 1e-6 rc rebuild the lists every step.
-int choose_skin(sdpd_ctx* c) {
+//
+// The automatic skin trades the candidates past the cut-off, which every
+// step's passes walk (about 3 skin / rc of the neighbours), against the
+// rebuilds: it is what a particle at speed v_est covers in kSkinSteps
+// steps, twice over (lists are rebuilt after skin / 2), between 1/20 of
+// the grid's largest and that largest.  v_est is the fastest upload speed,
+// or five thermal standard deviations sqrt(kT / m) of the lightest
+// particle when those are faster.  Results do not depend on the skin.
+constexpr double kSkinSteps = 256.0;
+
+double auto_skin(const DevParams& P, double smax, double v_est) {
+  const double want = 2.0 * v_est * P.dt * kSkinSteps;
+  return std::min(smax, std::max(0.05 * smax, want));
+}
+
+int choose_skin(sdpd_ctx* c, double v_est) {
   DevParams& P = c->P;
   const double smax = sdpd::grid_skin(P, c->geo.data(), (int)c->geo.size());
   if (c->skin_req > 0.0 && c->skin_req > smax)
     return fail(c, SDPD_ERR_ARG,
                 "sdpd_set_skin: the skin exceeds what the sort grid allows (2 cell edges - cut_off = " +
                     std::to_string(smax) + ")");
-  const double skin = c->skin_req < 0.0 ? smax : c->skin_req;
+  const double skin = c->skin_req < 0.0 ? auto_skin(P, smax, v_est) : c->skin_req;
   const bool on = P.le_g < 0 && P.ncoll == 0 && sdpd::skin_usable(P, skin);
   P.verlet = on ? 1 : 0;
   P.skin = on ? skin : 0.0;
@@ -893,7 +908,16 @@ int upload_impl(sdpd_ctx* c, const double* x, const double* v, const double* m,
       for (int d = 0; d < dim; ++d) vq *= g.L[d];
       vol += vq;
     }
-    int rc = choose_skin(c);
+    // the speed that sizes the automatic Verlet skin
+    double vmax2 = 0.0, mmin = 1e300;
+    for (int32_t i = 0; i < n; ++i) {
+      double v2 = 0.0;
+      for (int d = 0; d < dim; ++d) v2 += v[(size_t)d + (size_t)dim * i] * v[(size_t)d + (size_t)dim * i];
+      vmax2 = std::max(vmax2, v2);
+      mmin = std::min(mmin, m[i]);
+    }
+    const double v_th = c->P.noise ? 5.0 * std::sqrt(c->P.kt / mmin) : 0.0;
+    int rc = choose_skin(c, std::max(std::sqrt(vmax2), v_th));
     if (rc) return rc;
     c->P.list_cap = c->list_cap_req > 0 ? c->list_cap_req : sdpd::auto_list_capacity(c->P, vol, n);
   }

@@ -116,9 +116,10 @@ def test_verlet_rebuilds_and_wraps_against_oracle(oracle):
 
 
 def test_verlet_wraps_without_rebuild(oracle):
-    # the automatic skin (2 cs - rc = 1 dx here) is not used up in 60 steps,
-    # while the drift carries particles across the x faces: the images come
-    # from the positions, not from the build's cells
+    # the automatic skin (the grid's largest at this drift, 2 cs - rc = 1 dx
+    # here) is not used up in 60 steps, while the drift carries particles
+    # across the x faces: the images come from the positions, not from the
+    # build's cells
     box = drifting_box(drift=(2.0, 0.0, 0.0), dt=1e-3)
     announce("verlet wraps without rebuild")
     steps = 60
@@ -150,6 +151,14 @@ def test_verlet_overflowed_lists_over_steps(precision):
         assert out["builds"] == ref["builds"]
 
 
+def auto_skin(box, grid=1.0 - 12e-9):
+    # 2 v dt 256 for the fastest upload speed or five thermal deviations,
+    # between 1/20 of and the grid's largest skin (2 cs - rc = 1 here, less
+    # its margin)
+    v = max(np.sqrt((box.v ** 2).sum(1)).max(), 5.0 * np.sqrt(box.prm.kt / box.m.min()))
+    return min(grid, max(0.05 * grid, 2.0 * v * box.prm.dt * 256))
+
+
 def test_skin_arguments():
     box = W.tait_box_3d(12, kt=1e-3)
     with SDPD(box.prm) as g:
@@ -159,13 +168,19 @@ def test_skin_arguments():
             g.upload(box.x, box.v, box.m, box.pid)
         g.set_skin(-1.0)
         g.upload(box.x, box.v, box.m, box.pid)
-        assert 0.99 < g.skin() < 1.0
+        assert g.skin() == pytest.approx(auto_skin(box), rel=1e-12)
+        assert 0.0 < g.skin() <= 0.05  # thermal speeds at kT 1e-3: the floor, 1/20 of the grid's
         with pytest.raises(SDPDError):
             g.set_skin(0.1)  # after upload
         with pytest.raises(SDPDError):
             g.set_lees_edwards(2, [-0.3, 0, 0], [0.3, 0, 0])  # Verlet lists are for such boxes
         g.integrate_VV(2)
         assert g.list_builds() == 1
+    # a fast drift takes the grid's largest skin
+    fast = drifting_box(drift=(2.0, 0.0, 0.0), dt=1e-3)
+    with SDPD(fast.prm) as g:
+        g.upload(fast.x, fast.v, fast.m, fast.pid)
+        assert 0.99 < g.skin() < 1.0
     le = W.tait_box_3d(12, kt=1e-3)
     le.prm = le.prm.replace(le_dim=2, le_v_lower=[-0.3, 0, 0], le_v_upper=[0.3, 0, 0])
     with SDPD(le.prm) as g:

@@ -43,7 +43,7 @@ constexpr int kBuf = 16;          // LDS list entries per lane between flushes (
 constexpr int kTile = 224;        // staged candidate records per stencil row batch
 constexpr int kCodeShift = 25;    // list entry = j | (stencil code << 25)
 constexpr uint32_t kIndexMask = (1u << kCodeShift) - 1u;
-constexpr int kEvalBatch = 4;     // buffered FP32 density pairs evaluated per gather batch
+constexpr int kEvalBatch = 4;     // buffered density pairs evaluated per gather batch
 
 // Stencil code of a neighbour: offsets o in -2..2 per dimension,
 // code = (ox+2) + 5 (oy+2) + 25 (oz+2).
@@ -452,14 +452,15 @@ __device__ __forceinline__ int staged_scan(const DevParams& P, const DeviceState
   uint32_t* __restrict__ blk = list ? list_block(P, list, ch) : nullptr;
   auto flush = [&](bool final) {
     if constexpr (Pol::kBatched) {
-      // kEvalBatch entries at a time: the policy issues all their gathers
+      // Pol::kBatch entries at a time: the policy issues all their gathers
       // before it evaluates the first pair (slots past cnt read the lane's
       // own particle and add nothing)
-      for (int q = ev; __ballot(q < cnt) != 0ull; q += kEvalBatch) {
-        uint32_t e[kEvalBatch];
-        bool ok[kEvalBatch];
+      constexpr int NB = Pol::kBatch;
+      for (int q = ev; __ballot(q < cnt) != 0ull; q += NB) {
+        uint32_t e[NB];
+        bool ok[NB];
 #pragma unroll
-        for (int k = 0; k < kEvalBatch; ++k) {
+        for (int k = 0; k < NB; ++k) {
           ok[k] = q + k < cnt;
           e[k] = ok[k] ? buf[(q + k) * kWave + lane] : (uint32_t)h.ii;
         }
@@ -682,6 +683,7 @@ struct DensityF64 {
   int nin = 0, rechecks = 0;
   static constexpr bool kImages = true;
   static constexpr bool kBatched = DECIDE;
+  static constexpr int kBatch = kEvalBatch;
   __device__ __forceinline__ void body(uint32_t ent) {
     const int j = (int)(ent & kIndexMask);
     const int64_t n = P.n;
@@ -747,6 +749,7 @@ struct DensityMixed {
   int nin = 0, rechecks = 0;
   static constexpr bool kImages = false;
   static constexpr bool kBatched = true;
+  static constexpr int kBatch = kEvalBatch;
   template <int NB>
   __device__ __forceinline__ void body_batch(const uint32_t (&e)[NB], const bool (&ok)[NB]) {
     float4 q[NB];
@@ -845,7 +848,7 @@ __device__ __forceinline__ bool list_overflow(const DevParams& P, const DeviceSt
 
 // Walk the neighbours stored in a chunk's lists (waves without list
 // overflow): the in-cut-off pairs of the step's density pass, or the Verlet
-// lists' candidates.  Policies that evaluate in batches get kEvalBatch
+// lists' candidates.  Policies that evaluate in batches get Pol::kBatch
 // entries at a time from every lane (slots past a lane's count carry its
 // own particle and are flagged off).
 template <int DIM, class Pol>
@@ -865,12 +868,13 @@ __device__ __forceinline__ void for_each_listed(const DevParams& P, const Device
         hi = src[1];
       }
       const uint32_t e8[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+      constexpr int NB = Pol::kBatch;
 #pragma unroll
-      for (int b0 = 0; b0 < 8; b0 += kEvalBatch) {
-        uint32_t e[kEvalBatch];
-        bool ok[kEvalBatch];
+      for (int b0 = 0; b0 < 8; b0 += NB) {
+        uint32_t e[NB];
+        bool ok[NB];
 #pragma unroll
-        for (int k = 0; k < kEvalBatch; ++k) {
+        for (int k = 0; k < NB; ++k) {
           ok[k] = q + b0 + k < cnt;
           e[k] = ok[k] ? e8[b0 + k] : (uint32_t)h.ii;
         }
@@ -1072,18 +1076,51 @@ struct ForceF64 {
   double rxc_i[3] = {0.0, 0.0, 0.0}, vcoll_i[3] = {0.0, 0.0, 0.0};
   double le_t = 0.0;
   static constexpr bool kImages = true;
-  static constexpr bool kBatched = false;
+  // Verlet lists: the walk takes four entries at a time (two with the
+  // Oldroyd-B body's registers) and gathers their records and pids before
+  // it evaluates the first pair, so a wave keeps more loads in flight.
+  // Measured on the 128^3 box: forces 1.824 ms one entry at a time, 1.819
+  // two, 1.761 four; the Oldroyd-B batch of 256 boxes 6.62 -> 6.23 ms with
+  // two.
+  static constexpr bool kBatched = DECIDE;
+  static constexpr int kBatch = FT == 5 ? 2 : 4;
   __device__ __forceinline__ void body(uint32_t ent) {
     const int j = (int)(ent & kIndexMask);
-    const Images o = decode_images(ent);
-    const bool jb = COLL && (ent & kBoundaryBit) != 0u;
     if (COLL) {
       // two boundary particles do not interact (pp_interaction.inc: same
       // colloid CYCLE, colloid-colloid pair forces off)
-      if (sidi != 0 && jb) return;
+      if (sidi != 0 && (ent & kBoundaryBit) != 0u) return;
     }
+    // both halves of the record into registers up front (a reference into
+    // global memory would leave each field's load at its first use)
     const double4* rj = reinterpret_cast<const double4*>(S.rec64 + 8 * (int64_t)j);
     const double4 ra = rj[0], rb = rj[1];
+    eval<false>(ent, ra, rb, 0u);
+  }
+  template <int NB>
+  __device__ __forceinline__ void body_batch(const uint32_t (&e)[NB], const bool (&ok)[NB]) {
+    double4 ra[NB], rb[NB];
+    uint32_t pj[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      const int j = (int)(e[k] & kIndexMask);
+      const double4* rj = reinterpret_cast<const double4*>(S.rec64 + 8 * (int64_t)j);
+      ra[k] = rj[0];
+      rb[k] = rj[1];
+      pj[k] = P.noise ? (uint32_t)S.pid[j] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < NB; ++k)
+      if (ok[k]) eval<true>(e[k], ra[k], rb[k], pj[k]);
+  }
+  // One pair from j's record; PID: j's pid given (else gathered for the
+  // noise term).
+  template <bool PID>
+  __device__ __forceinline__ void eval(uint32_t ent, const double4& ra, const double4& rb,
+                                       uint32_t pid_in) {
+    const int j = (int)(ent & kIndexMask);
+    const Images o = decode_images(ent);
+    const bool jb = COLL && (ent & kBoundaryBit) != 0u;
     double d0, d1, d2, r2;
     if (DECIDE) {
       r2 = exact_d2_pos<DIM>(P, h, ra.x, ra.y, ra.z, d0, d1, d2);
@@ -1120,7 +1157,7 @@ struct ForceF64 {
       const double aij = (bi + bj) * gw;
       double sc = P.eta_coef * P.eta * aij * du * (ir * ir) - aij * pij;
       if (P.noise) {
-        const uint32_t pj_ = (uint32_t)S.pid[j];
+        const uint32_t pj_ = PID ? pid_in : (uint32_t)S.pid[j];
         float g[4];
         pair_normals(P.seed, pidi < pj_ ? pidi : pj_, pidi < pj_ ? pj_ : pidi, step, g);
         sc = sc + (double)g[0] * P.sqrt_2kt_eta * sqrt_nr(-P.eta_coef * aij * ir) * P.inv_sqrt_dt;
@@ -1168,7 +1205,7 @@ struct ForceF64 {
       if (P.noise) {
         // f_r W e with W e = s (ca g + cb (g.e) e), e = d / r, s = +1 when
         // pid_i < pid_j (noise_vector), written on d to save operations
-        const uint32_t pidj = (uint32_t)S.pid[j];
+        const uint32_t pidj = PID ? pid_in : (uint32_t)S.pid[j];
         float g[4];
         pair_normals(P.seed, pidi < pidj ? pidi : pidj, pidi < pidj ? pidj : pidi, step, g);
         // sqrt(-4 kT f_d) / sqrt(dt) as one root (the factor is uniform)
@@ -1421,10 +1458,11 @@ __device__ __forceinline__ void visit_neighbours(const DevParams& P, const Devic
   }
 }
 
-// The FP64 list walk (without colloids) takes 8 KB of LDS per workgroup,
-// its entry buffer plus a reserve.  That holds it to five waves per SIMD.
-// Its registers alone would allow six, but on the 128^3 box six waves'
-// gathers run 6 % slower (1.99 against 1.88 ms per pass).
+// The FP64 walk of per-step lists (without colloids) takes 8 KB of LDS per
+// workgroup, its entry buffer plus a reserve.  That holds it to five waves
+// per SIMD.  Its registers alone would allow six, but on the 128^3 box six
+// waves' gathers ran 6 % slower (1.99 against 1.88 ms per pass).  The
+// Verlet walk evaluates in batches from registers and takes no LDS.
 constexpr int kForceLdsWordsF64 = 2048;
 
 // Two launches over the same grid: SCAN = false takes the chunks whose

@@ -68,7 +68,9 @@ struct DeviceState {
   int32_t *rkey, *rslot, *rcount, *rstart, *runsorted, *rlpdx;
   // Verlet lists: positions at the last build (cell order), the step's
   // gate (gate[0] = 1: the lists are rebuilt this step; gate[1] counts the
-  // builds), the bits of the largest squared displacement since the build
+  // builds; gate[2] = 1: a chunk's lists overflowed at the last build, so
+  // the rescan passes have work), the bits of the largest squared
+  // displacement since the build
   // seen this step, and the in-cut-off neighbours of each particle in the
   // last density pass (every mode)
   double* x_ref;

@@ -941,6 +941,8 @@ template <int DIM, bool MIXED, int TILE, int BUF, bool LE, int MODE>
 __global__ __launch_bounds__(kWave) void k_density(DevParams P, DeviceState S) {
   if (MODE == kBuild && S.gate[0] == 0) return;
   if ((MODE == kWalk || MODE == kRescan) && S.gate[0] != 0) return;
+  // no list overflowed at the build: nothing to rescan
+  if (MODE == kRescan && S.gate[2] == 0) return;
   constexpr bool kDecide = MODE != kStore;
   constexpr bool kScan = MODE != kWalk;
   const int le_box = P.le_g;
@@ -997,6 +999,7 @@ __global__ __launch_bounds__(kWave) void k_density(DevParams P, DeviceState S) {
   S.an[i] = an;
   S.bn[i] = bn;
   if (MODE == kStore || MODE == kBuild) S.nbr_count[i] = cnt;
+  if (MODE == kBuild && cnt > P.list_cap) S.gate[2] = 1;  // every writer stores 1
   S.nbr_in[i] = nin;
   S.recheck[i] = rechecks;
   write_force_records<DIM>(P, S, i, an, bn, press);
@@ -1476,6 +1479,8 @@ constexpr int kForceLdsWordsF64 = 2048;
 // the Verlet candidates and the bodies decide in / out of the cut-off.
 template <int DIM, int FT, bool MIXED, bool COLL, bool LE, bool SCAN, bool VERLET = false>
 __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
+  // Verlet lists: the rescan has work only if a list overflowed at the build
+  if (SCAN && VERLET && S.gate[2] == 0) return;
   const int le_box = P.le_g;
   if (!LE) P.le_g = -1;
   __shared__ float4 tile[SCAN ? kTile + 1 : 1];

@@ -229,7 +229,8 @@ __global__ __launch_bounds__(256) void k_track(DevParams P, DeviceState S) {
 // The step's gate: rebuild the lists when a particle moved more than
 // skin / 2 since their build (a pair then may have closed in by more than
 // the skin); the displacement maximum starts again from zero.  gate[1]
-// counts the builds.
+// counts the builds; gate[2], set by the build, flags lists that
+// overflowed their capacity.
 __global__ void k_decide(DevParams P, DeviceState S) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   const double d2 = __longlong_as_double((long long)S.disp[0]);
@@ -238,6 +239,7 @@ __global__ void k_decide(DevParams P, DeviceState S) {
   // yet) stays pending
   S.gate[0] |= rebuild;
   S.gate[1] += rebuild;
+  if (rebuild) S.gate[2] = 0;
   S.disp[0] = 0ull;
 }
 
@@ -252,6 +254,7 @@ __global__ void k_force_build(DeviceState S) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   S.gate[0] = 1;
   S.gate[1] += 1;
+  S.gate[2] = 0;
   S.disp[0] = 0ull;
 }
 

@@ -43,7 +43,6 @@ constexpr int kBuf = 16;          // LDS list entries per lane between flushes (
 constexpr int kTile = 224;        // staged candidate records per stencil row batch
 constexpr int kCodeShift = 25;    // list entry = j | (stencil code << 25)
 constexpr uint32_t kIndexMask = (1u << kCodeShift) - 1u;
-constexpr int kEvalBatch = 4;     // buffered density pairs evaluated per gather batch
 
 // Stencil code of a neighbour: offsets o in -2..2 per dimension,
 // code = (ox+2) + 5 (oy+2) + 25 (oz+2).
@@ -669,11 +668,13 @@ __device__ __forceinline__ double pair_d2(const DevParams& P, const Home& h, con
 // DECIDE (Verlet lists): the entries are candidates within rc + skin; the
 // body takes the exact in / out decision on the current positions and
 // counts the in-cut-off neighbours (nin) and the FP64 re-decisions.
-// Verlet lists evaluate in batches: the positions of a batch of
+// Verlet lists evaluate in batches of eight: the positions of the batch's
 // neighbours are gathered before the first pair is evaluated (three 8-byte
 // gathers each, whose latency a one-at-a-time walk exposes); the sums keep
-// the entries' order.  The per-step lists go one at a time (the batch's
-// registers cost the candidate scan occupancy every step).
+// the entries' order.  On the 128^3 box the walk takes 0.89 ms in batches
+// of four, 0.72 in eight, 0.74 in sixteen.  The per-step lists go one at a
+// time (the batch's registers cost the candidate scan occupancy every
+// step).
 template <int DIM, bool DECIDE = false>
 struct DensityF64 {
   const DevParams& P;
@@ -683,7 +684,7 @@ struct DensityF64 {
   int nin = 0, rechecks = 0;
   static constexpr bool kImages = true;
   static constexpr bool kBatched = DECIDE;
-  static constexpr int kBatch = kEvalBatch;
+  static constexpr int kBatch = 8;
   __device__ __forceinline__ void body(uint32_t ent) {
     const int j = (int)(ent & kIndexMask);
     const int64_t n = P.n;
@@ -748,8 +749,10 @@ struct DensityMixed {
   KahanF acc;
   int nin = 0, rechecks = 0;
   static constexpr bool kImages = false;
+  // in batches of eight (C3: 0.59 ms in fours, 0.51 in eights, 0.55 in
+  // sixteens)
   static constexpr bool kBatched = true;
-  static constexpr int kBatch = kEvalBatch;
+  static constexpr int kBatch = 8;
   template <int NB>
   __device__ __forceinline__ void body_batch(const uint32_t (&e)[NB], const bool (&ok)[NB]) {
     float4 q[NB];
@@ -860,23 +863,37 @@ __device__ __forceinline__ void for_each_listed(const DevParams& P, const Device
   const int lane = threadIdx.x;
   if constexpr (Pol::kBatched) {
     (void)buf;
-    for (int q = 0; __ballot(q < cnt) != 0ull; q += 8) {
-      uint4 lo = make_uint4(0u, 0u, 0u, 0u), hi = lo;
-      if (q < cnt) {
-        const uint4* src = reinterpret_cast<const uint4*>(blk + group_offset(q, lane));
-        lo = src[0];
-        hi = src[1];
-      }
-      const uint32_t e8[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-      constexpr int NB = Pol::kBatch;
+    // G groups of eight entries per iteration (the batch may span two);
+    // a group past the lane's count is not read
+    constexpr int NB = Pol::kBatch;
+    constexpr int G = (NB + 7) / 8;
+    for (int q = 0; __ballot(q < cnt) != 0ull; q += 8 * G) {
+      uint32_t ee[8 * G];
 #pragma unroll
-      for (int b0 = 0; b0 < 8; b0 += NB) {
+      for (int g = 0; g < G; ++g) {
+        uint4 lo = make_uint4(0u, 0u, 0u, 0u), hi = lo;
+        if (q + 8 * g < cnt) {
+          const uint4* src = reinterpret_cast<const uint4*>(blk + group_offset(q + 8 * g, lane));
+          lo = src[0];
+          hi = src[1];
+        }
+        ee[8 * g + 0] = lo.x;
+        ee[8 * g + 1] = lo.y;
+        ee[8 * g + 2] = lo.z;
+        ee[8 * g + 3] = lo.w;
+        ee[8 * g + 4] = hi.x;
+        ee[8 * g + 5] = hi.y;
+        ee[8 * g + 6] = hi.z;
+        ee[8 * g + 7] = hi.w;
+      }
+#pragma unroll
+      for (int b0 = 0; b0 < 8 * G; b0 += NB) {
         uint32_t e[NB];
         bool ok[NB];
 #pragma unroll
         for (int k = 0; k < NB; ++k) {
           ok[k] = q + b0 + k < cnt;
-          e[k] = ok[k] ? e8[b0 + k] : (uint32_t)h.ii;
+          e[k] = ok[k] ? ee[b0 + k] : (uint32_t)h.ii;
         }
         pol.body_batch(e, ok);
       }
@@ -1079,14 +1096,14 @@ struct ForceF64 {
   double rxc_i[3] = {0.0, 0.0, 0.0}, vcoll_i[3] = {0.0, 0.0, 0.0};
   double le_t = 0.0;
   static constexpr bool kImages = true;
-  // Verlet lists: the walk takes four entries at a time (two with the
+  // Verlet lists: the walk takes eight entries at a time (two with the
   // Oldroyd-B body's registers) and gathers their records and pids before
   // it evaluates the first pair, so a wave keeps more loads in flight.
   // Measured on the 128^3 box: forces 1.824 ms one entry at a time, 1.819
-  // two, 1.761 four; the Oldroyd-B batch of 256 boxes 6.62 -> 6.23 ms with
-  // two.
+  // two, 1.761 four, 1.65 eight (193 VGPRs, two waves per SIMD); the
+  // Oldroyd-B batch of 256 boxes 6.62 -> 6.23 ms with two.
   static constexpr bool kBatched = DECIDE;
-  static constexpr int kBatch = FT == 5 ? 2 : 4;
+  static constexpr int kBatch = FT == 5 ? 2 : 8;
   __device__ __forceinline__ void body(uint32_t ent) {
     const int j = (int)(ent & kIndexMask);
     if (COLL) {
@@ -1286,24 +1303,56 @@ struct ForceMixed {
   float inv_ni;
   WideF L[DIM * DIM];
   static constexpr bool kImages = false;
-  static constexpr bool kBatched = false;
+  // Verlet lists: entries four at a time, the records of all four
+  // gathered before the first pair is evaluated (C3 forces 1.27 ms one at
+  // a time, 1.16 in fours, 1.19 in eights; C5 3.94 in twos, 3.87 in fours)
+  static constexpr bool kBatched = DECIDE;
+  static constexpr int kBatch = 4;
+  // j's records: FP32 position (w: p/n^2 or p, Newtonian), velocity (w:
+  // 1/n^2, or the pid bits for Oldroyd-B), pid, and the Oldroyd-B
+  // (1/n^2, Pxx, Pyy, Pzz) and (Pxy, Pxz, Pyz, 0)
+  struct Rec {
+    float4 x, v, q, q2;
+    uint32_t pid;
+  };
+  __device__ __forceinline__ Rec load_rec(int j) const {
+    Rec r;
+    if (FT == 5) {
+      r.v = S.vr[j];
+      r.x = load_xyz(S.xa + j);
+      r.q = S.qr[j];
+      r.q2 = S.qr2[j];
+      r.pid = __float_as_uint(r.v.w);
+    } else {
+      r.x = S.qr[j];
+      r.v = S.vr[j];
+      r.q = r.q2 = make_float4(0.f, 0.f, 0.f, 0.f);
+      r.pid = P.noise ? (uint32_t)S.pid[j] : 0u;
+    }
+    return r;
+  }
   __device__ __forceinline__ void body(uint32_t ent) {
     const int j = (int)(ent & kIndexMask);
-    if (FT == 5) {
-      const float4 vj = S.vr[j];
-      body_rec(j, load_xyz(S.xa + j), vj, __float_as_uint(vj.w), S.qr[j],
-               decode_code(ent >> kCodeShift));
-    } else {
-      const uint32_t pidj = P.noise ? (uint32_t)S.pid[j] : 0u;
-      body_rec(j, S.qr[j], S.vr[j], pidj, make_float4(0.f, 0.f, 0.f, 0.f),
-               decode_code(ent >> kCodeShift));
-    }
+    const Rec r = load_rec(j);
+    body_rec(j, r.x, r.v, r.pid, r.q, r.q2, decode_code(ent >> kCodeShift));
+  }
+  template <int NB>
+  __device__ __forceinline__ void body_batch(const uint32_t (&e)[NB], const bool (&ok)[NB]) {
+    Rec r[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) r[k] = load_rec((int)(e[k] & kIndexMask));
+#pragma unroll
+    for (int k = 0; k < NB; ++k)
+      if (ok[k])
+        body_rec((int)(e[k] & kIndexMask), r[k].x, r[k].v, r[k].pid, r[k].q, r[k].q2,
+                 decode_code(e[k] >> kCodeShift));
   }
   // xj: cell-relative position (w: p/n^2 for Hu-Adams, p for the angular
   // form); vj: velocity (w: 1/n^2, Newtonian); Oldroyd-B qj: (1/n^2, Pxx,
-  // Pyy, Pzz)
+  // Pyy, Pzz), q2: (Pxy, Pxz, Pyz, 0)
   __device__ __forceinline__ void body_rec(int jidx, const float4& xj, const float4& vj,
-                                           uint32_t pidj, const float4& qj, const Offsets& o) {
+                                           uint32_t pidj, const float4& qj, const float4& q2,
+                                           const Offsets& o) {
     float d0, d1, d2;
     rel_diff<DIM>(P, h, xj, o, d0, d1, d2);
     const float r2 = mixed_s2(d0, d1, d2);
@@ -1328,7 +1377,6 @@ struct ForceMixed {
     if (FT == 5) {
       // rhs_force_ff_non_Newtonian_HuAdams.F90:105-131 with the records'
       // Pi / n^2, and the velocity gradient of pp_vgt_ip.inc:16-20
-      const float4 q2 = S.qr2[jidx];
       const float sxx = pb[0] + qj.y, syy = pb[1] + qj.z, szz = pb[2] + qj.w;
       const float sxy = pb[3] + q2.x, sxz = pb[4] + q2.y, syz = pb[5] + q2.z;
       float c0, c1, c2;

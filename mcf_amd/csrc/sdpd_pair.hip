@@ -1141,6 +1141,8 @@ struct ForceF64 {
     const int j = (int)(ent & kIndexMask);
     const Images o = decode_images(ent);
     const bool jb = COLL && (ent & kBoundaryBit) != 0u;
+    // two boundary particles do not interact
+    if (COLL && sidi != 0 && jb) return;
     double d0, d1, d2, r2;
     if (DECIDE) {
       r2 = exact_d2_pos<DIM>(P, h, ra.x, ra.y, ra.z, d0, d1, d2);

@@ -57,10 +57,11 @@ def run(box, steps, skin=None, cap=0, signature=False):
         x, v = g.download()
         f = g.download_force()
         rho, _ = g.download_density()
+        ct = None if box.prm.Newtonian else g.download_tensors()[0]
         builds = g.list_builds()
         sig = g.neighbor_signature() if signature else None
         pairs, _ = g.counters()
-    return dict(x=x, v=v, f=f, rho=rho, builds=builds, sig=sig, pairs=pairs)
+    return dict(x=x, v=v, f=f, rho=rho, ct=ct, builds=builds, sig=sig, pairs=pairs)
 
 
 def oracle_run(oracle, box, steps):
@@ -133,20 +134,26 @@ def test_verlet_wraps_without_rebuild(oracle):
     assert np.array_equal(out["sig"][0], cnt) and np.array_equal(out["sig"][1], hs)
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_verlet_overflowed_lists_over_steps(precision):
+@pytest.mark.parametrize("precision,body", [(0, "hu_adams"), (1, "hu_adams"), (0, "angular"),
+                                            (1, "angular"), (0, "oldroyd"), (1, "oldroyd")])
+def test_verlet_overflowed_lists_over_steps(precision, body):
     # capacities below the candidate counts: chunks rescan their stencil in
-    # the density and the force pass of every step; the results must not
-    # move, bit for bit
+    # the density and the force pass of every step, one pair at a time,
+    # while the walk of the other chunks evaluates its entries in batches;
+    # the results must not move, bit for bit
     box = drifting_box(dt=1e-3)
+    if body == "angular":
+        box.prm = box.prm.replace(rhs_force_type=4)
+    if body == "oldroyd":
+        box.prm = box.prm.replace(Newtonian=False, tau=0.1, n_p=1.0, kt_p=0.5)
     if precision:
         box = W.mixed(box)
-    announce(f"verlet overflowed lists: precision {precision}")
+    announce(f"verlet overflowed lists: precision {precision}, {body}")
     ref = run(box, 12, skin=0.005)
     assert ref["builds"] >= 2  # a rebuild inside the run
     for cap in (48, 8):
         out = run(box, 12, skin=0.005, cap=cap)
-        for k in ("x", "v", "f", "rho"):
+        for k in ("x", "v", "f", "rho") + (() if box.prm.Newtonian else ("ct",)):
             assert np.array_equal(out[k], ref[k]), (cap, k)
         assert out["builds"] == ref["builds"]
 

@@ -219,14 +219,23 @@ struct Chunk {
   Geo G;
 };
 
+// Chunk slots of the pair launches: an upper bound of the chunks (one
+// partial chunk per row plus n / 64 full ones), rounded up to a multiple of
+// the eight XCDs.
+__host__ __device__ inline int chunk_slots(const DevParams& P) {
+  const int m = (P.n + kWave - 1) / kWave + P.nrows;
+  return (m + 7) / 8 * 8;
+}
+
 // Workgroups are handed to the eight XCDs round-robin.  With xcd_order,
-// workgroup b takes chunk (b % 8) ceil(chunks / 8) + b / 8, i.e. each XCD
+// slot b takes chunk (b % 8) ceil(chunks / 8) + b / 8, i.e. each XCD
 // works through a contiguous eighth of the chunk sequence and its L2 serves
-// a compact slab of space; otherwise chunk b.  The grid is a multiple of 8
-// and at least the number of chunks.
-__device__ __forceinline__ bool load_chunk(const DevParams& P, const DeviceState& S, Chunk& ch) {
+// a compact slab of space; otherwise chunk b.  The slots, a multiple of 8
+// (chunk_slots), are at least the number of chunks; a workgroup takes slot
+// blockIdx.x, or every gridDim.x-th slot on the reduced grid of a rescan.
+__device__ __forceinline__ bool load_chunk(const DevParams& P, const DeviceState& S, Chunk& ch,
+                                           int b) {
   const int total = S.row_chunk_off[P.nrows];
-  const int b = blockIdx.x;
   int c = b;
   if (P.xcd_order) {
     const int per = (total + 7) >> 3;
@@ -954,22 +963,15 @@ __device__ __forceinline__ bool chunk_meets_le_face(int le_g, const Chunk& ch, i
 // scans visit the candidates in the same order, so they give the same sums.
 enum DensityMode { kStore = 0, kBuild = 1, kWalk = 2, kRescan = 3, kScanAll = 4 };
 
+// One chunk slot of the density pass.
 template <int DIM, bool MIXED, int TILE, int BUF, bool LE, int MODE>
-__global__ __launch_bounds__(kWave) void k_density(DevParams P, DeviceState S) {
-  if (MODE == kBuild && S.gate[0] == 0) return;
-  if ((MODE == kWalk || MODE == kRescan) && S.gate[0] != 0) return;
-  // no list overflowed at the build: nothing to rescan
-  if (MODE == kRescan && S.gate[2] == 0) return;
+__device__ __forceinline__ void density_chunk(const DevParams& P, const DeviceState& S, int slot,
+                                              int le_box, float4* __restrict__ tile,
+                                              uint32_t* __restrict__ buf) {
   constexpr bool kDecide = MODE != kStore;
   constexpr bool kScan = MODE != kWalk;
-  const int le_box = P.le_g;
-  if (!LE) P.le_g = -1;
-  __shared__ float4 tile[kScan ? TILE + 1 : 1];
-  // whole groups of 8 rows: the final flush stores the last group of a
-  // lane's entries as 8 rows, of which those past its count are padding
-  __shared__ uint32_t buf[(BUF + 7) / 8 * 8 * kWave];
   Chunk ch;
-  if (!load_chunk(P, S, ch)) return;
+  if (!load_chunk(P, S, ch, slot)) return;
   if (le_box >= 0 && chunk_meets_le_face(le_box, ch, 2) != LE) return;
   if ((MODE == kWalk || MODE == kRescan) && list_overflow(P, S, ch) != (MODE == kRescan)) return;
   const Home h = load_home<DIM>(P, S, ch);
@@ -1020,6 +1022,33 @@ __global__ __launch_bounds__(kWave) void k_density(DevParams P, DeviceState S) {
   S.nbr_in[i] = nin;
   S.recheck[i] = rechecks;
   write_force_records<DIM>(P, S, i, an, bn, press);
+}
+
+// The rescan (kRescan) has work only after a list overflowed at the build;
+// it is launched on a reduced grid, whose workgroups take every gridDim.x-th
+// chunk slot, so that on the other steps its launch costs little.
+template <int DIM, bool MIXED, int TILE, int BUF, bool LE, int MODE>
+__global__ __launch_bounds__(kWave) void k_density(DevParams P, DeviceState S) {
+  if (MODE == kBuild && S.gate[0] == 0) return;
+  if ((MODE == kWalk || MODE == kRescan) && S.gate[0] != 0) return;
+  // no list overflowed at the build: nothing to rescan
+  if (MODE == kRescan && S.gate[2] == 0) return;
+  constexpr bool kScan = MODE != kWalk;
+  const int le_box = P.le_g;
+  if (!LE) P.le_g = -1;
+  __shared__ float4 tile[kScan ? TILE + 1 : 1];
+  // whole groups of 8 rows: the final flush stores the last group of a
+  // lane's entries as 8 rows, of which those past its count are padding
+  __shared__ uint32_t buf[(BUF + 7) / 8 * 8 * kWave];
+  if (MODE == kRescan) {
+    const int slots = chunk_slots(P);
+    for (int slot = blockIdx.x; slot < slots; slot += gridDim.x) {
+      density_chunk<DIM, MIXED, TILE, BUF, LE, MODE>(P, S, slot, le_box, tile, buf);
+      __syncthreads();
+    }
+  } else {
+    density_chunk<DIM, MIXED, TILE, BUF, LE, MODE>(P, S, blockIdx.x, le_box, tile, buf);
+  }
 }
 
 __global__ void k_eos(DevParams P, DeviceState S) {
@@ -1527,16 +1556,13 @@ constexpr int kForceLdsWordsF64 = 2048;
 // instantiation takes the rows within coll_le_cells of a face, the plain
 // one (a quarter fewer registers) the interior.  VERLET: the lists hold
 // the Verlet candidates and the bodies decide in / out of the cut-off.
-template <int DIM, int FT, bool MIXED, bool COLL, bool LE, bool SCAN, bool VERLET = false>
-__global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
-  // Verlet lists: the rescan has work only if a list overflowed at the build
-  if (SCAN && VERLET && S.gate[2] == 0) return;
-  const int le_box = P.le_g;
-  if (!LE) P.le_g = -1;
-  __shared__ float4 tile[SCAN ? kTile + 1 : 1];
-  __shared__ uint32_t buf[(!MIXED && !COLL && !SCAN) ? kForceLdsWordsF64 : kBuf * kWave];
+// One chunk slot of the force pass.
+template <int DIM, int FT, bool MIXED, bool COLL, bool LE, bool SCAN, bool VERLET>
+__device__ __forceinline__ void forces_chunk(const DevParams& P, const DeviceState& S, int slot,
+                                             int le_box, float4* __restrict__ tile,
+                                             uint32_t* __restrict__ buf) {
   Chunk ch;
-  if (!load_chunk(P, S, ch)) return;
+  if (!load_chunk(P, S, ch, slot)) return;
   if (list_overflow(P, S, ch) != SCAN) return;
   if (COLL && le_box >= 0 && chunk_meets_le_face(le_box, ch, P.coll_le_cells) != LE) return;
   const Home h = load_home<DIM>(P, S, ch);
@@ -1613,6 +1639,28 @@ __global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
   }
 }
 
+// With Verlet lists the rescan (SCAN) has work only if a list overflowed at
+// the build; it is launched on a reduced grid, whose workgroups take every
+// gridDim.x-th chunk slot, so that on the other steps its launch costs
+// little.
+template <int DIM, int FT, bool MIXED, bool COLL, bool LE, bool SCAN, bool VERLET = false>
+__global__ __launch_bounds__(kWave) void k_forces(DevParams P, DeviceState S) {
+  if (SCAN && VERLET && S.gate[2] == 0) return;
+  const int le_box = P.le_g;
+  if (!LE) P.le_g = -1;
+  __shared__ float4 tile[SCAN ? kTile + 1 : 1];
+  __shared__ uint32_t buf[(!MIXED && !COLL && !SCAN) ? kForceLdsWordsF64 : kBuf * kWave];
+  if (SCAN && VERLET) {
+    const int slots = chunk_slots(P);
+    for (int slot = blockIdx.x; slot < slots; slot += gridDim.x) {
+      forces_chunk<DIM, FT, MIXED, COLL, LE, SCAN, VERLET>(P, S, slot, le_box, tile, buf);
+      __syncthreads();
+    }
+  } else {
+    forces_chunk<DIM, FT, MIXED, COLL, LE, SCAN, VERLET>(P, S, blockIdx.x, le_box, tile, buf);
+  }
+}
+
 // ---------------------------------------------------------------------
 // neighbour-set signature: per particle, the number of in-cutoff
 // neighbours and the wrapping sum of a hash of (pid_i, pid_j)
@@ -1658,7 +1706,7 @@ __global__ __launch_bounds__(kWave) void k_signature(DevParams P, DeviceState S)
   __shared__ float4 tile[kTile + 1];
   __shared__ uint32_t buf[kBuf * kWave];
   Chunk ch;
-  if (!load_chunk(P, S, ch)) return;
+  if (!load_chunk(P, S, ch, blockIdx.x)) return;
   const Home h = load_home<DIM>(P, S, ch);
   SignaturePol<DIM, VERLET> pol{P, S, h, (uint32_t)S.pid[h.ii], 0u};
   int rechecks = 0;
@@ -1674,10 +1722,14 @@ __global__ __launch_bounds__(kWave) void k_signature(DevParams P, DeviceState S)
 // launchers: one 64-lane workgroup per chunk; the grid covers the upper
 // bound of chunks (one partial chunk per row plus n / 64 full ones)
 // ---------------------------------------------------------------------
-static inline int nchunks_max(const DevParams& P) {
-  const int m = (P.n + kWave - 1) / kWave + P.nrows;
-  return (m + 7) / 8 * 8;
-}
+static inline int nchunks_max(const DevParams& P) { return chunk_slots(P); }
+
+// Workgroups of a rescan launch (a multiple of 8, at most the slots): few
+// enough that the launch is cheap on the steps without an overflowed list
+// (on the 128^3 box, whose 33k slots each took a workgroup before, the two
+// rescans now cost the step 15 us less), and enough that a step with
+// overflowed lists checks its slots 16 at a time.
+static inline dim3 rescan_grid(const DevParams& P) { return dim3(std::min(chunk_slots(P), 2048)); }
 
 template <int DIM, bool MIXED, bool LE, int MODE>
 static void launch_density_shape(hipStream_t s, const DevParams& P, const DeviceState& S) {
@@ -1714,7 +1766,8 @@ static void launch_density_dim(hipStream_t s, const DevParams& P, const DeviceSt
   dim3 g(nchunks_max(P)), b(kWave);
   launch_density_shape<DIM, MIXED, false, kBuild>(s, P, S);
   hipLaunchKernelGGL((k_density<DIM, MIXED, kTile, kBuf, false, kWalk>), g, b, 0, s, P, S);
-  hipLaunchKernelGGL((k_density<DIM, MIXED, kTile, kBuf, false, kRescan>), g, b, 0, s, P, S);
+  hipLaunchKernelGGL((k_density<DIM, MIXED, kTile, kBuf, false, kRescan>), rescan_grid(P), b, 0,
+                     s, P, S);
   launch_lists_current(s, S);
 }
 
@@ -1747,7 +1800,8 @@ static void launch_forces_le(hipStream_t s, const DevParams& P, const DeviceStat
   if constexpr (!COLL) {
     if (P.verlet) {
       hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, false, false, false, true>), g, b, 0, s, P, S);
-      hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, false, false, true, true>), g, b, 0, s, P, S);
+      hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, false, false, true, true>), rescan_grid(P), b,
+                         0, s, P, S);
       return;
     }
   }

@@ -101,33 +101,41 @@ __global__ void k_conformation(DevParams P, DeviceState S, double lambda) {
     S.ct[q * n + i] = S.ct[q * n + i] + S.act[q * n + i] * P.dt * lambda;
 }
 
-// Verlet lists: particle i's displacement since the lists were built and
-// its FP32 position relative to the corner of the cell it was sorted into
-// then.  Per dimension the displacement x - x_ref is taken to the nearest
-// period, so a particle that crossed a periodic face keeps its build cell:
-// its FP32 position goes on from there, unwrapped, and stays consistent
-// with the stencil offsets stored in the lists.  Returns the squared
-// displacement.
+// Verlet lists: particle i's displacement since the lists were built and,
+// in mixed precision, its FP32 position relative to the corner of the cell
+// it was sorted into then.  Per dimension the displacement x - x_ref is
+// taken to the nearest period, so a particle that crossed a periodic face
+// keeps its build cell: its FP32 position goes on from there, unwrapped,
+// and stays consistent with the stencil offsets stored in the lists.
+// Returns the squared displacement.
+//
+// FP64 runs keep the FP32 positions of the build: their walks take
+// distances from the FP64 positions, and the only readers of the FP32 ones
+// between builds (the rescans of overflowed lists and the neighbour-set
+// signature) screen at rc + skin.  Every particle has moved at most skin / 2
+// since the build, so the build-time positions put every pair that is
+// within rc now inside that screen, and the exact FP64 decision follows.
 __device__ __forceinline__ double track_particle(const DevParams& P, const DeviceState& S,
                                                  const RepGeom& g, int i, const double* xv) {
   const int64_t n = P.n;
-  const int kl = S.key_sorted[i] - g.cell_base;
-  const int c0 = kl % g.nm[0];
-  const int c1 = (kl / g.nm[0]) % g.nm[1];
-  const int c2 = kl / (g.nm[0] * g.nm[1]);
-  const int cc[3] = {c0, c1, c2};
-  float a[3] = {0.f, 0.f, 0.f};
+  double w[3] = {0.0, 0.0, 0.0};
   double d2 = 0.0;
   for (int d = 0; d < P.dim; ++d) {
     const double L = g.L[d];
     const double raw = xv[d] - S.x_ref[d * n + i];
-    const double w = raw > 0.5 * L ? -L : (raw < -0.5 * L ? L : 0.0);
-    const double dd = raw + w;
+    w[d] = raw > 0.5 * L ? -L : (raw < -0.5 * L ? L : 0.0);
+    const double dd = raw + w[d];
     d2 = d2 + dd * dd;
-    a[d] = (float)((xv[d] - (g.lo[d] + (double)cc[d] * g.cs[d])) + w);
   }
-  const float4 old = S.xa[i];
-  S.xa[i] = make_float4(a[0], a[1], a[2], old.w);
+  if (P.mixed) {
+    const int kl = S.key_sorted[i] - g.cell_base;
+    const int cc[3] = {kl % g.nm[0], (kl / g.nm[0]) % g.nm[1], kl / (g.nm[0] * g.nm[1])};
+    float a[3] = {0.f, 0.f, 0.f};
+    for (int d = 0; d < P.dim; ++d)
+      a[d] = (float)((xv[d] - (g.lo[d] + (double)cc[d] * g.cs[d])) + w[d]);
+    const float4 old = S.xa[i];
+    S.xa[i] = make_float4(a[0], a[1], a[2], old.w);
+  }
   return d2;
 }
 

@@ -28,47 +28,51 @@ __global__ void k_kick(DevParams P, DeviceState S, double lambda) {
     S.v[d * n + i] = S.v[d * n + i] + lambda * S.f[d * n + i] * P.dt;
 }
 
+// One dimension D of the wrap; true when the particle crossed a
+// Lees-Edwards face, which ends its check.  Crossing that face moves x by
+// the shear length, modulo its period, and wraps the third coordinate
+// (compile-time dimensions throughout, so the coordinates stay in
+// registers).
+template <int DIM, int D, class Geo>
+__device__ __forceinline__ bool adjust_dim(const DevParams& P, const Geo& g, double* xv,
+                                           double* vv, double le_time) {
+  const double lo = g.lo[D], L = g.L[D];
+  int side = 0;  // -1: crossed the lower face, +1: the upper one
+  if (xv[D] < lo) {
+    xv[D] = xv[D] + L;
+    side = -1;
+  }
+  // (a periodic dimension checks the upper face on the shifted value too)
+  if (!(side < 0 && D == P.le_g) && xv[D] >= lo + L) {
+    xv[D] = xv[D] - L;
+    side = 1;
+  }
+  if (side == 0 || D != P.le_g) return false;
+  // shear_length(x, lower) = (v_lower - v_upper) t, shear_length(x, upper)
+  // = (v_upper - v_lower) t; the velocity jump has the opposite sign
+  const double du = P.le_vhi - P.le_vlo;
+  const double sl = side < 0 ? -du * le_time : du * le_time;
+  xv[0] = f_modulo(xv[0] - sl, g.L[0]);
+  if (DIM == 3) {
+    constexpr int K = D == 1 ? 2 : 1;
+    xv[K] = f_modulo(xv[K], g.L[K]);
+  }
+  vv[0] = side < 0 ? vv[0] + du : vv[0] - du;
+  return true;
+}
+
 // Periodic and Lees-Edwards faces after a drift (particles_adjust_particles
 // .F90:137-389): dimensions in order; crossing a Lees-Edwards face moves
 // every other coordinate k by MODULO(x_k - shear_length_k, L_k) (the box
 // starts at 0 then; the shear length is zero but along x, so for the third
 // coordinate this is a plain periodic wrap), adds the face velocity jump,
 // and ends the particle's check.
-template <int DIM>
-__device__ __forceinline__ void adjust_particle(const DevParams& P, const RepGeom& g, double* xv,
+template <int DIM, class Geo>
+__device__ __forceinline__ void adjust_particle(const DevParams& P, const Geo& g, double* xv,
                                                 double* vv, double le_time) {
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) {
-    const double lo = g.lo[d], L = g.L[d];
-    if (xv[d] < lo) {
-      if (d == P.le_g) {
-        xv[d] = xv[d] + L;
-        const double sl = (P.le_vlo - P.le_vhi) * le_time;  // shear_length(x, lower)
-        xv[0] = f_modulo(xv[0] - sl, g.L[0]);
-        if (DIM == 3) {
-          const int k = d == 1 ? 2 : 1;
-          xv[k] = f_modulo(xv[k], g.L[k]);
-        }
-        vv[0] = vv[0] + (P.le_vhi - P.le_vlo);
-        return;
-      }
-      xv[d] = xv[d] + L;
-    }
-    if (xv[d] >= lo + L) {
-      if (d == P.le_g) {
-        xv[d] = xv[d] - L;
-        const double sl = (P.le_vhi - P.le_vlo) * le_time;  // shear_length(x, upper)
-        xv[0] = f_modulo(xv[0] - sl, g.L[0]);
-        if (DIM == 3) {
-          const int k = d == 1 ? 2 : 1;
-          xv[k] = f_modulo(xv[k], g.L[k]);
-        }
-        vv[0] = vv[0] + (P.le_vlo - P.le_vhi);
-        return;
-      }
-      xv[d] = xv[d] - L;
-    }
-  }
+  if (adjust_dim<DIM, 0>(P, g, xv, vv, le_time)) return;
+  if (adjust_dim<DIM, 1>(P, g, xv, vv, le_time)) return;
+  if (DIM == 3) (void)adjust_dim<DIM, DIM == 3 ? 2 : 1>(P, g, xv, vv, le_time);
 }
 
 __global__ void k_drift_wrap(DevParams P, DeviceState S) {
@@ -115,13 +119,31 @@ __global__ void k_conformation(DevParams P, DeviceState S, double lambda) {
 // signature) screen at rc + skin.  Every particle has moved at most skin / 2
 // since the build, so the build-time positions put every pair that is
 // within rc now inside that screen, and the exact FP64 decision follows.
+// A replica's corner and edges in registers.
+struct BoxFrame {
+  double lo[3], L[3];
+};
+template <int DIM>
+__device__ __forceinline__ BoxFrame load_frame(const RepGeom& g) {
+  BoxFrame f;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    f.lo[d] = d < DIM ? g.lo[d] : 0.0;
+    f.L[d] = d < DIM ? g.L[d] : 1.0;
+  }
+  return f;
+}
+
+template <int DIM>
 __device__ __forceinline__ double track_particle(const DevParams& P, const DeviceState& S,
-                                                 const RepGeom& g, int i, const double* xv) {
+                                                 const RepGeom& g, const BoxFrame& f, int i,
+                                                 const double* xv) {
   const int64_t n = P.n;
   double w[3] = {0.0, 0.0, 0.0};
   double d2 = 0.0;
-  for (int d = 0; d < P.dim; ++d) {
-    const double L = g.L[d];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const double L = f.L[d];
     const double raw = xv[d] - S.x_ref[d * n + i];
     w[d] = raw > 0.5 * L ? -L : (raw < -0.5 * L ? L : 0.0);
     const double dd = raw + w[d];
@@ -131,7 +153,8 @@ __device__ __forceinline__ double track_particle(const DevParams& P, const Devic
     const int kl = S.key_sorted[i] - g.cell_base;
     const int cc[3] = {kl % g.nm[0], (kl / g.nm[0]) % g.nm[1], kl / (g.nm[0] * g.nm[1])};
     float a[3] = {0.f, 0.f, 0.f};
-    for (int d = 0; d < P.dim; ++d)
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
       a[d] = (float)((xv[d] - (g.lo[d] + (double)cc[d] * g.cs[d])) + w[d]);
     const float4 old = S.xa[i];
     S.xa[i] = make_float4(a[0], a[1], a[2], old.w);
@@ -164,17 +187,22 @@ __device__ __forceinline__ void block_max_disp(unsigned long long* disp, double 
 // next one.  Then either the cell key and cell histogram of the next sort
 // (lists rebuilt every step; cell counts must be zero on entry) or, with
 // Verlet lists, the displacement since the lists' build.
-template <bool VERLET>
+template <int DIM, bool VERLET>
 __global__ __launch_bounds__(256) void k_boundary(DevParams P, DeviceState S, int closing,
                                                   int opening) {
   const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
   const bool act = i0 < P.n;
   const int i = act ? i0 : P.n - 1;  // lanes past the end compute but store nothing
   const int64_t n = P.n;
-  const int dim = P.dim;
+  constexpr int dim = DIM;
+  // the replica's corner and edges in registers: their loads issue with
+  // the state's (read through a reference into global memory, they were
+  // issued one dimension after another inside the wrap)
   const RepGeom& g = S.geo[S.rep[i]];
+  const BoxFrame fr = load_frame<DIM>(g);
   double xv[3] = {0.0, 0.0, 0.0}, vw[3] = {0.0, 0.0, 0.0};
   const bool fluid = is_fluid(P, S, i);
+#pragma unroll
   for (int d = 0; d < dim; ++d) {
     double vv = S.v[d * n + i];
     double ff = S.f[d * n + i];
@@ -185,19 +213,16 @@ __global__ __launch_bounds__(256) void k_boundary(DevParams P, DeviceState S, in
     if (opening && fluid) x = x + vv * P.dt;
     xv[d] = x;
   }
-  if (opening) {
-    if (dim == 3)
-      adjust_particle<3>(P, g, xv, vw, S.time_dev[1]);
-    else
-      adjust_particle<2>(P, g, xv, vw, S.time_dev[1]);
-  }
+  if (opening) adjust_particle<DIM>(P, fr, xv, vw, S.time_dev[1]);
   if (act) {
+#pragma unroll
     for (int d = 0; d < dim; ++d) {
       S.v[d * n + i] = vw[d];
       S.x[d * n + i] = xv[d];
     }
     if (!P.newtonian) {
-      const int d2 = dim * dim;
+      constexpr int d2 = dim * dim;
+#pragma unroll
       for (int q = 0; q < d2; ++q) {
         double c = S.ct[q * n + i];
         double a = S.act[q * n + i];
@@ -208,7 +233,7 @@ __global__ __launch_bounds__(256) void k_boundary(DevParams P, DeviceState S, in
     }
   }
   if (VERLET) {
-    const double d2 = act ? track_particle(P, S, g, i, xv) : 0.0;
+    const double d2 = act ? track_particle<DIM>(P, S, g, fr, i, xv) : 0.0;
     block_max_disp(S.disp, d2);
     return;
   }
@@ -230,7 +255,10 @@ __global__ __launch_bounds__(256) void k_track(DevParams P, DeviceState S) {
   const int64_t n = P.n;
   double xv[3] = {0.0, 0.0, 0.0};
   for (int d = 0; d < P.dim; ++d) xv[d] = S.x[d * n + i];
-  const double d2 = act ? track_particle(P, S, S.geo[S.rep[i]], i, xv) : 0.0;
+  const RepGeom& g = S.geo[S.rep[i]];
+  const double d2 = !act ? 0.0
+                          : (P.dim == 3 ? track_particle<3>(P, S, g, load_frame<3>(g), i, xv)
+                                        : track_particle<2>(P, S, g, load_frame<2>(g), i, xv));
   block_max_disp(S.disp, d2);
 }
 
@@ -399,14 +427,19 @@ void launch_conformation(hipStream_t s, const DevParams& P, const DeviceState& S
 
 void launch_boundary(hipStream_t s, const DevParams& P, const DeviceState& S, int closing,
                      int opening) {
+  const dim3 g(nblk(P.n)), b(256);
   if (P.verlet) {
-    hipLaunchKernelGGL(k_boundary<true>, dim3(nblk(P.n)), dim3(256), 0, s, P, S, closing,
-                       opening);
+    if (P.dim == 3)
+      hipLaunchKernelGGL((k_boundary<3, true>), g, b, 0, s, P, S, closing, opening);
+    else
+      hipLaunchKernelGGL((k_boundary<2, true>), g, b, 0, s, P, S, closing, opening);
     return;
   }
   launch_fill_i32(s, S.count, 0, P.ncell);
-  hipLaunchKernelGGL(k_boundary<false>, dim3(nblk(P.n)), dim3(256), 0, s, P, S, closing,
-                     opening);
+  if (P.dim == 3)
+    hipLaunchKernelGGL((k_boundary<3, false>), g, b, 0, s, P, S, closing, opening);
+  else
+    hipLaunchKernelGGL((k_boundary<2, false>), g, b, 0, s, P, S, closing, opening);
 }
 
 void launch_track(hipStream_t s, const DevParams& P, const DeviceState& S) {

@@ -258,9 +258,12 @@ int sdpd_set_skin(sdpd_ctx *ctx, double skin);
 int sdpd_get_skin(sdpd_ctx *ctx, double *skin);
 int sdpd_get_list_builds(sdpd_ctx *ctx, int64_t *builds);
 
-/* Autotuner: times reps density + force passes at the current positions
- * for every launch shape (the simulation does not advance), applies the
- * fastest and returns it with its time per pass pair (ms). */
+/* Autotuner: times reps passes at the current positions for each launch
+ * shape (the simulation does not advance), applies the fastest and returns
+ * it with its time per density + force pass pair (ms).  Lists rebuilt every
+ * step: density + force pass pairs for every shape.  Verlet lists: the
+ * chunk order on pass pairs that walk the lists (each step's cost), then
+ * the density shape on candidate scans of every chunk (each build's). */
 int sdpd_autotune(sdpd_ctx *ctx, int32_t reps, sdpd_tuning *best, double *ms_best);
 
 /* Noise step counter (the reference's marching step). */

@@ -64,7 +64,7 @@ def save_cache(path: str, cache: dict) -> None:
     os.replace(tmp, path)
 
 
-def tune(sim, prm: SDPDParams, n: int, volume: float, nrep: int = 1, reps: int = 3,
+def tune(sim, prm: SDPDParams, n: int, volume: float, nrep: int = 1, reps: int = 5,
          cache_path: Optional[str] = None, retune: bool = False, tag: str = "") -> dict:
     """Apply the cached launch shape of this workload class to `sim` (an
     mcf_amd.sdpd.SDPD with particles uploaded), or run the autotuner and

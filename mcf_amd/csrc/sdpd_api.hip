@@ -1461,35 +1461,69 @@ int sdpd_autotune(sdpd_ctx* c, int32_t reps, sdpd_tuning* best, double* ms_best)
     release_saved();
     return fail(c, SDPD_ERR_DEVICE, "sdpd_autotune: event creation failed");
   }
+  // reps timed passes after an untimed one, at the current positions (the
+  // state is recomputed identically each time): SCAN times the candidate
+  // scan of every chunk (the density pass of a Verlet build), otherwise a
+  // density + force pass pair as a step runs it.  Returns the time in ms,
+  // or -1 after an error (rc set).
+  auto time_passes = [&](bool scan) -> float {
+    auto pass = [&]() -> int {
+      if (scan) return run_density(c, true);
+      int r = run_density(c);
+      if (!r) r = run_forces(c);
+      return r;
+    };
+    if ((rc = pass())) return -1.f;
+    (void)hipEventRecord(a, c->stream);
+    for (int r = 0; r < reps && rc == SDPD_OK; ++r) rc = pass();
+    (void)hipEventRecord(b, c->stream);
+    if (rc) return -1.f;
+    if (hipEventSynchronize(b) != hipSuccess) {
+      rc = fail(c, SDPD_ERR_DEVICE, "sdpd_autotune: event wait failed");
+      return -1.f;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+  };
   sdpd_tuning pick{0, 1};
   float tbest = -1.f;
-  for (int order = 1; order >= 0 && rc == SDPD_OK; --order)
-    for (int shape = 0; shape < 4 && rc == SDPD_OK; ++shape) {
-      c->P.density_shape = shape;
+  if (c->P.verlet) {
+    // Verlet lists: a step walks the lists, whose cost depends on the chunk
+    // order alone; the density shape serves the candidate scan of the
+    // builds.  The order is chosen on walk passes, then the shape on scans.
+    for (int order = 1; order >= 0 && rc == SDPD_OK; --order) {
       c->P.xcd_order = order;
-      // one untimed pass, then reps timed density + force passes at the
-      // current positions (the state is recomputed identically each time).
-      // With Verlet lists the timed density passes scan the stencil of
-      // every chunk (the launch shapes matter to the lists' builds), which
-      // gives the sums of the list walk.
-      if ((rc = run_density(c)) || (rc = run_forces(c))) break;
-      (void)hipEventRecord(a, c->stream);
-      for (int r = 0; r < reps && rc == SDPD_OK; ++r)
-        if (!(rc = run_density(c, c->P.verlet != 0))) rc = run_forces(c);
-      (void)hipEventRecord(b, c->stream);
-      if (rc) break;
-      if (hipEventSynchronize(b) != hipSuccess) {
-        rc = fail(c, SDPD_ERR_DEVICE, "sdpd_autotune: event wait failed");
-        break;
-      }
-      float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, a, b);
-      if (tbest < 0.f || ms < tbest) {
+      const float ms = time_passes(false);
+      if (ms >= 0.f && (tbest < 0.f || ms < tbest)) {
         tbest = ms;
-        pick.density_shape = shape;
         pick.chunk_order = order;
       }
     }
+    c->P.xcd_order = pick.chunk_order;
+    float sbest = -1.f;
+    for (int shape = 0; shape < 4 && rc == SDPD_OK; ++shape) {
+      c->P.density_shape = shape;
+      const float ms = time_passes(true);
+      if (ms >= 0.f && (sbest < 0.f || ms < sbest)) {
+        sbest = ms;
+        pick.density_shape = shape;
+      }
+    }
+  } else {
+    // lists rebuilt every step: the density pass is the scan itself
+    for (int order = 1; order >= 0 && rc == SDPD_OK; --order)
+      for (int shape = 0; shape < 4 && rc == SDPD_OK; ++shape) {
+        c->P.density_shape = shape;
+        c->P.xcd_order = order;
+        const float ms = time_passes(false);
+        if (ms >= 0.f && (tbest < 0.f || ms < tbest)) {
+          tbest = ms;
+          pick.density_shape = shape;
+          pick.chunk_order = order;
+        }
+      }
+  }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   c->timing = timing;

@@ -1486,43 +1486,62 @@ int sdpd_autotune(sdpd_ctx* c, int32_t reps, sdpd_tuning* best, double* ms_best)
     (void)hipEventElapsedTime(&ms, a, b);
     return ms;
   };
+  // Each choice is timed twice, in a sequence and then backwards, and keeps
+  // its faster time: the GPU's clock still settles during the first passes,
+  // which would otherwise favour whatever is timed last.  The linear chunk
+  // order is taken only when it is faster by more than kOrderMargin: the
+  // two orders often time within noise of each other, and the
+  // XCD-contiguous one reads less through the fabric (each XCD's L2 serves
+  // a compact slab).
+  constexpr float kOrderMargin = 0.99f;
+  auto keep_min = [](float& t, float ms) {
+    if (ms >= 0.f && (t < 0.f || ms < t)) t = ms;
+  };
   sdpd_tuning pick{0, 1};
   float tbest = -1.f;
   if (c->P.verlet) {
     // Verlet lists: a step walks the lists, whose cost depends on the chunk
     // order alone; the density shape serves the candidate scan of the
     // builds.  The order is chosen on walk passes, then the shape on scans.
-    for (int order = 1; order >= 0 && rc == SDPD_OK; --order) {
+    float t_order[2] = {-1.f, -1.f};
+    for (int k = 0; k < 4 && rc == SDPD_OK; ++k) {
+      const int order = k == 0 || k == 3 ? 1 : 0;
       c->P.xcd_order = order;
-      const float ms = time_passes(false);
-      if (ms >= 0.f && (tbest < 0.f || ms < tbest)) {
-        tbest = ms;
-        pick.chunk_order = order;
-      }
+      keep_min(t_order[order], time_passes(false));
+    }
+    if (rc == SDPD_OK) {
+      pick.chunk_order = t_order[0] < kOrderMargin * t_order[1] ? 0 : 1;
+      tbest = t_order[pick.chunk_order];
     }
     c->P.xcd_order = pick.chunk_order;
-    float sbest = -1.f;
-    for (int shape = 0; shape < 4 && rc == SDPD_OK; ++shape) {
+    float t_shape[4] = {-1.f, -1.f, -1.f, -1.f};
+    for (int k = 0; k < 8 && rc == SDPD_OK; ++k) {
+      const int shape = k < 4 ? k : 7 - k;
       c->P.density_shape = shape;
-      const float ms = time_passes(true);
-      if (ms >= 0.f && (sbest < 0.f || ms < sbest)) {
-        sbest = ms;
-        pick.density_shape = shape;
-      }
+      keep_min(t_shape[shape], time_passes(true));
     }
+    for (int shape = 1; shape < 4; ++shape)
+      if (t_shape[shape] < t_shape[pick.density_shape]) pick.density_shape = shape;
   } else {
     // lists rebuilt every step: the density pass is the scan itself
-    for (int order = 1; order >= 0 && rc == SDPD_OK; --order)
-      for (int shape = 0; shape < 4 && rc == SDPD_OK; ++shape) {
-        c->P.density_shape = shape;
-        c->P.xcd_order = order;
-        const float ms = time_passes(false);
-        if (ms >= 0.f && (tbest < 0.f || ms < tbest)) {
-          tbest = ms;
-          pick.density_shape = shape;
-          pick.chunk_order = order;
-        }
-      }
+    float t[2][4] = {{-1.f, -1.f, -1.f, -1.f}, {-1.f, -1.f, -1.f, -1.f}};
+    for (int k = 0; k < 16 && rc == SDPD_OK; ++k) {
+      const int m = k < 8 ? k : 15 - k;  // the sequence, then backwards
+      const int order = 1 - m / 4, shape = m % 4;
+      c->P.density_shape = shape;
+      c->P.xcd_order = order;
+      keep_min(t[order][shape], time_passes(false));
+    }
+    if (rc == SDPD_OK) {
+      int s_order[2] = {0, 0};
+      for (int order = 0; order < 2; ++order)
+        for (int shape = 1; shape < 4; ++shape)
+          if (t[order][shape] < t[order][s_order[order]]) s_order[order] = shape;
+      const float t0 = t[0][s_order[0]], t1 = t[1][s_order[1]];
+      pick.chunk_order = t0 < kOrderMargin * t1 ? 0 : 1;
+      pick.density_shape = s_order[pick.chunk_order];
+      tbest = pick.chunk_order ? t1 : t0;
+    }
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);

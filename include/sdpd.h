@@ -239,12 +239,15 @@ int sdpd_get_list_capacity(sdpd_ctx *ctx, int32_t *cap);
 
 /* Verlet lists.  No counterpart in the reference (technique_build_list
  * rebuilds PPM's cell list every step, marching_integrate_VV.F90:968-971).
- * In a periodic box without sheared faces or colloids the stored lists
- * hold every pair within cut_off + skin of their build; the particles keep
- * their order until the next build, which happens only once a particle has
- * moved more than skin / 2 since the last one.  Every step still decides
- * in or out of the cut-off on its own positions with the reference's FP64
- * formula, so the neighbour sets and the sums equal a per-step rebuild's.
+ * The stored lists hold every pair within cut_off + skin of their build;
+ * the particles keep their order until the next build, which happens once
+ * a particle has moved more than skin / 2 since the last one or, with
+ * sheared faces, once twice the largest displacement plus the drift of the
+ * sheared copies since the build (|v_upper - v_lower| times the time)
+ * exceeds the skin.  Every step still decides in or out of the cut-off on
+ * its own positions with the reference's FP64 formula, the image across a
+ * sheared face taken from them, so the neighbour sets equal a per-step
+ * rebuild's.  Mixed precision with sheared faces rebuilds every step.
  * skin < 0 (the default): automatic, 2 v dt 256 with v the fastest upload
  * speed or five thermal deviations sqrt(kT / m_min), whichever is larger,
  * kept between 1/20 of and the largest skin the sort grid allows (twice its

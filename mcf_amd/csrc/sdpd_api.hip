@@ -458,7 +458,7 @@ int alloc_state(sdpd_ctx* c, int n) {
                          hipMemcpyHostToDevice));
     ALLOC(rep_stats, 4 * (size_t)P.nrep);
     ALLOC(step_dev, 1);
-    ALLOC(time_dev, 3);
+    ALLOC(time_dev, 4);
   }
   {
     const size_t nrows = (size_t)P.nrows;
@@ -771,8 +771,8 @@ namespace {
 // Verlet lists or lists rebuilt every step.  The candidate scan reaches two
 // cells of the sort grid each way, so a build finds every pair within
 // 2 cs - rc of the cut-off: that is the largest skin (less a margin for the
-// rounding of cell keys).  Sheared faces and colloids, a request of 0 or a
-// skin thinner than 1e-6 rc rebuild the lists every step.
+// rounding of cell keys).  Mixed precision with sheared faces, a request
+// of 0 or a skin thinner than 1e-6 rc rebuild the lists every step.
 //
 // The automatic skin trades the candidates past the cut-off, which every
 // step's passes walk (about 3 skin / rc of the neighbours), against the
@@ -780,7 +780,8 @@ namespace {
 // steps, twice over (lists are rebuilt after skin / 2), between 1/20 of
 // the grid's largest and that largest.  v_est is the fastest upload speed,
 // or five thermal standard deviations sqrt(kT / m) of the lightest
-// particle when those are faster.  Results do not depend on the skin.
+// particle when those are faster.  The neighbour sets do not depend on
+// the skin.
 constexpr double kSkinSteps = 256.0;
 
 double auto_skin(const DevParams& P, double smax, double v_est) {
@@ -796,7 +797,8 @@ int choose_skin(sdpd_ctx* c, double v_est) {
                 "sdpd_set_skin: the skin exceeds what the sort grid allows (2 cell edges - cut_off = " +
                     std::to_string(smax) + ")");
   const double skin = c->skin_req < 0.0 ? auto_skin(P, smax, v_est) : c->skin_req;
-  const bool on = P.le_g < 0 && P.ncoll == 0 && sdpd::skin_usable(P, skin);
+  // (mixed precision in a sheared box keeps lists rebuilt every step)
+  const bool on = !(P.le_g >= 0 && P.mixed) && sdpd::skin_usable(P, skin);
   P.verlet = on ? 1 : 0;
   P.skin = on ? skin : 0.0;
   P.half_skin2 = 0.25 * P.skin * P.skin;

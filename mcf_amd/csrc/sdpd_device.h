@@ -77,11 +77,12 @@ struct DevParams {
   // plus a colloid's radius, outer layer and the cut-off: the nearest image
   // of a colloid seen from there may lie across the face)
   int32_t coll_le_cells;
-  // Verlet lists (periodic fluid boxes, single or batched): the stored
-  // lists hold every candidate within rc + skin at their build, the state
-  // keeps its order until the next build, and a build happens only once a
-  // particle has moved more than skin / 2 since the last one.  Every step
-  // decides in / out of the cut-off afresh on the current positions.
+  // Verlet lists: the stored lists hold every candidate within rc + skin
+  // at their build, the state keeps its order until the next build, and a
+  // build happens once a particle has moved more than skin / 2 since the
+  // last one (with sheared faces: once twice that displacement plus the
+  // sheared copies' drift exceeds the skin).  Every step decides in / out
+  // of the cut-off afresh on the current positions.
   // verlet = 0: lists of in-cut-off pairs, rebuilt every step.
   int32_t verlet;
   double skin;

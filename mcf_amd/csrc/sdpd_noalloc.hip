@@ -118,7 +118,7 @@ size_t carve_state(unsigned char* base, const DevParams& P, DeviceState& S) {
   S.row_len = c.take<int32_t>(nrows);
   S.row_rep = c.take<int32_t>(nrows);
   S.step_dev = c.take<int64_t>(1);
-  S.time_dev = c.take<double>(3);
+  S.time_dev = c.take<double>(4);
   const size_t nchunk = (N + 63) / 64 + nrows;
   const size_t ntile = (std::max((size_t)P.ncell, nrows) + 4095) / 4096 + 1;
   S.row_nchunk = c.take<int32_t>(nrows);

@@ -188,6 +188,48 @@ __device__ __forceinline__ double exact_d2_pos(const DevParams& P, const Home& h
   return DIM == 3 ? dist2_exact(d0, d1, d2) : dist2_exact2(d0, d1);
 }
 
+// The Verlet lists' decision: the exact squared distance on the current
+// positions with the image taken from them.  In a box with sheared faces a
+// pair whose coordinates along the faces' normal differ by more than half
+// the box meets across a face, and j's copy there is the one the candidate
+// scan sees: sigma 0, the copy above the upper face, or 1, below the lower
+// one (returned in sg; -1 when no sheared face lies between them), with the
+// formula and roundings of exact_d2_le.
+template <int DIM>
+__device__ __forceinline__ double exact_d2_verlet(const DevParams& P, const Home& h, double xj0,
+                                                  double xj1, double xj2, double& d0,
+                                                  double& d1, double& d2, int& sg) {
+  sg = -1;
+  if (P.le_g >= 0) {
+    const bool gy = P.le_g == 1;
+    const double raw = gy ? h.x1 - xj1 : h.x2 - xj2;
+    const double Lg = gy ? h.G.L1 : h.G.L2;
+    sg = raw > 0.5 * Lg ? 0 : (raw < -0.5 * Lg ? 1 : -1);
+  }
+  if (sg < 0) return exact_d2_pos<DIM>(P, h, xj0, xj1, xj2, d0, d1, d2);
+  const double L0 = h.G.L0;
+  const double su = (P.le_vhi - P.le_vlo) * h.le.t, sd = (P.le_vlo - P.le_vhi) * h.le.t;
+  double xi = h.x0, xj = xj0;
+  if (sg == 0)
+    xj = f_modulo(xj + su, L0);
+  else if (P.symmetry)
+    xi = f_modulo(xi + su, L0);
+  else
+    xj = f_modulo(xj + sd, L0);
+  const double rx = xi - xj;
+  const int mx = rx > 0.5 * L0 ? 1 : (rx < -0.5 * L0 ? (P.symmetry ? 2 : 3) : 0);
+  const int mg = sg == 0 ? 1 : (P.symmetry ? 2 : 3);
+  d0 = image_diff(xi, xj, L0, mx);
+  if (P.le_g == 1) {
+    d1 = image_diff(h.x1, xj1, h.G.L1, mg);
+    d2 = DIM == 3 ? image_diff_pos(h.x2, xj2, h.G.L2, P.symmetry) : 0.0;
+  } else {
+    d1 = image_diff_pos(h.x1, xj1, h.G.L1, P.symmetry);
+    d2 = image_diff(h.x2, xj2, h.G.L2, mg);
+  }
+  return DIM == 3 ? dist2_exact(d0, d1, d2) : dist2_exact2(d0, d1);
+}
+
 // FP32 squared distance, rounded as the candidate scan rounds it, so the
 // in / out decisions of the density and force passes coincide
 __device__ __forceinline__ float mixed_s2(float d0, float d1, float d2) {
@@ -304,7 +346,11 @@ __device__ __forceinline__ Home load_home(const DevParams& P, const DeviceState&
     for (int sg = 0; sg < 2; ++sg) {
       const double sl = sg == 0 ? (P.le_vhi - P.le_vlo) * h.le.t : (P.le_vlo - P.le_vhi) * h.le.t;
       const double sp = f_modulo(sl, ch.G.L0);
-      const int wc = (int)floor((h.x0 - sp) / cs);
+      // Verlet lists keep the build's cells, and the FP32 position goes on
+      // from the build cell unwrapped: the window centre is taken in that
+      // frame, so a lane that crossed an x face since stays with its wave
+      const double xw = P.verlet ? (double)h.cx * cs + (double)h.a.x : h.x0;
+      const int wc = (int)floor((xw - sp) / cs);
       const float dxf = (float)((double)(h.cx - wc) * cs - sp);
       int mn = h.active ? wc : INT_MAX, mx = h.active ? wc : INT_MIN;
 #pragma unroll
@@ -722,7 +768,8 @@ struct DensityF64 {
       if (DECIDE) {
         // the reference's decision (pp_interaction.inc:37-46) on the
         // current positions
-        r2 = exact_d2_pos<DIM>(P, h, x0[k], x1[k], x2[k], d0, d1, d2);
+        int sg;
+        r2 = exact_d2_verlet<DIM>(P, h, x0[k], x1[k], x2[k], d0, d1, d2, sg);
         use = use && r2 < P.cut2;
         nin += use ? 1 : 0;
       } else {
@@ -1019,6 +1066,9 @@ __device__ __forceinline__ void density_chunk(const DevParams& P, const DeviceSt
   S.bn[i] = bn;
   if (MODE == kStore || MODE == kBuild) S.nbr_count[i] = cnt;
   if (MODE == kBuild && cnt > P.list_cap) S.gate[2] = 1;  // every writer stores 1
+  // the shear time of the build (the copies across sheared faces drift
+  // from their build positions as it advances)
+  if (MODE == kBuild && i == 0) S.time_dev[3] = S.time_dev[1];
   S.nbr_in[i] = nin;
   S.recheck[i] = rechecks;
   write_force_records<DIM>(P, S, i, an, bn, press);
@@ -1131,7 +1181,10 @@ struct ForceF64 {
   // Measured on the 128^3 box: forces 1.824 ms one entry at a time, 1.819
   // two, 1.761 four, 1.65 eight (193 VGPRs, two waves per SIMD); the
   // Oldroyd-B batch of 256 boxes 6.62 -> 6.23 ms with two.
-  static constexpr bool kBatched = DECIDE;
+  // With colloids the walk goes one entry at a time (the substitution
+  // branch's registers leave no room for batches; in twos the per-step
+  // lists ran slower).
+  static constexpr bool kBatched = DECIDE && !COLL;
   static constexpr int kBatch = FT == 5 ? 2 : 8;
   __device__ __forceinline__ void body(uint32_t ent) {
     const int j = (int)(ent & kIndexMask);
@@ -1173,17 +1226,21 @@ struct ForceF64 {
     // two boundary particles do not interact
     if (COLL && sidi != 0 && jb) return;
     double d0, d1, d2, r2;
+    double jump;  // x velocity of j's image minus j's own
     if (DECIDE) {
-      r2 = exact_d2_pos<DIM>(P, h, ra.x, ra.y, ra.z, d0, d1, d2);
+      // the image and the sheared face crossed come from the positions
+      int sg;
+      r2 = exact_d2_verlet<DIM>(P, h, ra.x, ra.y, ra.z, d0, d1, d2, sg);
       if (!(r2 < P.cut2)) return;
+      jump = sg == 0 ? P.le_u : (sg == 1 ? -P.le_u : 0.0);
     } else {
       r2 = pair_d2<DIM>(P, h, o, ra.x, ra.y, ra.z, d0, d1, d2);
+      jump = le_dv(P, o);
     }
     const double ir = rsqrt_nr(r2);
     const double r = r2 * ir;
     double w, gw;
     kernel_eval(P, r * P.inv_h, w, gw);
-    const double jump = le_dv(P, o);  // x velocity of j's image minus j's own
     // (without sheared faces the jump is the constant 0 and no add is made)
     double u0 = vi0 - (P.le_g >= 0 ? rb.x + jump : rb.x), u1 = vi1 - rb.y,
            u2 = DIM == 3 ? vi2 - rb.z : 0.0;
@@ -1691,8 +1748,9 @@ struct SignaturePol {
     if (DECIDE) {
       const int64_t n = P.n;
       double d0, d1, d2;
-      if (!(exact_d2_pos<DIM>(P, h, S.x[j], S.x[n + j], DIM == 3 ? S.x[2 * n + j] : 0.0, d0, d1,
-                              d2) < P.cut2))
+      int sg;
+      if (!(exact_d2_verlet<DIM>(P, h, S.x[j], S.x[n + j], DIM == 3 ? S.x[2 * n + j] : 0.0, d0,
+                                 d1, d2, sg) < P.cut2))
         return;
     }
     ++nin;
@@ -1762,8 +1820,18 @@ static void launch_density_dim(hipStream_t s, const DevParams& P, const DeviceSt
     return;
   }
   // Verlet lists: each kernel returns at once unless the step's gate
-  // selects it (build, or walk + rescan of overflowed chunks)
+  // selects it (build, or walk + rescan of overflowed chunks); a box with
+  // sheared faces splits the chunks between the Lees-Edwards and the plain
+  // instantiations, as its per-step lists do
   dim3 g(nchunks_max(P)), b(kWave);
+  if constexpr (!MIXED) {  // mixed precision keeps per-step lists in sheared boxes
+    if (P.le_g >= 0) {
+      launch_density_shape<DIM, MIXED, true, kBuild>(s, P, S);
+      hipLaunchKernelGGL((k_density<DIM, MIXED, kTile, kBuf, true, kWalk>), g, b, 0, s, P, S);
+      hipLaunchKernelGGL((k_density<DIM, MIXED, kTile, kBuf, true, kRescan>), rescan_grid(P), b,
+                         0, s, P, S);
+    }
+  }
   launch_density_shape<DIM, MIXED, false, kBuild>(s, P, S);
   hipLaunchKernelGGL((k_density<DIM, MIXED, kTile, kBuf, false, kWalk>), g, b, 0, s, P, S);
   hipLaunchKernelGGL((k_density<DIM, MIXED, kTile, kBuf, false, kRescan>), rescan_grid(P), b, 0,
@@ -1797,13 +1865,23 @@ void launch_refresh_records(hipStream_t s, const DevParams& P, const DeviceState
 template <int DIM, int FT, bool MIXED, bool COLL>
 static void launch_forces_le(hipStream_t s, const DevParams& P, const DeviceState& S) {
   dim3 g(nchunks_max(P)), b(kWave);
-  if constexpr (!COLL) {
-    if (P.verlet) {
-      hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, false, false, false, true>), g, b, 0, s, P, S);
-      hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, false, false, true, true>), rescan_grid(P), b,
-                         0, s, P, S);
-      return;
+  if (P.verlet) {
+    // as the lists rebuilt every step: with colloids the chunks near a
+    // sheared face take the Lees-Edwards instantiation and the others the
+    // plain one; without, a sheared box runs the Lees-Edwards one alone
+    if constexpr (!MIXED) {  // mixed precision keeps per-step lists in sheared boxes
+      if (P.le_g >= 0) {
+        hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, true, false, true>), g, b, 0, s, P, S);
+        hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, true, true, true>), rescan_grid(P), b,
+                           0, s, P, S);
+      }
     }
+    if (P.le_g < 0 || COLL) {
+      hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, false, false, true>), g, b, 0, s, P, S);
+      hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, false, true, true>), rescan_grid(P), b,
+                         0, s, P, S);
+    }
+    return;
   }
   if (P.le_g >= 0) {
     hipLaunchKernelGGL((k_forces<DIM, FT, MIXED, COLL, true, false>), g, b, 0, s, P, S);
@@ -1854,14 +1932,18 @@ void launch_signature(hipStream_t s, const DevParams& P, const DeviceState& S, b
   dim3 g(nchunks_max(P)), b(kWave);
   const bool le = P.le_g >= 0;
   if (P.dim == 3) {
-    if (P.verlet)
+    if (P.verlet && le)
+      hipLaunchKernelGGL((k_signature<3, true, true>), g, b, 0, s, P, S);
+    else if (P.verlet)
       hipLaunchKernelGGL((k_signature<3, false, true>), g, b, 0, s, P, S);
     else if (le)
       hipLaunchKernelGGL((k_signature<3, true, false>), g, b, 0, s, P, S);
     else
       hipLaunchKernelGGL((k_signature<3, false, false>), g, b, 0, s, P, S);
   } else {
-    if (P.verlet)
+    if (P.verlet && le)
+      hipLaunchKernelGGL((k_signature<2, true, true>), g, b, 0, s, P, S);
+    else if (P.verlet)
       hipLaunchKernelGGL((k_signature<2, false, true>), g, b, 0, s, P, S);
     else if (le)
       hipLaunchKernelGGL((k_signature<2, true, false>), g, b, 0, s, P, S);

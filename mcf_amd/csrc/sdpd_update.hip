@@ -113,12 +113,14 @@ __global__ void k_conformation(DevParams P, DeviceState S, double lambda) {
 // and stays consistent with the stencil offsets stored in the lists.
 // Returns the squared displacement.
 //
-// FP64 runs keep the FP32 positions of the build: their walks take
-// distances from the FP64 positions, and the only readers of the FP32 ones
-// between builds (the rescans of overflowed lists and the neighbour-set
-// signature) screen at rc + skin.  Every particle has moved at most skin / 2
-// since the build, so the build-time positions put every pair that is
-// within rc now inside that screen, and the exact FP64 decision follows.
+// FP64 runs without sheared faces keep the FP32 positions of the build:
+// their walks take distances from the FP64 positions, and the only readers
+// of the FP32 ones between builds (the rescans of overflowed lists and the
+// neighbour-set signature) screen at rc + skin.  Every particle has moved
+// at most skin / 2 since the build, so the build-time positions put every
+// pair that is within rc now inside that screen, and the exact FP64
+// decision follows.  With sheared faces the scan's windows follow the
+// current shear length and FP32 positions, which are kept current.
 // A replica's corner and edges in registers.
 struct BoxFrame {
   double lo[3], L[3];
@@ -149,7 +151,7 @@ __device__ __forceinline__ double track_particle(const DevParams& P, const Devic
     const double dd = raw + w[d];
     d2 = d2 + dd * dd;
   }
-  if (P.mixed) {
+  if (P.mixed || P.le_g >= 0) {
     const int kl = S.key_sorted[i] - g.cell_base;
     const int cc[3] = {kl % g.nm[0], (kl / g.nm[0]) % g.nm[1], kl / (g.nm[0] * g.nm[1])};
     float a[3] = {0.f, 0.f, 0.f};
@@ -270,7 +272,21 @@ __global__ __launch_bounds__(256) void k_track(DevParams P, DeviceState S) {
 __global__ void k_decide(DevParams P, DeviceState S) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   const double d2 = __longlong_as_double((long long)S.disp[0]);
-  const int rebuild = d2 > P.half_skin2 ? 1 : 0;
+  double lim = P.half_skin2;
+  int rebuild = 0;
+  if (P.le_g >= 0) {
+    // sheared faces: the copies across a face move by |v_upper - v_lower|
+    // per unit time against the box, so a pair across it may have closed
+    // in by that drift since the build on top of the particles' own
+    // displacements: the lists hold while 2 max |dx| + drift < skin.  A
+    // list that overflowed rebuilds every step (the rescans then always
+    // see the positions of a build).
+    const double drift = fabs(P.le_vhi - P.le_vlo) * fabs(S.time_dev[1] - S.time_dev[3]);
+    const double room = 0.5 * (P.skin - drift);
+    lim = room > 0.0 ? room * room : -1.0;
+    rebuild = S.gate[2] != 0 ? 1 : 0;
+  }
+  rebuild = rebuild | (d2 > lim ? 1 : 0);
   // a build still pending (a sort whose lists no density pass has built
   // yet) stays pending
   S.gate[0] |= rebuild;

@@ -192,6 +192,12 @@ def test_skin_arguments():
     le.prm = le.prm.replace(le_dim=2, le_v_lower=[-0.3, 0, 0], le_v_upper=[0.3, 0, 0])
     with SDPD(le.prm) as g:
         g.upload(le.x, le.v, le.m, le.pid)
-        assert g.skin() == 0.0  # sheared faces: lists rebuilt every step
+        # sheared faces in FP64: Verlet lists with the automatic skin
+        assert g.skin() == pytest.approx(auto_skin(le), rel=1e-12)
+        g.integrate_VV(2)
+        assert g.list_builds() >= 1
+    with SDPD(W.mixed(le).prm) as g:
+        g.upload(le.x, le.v, le.m, le.pid)
+        assert g.skin() == 0.0  # mixed precision with sheared faces: rebuilt every step
         g.integrate_VV(2)
         assert g.list_builds() == -1

@@ -1181,11 +1181,11 @@ struct ForceF64 {
   // Measured on the 128^3 box: forces 1.824 ms one entry at a time, 1.819
   // two, 1.761 four, 1.65 eight (193 VGPRs, two waves per SIMD); the
   // Oldroyd-B batch of 256 boxes 6.62 -> 6.23 ms with two.
-  // With colloids the walk goes one entry at a time (the substitution
-  // branch's registers leave no room for batches; in twos the per-step
-  // lists ran slower).
-  static constexpr bool kBatched = DECIDE && !COLL;
-  static constexpr int kBatch = FT == 5 ? 2 : 8;
+  // With colloids the Verlet walk takes twos (C4 forces 3.66 ms one at a
+  // time, 3.31 in twos, 3.69 in fours; the substitution branch's registers
+  // leave no room for more).
+  static constexpr bool kBatched = DECIDE;
+  static constexpr int kBatch = COLL ? 2 : (FT == 5 ? 2 : 8);
   __device__ __forceinline__ void body(uint32_t ent) {
     const int j = (int)(ent & kIndexMask);
     if (COLL) {

@@ -6,7 +6,7 @@ displacements plus the drift of the sheared copies since the build could
 have closed a pair in by the skin.
 
 The bars are those of the per-step lists: trajectories against the oracle
-within 1e-10 with particles crossing the sheared faces between builds,
+within 1e-10 with particles crossing the sheared faces (3D and 2D boxes),
 neighbour sets equal to the oracle's bit for bit at the end, and the same
 pairs as the per-step lists at every checked step.
 
@@ -46,27 +46,38 @@ def oracle_signature_le(O, box, t):
     return cnt, (hs & np.uint64(0xFFFFFFFF)).astype(np.uint32)
 
 
-def sheared_box(kind="y3d", sym=True, n_side=13, dt_factor=20):
-    """A Tait box with sheared faces (shear rate 0.5), the solvent on the
-    linear profile and a time step large enough that a thin skin lasts a few
-    steps.  The lattice is shifted along the faces' normal so that its top
+def sheared_box(kind="y3d", sym=True, n_side=13):
+    """A box with sheared faces (shear rate 0.5), the solvent on the linear
+    profile and a time step large enough that a thin skin (box.skin, 0.05 dx)
+    lasts a few steps: the sheared copies drift by 0.003 to 0.0065 dx per
+    step.  The lattice is shifted along the faces' normal so that its top
     plane sits 0.05 dx below the upper face; that plane moves towards it by
-    0.004 dx per step and crosses it between builds."""
-    box = W.tait_box_3d(n_side, kt=1e-3, symmetry=sym)
-    L = float(n_side)
-    le_dim = 2 if kind == "y3d" else 3
+    0.004 dx per step (0.003 in 2D) and crosses it.  A crossing counts the
+    tangential shift it took as displacement, so it rebuilds the lists unless
+    that shift is smaller than what is left of the skin.
+    y3d / z3d: a 13^3 Tait box sheared across y or z.  y2d: the 24^2
+    shear-wave box (unit square) sheared across y, where the whole fluid
+    moves along the normal (its light particles would stop a single plane
+    within a few steps)."""
+    if kind == "y2d":
+        box = W.shear_wave_2d(24, brownian=True, kt=1e-6, symmetry=sym)
+        L, dx, le_dim, dt_factor, w = 1.0, 1.0 / 24, 2, 2.5, 0.003
+    else:
+        box = W.tait_box_3d(n_side, kt=1e-3, symmetry=sym)
+        L, dx, le_dim, dt_factor, w = float(n_side), 1.0, 2 if kind == "y3d" else 3, 20, 0.004
     g = le_dim - 1
     u = 0.5 * L / 2
     dt = box.prm.dt * dt_factor
     box.prm = box.prm.replace(le_dim=le_dim, le_v_lower=[-u, 0, 0], le_v_upper=[u, 0, 0], dt=dt)
     x = box.x.copy()
-    x[:, g] = np.mod(x[:, g] + 0.45, L)
+    x[:, g] = np.mod(x[:, g] + 0.45 * dx, L)
     v = box.v.copy()
     v[:, 0] += 0.5 * (x[:, g] - L / 2)
-    top = x[:, g] > L - 0.3
-    v[top, g] += 0.004 / dt
+    moving = x[:, g] > (L - 0.3 * dx if kind != "y2d" else -1.0)
+    v[moving, g] += w * dx / dt
     box.x = np.ascontiguousarray(x)
     box.v = np.ascontiguousarray(v)
+    box.skin = 0.05 * dx
     return box
 
 
@@ -90,13 +101,14 @@ def periodic_err(a, b, L):
     return np.minimum(d, L - d).max()
 
 
-@pytest.mark.parametrize("kind,sym", [("y3d", True), ("y3d", False), ("z3d", True)])
+@pytest.mark.parametrize("kind,sym",
+                         [("y3d", True), ("y3d", False), ("z3d", True), ("y2d", True), ("y2d", False)])
 def test_sheared_verlet_lists_against_oracle(oracle, kind, sym):
     box = sheared_box(kind, sym)
     steps = 30
     announce(f"sheared Verlet lists: {kind} sym={sym}, {steps} steps")
-    out = gpu_run(box, steps, skin=0.05)
-    assert out["skin"] == pytest.approx(0.05)
+    out = gpu_run(box, steps, skin=box.skin)
+    assert out["skin"] == pytest.approx(box.skin)
     assert out["builds"] >= 3  # rebuilt within the run, not every step
     assert out["builds"] < steps
     st = oracle.State(box.prm, box.x, box.v, box.m, box.pid)
@@ -105,7 +117,7 @@ def test_sheared_verlet_lists_against_oracle(oracle, kind, sym):
         st.vv_step(s)
     L = box.prm.max_phys[0]
     g = box.prm.le_dim - 1
-    # the crossers went through the sheared faces between builds
+    # the crossers went through the sheared faces
     assert (np.abs(out["x"][:, g] - box.x[:, g]) > 0.5 * L).sum() >= 4
     assert periodic_err(out["x"], st.x, L) < 1e-10 * L
     assert vec_err(out["v"], st.v) < 1e-10
@@ -115,12 +127,12 @@ def test_sheared_verlet_lists_against_oracle(oracle, kind, sym):
     assert np.array_equal(out["sig"][0], cnt) and np.array_equal(out["sig"][1], hs)
 
 
-@pytest.mark.parametrize("kind", ["y3d", "z3d"])
+@pytest.mark.parametrize("kind", ["y3d", "z3d", "y2d"])
 def test_sheared_verlet_agrees_with_per_step_lists(kind):
     box = sheared_box(kind, True)
     steps = 20
     announce(f"sheared Verlet vs per-step lists: {kind}")
-    a = gpu_run(box, steps, skin=0.05)
+    a = gpu_run(box, steps, skin=box.skin)
     b = gpu_run(box, steps, skin=0.0)
     assert a["builds"] >= 2 and b["builds"] == -1
     assert a["pairs"] == b["pairs"]

@@ -1,6 +1,7 @@
 """Lees-Edwards faces on the GPU against the oracle: neighbour sets
 bit-exact across the sheared faces, density / forces within the FP64 bar,
-trajectories with particles crossing the faces, mixed against FP64."""
+trajectories with particles crossing the faces, Oldroyd-B trajectories whose
+velocity gradient sees the sheared copies' jump, mixed against FP64."""
 import numpy as np
 import pytest
 
@@ -194,3 +195,70 @@ def test_le_corner_crossing(oracle):
         st.vv_step(s)
     assert np.all((x >= 0) & (x < L))
     assert np.abs(x - st.x).max() < 1e-10 * L
+
+
+def le_oldroyd_box(kind="y3d"):
+    """The 12^3 Oldroyd-B box (tau 0.1, eta_p 0.05) with sheared faces across
+    y or z (shear rate 0.5), the solvent on the linear profile and dt x 20,
+    so that the conformation leaves the identity within a few steps."""
+    box = W.oldroyd_box_3d(12)
+    L = 12.0
+    le_dim = 2 if kind == "y3d" else 3
+    u = 0.5 * L / 2
+    box.prm = box.prm.replace(le_dim=le_dim, le_v_lower=[-u, 0, 0], le_v_upper=[u, 0, 0],
+                              dt=box.prm.dt * 20)
+    v = box.v.copy()
+    v[:, 0] += 0.5 * (box.x[:, le_dim - 1] - L / 2)
+    box.v = np.ascontiguousarray(v)
+    return box
+
+
+@pytest.mark.parametrize("kind", ["y3d", "z3d"])
+def test_le_oldroyd_trajectory(oracle, kind):
+    # the sheared copies' velocity jump enters the velocity gradient
+    # (pp_vgt_ip.inc:16-20) as well as the forces, so the conformation
+    # follows the imposed shear; FP64 boxes with sheared faces walk Verlet
+    # lists
+    box = le_oldroyd_box(kind)
+    steps = 6
+    announce(f"LE Oldroyd-B trajectory: {kind}, {steps} steps")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.integrate_VV(steps)
+        x, v = g.download()
+        ct, vgt = g.download_tensors()
+        builds = g.list_builds()
+    assert builds >= 1
+    st = oracle.State(box.prm, box.x, box.v, box.m, box.pid)
+    st.prologue(0)
+    for s in range(1, steps + 1):
+        st.vv_step(s)
+    L = 12.0
+    dx = np.abs(x - st.x)
+    assert np.minimum(dx, L - dx).max() < 1e-10 * L
+    assert vec_err(v, st.v) < 1e-10
+    assert vec_err(ct, st.ct) < 1e-10
+    assert vec_err(vgt, st.vgt) < 1e-9
+    # the gradient is the imposed shear (0.5), not the periodic jump of the
+    # velocity profile that a box without sheared faces would see
+    g_dim = box.prm.le_dim - 1
+    shear = vgt[:, 0 + 3 * g_dim]  # d v_x / d x_g
+    assert abs(np.median(shear) - 0.5) < 0.1 and np.abs(vgt).max() < 1.5
+
+
+def test_le_oldroyd_mixed_close_to_fp64():
+    # mixed precision with sheared faces rebuilds its lists every step
+    box = le_oldroyd_box("y3d")
+    announce("LE Oldroyd-B mixed vs fp64")
+    out = []
+    for b in (box, W.mixed(box)):
+        with SDPD(b.prm) as g:
+            g.upload(b.x, b.v, b.m, b.pid)
+            g.integrate_VV(4)
+            x, v = g.download()
+            ct, vgt = g.download_tensors()
+        out.append((v, ct, vgt))
+    eye = np.tile(np.eye(3).reshape(-1), (box.n, 1))
+    assert vec_err(out[1][0], out[0][0]) < 1e-5
+    assert vec_err(out[1][1] - eye, out[0][1] - eye) < 1e-4
+    assert vec_err(out[1][2], out[0][2]) < 1e-4

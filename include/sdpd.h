@@ -194,8 +194,11 @@ typedef struct sdpd_colloid_params {
 /* Declares num colloids before sdpd_upload (physics_new colloid set-up,
  * colloid_new.F90).  radius, m: (num); mmi, x, v, omega: (3, num) (the
  * moment of inertia about the principal axes, centre, translational and
- * angular velocity).  One box (no replicas), FP64, Newtonian flow; the
- * boundary particles then come with sdpd_upload (sid = colloid index).
+ * angular velocity).  One box (no replicas), FP64, Newtonian or Oldroyd-B
+ * flow (in the latter both sides of a fluid-boundary pair take the fluid
+ * particle's number density and pressure tensor, pp_interaction_fc.inc:
+ * 80-113); the boundary particles then come with sdpd_upload (sid = colloid
+ * index).
  * Colloid-colloid and colloid-wall forces are not modelled (the
  * reference's pp_interact_cc / cw off).  num = 0 removes the colloids. */
 int sdpd_set_colloids(sdpd_ctx *ctx, const sdpd_colloid_params *prm, int32_t num,

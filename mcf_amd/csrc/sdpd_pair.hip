@@ -1249,13 +1249,20 @@ struct ForceF64 {
     double psum = 0.0, pij = 0.0;
     if (FT == 3) psum = ai + ra.w;
     if (FT == 4) pij = (pi + S.p[j]) * 0.5;
-    if (COLL && (sidi != 0 || jb)) {
-      // a fluid particle and a colloid boundary particle: only the
-      // boundary particle's velocity and pressure change
+    // with a colloid boundary particle in an Oldroyd-B flow both sides take
+    // the fluid particle's 1/n^2 and Pi/n^2
+    // (pp_interaction_fc.inc:80-113, pp_interaction_cf.inc:80-110)
+    double bsum = bi + bj;
+    const bool cpair = COLL && (sidi != 0 || jb);
+    if (cpair) {
+      // a fluid particle and a colloid boundary particle: the boundary
+      // particle's velocity changes, and its pressure (Newtonian) or its
+      // number density and pressure tensor (Oldroyd-B)
       double pb;
       colloid_substitute(j, jb ? S.sid[j] : 0, ra, rb, d0, d1, d2, jump, u0, u1, u2, pb);
       if (FT == 3) psum = sidi == 0 ? ai + pb * bj : pb * bi + ra.w;
       if (FT == 4) pij = pb;
+      if (FT == 5) bsum = sidi == 0 ? bi + bi : bj + bj;
     }
     double a0, a1, a2;
     if (FT == 4) {
@@ -1276,7 +1283,7 @@ struct ForceF64 {
       a2 = sc * d2;
     } else {
       const double gir = gw * ir;
-      const double fd = P.eta * (bi + bj) * gir;
+      const double fd = P.eta * (FT == 5 ? bsum : bi + bj) * gir;
       if (FT == 3) {
         // rhs_force_ff_Newtonian_HuAdams.F90:140-237
         const double fcr = psum * gir;
@@ -1289,7 +1296,8 @@ struct ForceF64 {
         // and the velocity gradient of pp_vgt_ip.inc:16-20
         const double du[3] = {-u0, -u1, -u2};
         const double dx[3] = {d0, d1, d2};
-        const double gfac = gw * inv_ni * ir;
+        // a boundary particle's gradient divides by its fluid partner's n
+        const double gfac = gw * ((cpair && sidi != 0) ? 1.0 / S.rho[j] : inv_ni) * ir;
 #pragma unroll
         for (int b = 0; b < DIM; ++b)
 #pragma unroll
@@ -1304,7 +1312,10 @@ struct ForceF64 {
 #pragma unroll
           for (int a = 0; a < DIM; ++a) {
             const double ptjb = DIM == 3 ? pjb[a + 3 * jj] : pjb[(a == jj ? (a == 0 ? 0 : 4) : 1)];
-            fc[jj] = fc[jj] + (pti[a + DIM * jj] * bi + ptjb) * dx[a];
+            // colloid pairs: the fluid particle's Pi / n^2 on both sides
+            const double t = cpair ? (sidi == 0 ? 2.0 * (pti[a + DIM * jj] * bi) : ptjb + ptjb)
+                                   : pti[a + DIM * jj] * bi + ptjb;
+            fc[jj] = fc[jj] + t * dx[a];
           }
         a0 = fd * u0 - fc[0] * gir;
         a1 = fd * u1 - fc[1] * gir;
@@ -1905,11 +1916,13 @@ static void launch_forces_dim(hipStream_t s, const DevParams& P, const DeviceSta
     else
       launch_forces_le<DIM, 5, true, false>(s, P, S);
   } else if (P.ncoll) {
-    // colloids: Newtonian FP64 only (checked by the API)
+    // colloids: FP64 only (checked by the API)
     if (ft == 3)
       launch_forces_le<DIM, 3, false, true>(s, P, S);
-    else
+    else if (ft == 4)
       launch_forces_le<DIM, 4, false, true>(s, P, S);
+    else
+      launch_forces_le<DIM, 5, false, true>(s, P, S);
   } else {
     if (ft == 3)
       launch_forces_le<DIM, 3, false, false>(s, P, S);

@@ -1154,10 +1154,25 @@ static void visit_force(void *vctx, int32_t ip, int32_t jp, double dij,
   }
   const int brown = p->brownian && p->kt > ORC_MACHINE_ZERO;
   if (!p->newtonian) {
+    const int d2n = dim * dim;
+    const double *pti = fc->pt + (size_t)d2n * ip, *ptj = fc->pt + (size_t)d2n * jp;
+    if (si != 0 || sj != 0) {
+      /* pp_interaction_fc.inc:80-113, pp_interaction_cf.inc:80-110: in a
+       * non-Newtonian flow both sides of a fluid-boundary pair take the
+       * fluid particle's number density (rho_ip = rho_jp) and pressure
+       * tensor (pt of the boundary particle := pt of the fluid one), for
+       * the velocity gradients and the force alike */
+      if (sj != 0) {
+        numj = numi;
+        ptj = pti;
+      } else {
+        numi = numj;
+        pti = ptj;
+      }
+    }
     /* pp_vgt_ip.inc / pp_vgt_jp.inc */
     const double *xi = fc->c->xall + (size_t)dim * ip;
     const double *xj = fc->c->xall + (size_t)dim * jp;
-    const int d2n = dim * dim;
     for (int bb = 0; bb < dim; ++bb)
       for (int aa = 0; aa < dim; ++aa)
         fc->vgt[aa + dim * bb + (size_t)d2n * ip] +=
@@ -1168,7 +1183,6 @@ static void visit_force(void *vctx, int32_t ip, int32_t jp, double dij,
           fc->vgt[aa + dim * bb + (size_t)d2n * jp] +=
               (vi[aa] - vj[aa]) * (xj[bb] - xi[bb]) * gradw / numj / dij;
     /* rhs_force_ff_non_Newtonian_HuAdams.F90:105-131 */
-    const double *pti = fc->pt + (size_t)d2n * ip, *ptj = fc->pt + (size_t)d2n * jp;
     double fcv[3] = {0, 0, 0};
     for (int jj = 0; jj < dim; ++jj)
       for (int ii = 0; ii < dim; ++ii)
@@ -1353,7 +1367,6 @@ int orc_interaction(const orc_params *p, int64_t step, const double *x,
   if (check_sid(p, sid, n)) return -1;
   if (p->rhs_force_type != 3 && p->rhs_force_type != 4) return -1;
   if (!p->newtonian && p->rhs_force_type != 3) return -1;
-  if (!p->newtonian && p->coll) return -1; /* colloids in Newtonian flows only */
   orc_cells c;
   if (cells_build(p, x, n, &c)) return -1;
   int rc = interaction_on_cells(p, &c, step, v, rho, press, m, pid, sid, ct, f, vgt);
@@ -1672,7 +1685,6 @@ static int evaluate(const orc_params *p, int64_t step, int32_t n,
                     const int32_t *pid, double *rho, double *press, double *f,
                     const int32_t *sid, double *ct, double *act, double *vgt) {
   orc_cells c;
-  if (!p->newtonian && p->coll) return -1;
   if (cells_build(p, x, n, &c)) return -1;
   int rc = density_on_cells(p, &c, m, sid, rho);
   if (!rc) rc = orc_pressure(p, rho, m, n, press);

@@ -107,6 +107,76 @@ def test_colloid_le_face_and_interior_chunks(oracle):
     assert np.abs(torque - to).max() < 1e-11 * (np.abs(to).max() + 1e-300)
 
 
+def oldroyd_colloids(kind, sym, le, **kw):
+    """The colloid boxes of make() in an Oldroyd-B fluid (tau 0.1, G 0.5)."""
+    box, cs = make(kind, sym, 2, True, **kw)
+    box.prm = box.prm.replace(Newtonian=False, tau=0.1, n_p=1.0, kt_p=0.5)
+    if le:
+        box = le_variant(box)
+    return box, cs
+
+
+def perturbed_ct(n, dim, seed=5):
+    rng = np.random.default_rng(seed)
+    a = rng.normal(0, 0.05, size=(n, dim, dim))
+    return (np.tile(np.eye(dim), (n, 1, 1)) + a + a.transpose(0, 2, 1)).reshape(n, dim * dim)
+
+
+@pytest.mark.parametrize("kind,sym,le", [("sphere", True, False), ("sphere", False, True),
+                                         ("disk", True, False)])
+def test_colloid_oldroyd_forces(oracle, kind, sym, le):
+    # Oldroyd-B around colloids (pp_interaction_fc.inc:80-113,
+    # pp_interaction_cf.inc:80-110): both sides of a fluid-boundary pair
+    # take the fluid particle's number density and pressure tensor, for the
+    # forces and the velocity gradients alike
+    box, cs = oldroyd_colloids(kind, sym, le)
+    ct = perturbed_ct(box.n, box.prm.num_dim)
+    step = 3
+    announce(f"colloid Oldroyd-B forces: {kind} sym={sym} le={le} n={box.n}")
+    with gpu_context(box, cs) as g:
+        g.upload_ct(ct)
+        g.step = step
+        g.build_list()
+        g.compute_density()
+        g.compute_interaction()
+        f = g.download_force()
+        _, vgt = g.download_tensors()
+    c = to_oracle(oracle, cs)
+    t = marching_time(step, box.prm.dt)
+    ro = oracle.density(box.prm, box.x, box.m, box.sid, coll=c, le_time=t)
+    po = oracle.pressure(box.prm, ro, box.m)
+    fo, vgto = oracle.interaction(box.prm, step, box.x, box.v, ro, po, box.m, box.pid, box.sid,
+                                  ct=ct, coll=c, le_time=t)
+    assert vec_err(f, fo) < 1e-12
+    assert vec_err(vgt, vgto) < 1e-12
+    assert np.abs(vgt[box.sid > 0]).max() > 0
+
+
+def test_colloid_oldroyd_trajectory(oracle):
+    box, cs = oldroyd_colloids("sphere", True, True, translate=True)
+    box.prm = box.prm.replace(dt=box.prm.dt * 20)
+    nsteps = 6
+    announce(f"colloid Oldroyd-B trajectory, {nsteps} steps")
+    with gpu_context(box, cs) as g:
+        g.integrate_VV(nsteps)
+        x, v = g.download()
+        ct, vgt = g.download_tensors()
+        cx = g.colloids()[0]
+    c = to_oracle(oracle, cs)
+    st = oracle.State(box.prm, box.x, box.v, box.m, box.pid, box.sid, coll=c)
+    st.prologue(0)
+    for s in range(1, nsteps + 1):
+        st.vv_step(s)
+    L = np.array(box.prm.length[:3])
+    dx = np.abs(x - st.x)
+    assert np.minimum(dx, L[None, :] - dx).max() < 1e-10 * L.max()
+    assert vec_err(v, st.v) < 1e-10
+    assert vec_err(ct, st.ct) < 1e-10
+    assert vec_err(vgt, st.vgt) < 1e-9
+    dcx = np.abs(cx - c.x)
+    assert np.minimum(dcx, L[None, :] - dcx).max() < 1e-10 * L.max()
+
+
 @pytest.mark.parametrize("rho_type", [0, 1])
 def test_colloid_density_types(oracle, rho_type):
     box, cs = two_spheres(rho_type=rho_type)

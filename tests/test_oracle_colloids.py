@@ -261,6 +261,49 @@ def test_fluid_boundary_pair_force(oracle):
     assert np.allclose(f[1], -fi, rtol=1e-13)
 
 
+@pytest.mark.parametrize("sym", [True, False])
+def test_fluid_boundary_pair_oldroyd(oracle, sym):
+    # Oldroyd-B (pp_interaction_fc.inc:80-113, pp_interaction_cf.inc:80-110):
+    # both sides of a fluid-boundary pair take the fluid particle's number
+    # density and pressure tensor Pi = (p + G) I - G c, the boundary particle
+    # moving at the Morris velocity (kT = 0).  The boundary particle's own
+    # number density (rho / m = 1), pressure and conformation differ from
+    # the fluid particle's, so taking any of them would move the result.
+    G = 0.5
+    prm = unit_box(L=20.0, cut_off=3.0, stateEquation_type=2, c=10.0, rho=1.0, rho_ref=1.0,
+                   gamma=7.0, eta=0.1, Newtonian=False, tau=0.1, n_p=1.0, kt_p=G, symmetry=sym)
+    o = np.array([10.0, 10.0, 10.0])
+    c = one_colloid(oracle, o, radius=2.0)
+    x = np.array([o + [3.0, 0.2, 0.1], o + [1.5, 0.0, 0.0]])
+    v = np.array([[0.3, -0.2, 0.1], [7.0, 7.0, 7.0]])
+    m = np.array([1.0, 1.0])
+    pid = np.array([1, 2], np.int32)
+    sid = np.array([0, 1], np.int32)
+    cf = np.array([[1.2, 0.1, -0.05], [0.1, 0.9, 0.2], [-0.05, 0.2, 1.1]])
+    cb = np.array([[0.7, -0.3, 0.0], [-0.3, 1.4, 0.1], [0.0, 0.1, 0.8]])
+    ct = np.stack([cf.reshape(-1, order="F"), cb.reshape(-1, order="F")])
+    rho = oracle.density(prm, x, m, sid, coll=c)
+    p = oracle.pressure(prm, rho, m)
+    f, vgt = oracle.interaction(prm, 0, x, v, rho, p, m, pid, sid, ct=ct, coll=c)
+    vc = oracle.colloid_noslip(prm, c, x[0], x[1], v[0], 1)
+    nf = rho[0]
+    assert rho[1] == 1.0 and abs(nf - 1.0) > 0.01
+    d = x[0] - x[1]
+    r = np.linalg.norm(d)
+    e = d / r
+    _, gw = oracle.kernel(prm, r)
+    pi_f = (p[0] + G) * np.eye(3) - G * cf
+    # rhs_force_ff_non_Newtonian_HuAdams.F90:105-131 with n_j = n_i, Pi_j = Pi_i
+    fi = -(2.0 * pi_f @ e / nf ** 2) * gw + prm.eta * (2.0 / nf ** 2) * gw / r * (v[0] - vc)
+    assert np.allclose(f[0], fi, rtol=1e-13, atol=0)
+    assert np.allclose(f[1], -fi, rtol=1e-13, atol=0)
+    # pp_vgt_ip.inc / pp_vgt_jp.inc: L(a, b) += (v_j - v_i)_a (x_i - x_j)_b W' / (n r)
+    li = np.outer(vc - v[0], d) * gw / (nf * r)
+    lb = np.outer(v[0] - vc, -d) * gw / (nf * r)
+    assert np.allclose(vgt[0].reshape(3, 3, order="F"), li, rtol=1e-13, atol=1e-16)
+    assert np.allclose(vgt[1].reshape(3, 3, order="F"), lb, rtol=1e-13, atol=1e-16)
+
+
 # ----------------------------------------------------------------------------
 # rigid-body motion
 # ----------------------------------------------------------------------------

@@ -50,7 +50,9 @@ typedef struct sdpd_params {
   int32_t state_equation_type;/* 1 Morris, 2 Batchelor (Tait)                */
   int32_t newtonian;          /* 0: Oldroyd-B conformation tensor            */
   int32_t brownian;           /* thermal noise                               */
-  int32_t body_force_type;    /* 0 none, 1 constant                          */
+  int32_t body_force_type;    /* 0 none, 1 constant, 2 +-x on the halves of
+                                 the last dimension, 3 sine along y (4 is
+                                 refused)                                    */
   int32_t precision;          /* SDPD_PRECISION_FP64 or _MIXED               */
   double min_phys[3];
   double max_phys[3];

@@ -148,10 +148,23 @@ int derive_params(const sdpd_params& p, DevParams& P, const char*& err) {
   P.newtonian = p.newtonian ? 1 : 0;
   const double kt = p.brownian ? p.kt : 0.0;  // physics_adjust_parameters.F90:415-417
   P.noise = (p.brownian && kt > kMachineZero) ? 1 : 0;
+  // particles_apply_body_force.F90:96-108: 1 constant, 2 +-x on the two
+  // halves of the last dimension, 3 sine along y; type 4 (the stochastic
+  // model, a Langevin force from the reference's sequential random stream,
+  // with its own tau_sm / k_sm keys) is not provided
+  if (p.body_force_type < 0 || p.body_force_type > 3)
+    return err = "body_force_type must be 0..3 (4, the stochastic model, is not supported)",
+           SDPD_ERR_ARG;
   P.body_force = 0;
   if (p.body_force_type == 1)
     for (int d = 0; d < p.num_dim; ++d)
       if (std::fabs(p.body_force[d]) > kMachineZero) P.body_force = 1;
+  // types 2 and 3 act along x only, when that component is not zero
+  if ((p.body_force_type == 2 || p.body_force_type == 3) &&
+      std::fabs(p.body_force[0]) > kMachineZero)
+    P.body_force = p.body_force_type;
+  P.bf_split = p.max_phys[p.num_dim - 1] * 0.5;                // :245-252
+  P.bf_k = 2.0 * kPi / (p.max_phys[1] - p.min_phys[1]);        // :323-324
   const double cell = p.cut_off * 1.0001;  // mcf.F90:617
   int64_t ncell = 1, ncell_ref = 1;
   double lmax = 0.0;
@@ -999,6 +1012,9 @@ int sdpd_set_replicas(sdpd_ctx* c, int32_t nrep, const double* box_lo, const dou
     return fail(c, SDPD_ERR_STATE, "sdpd_set_replicas: Lees-Edwards faces are for one box only");
   if (c->P.ncoll > 0)
     return fail(c, SDPD_ERR_STATE, "sdpd_set_replicas: colloids are for one box only");
+  if (c->P.body_force >= 2)
+    return fail(c, SDPD_ERR_STATE,
+                "sdpd_set_replicas: body_force_type 2 and 3 depend on the box bounds (one box only)");
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   drop_graphs(c);
   free_state(c->S);

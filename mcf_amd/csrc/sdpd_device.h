@@ -17,7 +17,7 @@ struct DevParams {
   int32_t eos_type;
   int32_t newtonian;
   int32_t noise;          // Brownian and kt > machine zero
-  int32_t body_force;     // constant body force present
+  int32_t body_force;     // body force type present: 1 constant, 2 two-way, 3 sine
   int32_t n;              // particles
   // sort / gather grid: half of PPM's cell edge, so the stencil is 5 cells
   // wide per dimension (at least 6 cells per dimension)
@@ -38,6 +38,8 @@ struct DevParams {
   double eta, eta_coef, kt, dt, sqrt_dt, inv_sqrt_dt, sqrt_2kt_eta;
   double tau, G;
   double bf[3];
+  double bf_split;        // type 2: half of max_phys along the last dimension
+  double bf_k;            // type 3: 2 pi / (max_phys(2) - min_phys(2))
   uint64_t seed;
   int64_t step;
   int32_t mixed;          // FP32 pair arithmetic

@@ -1554,12 +1554,23 @@ __device__ __forceinline__ void finish_force(const DevParams& P, const DeviceSta
                                              double f0, double f1, double f2,
                                              bool fluid = true) {
   const int64_t n = P.n;
-  // fluid particles only (particles_apply_body_force.F90:173)
-  if (P.body_force && fluid) {
+  // fluid particles only (particles_apply_body_force.F90:173,246,252,330)
+  if (P.body_force == 1 && fluid) {
     // particles_apply_body_force.F90:125-193
     if (fabs(P.bf[0]) > 1e-14) f0 = f0 + P.bf[0];
     if (fabs(P.bf[1]) > 1e-14) f1 = f1 + P.bf[1];
     if (DIM == 3 && fabs(P.bf[2]) > 1e-14) f2 = f2 + P.bf[2];
+  } else if (P.body_force == 2 && fluid) {
+    // :196-273: +g_x below half of max_phys along the last dimension, -g_x
+    // above it, nothing on the line itself
+    const double xl = S.x[(int64_t)(DIM - 1) * n + i];
+    if (xl < P.bf_split)
+      f0 = f0 + P.bf[0];
+    else if (xl > P.bf_split)
+      f0 = f0 - P.bf[0];
+  } else if (P.body_force == 3 && fluid) {
+    // :276-353: g_x sin(k y), k = 2 pi / (max_phys(2) - min_phys(2))
+    f0 = f0 + P.bf[0] * sin(P.bf_k * S.x[n + i]);
   }
   S.f[i] = f0;
   S.f[n + i] = f1;

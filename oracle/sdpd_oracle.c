@@ -1352,6 +1352,27 @@ static int interaction_on_cells(const orc_params *p, const orc_cells *c,
       if (fabs(p->body_force[d]) > ORC_MACHINE_ZERO)
         for (int32_t i = 0; i < c->nreal; ++i)
           if (sid[i] == 0) fa[d + (size_t)dim * i] += p->body_force[d];
+  /* :196-273 (type 2): +g_x where the last coordinate lies below
+   * max_phys(num_dim) / 2, -g_x above it; :276-353 (type 3): g_x sin(k y)
+   * with k = 2 pi / (max_phys(2) - min_phys(2)); fluid particles only */
+  if ((p->body_force_type == 2 || p->body_force_type == 3) &&
+      fabs(p->body_force[0]) > ORC_MACHINE_ZERO) {
+    const double half = p->max_phys[dim - 1] * 0.5;
+    const double k = 2.0 * ORC_PI / (p->max_phys[1] - p->min_phys[1]);
+    for (int32_t i = 0; i < c->nreal; ++i) {
+      if (sid[i] != 0) continue;
+      const double *xi = c->xall + (size_t)dim * i;
+      double *fi = fa + (size_t)dim * i;
+      if (p->body_force_type == 2) {
+        if (xi[dim - 1] < half)
+          fi[0] += p->body_force[0];
+        else if (xi[dim - 1] > half)
+          fi[0] -= p->body_force[0];
+      } else {
+        fi[0] += p->body_force[0] * sin(k * xi[1]);
+      }
+    }
+  }
   memcpy(f_real, fa, sizeof(double) * (size_t)dim * c->nreal);
   if (!p->newtonian && vgt_real)
     memcpy(vgt_real, vg, sizeof(double) * (size_t)d2n * c->nreal);

@@ -48,6 +48,8 @@ def test_missing_library_fails_loudly(tmp_path):
     dict(Newtonian=False, tau=0.0),
     dict(max_phys=[5.0, 5.0, 5.0], cut_off=3.0),  # fewer than 3 cells
     dict(rhs_force_type=1),
+    dict(body_force_type=4, body_force=[0.1, 0.0, 0.0]),  # the stochastic model
+    dict(body_force_type=-1),
 ])
 def test_create_rejects_bad_parameters(bad):
     lib = S.load_library()

@@ -174,15 +174,49 @@ def test_force_parity(oracle, name, force_type):
     assert pairs == (2 if box.prm.symmetry else 1) * len(on)
 
 
-def test_body_force_parity(oracle):
-    box = W.shear_wave_2d(16)
-    box.prm = box.prm.replace(body_force_type=1, body_force=[0.5, 0.0, 0.0])
-    announce("forces: body force")
+@pytest.mark.parametrize("kind,btype", [("2d", 1), ("2d", 2), ("2d", 3), ("3d", 2), ("3d", 3)])
+def test_body_force_parity(oracle, kind, btype):
+    # particles_apply_body_force.F90: 1 constant, 2 +-g_x on the halves of
+    # the last dimension, 3 g_x sin(k y); mixed precision shares the epilogue
+    box = W.shear_wave_2d(16) if kind == "2d" else W.tait_box_3d(10, kt=1e-3)
+    box.prm = box.prm.replace(body_force_type=btype, body_force=[0.5, 0.2, -0.1])
+    announce(f"forces: body force type {btype}, {kind}")
     g = gpu_evaluate(box)
     f = g.download_force()
     g.close()
     _, _, fo, _ = oracle_evaluate(oracle, box)
     assert vec_err(f, fo) < 1e-12
+    gm = gpu_evaluate(W.mixed(box))
+    fm = gm.download_force()
+    gm.close()
+    assert vec_err(fm, fo) < (5e-5 if kind == "2d" else 1e-5)
+
+
+@pytest.mark.parametrize("btype", [2, 3])
+def test_body_force_trajectory(oracle, btype):
+    box = W.tait_box_3d(10, kt=1e-3)
+    box.prm = box.prm.replace(body_force_type=btype, body_force=[2.0, 0.0, 0.0], dt=box.prm.dt * 20)
+    announce(f"body force type {btype}: 6-step trajectory")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.integrate_VV(6)
+        x, v = g.download()
+    st = oracle.State(box.prm, box.x, box.v, box.m, box.pid)
+    st.prologue(0)
+    for s in range(1, 7):
+        st.vv_step(s)
+    assert vec_err(v, st.v) < 1e-10
+    L = 10.0
+    dx = np.abs(x - st.x)
+    assert np.minimum(dx, L - dx).max() < 1e-10 * L
+
+
+def test_body_force_types_2_3_are_for_one_box():
+    prm = W.tait_box_3d(10).prm.replace(body_force_type=2, body_force=[0.1, 0.0, 0.0])
+    ens = W.ensemble_of_cubes((10, 11), prm, kt=0.0)
+    with SDPD(ens.prm) as g:
+        with pytest.raises(SDPDError):
+            g.set_replicas(ens.box_lo, ens.box_hi)
 
 
 @pytest.mark.parametrize("name", ["shear2d_sym", "tait3d_sym_noise", "tait3d_full_noise"])

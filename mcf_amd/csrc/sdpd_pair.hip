@@ -1554,23 +1554,13 @@ __device__ __forceinline__ void finish_force(const DevParams& P, const DeviceSta
                                              double f0, double f1, double f2,
                                              bool fluid = true) {
   const int64_t n = P.n;
-  // fluid particles only (particles_apply_body_force.F90:173,246,252,330)
+  // fluid particles only (particles_apply_body_force.F90:173); the body
+  // forces of types 2 and 3 follow in k_body_force
   if (P.body_force == 1 && fluid) {
     // particles_apply_body_force.F90:125-193
     if (fabs(P.bf[0]) > 1e-14) f0 = f0 + P.bf[0];
     if (fabs(P.bf[1]) > 1e-14) f1 = f1 + P.bf[1];
     if (DIM == 3 && fabs(P.bf[2]) > 1e-14) f2 = f2 + P.bf[2];
-  } else if (P.body_force == 2 && fluid) {
-    // :196-273: +g_x below half of max_phys along the last dimension, -g_x
-    // above it, nothing on the line itself
-    const double xl = S.x[(int64_t)(DIM - 1) * n + i];
-    if (xl < P.bf_split)
-      f0 = f0 + P.bf[0];
-    else if (xl > P.bf_split)
-      f0 = f0 - P.bf[0];
-  } else if (P.body_force == 3 && fluid) {
-    // :276-353: g_x sin(k y), k = 2 pi / (max_phys(2) - min_phys(2))
-    f0 = f0 + P.bf[0] * sin(P.bf_k * S.x[n + i]);
   }
   S.f[i] = f0;
   S.f[n + i] = f1;
@@ -1915,6 +1905,30 @@ static void launch_forces_le(hipStream_t s, const DevParams& P, const DeviceStat
   }
 }
 
+// Body forces that depend on the position (particles_apply_body_force.F90:
+// 196-353), added to the accelerations of fluid particles after the force
+// pass: type 2, +g_x below half of max_phys along the last dimension and
+// -g_x above it (nothing on the line itself); type 3, g_x sin(k y) with
+// k = 2 pi / (max_phys(2) - min_phys(2)).  Kept out of the pair kernels,
+// whose registers the sine would otherwise take.
+__global__ void k_body_force(DevParams P, DeviceState S) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  if (P.ncoll && S.sid[i] != 0) return;  // :246, :252, :330
+  const int64_t n = P.n;
+  double f0 = S.f[i];
+  if (P.body_force == 2) {
+    const double xl = S.x[(int64_t)(P.dim - 1) * n + i];
+    if (xl < P.bf_split)
+      f0 = f0 + P.bf[0];
+    else if (xl > P.bf_split)
+      f0 = f0 - P.bf[0];
+  } else {
+    f0 = f0 + P.bf[0] * sin(P.bf_k * S.x[n + i]);
+  }
+  S.f[i] = f0;
+}
+
 template <int DIM>
 static void launch_forces_dim(hipStream_t s, const DevParams& P, const DeviceState& S,
                               bool mixed) {
@@ -1949,6 +1963,8 @@ void launch_forces(hipStream_t s, const DevParams& P, const DeviceState& S, bool
     launch_forces_dim<3>(s, P, S, mixed);
   else
     launch_forces_dim<2>(s, P, S, mixed);
+  if (P.body_force >= 2)
+    hipLaunchKernelGGL(k_body_force, dim3((P.n + 255) / 256), dim3(256), 0, s, P, S);
 }
 
 void launch_signature(hipStream_t s, const DevParams& P, const DeviceState& S, bool mixed) {

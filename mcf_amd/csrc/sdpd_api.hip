@@ -119,8 +119,16 @@ int derive_params(const sdpd_params& p, DevParams& P, const char*& err) {
     return err = "kernel_type must be 1 (quintic spline) or 2 (Lucy)", SDPD_ERR_ARG;
   if (p.rhs_density_type != 1 && p.rhs_density_type != 2)
     return err = "rhs_density_type must be 1 or 2", SDPD_ERR_ARG;
-  if (p.rhs_force_type != 3 && p.rhs_force_type != 4)
-    return err = "rhs_force_type must be 3 (Hu-Adams) or 4 (angular)", SDPD_ERR_ARG;
+  if (p.rhs_force_type < 1 || p.rhs_force_type > 4)
+    return err = "rhs_force_type must be 1 (Morris), 2 (Espanol), 3 (Hu-Adams) or 4 (angular)",
+           SDPD_ERR_ARG;
+  // rhs_force_ff_Newtonian_Morris.F90 has no thermal noise (ctrl.mcf:
+  // Brownian needs rhs_force_type 2, 3 or 4)
+  if (p.brownian && p.rhs_force_type == 1)
+    return err = "Brownian motion requires rhs_force_type 2, 3 or 4", SDPD_ERR_ARG;
+  if (p.precision == SDPD_PRECISION_MIXED && p.rhs_force_type < 3)
+    return err = "mixed precision: rhs_force_type 3 or 4 (Morris and Espanol run in FP64)",
+           SDPD_ERR_ARG;
   if (p.state_equation_type != 1 && p.state_equation_type != 2)
     return err = "stateEquation_type must be 1 or 2", SDPD_ERR_ARG;
   if (p.state_equation_type == 1 && p.c == 0.0)
@@ -1287,6 +1295,8 @@ int sdpd_set_colloids(sdpd_ctx* c, const sdpd_colloid_params* prm, int32_t num,
   if (c->P.nrep != 1 || !c->rep_lo.empty())
     return fail(c, SDPD_ERR_STATE, "sdpd_set_colloids: one box only (batched context)");
   if (c->P.mixed) return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: FP64 only");
+  if (c->P.force_type < 3)
+    return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: rhs_force_type 3 or 4");
   if (prm->noslip != 1 && prm->noslip != 2)
     return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: noslip must be 1 (frozen) or 2 (Morris)");
   if (prm->rho_type != 0 && prm->rho_type != 1)

@@ -1059,9 +1059,10 @@ __device__ __forceinline__ void density_chunk(const DevParams& P, const DeviceSt
   const double press = eos_pressure(P, P.density_type == 2 ? rho * mi : rho);
   S.rho[i] = rho;
   S.p[i] = press;
-  // per-particle factors of the pair forces: p/n^2 and 1/n^2
+  // per-particle factors of the pair forces: p/n^2 and 1/n^2 (Morris and
+  // Espanol, whose viscous terms divide by n_i n_j: 1/n)
   const double an = press / (rho * rho);
-  const double bn = 1.0 / (rho * rho);
+  const double bn = P.force_type <= 2 ? 1.0 / rho : 1.0 / (rho * rho);
   S.an[i] = an;
   S.bn[i] = bn;
   if (MODE == kStore || MODE == kBuild) S.nbr_count[i] = cnt;
@@ -1265,7 +1266,54 @@ struct ForceF64 {
       if (FT == 5) bsum = sidi == 0 ? bi + bi : bj + bj;
     }
     double a0, a1, a2;
-    if (FT == 4) {
+    if (FT == 1) {
+      // rhs_force_ff_Newtonian_Morris.F90:112-136: accelerations, each term
+      // carrying the partner's mass (the caller skips the division by m_i);
+      // bi, bj = 1/rho
+      const double mj = S.m[j];
+      const double fcm = mj * (ai + ra.w) * gw * ir;
+      const double fdm = mj * 2.0 * P.eta * gw * bi * bj * ir;
+      a0 = fdm * u0 - fcm * d0;
+      a1 = fdm * u1 - fcm * d1;
+      a2 = fdm * u2 - fcm * d2;
+    } else if (FT == 2) {
+      // rhs_force_ff_Newtonian_Espanol.F90:152-251; bi, bj = 1/n
+      const double gnn = gw * bi * bj * ir;
+      const double ev = (DIM == 3 ? fma(d2, u2, fma(d1, u1, d0 * u0)) : fma(d1, u1, d0 * u0)) * ir;
+      const double fd1 = (2.0 - 1.0 / DIM) * P.eta * gnn;
+      const double fd2 = P.eta * (DIM + 2.0) / DIM * gnn * ev;
+      // pressure and the dissipative term along e, both along d / r
+      const double se = ((ai + ra.w) * gw * -1.0 + fd2) * ir;
+      a0 = fd1 * u0 + se * d0;
+      a1 = fd1 * u1 + se * d1;
+      a2 = fd1 * u2 + se * d2;
+      if (P.noise) {
+        // (A S e + B tr e) / sqrt(dt): S e as in the Hu-Adams body, oriented
+        // by the lower pid; tr = g[3] / sqrt(num_dim), shared by the pair
+        const uint32_t pidj = PID ? pid_in : (uint32_t)S.pid[j];
+        float g[4];
+        pair_normals(P.seed, pidi < pidj ? pidi : pidj, pidi < pidj ? pidj : pidi, step, g);
+        const double zij = 4.0 * P.kt * gnn;
+        const double ea = 5.0 * P.eta / 3.0, eb = (DIM + 2.0) * P.eta / 3.0;
+        const double aij = sqrt_nr(-zij * ea) * P.inv_sqrt_dt;
+        const double bij = sqrt_nr(-zij * DIM / 2.0 * (eb + ea * (2.0 / DIM - 1.0))) * P.inv_sqrt_dt;
+        const double saij = pidi < pidj ? aij : -aij;
+        const double ca = 0.70710678118654752440;
+        const double g0 = (double)g[0], g1 = (double)g[1];
+        const double btr = bij * (double)g[3] * (DIM == 3 ? 0.57735026918962576451 : ca) * ir;
+        if (DIM == 3) {
+          const double cb = 0.81649658092772603273 - 0.70710678118654752440;
+          const double g2 = (double)g[2];
+          const double k = cb * fma(g2, d2, fma(g1, d1, g0 * d0)) * (ir * ir);
+          a0 = fma(saij, fma(k, d0, ca * g0), fma(btr, d0, a0));
+          a1 = fma(saij, fma(k, d1, ca * g1), fma(btr, d1, a1));
+          a2 = fma(saij, fma(k, d2, ca * g2), fma(btr, d2, a2));
+        } else {
+          a0 = fma(saij, ca * g0, fma(btr, d0, a0));
+          a1 = fma(saij, ca * g1, fma(btr, d1, a1));
+        }
+      }
+    } else if (FT == 4) {
       // rhs_force_ff_Newtonian_HuAdams_angular.F90:166-229: everything
       // along e = d / r
       const double du = DIM == 3 ? fma(d2, u2, fma(d1, u1, d0 * u0)) : fma(d1, u1, d0 * u0);
@@ -1637,7 +1685,8 @@ __device__ __forceinline__ void forces_chunk(const DevParams& P, const DeviceSta
   const Home h = load_home<DIM>(P, S, ch);
   const int64_t n = P.n;
   const int ii = h.ii;
-  const double inv_mi = 1.0 / S.m[ii];
+  // the Morris body sums accelerations already
+  const double inv_mi = (!MIXED && FT == 1) ? 1.0 : 1.0 / S.m[ii];
   if (MIXED) {
     const float4 vi = S.vr[ii];
     const float4 qi = S.qr[ii];
@@ -1953,6 +2002,10 @@ static void launch_forces_dim(hipStream_t s, const DevParams& P, const DeviceSta
       launch_forces_le<DIM, 3, false, false>(s, P, S);
     else if (ft == 4)
       launch_forces_le<DIM, 4, false, false>(s, P, S);
+    else if (ft == 1)
+      launch_forces_le<DIM, 1, false, false>(s, P, S);  // Morris (FP64 only)
+    else if (ft == 2)
+      launch_forces_le<DIM, 2, false, false>(s, P, S);  // Espanol (FP64 only)
     else
       launch_forces_le<DIM, 5, false, false>(s, P, S);
   }

@@ -19,7 +19,9 @@
  *       (ppm.tgz:src/ppm_neighlist_MkNeighIdx.F90:176-270,
  *        mcf/src/particles_compute_density.F90:578-867,
  *        particles_compute_interaction.F90:618-911, pp_interaction.inc:37-49);
- *   - density, equation of state, Hu-Adams / angular / Oldroyd-B pair forces
+ *   - density, equation of state, Morris / Espanol / Hu-Adams / angular /
+ *     Oldroyd-B pair forces (rhs_force_ff_Newtonian_Morris.F90:112-136,
+ *        rhs_force_ff_Newtonian_Espanol.F90:152-251)
  *       (rhs_density.F90:1-102, stateEquation_compute_pressure.F90:87-180,
  *        stateEquation_new.F90:102, particles_compute_pressure.F90:1-106,
  *        particles_compute_pressure_tensor.F90:94-133,
@@ -34,7 +36,7 @@
  *        particles_integrate_velocity.F90:84-94,
  *        particles_integrate_position.F90:82-123,
  *        particles_adjust_particles.F90:154-161,272-279,
- *        particles_apply_body_force.F90:125-193);
+ *        particles_apply_body_force.F90:125-353);
  *   - the statistics sums (statistic_compute_statistic.F90:230-283).
  *
  * Thermal noise: the reference draws a sequential RANDOM_NUMBER stream in
@@ -1110,6 +1112,18 @@ static void noise_vector(const force_ctx *fc, int32_t ip, int32_t jp,
   }
 }
 
+/* rhs_force_ff_Newtonian_Espanol.F90:198-249: the reference draws a full
+ * dW, takes its trace / num_dim and its symmetric traceless part S.  The
+ * two are independent Gaussians: S e has the covariance noise_vector
+ * draws, and the trace is N(0, 1 / num_dim), here the fourth normal of the
+ * same pair block over sqrt(num_dim). */
+static double noise_trace(const force_ctx *fc, int32_t ip, int32_t jp) {
+  uint32_t a = (uint32_t)fc->pid[ip], b = (uint32_t)fc->pid[jp];
+  float g[4];
+  force_normals(fc->p->seed, a < b ? a : b, a < b ? b : a, fc->step, g);
+  return (double)g[3] / sqrt((double)fc->p->num_dim);
+}
+
 static void visit_force(void *vctx, int32_t ip, int32_t jp, double dij,
                         const double *rij) {
   force_ctx *fc = (force_ctx *)vctx;
@@ -1227,6 +1241,53 @@ static void visit_force(void *vctx, int32_t ip, int32_t jp, double dij,
       for (int d = 0; d < dim; ++d) {
         fi[d] = fi[d] + f_r * we[d] / mi;
         fj[d] = fj[d] - f_r * we[d] / mj;
+      }
+    }
+  } else if (p->rhs_force_type == 1) {
+    /* rhs_force_ff_Newtonian_Morris.F90:112-136: accelerations carry the
+     * partner's mass; rho is the density pass's density */
+    double f_c = (pi / (numi * numi) + pj / (numj * numj)) * gradw;
+    for (int d = 0; d < dim; ++d) {
+      fi[d] = fi[d] - mj * f_c * eij[d];
+      fj[d] = fj[d] + mi * f_c * eij[d];
+    }
+    double f_d = 2.0 * p->eta * gradw / numi / numj / dij;
+    for (int d = 0; d < dim; ++d) {
+      fi[d] = fi[d] + mj * f_d * vij[d];
+      fj[d] = fj[d] - mi * f_d * vij[d];
+    }
+  } else if (p->rhs_force_type == 2) {
+    /* rhs_force_ff_Newtonian_Espanol.F90:152-251 */
+    double ev = 0.0;
+    for (int d = 0; d < dim; ++d) ev = ev + eij[d] * vij[d];
+    double f_c = (pi / (numi * numi) + pj / (numj * numj)) * gradw;
+    for (int d = 0; d < dim; ++d) {
+      fi[d] = fi[d] - f_c * eij[d] / mi;
+      fj[d] = fj[d] + f_c * eij[d] / mj;
+    }
+    double f_d1 = (2.0 - 1.0 / dim) * p->eta * gradw / numi / numj / dij;
+    for (int d = 0; d < dim; ++d) {
+      fi[d] = fi[d] + f_d1 * vij[d] / mi;
+      fj[d] = fj[d] - f_d1 * vij[d] / mj;
+    }
+    double f_d2 = p->eta * (dim + 2) / dim * gradw * ev / numi / numj / dij;
+    for (int d = 0; d < dim; ++d) {
+      fi[d] = fi[d] + f_d2 * eij[d] / mi;
+      fj[d] = fj[d] - f_d2 * eij[d] / mj;
+    }
+    if (brown) {
+      double we[3];
+      noise_vector(fc, ip, jp, eij, we);
+      const double tr = noise_trace(fc, ip, jp);
+      double zij = 4.0 * p->kt * gradw / dij / numi / numj;
+      double a = 5.0 * p->eta / 3.0;
+      double b = ((double)dim + 2.0) * p->eta / 3.0;
+      double aij = sqrt(-zij * a);
+      double bij = sqrt(-zij * (double)dim / 2.0 * (b + a * (2.0 / (double)dim - 1.0)));
+      for (int d = 0; d < dim; ++d) {
+        double f_r = (aij * we[d] + bij * tr * eij[d]) / sqrt(p->dt);
+        fi[d] = fi[d] + f_r / mi;
+        fj[d] = fj[d] - f_r / mj;
       }
     }
   } else {
@@ -1386,7 +1447,7 @@ int orc_interaction(const orc_params *p, int64_t step, const double *x,
                     const double *m, const int32_t *pid, const int32_t *sid,
                     const double *ct, int32_t n, double *f, double *vgt) {
   if (check_sid(p, sid, n)) return -1;
-  if (p->rhs_force_type != 3 && p->rhs_force_type != 4) return -1;
+  if (p->rhs_force_type < 1 || p->rhs_force_type > 4) return -1;
   if (!p->newtonian && p->rhs_force_type != 3) return -1;
   orc_cells c;
   if (cells_build(p, x, n, &c)) return -1;

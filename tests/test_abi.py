@@ -47,7 +47,9 @@ def test_missing_library_fails_loudly(tmp_path):
     dict(Brownian=True, rhs_density_type=1),
     dict(Newtonian=False, tau=0.0),
     dict(max_phys=[5.0, 5.0, 5.0], cut_off=3.0),  # fewer than 3 cells
-    dict(rhs_force_type=1),
+    dict(rhs_force_type=5),
+    dict(rhs_force_type=1, Brownian=True),  # Morris has no thermal noise
+    dict(rhs_force_type=2, precision=1),  # Espanol runs in FP64
     dict(body_force_type=4, body_force=[0.1, 0.0, 0.0]),  # the stochastic model
     dict(body_force_type=-1),
 ])

@@ -174,6 +174,47 @@ def test_force_parity(oracle, name, force_type):
     assert pairs == (2 if box.prm.symmetry else 1) * len(on)
 
 
+@pytest.mark.parametrize("kind,ft,dens,brown,sym", [
+    ("3d", 1, 1, False, True), ("3d", 1, 2, False, False),
+    ("3d", 2, 2, True, True), ("3d", 2, 2, True, False), ("2d", 2, 2, True, True)])
+def test_morris_espanol_parity(oracle, kind, ft, dens, brown, sym):
+    # rhs_force_ff_Newtonian_Morris.F90 (type 1, no noise) and
+    # rhs_force_ff_Newtonian_Espanol.F90 (type 2, its trace noise from the
+    # fourth normal of the pair block), FP64
+    if kind == "3d":
+        box = W.tait_box_3d(10, kt=1e-3, brownian=brown, symmetry=sym)
+    else:
+        box = W.shear_wave_2d(16, brownian=brown, kt=1e-4, symmetry=sym)
+    box.prm = box.prm.replace(rhs_force_type=ft, rhs_density_type=dens)
+    announce(f"forces: rhs_force_type {ft}, density type {dens}, {kind} brownian={brown} sym={sym}")
+    g = gpu_evaluate(box, step=4)
+    rho, _ = g.download_density()
+    f = g.download_force()
+    g.close()
+    ro, _, fo, _ = oracle_evaluate(oracle, box, step=4)
+    assert rel_err(rho, ro) < 1e-13
+    assert vec_err(f, fo) < 1e-12
+
+
+@pytest.mark.parametrize("ft", [1, 2])
+def test_morris_espanol_trajectory(oracle, ft):
+    box = W.tait_box_3d(10, kt=1e-3, brownian=ft == 2)
+    box.prm = box.prm.replace(rhs_force_type=ft, dt=box.prm.dt * 10)
+    announce(f"rhs_force_type {ft}: 6-step trajectory")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.integrate_VV(6)
+        x, v = g.download()
+    st = oracle.State(box.prm, box.x, box.v, box.m, box.pid)
+    st.prologue(0)
+    for s in range(1, 7):
+        st.vv_step(s)
+    assert vec_err(v, st.v) < 1e-10
+    L = 10.0
+    dx = np.abs(x - st.x)
+    assert np.minimum(dx, L - dx).max() < 1e-10 * L
+
+
 @pytest.mark.parametrize("kind,btype", [("2d", 1), ("2d", 2), ("2d", 3), ("3d", 2), ("3d", 3)])
 def test_body_force_parity(oracle, kind, btype):
     # particles_apply_body_force.F90: 1 constant, 2 +-g_x on the halves of

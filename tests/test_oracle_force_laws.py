@@ -1,0 +1,86 @@
+"""The Morris (rhs_force_type 1) and Espanol (2) pair forces in the oracle
+(rhs_force_ff_Newtonian_Morris.F90:112-136,
+rhs_force_ff_Newtonian_Espanol.F90:152-251).  The reference holds no test
+vectors for them: each is pinned by one pair worked out by hand from the
+reference's formulas, with unequal masses and densities so that a swapped
+mass or density fails, and the Espanol thermal force by its covariance over
+many steps, which the trace term changes."""
+import numpy as np
+import pytest
+
+from mcf_amd.params import SDPDParams
+
+
+def pair_box(force_type, density_type, **kw):
+    return SDPDParams(num_dim=3, min_phys=[0, 0, 0], max_phys=[20.0] * 3, cut_off=3.0,
+                      rhs_density_type=density_type, rhs_force_type=force_type,
+                      stateEquation_type=2, c=10.0, rho=1.0, rho_ref=1.0, gamma=7.0, eta=0.3, **kw)
+
+
+def one_pair(O, prm, step=0, m=(1.0, 1.7)):
+    x = np.array([[10.0, 10.0, 10.0], [11.2, 10.5, 9.6]])
+    v = np.array([[0.3, -0.2, 0.1], [-0.1, 0.25, 0.05]])
+    m = np.array(m)
+    pid = np.array([3, 8], np.int32)
+    rho = O.density(prm, x, m)
+    p = O.pressure(prm, rho, m)
+    f, _ = O.interaction(prm, step, x, v, rho, p, m, pid)
+    d = x[0] - x[1]
+    r = np.linalg.norm(d)
+    _, gw = O.kernel(prm, r)
+    return f, x, v, m, rho, p, d / r, r, gw
+
+
+@pytest.mark.parametrize("density_type", [1, 2])
+def test_morris_pair(oracle, density_type):
+    prm = pair_box(1, density_type)
+    f, x, v, m, rho, p, e, r, gw = one_pair(oracle, prm)
+    if density_type == 1:  # (number densities of a lone pair are equal)
+        assert abs(rho[0] - rho[1]) > 1e-3 * rho[0]
+    vij = v[0] - v[1]
+    f_c = (p[0] / rho[0] ** 2 + p[1] / rho[1] ** 2) * gw
+    f_d = 2.0 * prm.eta * gw / rho[0] / rho[1] / r
+    fi = -m[1] * f_c * e + m[1] * f_d * vij
+    fj = m[0] * f_c * e - m[0] * f_d * vij
+    assert np.allclose(f[0], fi, rtol=1e-13, atol=0)
+    assert np.allclose(f[1], fj, rtol=1e-13, atol=0)
+
+
+def test_espanol_pair(oracle):
+    prm = pair_box(2, 2)
+    f, x, v, m, rho, p, e, r, gw = one_pair(oracle, prm)
+    vij = v[0] - v[1]
+    ev = e @ vij
+    f_c = (p[0] / rho[0] ** 2 + p[1] / rho[1] ** 2) * gw
+    f_d1 = (2.0 - 1.0 / 3.0) * prm.eta * gw / rho[0] / rho[1] / r
+    f_d2 = prm.eta * 5.0 / 3.0 * gw * ev / rho[0] / rho[1] / r
+    fpair = -f_c * e + f_d1 * vij + f_d2 * e
+    assert np.allclose(f[0], fpair / m[0], rtol=1e-13, atol=0)
+    assert np.allclose(f[1], -fpair / m[1], rtol=1e-13, atol=0)
+
+
+def test_espanol_noise_covariance(oracle):
+    # f_r = (A S e + B tr e) / sqrt(dt) with S the symmetric traceless part
+    # of a Gaussian dW and tr its trace / 3: along e the variance is
+    # A^2 2/3 + B^2 / 3, across e A^2 / 2 (A = B in 3D)
+    kt, dt = 1e-3, 1e-4
+    prm = pair_box(2, 2, Brownian=True, kt=kt, dt=dt)
+    cold = pair_box(2, 2, dt=dt)
+    f0, x, v, m, rho, p, e, r, gw = one_pair(oracle, cold)
+    z = 4.0 * kt * gw / r / rho[0] / rho[1]
+    a, b = 5.0 * prm.eta / 3.0, 5.0 * prm.eta / 3.0
+    A = np.sqrt(-z * a)
+    B = np.sqrt(-z * 3.0 / 2.0 * (b + a * (2.0 / 3.0 - 1.0)))
+    samples = []
+    for step in range(1, 3001):
+        f, *_ = one_pair(oracle, prm, step=step)
+        fr0 = (f[0] - f0[0]) * m[0] * np.sqrt(dt)
+        fr1 = (f[1] - f0[1]) * m[1] * np.sqrt(dt)
+        assert np.allclose(fr0, -fr1, rtol=1e-9, atol=1e-14 * A)  # antisymmetric
+        samples.append(fr0)
+    s = np.array(samples)
+    along = s @ e
+    across = s - along[:, None] * e[None, :]
+    assert abs(along.mean()) < 0.1 * A
+    assert np.var(along) == pytest.approx(A ** 2 * 2.0 / 3.0 + B ** 2 / 3.0, rel=0.1)
+    assert np.mean((across ** 2).sum(1)) / 2.0 == pytest.approx(A ** 2 / 2.0, rel=0.1)

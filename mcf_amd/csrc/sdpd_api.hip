@@ -136,8 +136,14 @@ int derive_params(const sdpd_params& p, DevParams& P, const char*& err) {
   if (p.state_equation_type == 2 && (p.rho_ref == 0.0 || p.gamma == 0.0))
     return err = "stateEquation_init_Batchelor: neither rho_ref nor gamma can be zero",
            SDPD_ERR_ARG;
-  if (!p.newtonian && p.rhs_force_type != 3)
-    return err = "non-Newtonian flow needs rhs_force_type = 3", SDPD_ERR_ARG;
+  // rhs_force_ff_non_Newtonian.F90: Espanol (2), Hu-Adams (3), angular (4)
+  if (!p.newtonian && p.rhs_force_type == 1)
+    return err = "non-Newtonian flow needs rhs_force_type 2, 3 or 4", SDPD_ERR_ARG;
+  if (!p.newtonian && p.rhs_force_type == 4 && p.brownian)
+    return err = "rhs_force_ff_non_Newtonian_HuAdams_angular: Brownian force is not available",
+           SDPD_ERR_ARG;
+  if (!p.newtonian && p.rhs_force_type != 3 && p.precision == SDPD_PRECISION_MIXED)
+    return err = "mixed precision Oldroyd-B: rhs_force_type 3", SDPD_ERR_ARG;
   if (!p.newtonian && std::fabs(p.tau) < kMachineZero)
     return err = "particles_compute_act: tau should not be zero", SDPD_ERR_ARG;
   if (p.brownian && p.rhs_density_type != 2)
@@ -1295,8 +1301,9 @@ int sdpd_set_colloids(sdpd_ctx* c, const sdpd_colloid_params* prm, int32_t num,
   if (c->P.nrep != 1 || !c->rep_lo.empty())
     return fail(c, SDPD_ERR_STATE, "sdpd_set_colloids: one box only (batched context)");
   if (c->P.mixed) return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: FP64 only");
-  if (c->P.force_type < 3)
-    return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: rhs_force_type 3 or 4");
+  if (c->P.force_type < 3 || (!c->P.newtonian && c->P.force_type != 3))
+    return fail(c, SDPD_ERR_ARG,
+                "sdpd_set_colloids: rhs_force_type 3 or 4 (3 in Oldroyd-B flows)");
   if (prm->noslip != 1 && prm->noslip != 2)
     return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: noslip must be 1 (frozen) or 2 (Morris)");
   if (prm->rho_type != 0 && prm->rho_type != 1)

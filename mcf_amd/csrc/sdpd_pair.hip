@@ -1059,10 +1059,10 @@ __device__ __forceinline__ void density_chunk(const DevParams& P, const DeviceSt
   const double press = eos_pressure(P, P.density_type == 2 ? rho * mi : rho);
   S.rho[i] = rho;
   S.p[i] = press;
-  // per-particle factors of the pair forces: p/n^2 and 1/n^2 (Morris and
-  // Espanol, whose viscous terms divide by n_i n_j: 1/n)
+  // per-particle factors of the pair forces: p/n^2 and 1/n^2 (Newtonian
+  // Morris and Espanol, whose viscous terms divide by n_i n_j: 1/n)
   const double an = press / (rho * rho);
-  const double bn = P.force_type <= 2 ? 1.0 / rho : 1.0 / (rho * rho);
+  const double bn = (P.newtonian && P.force_type <= 2) ? 1.0 / rho : 1.0 / (rho * rho);
   S.an[i] = an;
   S.bn[i] = bn;
   if (MODE == kStore || MODE == kBuild) S.nbr_count[i] = cnt;
@@ -1186,7 +1186,7 @@ struct ForceF64 {
   // time, 3.31 in twos, 3.69 in fours; the substitution branch's registers
   // leave no room for more).
   static constexpr bool kBatched = DECIDE;
-  static constexpr int kBatch = COLL ? 2 : (FT == 5 ? 2 : 8);
+  static constexpr int kBatch = COLL ? 2 : (FT >= 5 ? 2 : 8);
   __device__ __forceinline__ void body(uint32_t ent) {
     const int j = (int)(ent & kIndexMask);
     if (COLL) {
@@ -1287,32 +1287,7 @@ struct ForceF64 {
       a0 = fd1 * u0 + se * d0;
       a1 = fd1 * u1 + se * d1;
       a2 = fd1 * u2 + se * d2;
-      if (P.noise) {
-        // (A S e + B tr e) / sqrt(dt): S e as in the Hu-Adams body, oriented
-        // by the lower pid; tr = g[3] / sqrt(num_dim), shared by the pair
-        const uint32_t pidj = PID ? pid_in : (uint32_t)S.pid[j];
-        float g[4];
-        pair_normals(P.seed, pidi < pidj ? pidi : pidj, pidi < pidj ? pidj : pidi, step, g);
-        const double zij = 4.0 * P.kt * gnn;
-        const double ea = 5.0 * P.eta / 3.0, eb = (DIM + 2.0) * P.eta / 3.0;
-        const double aij = sqrt_nr(-zij * ea) * P.inv_sqrt_dt;
-        const double bij = sqrt_nr(-zij * DIM / 2.0 * (eb + ea * (2.0 / DIM - 1.0))) * P.inv_sqrt_dt;
-        const double saij = pidi < pidj ? aij : -aij;
-        const double ca = 0.70710678118654752440;
-        const double g0 = (double)g[0], g1 = (double)g[1];
-        const double btr = bij * (double)g[3] * (DIM == 3 ? 0.57735026918962576451 : ca) * ir;
-        if (DIM == 3) {
-          const double cb = 0.81649658092772603273 - 0.70710678118654752440;
-          const double g2 = (double)g[2];
-          const double k = cb * fma(g2, d2, fma(g1, d1, g0 * d0)) * (ir * ir);
-          a0 = fma(saij, fma(k, d0, ca * g0), fma(btr, d0, a0));
-          a1 = fma(saij, fma(k, d1, ca * g1), fma(btr, d1, a1));
-          a2 = fma(saij, fma(k, d2, ca * g2), fma(btr, d2, a2));
-        } else {
-          a0 = fma(saij, ca * g0, fma(btr, d0, a0));
-          a1 = fma(saij, ca * g1, fma(btr, d1, a1));
-        }
-      }
+      if (P.noise) espanol_noise(PID ? pid_in : (uint32_t)S.pid[j], d0, d1, d2, ir, gnn, a0, a1, a2);
     } else if (FT == 4) {
       // rhs_force_ff_Newtonian_HuAdams_angular.F90:166-229: everything
       // along e = d / r
@@ -1331,7 +1306,7 @@ struct ForceF64 {
       a2 = sc * d2;
     } else {
       const double gir = gw * ir;
-      const double fd = P.eta * (FT == 5 ? bsum : bi + bj) * gir;
+      const double fd = P.eta * (FT >= 5 ? bsum : bi + bj) * gir;
       if (FT == 3) {
         // rhs_force_ff_Newtonian_HuAdams.F90:140-237
         const double fcr = psum * gir;
@@ -1339,7 +1314,8 @@ struct ForceF64 {
         a1 = fd * u1 - fcr * d1;
         a2 = fd * u2 - fcr * d2;
       } else {
-        // rhs_force_ff_non_Newtonian_HuAdams.F90:105-131 with
+        // rhs_force_ff_non_Newtonian_HuAdams.F90:105-131 (FT 5; Espanol's,
+        // FT 6, differs only in the noise; the angular form, FT 7, below) with
         // Pi = (p + G) I - G c (particles_compute_pressure_tensor.F90:114-133),
         // and the velocity gradient of pp_vgt_ip.inc:16-20
         const double du[3] = {-u0, -u1, -u2};
@@ -1361,15 +1337,31 @@ struct ForceF64 {
           for (int a = 0; a < DIM; ++a) {
             const double ptjb = DIM == 3 ? pjb[a + 3 * jj] : pjb[(a == jj ? (a == 0 ? 0 : 4) : 1)];
             // colloid pairs: the fluid particle's Pi / n^2 on both sides
-            const double t = cpair ? (sidi == 0 ? 2.0 * (pti[a + DIM * jj] * bi) : ptjb + ptjb)
-                                   : pti[a + DIM * jj] * bi + ptjb;
+            double t = cpair ? (sidi == 0 ? 2.0 * (pti[a + DIM * jj] * bi) : ptjb + ptjb)
+                             : pti[a + DIM * jj] * bi + ptjb;
+            // rhs_force_ff_non_Newtonian_HuAdams_angular.F90: the diagonal
+            // averaged, (Pi_i + Pi_j) / 2 (1/n_i^2 + 1/n_j^2)
+            if (FT == 7 && a == jj) t = 0.5 * (pti[a + DIM * jj] + ptjb / bj) * (bi + bj);
             fc[jj] = fc[jj] + t * dx[a];
           }
-        a0 = fd * u0 - fc[0] * gir;
-        a1 = fd * u1 - fc[1] * gir;
-        a2 = DIM == 3 ? fd * u2 - fc[2] * gir : 0.0;
+        if (FT == 7) {
+          // its dissipative term acts along e: eta_coef eta (1/n_i^2 +
+          // 1/n_j^2) W' (e . v_ij) / r
+          const double dv = DIM == 3 ? fma(d2, u2, fma(d1, u1, d0 * u0)) : fma(d1, u1, d0 * u0);
+          const double fde = P.eta_coef * P.eta * (bi + bj) * gw * dv * (ir * ir * ir);
+          a0 = fde * d0 - fc[0] * gir;
+          a1 = fde * d1 - fc[1] * gir;
+          a2 = DIM == 3 ? fde * d2 - fc[2] * gir : 0.0;
+        } else {
+          a0 = fd * u0 - fc[0] * gir;
+          a1 = fd * u1 - fc[1] * gir;
+          a2 = DIM == 3 ? fd * u2 - fc[2] * gir : 0.0;
+        }
       }
-      if (P.noise) {
+      if (FT == 6 && P.noise) {
+        espanol_noise(PID ? pid_in : (uint32_t)S.pid[j], d0, d1, d2, ir,
+                      gw * ir * inv_ni / S.rho[j], a0, a1, a2);
+      } else if (FT != 7 && P.noise) {
         // f_r W e with W e = s (ca g + cb (g.e) e), e = d / r, s = +1 when
         // pid_i < pid_j (noise_vector), written on d to save operations
         const uint32_t pidj = PID ? pid_in : (uint32_t)S.pid[j];
@@ -1396,6 +1388,39 @@ struct ForceF64 {
     f0 = f0 + a0;
     f1 = f1 + a1;
     if (DIM == 3) f2 = f2 + a2;
+  }
+
+  // Espanol's thermal force (rhs_force_ff_Newtonian_Espanol.F90:198-249,
+  // rhs_force_ff_non_Newtonian_Espanol.F90): (A S e + B tr e) / sqrt(dt)
+  // with S the symmetric traceless part of the pair's Gaussian dW and tr
+  // its trace / num_dim, Z = 4 kT W' / (r n_i n_j) (gnn = W' / (r n_i n_j)).
+  // S and tr are independent: S e is drawn as in the Hu-Adams body, oriented
+  // by the lower pid, and tr is the pair block's fourth normal over
+  // sqrt(num_dim), shared by both sides.
+  __device__ __forceinline__ void espanol_noise(uint32_t pidj, double d0, double d1, double d2,
+                                                double ir, double gnn, double& a0, double& a1,
+                                                double& a2) const {
+    float g[4];
+    pair_normals(P.seed, pidi < pidj ? pidi : pidj, pidi < pidj ? pidj : pidi, step, g);
+    const double zij = 4.0 * P.kt * gnn;
+    const double ea = 5.0 * P.eta / 3.0, eb = (DIM + 2.0) * P.eta / 3.0;
+    const double aij = sqrt_nr(-zij * ea) * P.inv_sqrt_dt;
+    const double bij = sqrt_nr(-zij * DIM / 2.0 * (eb + ea * (2.0 / DIM - 1.0))) * P.inv_sqrt_dt;
+    const double saij = pidi < pidj ? aij : -aij;
+    const double ca = 0.70710678118654752440;
+    const double g0 = (double)g[0], g1 = (double)g[1];
+    const double btr = bij * (double)g[3] * (DIM == 3 ? 0.57735026918962576451 : ca) * ir;
+    if (DIM == 3) {
+      const double cb = 0.81649658092772603273 - 0.70710678118654752440;
+      const double g2 = (double)g[2];
+      const double k = cb * fma(g2, d2, fma(g1, d1, g0 * d0)) * (ir * ir);
+      a0 = fma(saij, fma(k, d0, ca * g0), fma(btr, d0, a0));
+      a1 = fma(saij, fma(k, d1, ca * g1), fma(btr, d1, a1));
+      a2 = fma(saij, fma(k, d2, ca * g2), fma(btr, d2, a2));
+    } else {
+      a0 = fma(saij, ca * g0, fma(btr, d0, a0));
+      a1 = fma(saij, ca * g1, fma(btr, d1, a1));
+    }
   }
 
   // A fluid particle and a colloid boundary particle
@@ -1737,7 +1762,7 @@ __device__ __forceinline__ void forces_chunk(const DevParams& P, const DeviceSta
     pol.step = S.step_dev[0];
 #pragma unroll
     for (int q = 0; q < 9; ++q) pol.L[q] = 0.0;
-    if (FT == 5) {
+    if (FT >= 5) {
       // particles_compute_pressure_tensor.F90:114-133
 #pragma unroll
       for (int b = 0; b < DIM; ++b)
@@ -1753,7 +1778,7 @@ __device__ __forceinline__ void forces_chunk(const DevParams& P, const DeviceSta
     const int i = h.i;
     finish_force<DIM>(P, S, i, pol.f0 * inv_mi, pol.f1 * inv_mi, pol.f2 * inv_mi,
                       !COLL || pol.sidi == 0);
-    if (FT == 5) conformation_rate<DIM>(P, S, i, pol.L);
+    if (FT >= 5) conformation_rate<DIM>(P, S, i, pol.L);
   }
 }
 
@@ -1981,7 +2006,8 @@ __global__ void k_body_force(DevParams P, DeviceState S) {
 template <int DIM>
 static void launch_forces_dim(hipStream_t s, const DevParams& P, const DeviceState& S,
                               bool mixed) {
-  const int ft = P.newtonian ? P.force_type : 5;
+  // Oldroyd-B: 5 Hu-Adams, 6 Espanol, 7 angular (rhs_force_ff_non_Newtonian.F90)
+  const int ft = P.newtonian ? P.force_type : (P.force_type == 3 ? 5 : (P.force_type == 2 ? 6 : 7));
   if (mixed) {
     if (ft == 3)
       launch_forces_le<DIM, 3, true, false>(s, P, S);
@@ -2006,6 +2032,10 @@ static void launch_forces_dim(hipStream_t s, const DevParams& P, const DeviceSta
       launch_forces_le<DIM, 1, false, false>(s, P, S);  // Morris (FP64 only)
     else if (ft == 2)
       launch_forces_le<DIM, 2, false, false>(s, P, S);  // Espanol (FP64 only)
+    else if (ft == 6)
+      launch_forces_le<DIM, 6, false, false>(s, P, S);  // Oldroyd-B Espanol (FP64 only)
+    else if (ft == 7)
+      launch_forces_le<DIM, 7, false, false>(s, P, S);  // Oldroyd-B angular (FP64 only)
     else
       launch_forces_le<DIM, 5, false, false>(s, P, S);
   }

@@ -1196,22 +1196,59 @@ static void visit_force(void *vctx, int32_t ip, int32_t jp, double dij,
         for (int aa = 0; aa < dim; ++aa)
           fc->vgt[aa + dim * bb + (size_t)d2n * jp] +=
               (vi[aa] - vj[aa]) * (xj[bb] - xi[bb]) * gradw / numj / dij;
-    /* rhs_force_ff_non_Newtonian_HuAdams.F90:105-131 */
+    /* rhs_force_ff_non_Newtonian_HuAdams.F90:105-131 (type 3) and
+     * rhs_force_ff_non_Newtonian_Espanol.F90 (type 2, the same but for its
+     * thermal force); rhs_force_ff_non_Newtonian_HuAdams_angular.F90 (type
+     * 4) averages the diagonal, (Pi_i + Pi_j) / 2 (1/n_i^2 + 1/n_j^2), and
+     * dissipates along e */
+    const int ang = p->rhs_force_type == 4;
     double fcv[3] = {0, 0, 0};
     for (int jj = 0; jj < dim; ++jj)
-      for (int ii = 0; ii < dim; ++ii)
-        fcv[jj] = fcv[jj] + pti[ii + dim * jj] * eij[ii] / (numi * numi) +
-                  ptj[ii + dim * jj] * eij[ii] / (numj * numj);
+      for (int ii = 0; ii < dim; ++ii) {
+        if (ang && ii == jj) {
+          double pij = (pti[ii + dim * jj] + ptj[ii + dim * jj]) / 2.0;
+          fcv[jj] = fcv[jj] + pij * eij[ii] / (numi * numi) + pij * eij[ii] / (numj * numj);
+        } else {
+          fcv[jj] = fcv[jj] + pti[ii + dim * jj] * eij[ii] / (numi * numi) +
+                    ptj[ii + dim * jj] * eij[ii] / (numj * numj);
+        }
+      }
     for (int d = 0; d < dim; ++d) {
       fi[d] = fi[d] - fcv[d] * gradw / mi;
       fj[d] = fj[d] + fcv[d] * gradw / mj;
+    }
+    if (ang) {
+      double ev = 0.0;
+      for (int d = 0; d < dim; ++d) ev = ev + eij[d] * vij[d];
+      double f_da = p->eta_coef * p->eta * (1.0 / (numi * numi) + 1.0 / (numj * numj)) *
+                    gradw * ev / dij;
+      for (int d = 0; d < dim; ++d) {
+        fi[d] = fi[d] + f_da * eij[d] / mi;
+        fj[d] = fj[d] - f_da * eij[d] / mj;
+      }
+      if (brown) { fc->fail = 1; return; } /* "Brownian force is not available" */
+      goto accumulate;
     }
     double f_d = p->eta * (1.0 / (numi * numi) + 1.0 / (numj * numj)) * gradw / dij;
     for (int d = 0; d < dim; ++d) {
       fi[d] = fi[d] + f_d * vij[d] / mi;
       fj[d] = fj[d] - f_d * vij[d] / mj;
     }
-    if (brown) {
+    if (brown && p->rhs_force_type == 2) {
+      double we[3];
+      noise_vector(fc, ip, jp, eij, we);
+      const double tr = noise_trace(fc, ip, jp);
+      double zij = 4.0 * p->kt * gradw / dij / numi / numj;
+      double a = 5.0 * p->eta / 3.0;
+      double b = ((double)dim + 2.0) * p->eta / 3.0;
+      double aij = sqrt(-zij * a);
+      double bij = sqrt(-zij * (double)dim / 2.0 * (b + a * (2.0 / (double)dim - 1.0)));
+      for (int d = 0; d < dim; ++d) {
+        double f_r = (aij * we[d] + bij * tr * eij[d]) / sqrt(p->dt);
+        fi[d] = fi[d] + f_r / mi;
+        fj[d] = fj[d] - f_r / mj;
+      }
+    } else if (brown) {
       if (f_d > 0.0) { fc->fail = 1; return; }
       double we[3];
       noise_vector(fc, ip, jp, eij, we);
@@ -1319,6 +1356,7 @@ static void visit_force(void *vctx, int32_t ip, int32_t jp, double dij,
       }
     }
   }
+accumulate:;
   double *f = fc->f;
   for (int d = 0; d < dim; ++d) f[d + (size_t)dim * ip] += fi[d];
   if (p->symmetry)
@@ -1448,7 +1486,7 @@ int orc_interaction(const orc_params *p, int64_t step, const double *x,
                     const double *ct, int32_t n, double *f, double *vgt) {
   if (check_sid(p, sid, n)) return -1;
   if (p->rhs_force_type < 1 || p->rhs_force_type > 4) return -1;
-  if (!p->newtonian && p->rhs_force_type != 3) return -1;
+  if (!p->newtonian && p->rhs_force_type == 1) return -1;
   orc_cells c;
   if (cells_build(p, x, n, &c)) return -1;
   int rc = interaction_on_cells(p, &c, step, v, rho, press, m, pid, sid, ct, f, vgt);

@@ -410,6 +410,47 @@ def test_oldroyd_2d_parity(oracle):
     assert vec_err(fm, fo) < 5e-5  # the 2D mixed bar (FORCE_BAR)
 
 
+@pytest.mark.parametrize("kind,ft,brown", [("3d", 2, True), ("3d", 4, False), ("2d", 2, True),
+                                           ("2d", 4, False)])
+def test_oldroyd_force_laws_parity(oracle, kind, ft, brown):
+    # rhs_force_ff_non_Newtonian.F90: Espanol (2: the Hu-Adams terms and
+    # Espanol's thermal force) and the angular form (4: the diagonal of the
+    # pressure tensors averaged, dissipation along e, no thermal force)
+    box = W.oldroyd_box_3d(12) if kind == "3d" else _oldroyd_2d()
+    box.prm = box.prm.replace(rhs_force_type=ft, Brownian=brown)
+    announce(f"Oldroyd-B rhs_force_type {ft}, {kind}, brownian={brown}")
+    d = box.prm.num_dim
+    rng = np.random.default_rng(13)
+    a = rng.normal(0, 0.05, size=(box.n, d, d))
+    ct = (np.tile(np.eye(d), (box.n, 1, 1)) + a + a.transpose(0, 2, 1)).reshape(box.n, d * d)
+    g = gpu_evaluate(box, step=2, ct=ct)
+    f = g.download_force()
+    _, vgt = g.download_tensors()
+    g.close()
+    _, _, fo, vgto = oracle_evaluate(oracle, box, step=2, ct=ct)
+    assert vec_err(f, fo) < 1e-12
+    assert vec_err(vgt, vgto) < 1e-12
+
+
+@pytest.mark.parametrize("ft", [2, 4])
+def test_oldroyd_force_laws_trajectory(oracle, ft):
+    box = W.oldroyd_box_3d(12)
+    box.prm = box.prm.replace(rhs_force_type=ft, Brownian=ft == 2)
+    announce(f"Oldroyd-B rhs_force_type {ft}: trajectory")
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.integrate_VV(4)
+        x, v = g.download()
+        ct, vgt = g.download_tensors()
+    st = oracle.State(box.prm, box.x, box.v, box.m, box.pid)
+    st.prologue(0)
+    for s in range(1, 5):
+        st.vv_step(s)
+    assert vec_err(v, st.v) < 1e-10
+    assert vec_err(ct, st.ct) < 1e-10
+    assert vec_err(vgt, st.vgt) < 1e-9
+
+
 def test_oldroyd_trajectory(oracle):
     box = W.oldroyd_box_3d(12)
     announce("Oldroyd-B trajectory")

@@ -1,6 +1,8 @@
 """The Morris (rhs_force_type 1) and Espanol (2) pair forces in the oracle
 (rhs_force_ff_Newtonian_Morris.F90:112-136,
-rhs_force_ff_Newtonian_Espanol.F90:152-251).  The reference holds no test
+rhs_force_ff_Newtonian_Espanol.F90:152-251), and the Oldroyd-B Espanol and
+angular forms (rhs_force_ff_non_Newtonian_Espanol.F90,
+rhs_force_ff_non_Newtonian_HuAdams_angular.F90).  The reference holds no test
 vectors for them: each is pinned by one pair worked out by hand from the
 reference's formulas, with unequal masses and densities so that a swapped
 mass or density fails, and the Espanol thermal force by its covariance over
@@ -17,14 +19,18 @@ def pair_box(force_type, density_type, **kw):
                       stateEquation_type=2, c=10.0, rho=1.0, rho_ref=1.0, gamma=7.0, eta=0.3, **kw)
 
 
-def one_pair(O, prm, step=0, m=(1.0, 1.7)):
+CF = np.array([[1.2, 0.1, -0.05], [0.1, 0.9, 0.2], [-0.05, 0.2, 1.1]])
+CB = np.array([[0.7, -0.3, 0.0], [-0.3, 1.4, 0.1], [0.0, 0.1, 0.8]])
+
+
+def one_pair(O, prm, step=0, m=(1.0, 1.7), ct=None):
     x = np.array([[10.0, 10.0, 10.0], [11.2, 10.5, 9.6]])
     v = np.array([[0.3, -0.2, 0.1], [-0.1, 0.25, 0.05]])
     m = np.array(m)
     pid = np.array([3, 8], np.int32)
     rho = O.density(prm, x, m)
     p = O.pressure(prm, rho, m)
-    f, _ = O.interaction(prm, step, x, v, rho, p, m, pid)
+    f, _ = O.interaction(prm, step, x, v, rho, p, m, pid, ct=ct)
     d = x[0] - x[1]
     r = np.linalg.norm(d)
     _, gw = O.kernel(prm, r)
@@ -84,3 +90,61 @@ def test_espanol_noise_covariance(oracle):
     assert abs(along.mean()) < 0.1 * A
     assert np.var(along) == pytest.approx(A ** 2 * 2.0 / 3.0 + B ** 2 / 3.0, rel=0.1)
     assert np.mean((across ** 2).sum(1)) / 2.0 == pytest.approx(A ** 2 / 2.0, rel=0.1)
+
+
+def ob_box(force_type, **kw):
+    return pair_box(force_type, 2, Newtonian=False, tau=0.1, n_p=1.0, kt_p=0.5, **kw)
+
+
+def ct_of(*cs):
+    return np.stack([c.reshape(-1, order="F") for c in cs])
+
+
+def test_oldroyd_angular_reduces_to_newtonian_angular(oracle):
+    # with c = I, Pi = p I, and the Oldroyd-B angular form is the Newtonian
+    # angular one (rhs_force_ff_Newtonian_HuAdams_angular.F90:166-229)
+    eye = ct_of(np.eye(3), np.eye(3))
+    f_ob, *_ = one_pair(oracle, ob_box(4), ct=eye)
+    f_n, *_ = one_pair(oracle, pair_box(4, 2))
+    assert np.allclose(f_ob, f_n, rtol=1e-13, atol=0)
+
+
+def test_oldroyd_angular_pair(oracle):
+    G = 0.5
+    prm = ob_box(4)
+    f, x, v, m, rho, p, e, r, gw = one_pair(oracle, prm, ct=ct_of(CF, CB))
+    pi_i = (p[0] + G) * np.eye(3) - G * CF
+    pi_j = (p[1] + G) * np.eye(3) - G * CB
+    bi, bj = 1 / rho[0] ** 2, 1 / rho[1] ** 2
+    f_c = np.zeros(3)
+    for j in range(3):
+        for i in range(3):
+            if i == j:
+                f_c[j] += 0.5 * (pi_i[i, j] + pi_j[i, j]) * e[i] * (bi + bj)
+            else:
+                f_c[j] += pi_i[i, j] * e[i] * bi + pi_j[i, j] * e[i] * bj
+    f_d = prm.eta_coef * prm.eta * (bi + bj) * gw * (e @ (v[0] - v[1])) / r
+    fpair = -f_c * gw + f_d * e
+    assert np.allclose(f[0], fpair / m[0], rtol=1e-13, atol=0)
+    assert np.allclose(f[1], -fpair / m[1], rtol=1e-13, atol=0)
+
+
+def test_oldroyd_espanol(oracle):
+    # the deterministic part is the Oldroyd-B Hu-Adams force; the thermal
+    # force is Espanol's, Z = 4 kT W' / (r n_i n_j)
+    ct = ct_of(CF, CB)
+    f2, *_ = one_pair(oracle, ob_box(2), ct=ct)
+    f3, *_ = one_pair(oracle, ob_box(3), ct=ct)
+    assert np.array_equal(f2, f3)
+    kt, dt = 1e-3, 1e-4
+    f0, x, v, m, rho, p, e, r, gw = one_pair(oracle, ob_box(2, dt=dt), ct=ct)
+    z = 4.0 * kt * gw / r / rho[0] / rho[1]
+    A = np.sqrt(-z * 5.0 * 0.3 / 3.0)
+    along, across = [], []
+    for step in range(1, 2001):
+        f, *_ = one_pair(oracle, ob_box(2, Brownian=True, kt=kt, dt=dt), step=step, ct=ct)
+        fr = (f[0] - f0[0]) * m[0] * np.sqrt(dt)
+        along.append(fr @ e)
+        across.append(((fr - (fr @ e) * e) ** 2).sum() / 2.0)
+    assert np.var(along) == pytest.approx(A ** 2, rel=0.12)  # A = B in 3D
+    assert np.mean(across) == pytest.approx(A ** 2 / 2.0, rel=0.12)

@@ -13,7 +13,7 @@ def pytest_configure(config):
 
 
 def pytest_collection_modifyitems(config, items):
-    # GPU tests run only where a GPU is present (the driver selects them with -m gpu)
+    # GPU tests run only where a GPU is present (select them with -m gpu)
     pass
 
 

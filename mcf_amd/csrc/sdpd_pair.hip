@@ -805,8 +805,8 @@ struct DensityMixed {
   KahanF acc;
   int nin = 0, rechecks = 0;
   static constexpr bool kImages = false;
-  // in batches of eight (C3: 0.59 ms in fours, 0.51 in eights, 0.55 in
-  // sixteens)
+  // in batches of eight (the 128^3 Tait box in mixed precision: 0.59 ms in
+  // fours, 0.51 in eights, 0.55 in sixteens)
   static constexpr bool kBatched = true;
   static constexpr int kBatch = 8;
   template <int NB>
@@ -1182,9 +1182,9 @@ struct ForceF64 {
   // Measured on the 128^3 box: forces 1.824 ms one entry at a time, 1.819
   // two, 1.761 four, 1.65 eight (193 VGPRs, two waves per SIMD); the
   // Oldroyd-B batch of 256 boxes 6.62 -> 6.23 ms with two.
-  // With colloids the Verlet walk takes twos (C4 forces 3.66 ms one at a
-  // time, 3.31 in twos, 3.69 in fours; the substitution branch's registers
-  // leave no room for more).
+  // With colloids the Verlet walk takes twos (512 spheres of radius 5 in
+  // the sheared 128^3 box: forces 3.66 ms one at a time, 3.31 in twos, 3.69
+  // in fours; the substitution branch's registers leave no room for more).
   static constexpr bool kBatched = DECIDE;
   static constexpr int kBatch = COLL ? 2 : (FT >= 5 ? 2 : 8);
   __device__ __forceinline__ void body(uint32_t ent) {
@@ -1476,8 +1476,9 @@ struct ForceMixed {
   WideF L[DIM * DIM];
   static constexpr bool kImages = false;
   // Verlet lists: entries four at a time, the records of all four
-  // gathered before the first pair is evaluated (C3 forces 1.27 ms one at
-  // a time, 1.16 in fours, 1.19 in eights; C5 3.94 in twos, 3.87 in fours)
+  // gathered before the first pair is evaluated (the 128^3 Tait box: forces
+  // 1.27 ms one at a time, 1.16 in fours, 1.19 in eights; 256 Oldroyd-B
+  // boxes of 2k-60k particles: 3.94 in twos, 3.87 in fours)
   static constexpr bool kBatched = DECIDE;
   static constexpr int kBatch = 4;
   // j's records: FP32 position (w: p/n^2 or p, Newtonian), velocity (w:

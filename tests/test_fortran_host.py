@@ -140,6 +140,28 @@ def test_host_matches_python_binding(tmp_path, fused):
     assert np.allclose(xs, xr, rtol=5.1e-8) and np.array_equal(pid, box.pid)
 
 
+@needs_host
+@pytest.mark.gpu
+@pytest.mark.parametrize("ft,bft", [(2, 3), (1, 2)])
+def test_host_force_laws_and_body_forces(tmp_path, ft, bft):
+    # the keys rhs_force_type (1 Morris, 2 Espanol) and body_force_type
+    # (2 two-way, 3 sine) reach the library through the host unchanged
+    box = W.tait_box_3d(12, kt=1e-3, brownian=ft == 2)
+    box.prm = box.prm.replace(rhs_force_type=ft, body_force_type=bft, body_force=[0.5, 0.0, 0.0])
+    nsteps = 4
+    announce(f"Fortran host, rhs_force_type {ft}, body_force_type {bft}")
+    extra = f"body_force_type = {bft}\nbody_force = 0.5, 0.0, 0.0\n"
+    ctrl = write_configs(str(tmp_path), box, nsteps, extra=extra)
+    r = run_host(ctrl, str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    x, v = read_parts(str(tmp_path / parts_name(nsteps)), 3)
+    with SDPD(box.prm) as g:
+        g.upload(box.x, box.v, box.m, box.pid)
+        g.integrate_VV(nsteps)
+        xr, vr = g.download()
+    assert np.array_equal(x, xr) and np.array_equal(v, vr)
+
+
 def shear_v_line(d, le_dim, v_lo, v_hi):
     """The reference's shear_v key (io_read_physics.F90:1378-1402): per face,
     its velocity components along the face, faces x1, x2, y1, y2[, z1, z2]."""

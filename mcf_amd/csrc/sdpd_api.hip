@@ -126,9 +126,6 @@ int derive_params(const sdpd_params& p, DevParams& P, const char*& err) {
   // Brownian needs rhs_force_type 2, 3 or 4)
   if (p.brownian && p.rhs_force_type == 1)
     return err = "Brownian motion requires rhs_force_type 2, 3 or 4", SDPD_ERR_ARG;
-  if (p.precision == SDPD_PRECISION_MIXED && p.rhs_force_type < 3)
-    return err = "mixed precision: rhs_force_type 3 or 4 (Morris and Espanol run in FP64)",
-           SDPD_ERR_ARG;
   if (p.state_equation_type != 1 && p.state_equation_type != 2)
     return err = "stateEquation_type must be 1 or 2", SDPD_ERR_ARG;
   if (p.state_equation_type == 1 && p.c == 0.0)

@@ -1130,13 +1130,15 @@ __global__ void k_refresh_records(DevParams P, DeviceState S) {
 // Random force direction seen by particle i for neighbour j: a Gaussian
 // vector with the covariance of the reference's symmetric traceless
 // dW . e_ij (I/2 + e e^T/6 in 3D, I/2 in 2D), oriented by the lower pid.
+// g3 (Espanol's trace draw): the block's fourth normal, shared by the pair.
 template <int DIM, typename T>
 __device__ __forceinline__ void noise_vector(const DevParams& P, int64_t step, uint32_t pidi,
                                              uint32_t pidj, T e0, T e1, T e2, T& w0, T& w1,
-                                             T& w2) {
+                                             T& w2, T* g3 = nullptr) {
   uint32_t lo = pidi < pidj ? pidi : pidj, hi = pidi < pidj ? pidj : pidi;
   float g[4];
   pair_normals(P.seed, lo, hi, step, g);
+  if (g3) *g3 = (T)g[3];
   const T sgn = pidi < pidj ? T(1) : T(-1);
   const T ca = T(0.70710678118654752440);
   if (DIM == 3) {
@@ -1584,6 +1586,39 @@ struct ForceMixed {
         fi1 = fi1 + f_r * w1;
         fi2 = fi2 + f_r * w2;
       }
+    } else if (FT == 1) {
+      // rhs_force_ff_Newtonian_Morris.F90:112-136 (accelerations carrying
+      // the partner's mass; bi, vj.w = 1/rho)
+      const float mj = (float)S.m[jidx];
+      const float fcm = mj * (ai + xjw) * gw * ir;
+      const float fdm = mj * 2.0f * P.etaf * gw * bi * vj.w * ir;
+      fi0 = fdm * u0 - fcm * d0;
+      fi1 = fdm * u1 - fcm * d1;
+      fi2 = fdm * u2 - fcm * d2;
+    } else if (FT == 2) {
+      // rhs_force_ff_Newtonian_Espanol.F90:152-251 (bi, vj.w = 1/n)
+      const float gnn = gw * bi * vj.w * ir;
+      const float ev = (d0 * u0 + d1 * u1 + d2 * u2) * ir;
+      const float fd1 = (2.0f - 1.0f / DIM) * P.etaf * gnn;
+      const float fd2 = P.etaf * (DIM + 2.0f) / DIM * gnn * ev;
+      const float se = (fd2 - (ai + xjw) * gw) * ir;
+      fi0 = fd1 * u0 + se * d0;
+      fi1 = fd1 * u1 + se * d1;
+      fi2 = fd1 * u2 + se * d2;
+      if (P.noise) {
+        // (A S e + B tr e) / sqrt(dt), tr = g[3] / sqrt(num_dim)
+        float w0, w1, w2, g3;
+        noise_vector<DIM, float>(P, step, pidi, pidj, d0 * ir, d1 * ir, d2 * ir, w0, w1, w2, &g3);
+        const float zij = 4.0f * P.ktf * gnn;
+        const float ea = 5.0f * P.etaf / 3.0f, eb = (DIM + 2.0f) * P.etaf / 3.0f;
+        const float aij = __builtin_amdgcn_sqrtf(-zij * ea) * P.inv_sqrt_dtf;
+        const float bij = __builtin_amdgcn_sqrtf(-zij * DIM / 2.0f * (eb + ea * (2.0f / DIM - 1.0f))) *
+                          P.inv_sqrt_dtf;
+        const float btr = bij * g3 * (DIM == 3 ? 0.57735026918962576f : 0.70710678118654752f) * ir;
+        fi0 = fi0 + aij * w0 + btr * d0;
+        fi1 = fi1 + aij * w1 + btr * d1;
+        fi2 = fi2 + aij * w2 + btr * d2;
+      }
     } else if (FT == 3) {
       const float fcr = (ai + xjw) * gw * ir;
       const float f_d = P.etaf * (bi + vj.w) * gw * ir;
@@ -1712,12 +1747,12 @@ __device__ __forceinline__ void forces_chunk(const DevParams& P, const DeviceSta
   const int64_t n = P.n;
   const int ii = h.ii;
   // the Morris body sums accelerations already
-  const double inv_mi = (!MIXED && FT == 1) ? 1.0 : 1.0 / S.m[ii];
+  const double inv_mi = FT == 1 ? 1.0 : 1.0 / S.m[ii];
   if (MIXED) {
     const float4 vi = S.vr[ii];
     const float4 qi = S.qr[ii];
     // Newtonian records: qi.w = p/n^2 (Hu-Adams) or p (angular), vi.w = 1/n^2
-    ForceMixed<DIM, FT, VERLET> pol{P, S, h, vi.x, vi.y, vi.z, FT == 3 ? qi.w : 0.f, vi.w,
+    ForceMixed<DIM, FT, VERLET> pol{P, S, h, vi.x, vi.y, vi.z, FT <= 3 ? qi.w : 0.f, vi.w,
                             FT == 4 ? qi.w : 0.f,
                             FT == 5 ? __float_as_uint(vi.w) : (uint32_t)S.pid[ii],
                             S.step_dev[0]};
@@ -2014,6 +2049,10 @@ static void launch_forces_dim(hipStream_t s, const DevParams& P, const DeviceSta
       launch_forces_le<DIM, 3, true, false>(s, P, S);
     else if (ft == 4)
       launch_forces_le<DIM, 4, true, false>(s, P, S);
+    else if (ft == 1)
+      launch_forces_le<DIM, 1, true, false>(s, P, S);
+    else if (ft == 2)
+      launch_forces_le<DIM, 2, true, false>(s, P, S);
     else
       launch_forces_le<DIM, 5, true, false>(s, P, S);
   } else if (P.ncoll) {

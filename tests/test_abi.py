@@ -49,7 +49,6 @@ def test_missing_library_fails_loudly(tmp_path):
     dict(max_phys=[5.0, 5.0, 5.0], cut_off=3.0),  # fewer than 3 cells
     dict(rhs_force_type=5),
     dict(rhs_force_type=1, Brownian=True),  # Morris has no thermal noise
-    dict(rhs_force_type=2, precision=1),  # Espanol runs in FP64
     dict(Newtonian=False, tau=0.1, rhs_force_type=1),  # no Oldroyd-B Morris
     dict(Newtonian=False, tau=0.1, rhs_force_type=4, Brownian=True),  # no noise for it
     dict(Newtonian=False, tau=0.1, rhs_force_type=2, precision=1),  # FP64 only

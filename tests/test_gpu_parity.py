@@ -194,6 +194,11 @@ def test_morris_espanol_parity(oracle, kind, ft, dens, brown, sym):
     ro, _, fo, _ = oracle_evaluate(oracle, box, step=4)
     assert rel_err(rho, ro) < 1e-13
     assert vec_err(f, fo) < 1e-12
+    # mixed precision: FP32 pair arithmetic on the same records and normals
+    gm = gpu_evaluate(W.mixed(box), step=4)
+    fm = gm.download_force()
+    gm.close()
+    assert vec_err(fm, fo) < (5e-5 if kind == "2d" else 1e-5)
 
 
 @pytest.mark.parametrize("ft", [1, 2])

@@ -48,8 +48,7 @@ typedef struct sdpd_params {
   int32_t rhs_density_type;   /* 1 mass density, 2 number density            */
   int32_t rhs_force_type;     /* 1 Morris (without noise), 2 Espanol, 3
                                  Hu-Adams, 4 Hu-Adams angular; Oldroyd-B:
-                                 2, 3 or 4 (2 and 4 in FP64, 4 without
-                                 noise)                                      */
+                                 2, 3 or 4 (4 without noise)                 */
   int32_t state_equation_type;/* 1 Morris, 2 Batchelor (Tait)                */
   int32_t newtonian;          /* 0: Oldroyd-B conformation tensor            */
   int32_t brownian;           /* thermal noise                               */

@@ -139,8 +139,6 @@ int derive_params(const sdpd_params& p, DevParams& P, const char*& err) {
   if (!p.newtonian && p.rhs_force_type == 4 && p.brownian)
     return err = "rhs_force_ff_non_Newtonian_HuAdams_angular: Brownian force is not available",
            SDPD_ERR_ARG;
-  if (!p.newtonian && p.rhs_force_type != 3 && p.precision == SDPD_PRECISION_MIXED)
-    return err = "mixed precision Oldroyd-B: rhs_force_type 3", SDPD_ERR_ARG;
   if (!p.newtonian && std::fabs(p.tau) < kMachineZero)
     return err = "particles_compute_act: tau should not be zero", SDPD_ERR_ARG;
   if (p.brownian && p.rhs_density_type != 2)

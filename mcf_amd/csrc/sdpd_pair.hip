@@ -1492,7 +1492,7 @@ struct ForceMixed {
   };
   __device__ __forceinline__ Rec load_rec(int j) const {
     Rec r;
-    if (FT == 5) {
+    if (FT >= 5) {
       r.v = S.vr[j];
       r.x = load_xyz(S.xa + j);
       r.q = S.qr[j];
@@ -1549,10 +1549,19 @@ struct ForceMixed {
     kernel_w_gradw_f32(P.kernel_type, r * P.inv_hf, P.coeff, P.coef_gradf, w, gw);
     const float u0 = vi0 - (vj.x + le_dvf(P, h, o)), u1 = vi1 - vj.y, u2 = vi2 - vj.z;
     float fi0, fi1, fi2;
-    if (FT == 5) {
+    if (FT >= 5) {
       // rhs_force_ff_non_Newtonian_HuAdams.F90:105-131 with the records'
-      // Pi / n^2, and the velocity gradient of pp_vgt_ip.inc:16-20
-      const float sxx = pb[0] + qj.y, syy = pb[1] + qj.z, szz = pb[2] + qj.w;
+      // Pi / n^2, and the velocity gradient of pp_vgt_ip.inc:16-20 (FT 6,
+      // Espanol: its thermal force below; FT 7, angular: the diagonal
+      // averaged, (Pi_i + Pi_j) / 2 (1/n_i^2 + 1/n_j^2), and dissipation
+      // along e)
+      float sxx = pb[0] + qj.y, syy = pb[1] + qj.z, szz = pb[2] + qj.w;
+      if (FT == 7) {
+        const float bsum = bi + qj.x;
+        sxx = 0.5f * (pb[0] / bi + qj.y / qj.x) * bsum;
+        syy = 0.5f * (pb[1] / bi + qj.z / qj.x) * bsum;
+        szz = 0.5f * (pb[2] / bi + qj.w / qj.x) * bsum;
+      }
       const float sxy = pb[3] + q2.x, sxz = pb[4] + q2.y, syz = pb[5] + q2.z;
       float c0, c1, c2;
       if (DIM == 3) {
@@ -1566,9 +1575,17 @@ struct ForceMixed {
       }
       const float gir = gw * ir;
       const float f_d = P.etaf * (bi + qj.x) * gir;
-      fi0 = f_d * u0 - c0 * gir;
-      fi1 = f_d * u1 - c1 * gir;
-      fi2 = f_d * u2 - c2 * gir;
+      if (FT == 7) {
+        const float fde = P.eta_coeff * P.etaf * (bi + qj.x) * gw * (d0 * u0 + d1 * u1 + d2 * u2) *
+                          (ir * ir * ir);
+        fi0 = fde * d0 - c0 * gir;
+        fi1 = fde * d1 - c1 * gir;
+        fi2 = fde * d2 - c2 * gir;
+      } else {
+        fi0 = f_d * u0 - c0 * gir;
+        fi1 = f_d * u1 - c1 * gir;
+        fi2 = f_d * u2 - c2 * gir;
+      }
       const float gfac = gir * inv_ni;
       const float du[3] = {-u0, -u1, -u2};
       const float dx[3] = {d0, d1, d2};
@@ -1578,7 +1595,20 @@ struct ForceMixed {
 #pragma unroll
           for (int a = 0; a < DIM; ++a) L[a + DIM * b].add(du[a] * dx[b] * gfac);
       }
-      if (P.noise) {
+      if (FT == 6 && P.noise) {
+        // Espanol's (A S e + B tr e) / sqrt(dt), Z = 4 kT W' / (r n_i n_j)
+        float w0, w1, w2, g3;
+        noise_vector<DIM, float>(P, step, pidi, pidj, d0 * ir, d1 * ir, d2 * ir, w0, w1, w2, &g3);
+        const float zij = 4.0f * P.ktf * gir * inv_ni * __builtin_amdgcn_sqrtf(qj.x);
+        const float ea = 5.0f * P.etaf / 3.0f, eb = (DIM + 2.0f) * P.etaf / 3.0f;
+        const float aij = __builtin_amdgcn_sqrtf(-zij * ea) * P.inv_sqrt_dtf;
+        const float bij = __builtin_amdgcn_sqrtf(-zij * DIM / 2.0f * (eb + ea * (2.0f / DIM - 1.0f))) *
+                          P.inv_sqrt_dtf;
+        const float btr = bij * g3 * (DIM == 3 ? 0.57735026918962576f : 0.70710678118654752f) * ir;
+        fi0 = fi0 + aij * w0 + btr * d0;
+        fi1 = fi1 + aij * w1 + btr * d1;
+        fi2 = fi2 + aij * w2 + btr * d2;
+      } else if (FT == 5 && P.noise) {
         float w0, w1, w2;
         noise_vector<DIM, float>(P, step, pidi, pidj, d0 * ir, d1 * ir, d2 * ir, w0, w1, w2);
         const float f_r = __builtin_amdgcn_sqrtf(-4.0f * P.ktf * f_d) * P.inv_sqrt_dtf;
@@ -1754,9 +1784,9 @@ __device__ __forceinline__ void forces_chunk(const DevParams& P, const DeviceSta
     // Newtonian records: qi.w = p/n^2 (Hu-Adams) or p (angular), vi.w = 1/n^2
     ForceMixed<DIM, FT, VERLET> pol{P, S, h, vi.x, vi.y, vi.z, FT <= 3 ? qi.w : 0.f, vi.w,
                             FT == 4 ? qi.w : 0.f,
-                            FT == 5 ? __float_as_uint(vi.w) : (uint32_t)S.pid[ii],
+                            FT >= 5 ? __float_as_uint(vi.w) : (uint32_t)S.pid[ii],
                             S.step_dev[0]};
-    if (FT == 5) {
+    if (FT >= 5) {
       const float4 q2 = S.qr2[ii];
       pol.bi = qi.x;
       pol.pb[0] = qi.y;
@@ -1771,7 +1801,7 @@ __device__ __forceinline__ void forces_chunk(const DevParams& P, const DeviceSta
     if (!h.active) return;
     finish_force<DIM>(P, S, h.i, pol.f0.value() * inv_mi, pol.f1.value() * inv_mi,
                       DIM == 3 ? pol.f2.value() * inv_mi : 0.0);
-    if (FT == 5) {
+    if (FT >= 5) {
       double Ld[DIM * DIM];
 #pragma unroll
       for (int q = 0; q < DIM * DIM; ++q) Ld[q] = pol.L[q].value();
@@ -2053,6 +2083,10 @@ static void launch_forces_dim(hipStream_t s, const DevParams& P, const DeviceSta
       launch_forces_le<DIM, 1, true, false>(s, P, S);
     else if (ft == 2)
       launch_forces_le<DIM, 2, true, false>(s, P, S);
+    else if (ft == 6)
+      launch_forces_le<DIM, 6, true, false>(s, P, S);
+    else if (ft == 7)
+      launch_forces_le<DIM, 7, true, false>(s, P, S);
     else
       launch_forces_le<DIM, 5, true, false>(s, P, S);
   } else if (P.ncoll) {

@@ -51,7 +51,6 @@ def test_missing_library_fails_loudly(tmp_path):
     dict(rhs_force_type=1, Brownian=True),  # Morris has no thermal noise
     dict(Newtonian=False, tau=0.1, rhs_force_type=1),  # no Oldroyd-B Morris
     dict(Newtonian=False, tau=0.1, rhs_force_type=4, Brownian=True),  # no noise for it
-    dict(Newtonian=False, tau=0.1, rhs_force_type=2, precision=1),  # FP64 only
     dict(body_force_type=4, body_force=[0.1, 0.0, 0.0]),  # the stochastic model
     dict(body_force_type=-1),
 ])

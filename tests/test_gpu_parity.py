@@ -435,6 +435,13 @@ def test_oldroyd_force_laws_parity(oracle, kind, ft, brown):
     _, _, fo, vgto = oracle_evaluate(oracle, box, step=2, ct=ct)
     assert vec_err(f, fo) < 1e-12
     assert vec_err(vgt, vgto) < 1e-12
+    # mixed precision: FP32 pair arithmetic on the Pi / n^2 records
+    gm = gpu_evaluate(W.mixed(box), step=2, ct=ct)
+    fm = gm.download_force()
+    _, vgtm = gm.download_tensors()
+    gm.close()
+    assert vec_err(fm, fo) < (5e-5 if kind == "2d" else 1e-5)
+    assert vec_err(vgtm, vgto) < 1e-5
 
 
 @pytest.mark.parametrize("ft", [2, 4])

@@ -107,6 +107,26 @@ def test_colloid_le_face_and_interior_chunks(oracle):
     assert np.abs(torque - to).max() < 1e-11 * (np.abs(to).max() + 1e-300)
 
 
+@pytest.mark.parametrize("btype", [2, 3])
+def test_colloid_position_body_forces(oracle, btype):
+    # body forces of types 2 and 3 act on fluid particles only
+    box, cs = make("sphere", True, 2, True)
+    box.prm = box.prm.replace(body_force_type=btype, body_force=[0.3, 0.0, 0.0])
+    announce(f"colloids with body force type {btype}")
+    step = 2
+    with gpu_context(box, cs) as g:
+        g.step = step
+        g.build_list()
+        g.compute_density()
+        g.compute_interaction()
+        f = g.download_force()
+    c = to_oracle(oracle, cs)
+    ro = oracle.density(box.prm, box.x, box.m, box.sid, coll=c)
+    po = oracle.pressure(box.prm, ro, box.m)
+    fo, _ = oracle.interaction(box.prm, step, box.x, box.v, ro, po, box.m, box.pid, box.sid, coll=c)
+    assert vec_err(f, fo) < 1e-12
+
+
 def oldroyd_colloids(kind, sym, le, **kw):
     """The colloid boxes of make() in an Oldroyd-B fluid (tau 0.1, G 0.5)."""
     box, cs = make(kind, sym, 2, True, **kw)
@@ -304,3 +324,7 @@ def test_colloid_errors():
     with SDPD(W.mixed(box).prm) as g:
         with pytest.raises(SDPDError):
             g.set_colloids(cs)
+    for ft in (1, 2):  # the Morris and Espanol laws have no colloid pairs here
+        with SDPD(box.prm.replace(rhs_force_type=ft, Brownian=ft == 2)) as g:
+            with pytest.raises(SDPDError):
+                g.set_colloids(cs)

@@ -73,10 +73,11 @@ def test_le_neighbor_sets(oracle, kind, sym, step):
 
 
 @pytest.mark.parametrize("kind,sym", CASES)
-@pytest.mark.parametrize("force_type", [3, 4])
+@pytest.mark.parametrize("force_type", [3, 4, 1, 2])
 def test_le_density_forces(oracle, kind, sym, force_type):
     box = le_box(kind, sym)
-    box.prm = box.prm.replace(rhs_force_type=force_type)
+    # Morris (1) has no thermal force
+    box.prm = box.prm.replace(rhs_force_type=force_type, Brownian=force_type != 1)
     step = 5
     t = marching_time(step, box.prm.dt)
     rng = np.random.default_rng(4)

@@ -1296,9 +1296,8 @@ int sdpd_set_colloids(sdpd_ctx* c, const sdpd_colloid_params* prm, int32_t num,
   if (c->P.nrep != 1 || !c->rep_lo.empty())
     return fail(c, SDPD_ERR_STATE, "sdpd_set_colloids: one box only (batched context)");
   if (c->P.mixed) return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: FP64 only");
-  if (c->P.force_type < 3 || (!c->P.newtonian && c->P.force_type != 3))
-    return fail(c, SDPD_ERR_ARG,
-                "sdpd_set_colloids: rhs_force_type 3 or 4 (3 in Oldroyd-B flows)");
+  if (!c->P.newtonian && c->P.force_type != 3)
+    return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: rhs_force_type 3 in Oldroyd-B flows");
   if (prm->noslip != 1 && prm->noslip != 2)
     return fail(c, SDPD_ERR_ARG, "sdpd_set_colloids: noslip must be 1 (frozen) or 2 (Morris)");
   if (prm->rho_type != 0 && prm->rho_type != 1)

@@ -1250,7 +1250,7 @@ struct ForceF64 {
     const double bj = rb.w;
     // pressure terms: p_i / n_i^2 + p_j / n_j^2 (type 3), (p_i + p_j) / 2 (type 4)
     double psum = 0.0, pij = 0.0;
-    if (FT == 3) psum = ai + ra.w;
+    if (FT <= 3) psum = ai + ra.w;
     if (FT == 4) pij = (pi + S.p[j]) * 0.5;
     // with a colloid boundary particle in an Oldroyd-B flow both sides take
     // the fluid particle's 1/n^2 and Pi/n^2
@@ -1264,6 +1264,8 @@ struct ForceF64 {
       double pb;
       colloid_substitute(j, jb ? S.sid[j] : 0, ra, rb, d0, d1, d2, jump, u0, u1, u2, pb);
       if (FT == 3) psum = sidi == 0 ? ai + pb * bj : pb * bi + ra.w;
+      // Morris and Espanol: bi, bj = 1/rho
+      if (FT <= 2) psum = sidi == 0 ? ai + pb * (bj * bj) : pb * (bi * bi) + ra.w;
       if (FT == 4) pij = pb;
       if (FT == 5) bsum = sidi == 0 ? bi + bi : bj + bj;
     }
@@ -1273,7 +1275,7 @@ struct ForceF64 {
       // carrying the partner's mass (the caller skips the division by m_i);
       // bi, bj = 1/rho
       const double mj = S.m[j];
-      const double fcm = mj * (ai + ra.w) * gw * ir;
+      const double fcm = mj * psum * gw * ir;
       const double fdm = mj * 2.0 * P.eta * gw * bi * bj * ir;
       a0 = fdm * u0 - fcm * d0;
       a1 = fdm * u1 - fcm * d1;
@@ -1285,7 +1287,7 @@ struct ForceF64 {
       const double fd1 = (2.0 - 1.0 / DIM) * P.eta * gnn;
       const double fd2 = P.eta * (DIM + 2.0) / DIM * gnn * ev;
       // pressure and the dissipative term along e, both along d / r
-      const double se = ((ai + ra.w) * gw * -1.0 + fd2) * ir;
+      const double se = (psum * gw * -1.0 + fd2) * ir;
       a0 = fd1 * u0 + se * d0;
       a1 = fd1 * u1 + se * d1;
       a2 = fd1 * u2 + se * d2;
@@ -2095,6 +2097,10 @@ static void launch_forces_dim(hipStream_t s, const DevParams& P, const DeviceSta
       launch_forces_le<DIM, 3, false, true>(s, P, S);
     else if (ft == 4)
       launch_forces_le<DIM, 4, false, true>(s, P, S);
+    else if (ft == 1)
+      launch_forces_le<DIM, 1, false, true>(s, P, S);
+    else if (ft == 2)
+      launch_forces_le<DIM, 2, false, true>(s, P, S);
     else
       launch_forces_le<DIM, 5, false, true>(s, P, S);
   } else {

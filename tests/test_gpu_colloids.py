@@ -127,6 +127,33 @@ def test_colloid_position_body_forces(oracle, btype):
     assert vec_err(f, fo) < 1e-12
 
 
+@pytest.mark.parametrize("ft,kind,le", [(1, "sphere", False), (2, "sphere", True), (2, "disk", False)])
+def test_colloid_morris_espanol_forces(oracle, ft, kind, le):
+    # pp_interaction_fc.inc / _cf.inc call the same force law for
+    # fluid-boundary pairs, the boundary particle at the no-slip velocity
+    # and with the fluid particle's pressure
+    box, cs = make(kind, True, 2, True)
+    box.prm = box.prm.replace(rhs_force_type=ft, Brownian=ft == 2)
+    if le:
+        box = le_variant(box)
+    announce(f"colloids with rhs_force_type {ft}: {kind} le={le}")
+    step = 3
+    with gpu_context(box, cs) as g:
+        g.step = step
+        g.build_list()
+        g.compute_density()
+        g.compute_interaction()
+        f = g.download_force()
+    c = to_oracle(oracle, cs)
+    t = marching_time(step, box.prm.dt)
+    ro = oracle.density(box.prm, box.x, box.m, box.sid, coll=c, le_time=t)
+    po = oracle.pressure(box.prm, ro, box.m)
+    fo, _ = oracle.interaction(box.prm, step, box.x, box.v, ro, po, box.m, box.pid, box.sid,
+                               coll=c, le_time=t)
+    assert vec_err(f, fo) < 1e-12
+    assert np.abs(f[box.sid > 0]).max() > 0
+
+
 def oldroyd_colloids(kind, sym, le, **kw):
     """The colloid boxes of make() in an Oldroyd-B fluid (tau 0.1, G 0.5)."""
     box, cs = make(kind, sym, 2, True, **kw)
@@ -324,7 +351,7 @@ def test_colloid_errors():
     with SDPD(W.mixed(box).prm) as g:
         with pytest.raises(SDPDError):
             g.set_colloids(cs)
-    for ft in (1, 2):  # the Morris and Espanol laws have no colloid pairs here
-        with SDPD(box.prm.replace(rhs_force_type=ft, Brownian=ft == 2)) as g:
-            with pytest.raises(SDPDError):
-                g.set_colloids(cs)
+    with SDPD(box.prm.replace(Newtonian=False, tau=0.1, n_p=1.0, kt_p=0.5,
+                              rhs_force_type=2)) as g:
+        with pytest.raises(SDPDError):  # Oldroyd-B colloid pairs: the Hu-Adams form
+            g.set_colloids(cs)
